@@ -1,0 +1,188 @@
+"""Benchmark of the DEM contact step on one MI355X.
+
+Workload (default): the 1M-sphere polydisperse hopper (radii U[2,4] mm, 6
+planes, dt = 5e-6 s, cell 8 mm, 128^3 grid), synthetic seeded fill.  One
+"step" = one full DEM time step (Verlet position, hash, sort, cell ranges,
+particle-particle and wall contacts, Verlet velocity) over all particles.
+
+Prints ONE JSON line (rank 0, N = 1).  `value` is the throughput of the fused
+FP32c path when it is built, else of the plain FP64 path; both are reported.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from metakonga_amd import abi, scenes  # noqa: E402
+
+METRIC = "particle-steps/s (FP32-comp batched vs plain FP64) + p50/p99 scene latency"
+HBM_PEAK = 8.0e12
+PHASES = ["vv_position", "hash", "sort", "cell_ranges", "p2p", "walls", "triangles", "vv_velocity"]
+# Algorithmic bytes per particle of the plain FP64 particle-particle kernel:
+# own pos+r 32, vel 24, omega 24, mass 8, sorted index 4, force and moment
+# read+write 96 (neighbour re-reads excluded).
+P2P_F64_BYTES = 188.0
+# Algorithmic bytes per particle-step of the whole FP64 step (reference data model).
+STEP_F64_BYTES = 300.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def run_fp64(scene, steps, warmup, lib):
+    from metakonga_amd.xdem import Simulation
+    sim = Simulation(scene, lib)
+    t0 = time.perf_counter()
+    sim.step(warmup)
+    sim.sync()
+    log(f"fp64 warmup {warmup} steps {time.perf_counter() - t0:.2f}s")
+    abi.check(lib, lib.xdem_sim_enable_timing(sim.h, 1))
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    ms = np.zeros(8)
+    cnt = np.zeros(8, np.uint32)
+    abi.check(lib, lib.xdem_sim_phase_times(sim.h, abi.ptr(ms), abi.ptr(cnt)))
+    abi.check(lib, lib.xdem_sim_enable_timing(sim.h, 0))
+    # untimed check: the run stayed finite
+    st = sim.state()
+    assert np.isfinite(st["pos"]).all()
+    sim.close()
+    n = scene.np
+    return dict(wall_s=wall, ms_per_step=wall * 1e3 / steps, value=n * steps / wall,
+                phase_ms=dict(zip(PHASES, (ms / max(steps, 1)).tolist())),
+                phase_launches=dict(zip(PHASES, cnt.tolist())))
+
+
+def cpu_baseline(scene, budget_s=15.0, threads=1):
+    """Oracle CPU path (serial, the reference's CPU semantics) on a bounded
+    sample of the same workload: all particles, as many steps as fit ~budget."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.load()
+    w = pyoracle.World(scene, semantics=pyoracle.ORC_HOST)
+    t0 = time.perf_counter()
+    w.step(1, threads=threads)
+    t1 = time.perf_counter() - t0
+    k = max(1, min(50, int(budget_s / max(t1, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    w.step(k, threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=scene.np * k / dt, steps=k, seconds=dt)
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--np", type=int, default=1_000_000, help="particles of the hopper workload")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    lib = abi.load()
+    if lib.xdem_device_count() < 1:
+        raise SystemExit("no GPU visible to libxdem.so")
+    t0 = time.perf_counter()
+    scene = scenes.hopper(n=args.np)
+    log(f"scene {scene.name} np={scene.np} grid={scene.grid_size} cs={scene.cell_size:.4g} "
+        f"built in {time.perf_counter() - t0:.1f}s")
+
+    f64 = run_fp64(scene, args.steps, args.warmup, lib)
+    log(f"fp64: {f64['value']:.4g} particle-steps/s, {f64['ms_per_step']:.3f} ms/step")
+    log("fp64 phases ms/step: " + ", ".join(f"{k}={v:.4f}" for k, v in f64["phase_ms"].items()))
+
+    fused = None
+    try:
+        from metakonga_amd import batch
+        if batch.available(lib):
+            fused = batch.bench_fused(scene, args.steps, args.warmup, lib)
+            log(f"fp32c: {fused['value']:.4g} particle-steps/s, {fused['ms_per_step']:.3f} ms/step")
+    except ImportError:
+        fused = None
+
+    if fused is not None:
+        head = fused
+        dtype = "f32 contact + f64 positions"
+        dom_kernel = fused["dominant"]
+        roof = fused["roofline"]
+    else:
+        head = f64
+        dtype = "f64"
+        p2p_ms = f64["phase_ms"]["p2p"]
+        achieved = P2P_F64_BYTES * scene.np / (p2p_ms * 1e-3)
+        dom_kernel = "k_p2p<1> (plain FP64 particle-particle)"
+        roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None,
+                "kernel": dom_kernel, "bytes_per_particle": P2P_F64_BYTES,
+                "kernel_ms": p2p_ms}
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle, CPU-path semantics, 1 thread) ...")
+        c = cpu_baseline(scene, budget_s=args.cpu_budget)
+        cpu = {"value": c["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{scene.name} all {scene.np} particles, {c['steps']} steps "
+                         f"({c['seconds']:.1f}s), serial CPU-path semantics",
+               "cpu": cpu_model()}
+        log(f"cpu: {c['value']:.4g} particle-steps/s over {c['steps']} steps")
+
+    out = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded jittered-lattice fill, radii U[2,4] mm)",
+        "config": {"workload": f"hopper {scene.np} polydisperse spheres, 6 planes, dt 5e-6 s, "
+                               f"cell 8 mm, grid {scene.grid_size[0]}^3",
+                   "np": scene.np, "path": "fp32c-fused" if fused else "fp64-plain"},
+        "fp64_plain": {"value": f64["value"], "ms_per_step": f64["ms_per_step"],
+                       "phase_ms": f64["phase_ms"],
+                       "step_roofline_frac": STEP_F64_BYTES * f64["value"] / HBM_PEAK},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if fused is not None:
+        out["fp32c_fused"] = {k: v for k, v in fused.items() if k not in ("roofline",)}
+        out["speedup_vs_fp64"] = fused["value"] / f64["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

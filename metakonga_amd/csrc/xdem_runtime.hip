@@ -1,0 +1,221 @@
+// xdem_runtime.hip — library state: last error, stream, symbolic parameters,
+// workspace, and the memory helpers of the C ABI.
+#include <cstdarg>
+#include <mutex>
+
+#include "xdem_common.hpp"
+
+namespace xdem {
+
+namespace {
+thread_local std::string g_err;
+hipStream_t g_stream = nullptr;
+xdem_device_parameters g_params = {};
+bool g_params_set = false;
+Workspace g_ws[4];
+}  // namespace
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+hipStream_t stream() { return g_stream; }
+
+KParams params_from(const xdem_device_parameters& p) {
+    KParams k{};
+    k.np = p.np;
+    k.nsphere = p.nsphere;
+    k.ncell = p.ncell;
+    k.gx = p.grid_size.x;
+    k.gy = p.grid_size.y;
+    k.gz = p.grid_size.z;
+    k.rolling = p.rollingCondition ? 1 : 0;
+    k.dt = p.dt;
+    k.half2dt = p.half2dt;
+    k.cs = p.cell_size;
+    k.coh = p.cohesion;
+    k.gxv = p.gravity.x;
+    k.gyv = p.gravity.y;
+    k.gzv = p.gravity.z;
+    k.wox = p.world_origin.x;
+    k.woy = p.world_origin.y;
+    k.woz = p.world_origin.z;
+    return k;
+}
+
+const xdem_device_parameters& host_params() { return g_params; }
+bool params_set() { return g_params_set; }
+
+int32_t Workspace::reserve(size_t need) {
+    if (need <= bytes) return XDEM_OK;
+    if (ptr) {
+        XDEM_HIP(hipStreamSynchronize(g_stream));
+        XDEM_HIP(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+    }
+    size_t b = need + need / 4;
+    XDEM_HIP(hipMalloc(&ptr, b));
+    bytes = b;
+    return XDEM_OK;
+}
+
+Workspace& workspace(int slot) { return g_ws[slot]; }
+
+int32_t PhaseTimer::begin(int ph, hipStream_t s) {
+    if (!on) return XDEM_OK;
+    if (used == phase.size()) {
+        hipEvent_t a, b;
+        XDEM_HIP(hipEventCreate(&a));
+        XDEM_HIP(hipEventCreate(&b));
+        ev.push_back(a);
+        ev.push_back(b);
+        phase.push_back(ph);
+    }
+    phase[used] = ph;
+    open = (int)used;
+    XDEM_HIP(hipEventRecord(ev[2 * used], s));
+    ++used;
+    return XDEM_OK;
+}
+
+int32_t PhaseTimer::end(hipStream_t s) {
+    if (!on || open < 0) return XDEM_OK;
+    XDEM_HIP(hipEventRecord(ev[2 * open + 1], s));
+    open = -1;
+    return XDEM_OK;
+}
+
+int32_t PhaseTimer::flush() {
+    for (size_t i = 0; i < used; ++i) {
+        XDEM_HIP(hipEventSynchronize(ev[2 * i + 1]));
+        float t = 0.f;
+        XDEM_HIP(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+        ms[phase[i]] += t;
+        count[phase[i]] += 1;
+    }
+    used = 0;
+    return XDEM_OK;
+}
+
+void PhaseTimer::reset() {
+    used = 0;
+    open = -1;
+    for (int i = 0; i < NPHASE; ++i) {
+        ms[i] = 0;
+        count[i] = 0;
+    }
+}
+
+PhaseTimer::~PhaseTimer() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+}
+
+static bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
+
+int32_t validate_params(const xdem_device_parameters& p) {
+    XDEM_REQUIRE(is_pow2(p.grid_size.x) && is_pow2(p.grid_size.y) && is_pow2(p.grid_size.z),
+                 "grid_size (%u,%u,%u) must be powers of two (the hash wraps by mask)",
+                 p.grid_size.x, p.grid_size.y, p.grid_size.z);
+    uint64_t nc = (uint64_t)p.grid_size.x * p.grid_size.y * p.grid_size.z;
+    XDEM_REQUIRE(nc <= (1ull << 24), "grid of %llu cells exceeds the 24-bit hash range",
+                 (unsigned long long)nc);
+    XDEM_REQUIRE(p.ncell == nc, "ncell %u != gx*gy*gz %llu", p.ncell, (unsigned long long)nc);
+    XDEM_REQUIRE(p.cell_size > 0, "cell_size must be > 0");
+    return XDEM_OK;
+}
+
+}  // namespace xdem
+
+using namespace xdem;
+
+extern "C" {
+
+const char* xdem_last_error(void) { return g_err.c_str(); }
+
+int32_t xdem_version(void) { return 10000; }
+
+int32_t xdem_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int32_t xdem_synchronize(void) {
+    XDEM_HIP(hipStreamSynchronize(g_stream));
+    return XDEM_OK;
+}
+
+int32_t xdem_malloc(void** dptr, size_t bytes) {
+    XDEM_REQUIRE(dptr, "null dptr");
+    *dptr = nullptr;
+    if (bytes == 0) return XDEM_OK;
+    XDEM_HIP(hipMalloc(dptr, bytes));
+    return XDEM_OK;
+}
+
+int32_t xdem_free(void* dptr) {
+    if (!dptr) return XDEM_OK;
+    XDEM_HIP(hipFree(dptr));
+    return XDEM_OK;
+}
+
+int32_t xdem_memset(void* dptr, int value, size_t bytes) {
+    if (!bytes) return XDEM_OK;
+    XDEM_HIP(hipMemsetAsync(dptr, value, bytes, g_stream));
+    return XDEM_OK;
+}
+
+int32_t xdem_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return XDEM_OK;
+    XDEM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    XDEM_HIP(hipStreamSynchronize(g_stream));
+    return XDEM_OK;
+}
+
+int32_t xdem_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return XDEM_OK;
+    XDEM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+    XDEM_HIP(hipStreamSynchronize(g_stream));
+    return XDEM_OK;
+}
+
+int32_t xdem_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return XDEM_OK;
+    XDEM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+    return XDEM_OK;
+}
+
+// setSymbolicParameter — mphysics_cuda_def.cu:13-16 (cudaMemcpyToSymbol(cte)).
+int32_t xdem_set_symbolic_parameter(const xdem_device_parameters* h) {
+    XDEM_REQUIRE(h, "null parameters");
+    XDEM_TRY(validate_params(*h));
+    g_params = *h;
+    g_params_set = true;
+    return XDEM_OK;
+}
+
+int32_t xdem_get_symbolic_parameter(xdem_device_parameters* out) {
+    XDEM_REQUIRE(out, "null out");
+    if (!g_params_set) {
+        set_error("parameters never set (xdem_set_symbolic_parameter)");
+        return XDEM_ERR_STATE;
+    }
+    *out = g_params;
+    return XDEM_OK;
+}
+
+int32_t xdem_set_stream(void* s) {
+    g_stream = static_cast<hipStream_t>(s);
+    return XDEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+"""Python mirror of the reference's host objects over the C ABI (libxdem.so).
+
+`Simulation` follows dem_simulation (source/simulation/dem_simulation.cpp:204-325,
+447-469): walls are applied in the order they are added, like the reference's
+contact map; `Grid` follows grid_base/neighborhood_cell (include/grid_base.h).
+These are thin ctypes wrappers: every computation runs in libxdem.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+def _desc(scene) -> abi.SimDesc:
+    d = abi.SimDesc()
+    d.np = scene.np
+    d.dt = scene.dt
+    d.gravity = abi._d3(scene.gravity)
+    d.world_origin = abi._d3(scene.world_origin)
+    d.cell_size = scene.cell_size
+    d.grid_size = abi.uint3(*scene.grid_size)
+    d.rolling = scene.rolling
+    d.tcm = scene.tcm
+    m = scene.material
+    d.particle = abi.Material(m.density, m.youngs, m.poisson, m.shear)
+    c = scene.contact
+    d.pp = abi.ContactPairDesc(c.restitution, c.friction, c.rolling_friction, c.cohesion,
+                               c.stiffness_ratio, d.particle)
+    return d
+
+
+class Simulation:
+    """One scene on the plain FP64 GPU path (xdem_sim_*)."""
+
+    def __init__(self, scene, lib=None):
+        self.lib = lib or abi.load()
+        self.scene = scene
+        self.desc = _desc(scene)
+        h = C.c_void_p()
+        pos = abi.as_f64(scene.pos)
+        abi.check(self.lib, self.lib.xdem_sim_create(C.byref(self.desc), abi.ptr(pos), C.byref(h)))
+        self.h = h
+        c = scene.contact
+        wall = abi.ContactPairDesc(c.restitution, c.friction, c.rolling_friction, c.cohesion,
+                                   c.stiffness_ratio, self.desc.particle)
+        for pl in scene.planes:
+            abi.check(self.lib, self.lib.xdem_sim_add_plane(h, C.byref(pl), C.byref(wall)))
+        for cy in scene.cylinders:
+            abi.check(self.lib, self.lib.xdem_sim_add_cylinder(h, C.byref(cy), C.byref(wall)))
+
+    @property
+    def np(self):
+        return int(self.lib.xdem_sim_np(self.h))
+
+    def step(self, n=1):
+        abi.check(self.lib, self.lib.xdem_sim_step(self.h, n))
+
+    def sync(self):
+        abi.check(self.lib, self.lib.xdem_synchronize())
+
+    def state(self):
+        n = self.np
+        out = dict(pos=np.empty((n, 4)), vel=np.empty((n, 3)), omega=np.empty((n, 3)),
+                   acc=np.empty((n, 3)), alpha=np.empty((n, 3)), force=np.empty((n, 3)),
+                   moment=np.empty((n, 3)))
+        abi.check(self.lib, self.lib.xdem_sim_get_state(
+            self.h, *(abi.ptr(out[k]) for k in ("pos", "vel", "omega", "acc", "alpha", "force",
+                                                 "moment"))))
+        return out
+
+    def set_state(self, **kw):
+        n = self.np
+        arrs = []
+        for k, shp in (("pos", 4), ("vel", 3), ("omega", 3), ("acc", 3), ("alpha", 3),
+                       ("force", 3), ("moment", 3)):
+            a = kw.get(k)
+            arrs.append(None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(n, shp))
+        abi.check(self.lib, self.lib.xdem_sim_set_state(self.h, *(abi.ptr(a) for a in arrs)))
+
+    def mass_inertia(self):
+        ptrs = [C.c_void_p() for _ in range(9)]
+        abi.check(self.lib, self.lib.xdem_sim_device_arrays(self.h, *(C.byref(p) for p in ptrs)))
+        n = self.np
+        m = np.empty(n)
+        i = np.empty(n)
+        abi.check(self.lib, self.lib.xdem_memcpy_dtoh(abi.ptr(m), ptrs[7], 8 * n))
+        abi.check(self.lib, self.lib.xdem_memcpy_dtoh(abi.ptr(i), ptrs[8], 8 * n))
+        return m, i
+
+    def close(self):
+        if self.h:
+            self.lib.xdem_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,296 @@
+"""ctypes binding of oracle/liboracle.so — the CPU FP64 restatement of the
+reference step.  TEST INFRASTRUCTURE: loaded only by tests/, smoke() and the
+cpu_baseline leg of bench.py, as the checker / the timed CPU baseline."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from metakonga_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+ORC_GPU, ORC_HOST = 0, 1
+ORC_WALL_PLANE, ORC_WALL_CYLINDER = 0, 1
+
+
+class OrcWall(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("index", C.c_int32), ("cube_member", C.c_int32),
+                ("pad_", C.c_int32), ("cp", abi.ContactProperty)]
+
+
+class OrcWorld(C.Structure):
+    _fields_ = [("prm", abi.DeviceParameters), ("tcm", C.c_int32), ("semantics", C.c_int32),
+                ("np", C.c_uint32)] + [(n, C.c_void_p) for n in (
+                    "pos", "vel", "omega", "acc", "alpha", "force", "moment", "mass", "iner")] + [
+        ("cp_pp", abi.ContactProperty), ("nwall", C.c_uint32), ("walls", C.c_void_p),
+        ("planes", C.c_void_p), ("cylinders", C.c_void_p), ("ntri", C.c_uint32),
+        ("tri", C.c_void_p), ("tri_sphere", C.c_void_p), ("nobj", C.c_uint32), ("obj", C.c_void_p),
+        ("cp_obj", C.c_void_p)] + [(n, C.c_void_p) for n in ("hash", "index", "sorted", "cstart",
+                                                              "cend")]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    vp, u32, i32, f64, u64 = C.c_void_p, C.c_uint32, C.c_int32, C.c_double, C.c_uint64
+    sig = {
+        "orc_cell_of": ([vp, C.c_int, f64, f64, f64, vp], None),
+        "orc_hash_of": ([vp, vp], u32),
+        "orc_hash_and_index": ([vp, C.c_int, vp, u32, u32, vp, vp], None),
+        "orc_reorder_and_find_cell_start": ([vp, vp, u32, u32, vp, vp, vp], None),
+        "orc_force_constant": ([i32, f64, f64, f64, f64, vp, vp], None),
+        "orc_p2p": ([vp, i32, C.c_int] + [vp] * 10 + [u32, vp, u64], u64),
+        "orc_pairs_bruteforce": ([vp, u32, vp, u64], u64),
+        "orc_plane_detect": ([vp, C.c_int, vp, vp, vp, f64], f64),
+        "orc_plane": ([vp, i32, vp] + [vp] * 6 + [u32, vp], None),
+        "orc_cylinder_detect": ([vp, vp, vp, vp], f64),
+        "orc_cylinder": ([vp, i32, vp] + [vp] * 6 + [u32, vp, vp, vp, vp], None),
+        "orc_closest_pt_triangle": ([vp, vp, vp], None),
+        "orc_polygon": ([vp, i32, vp, vp, u32, C.c_int] + [vp] * 9 + [vp, u32], None),
+        "orc_vv_position": ([vp, C.c_int, vp, vp, vp, u32], None),
+        "orc_vv_velocity": ([vp, C.c_int] + [vp] * 8 + [u32], None),
+        "orc_reduction_d3": ([vp, u32, vp], None),
+        "orc_update_polygon": ([vp, vp, vp, vp, u32], None),
+        "orc_circumcenter": ([vp, vp, vp, vp], None),
+        "orc_step": ([vp, u32], None),
+        "orc_step_omp": ([vp, u32, i32], None),
+    }
+    for n, (a, r) in sig.items():
+        f = getattr(L, n)
+        f.argtypes = a
+        f.restype = r
+
+
+P = abi.ptr
+
+
+def byref(s):
+    return C.cast(C.pointer(s), C.c_void_p)
+
+
+# ------------------------------------------------------------------ wrappers
+def cell_of(prm, x, y, z, semantics=ORC_GPU):
+    out = np.zeros(3, np.int32)
+    load().orc_cell_of(byref(prm), semantics, x, y, z, P(out))
+    return out
+
+
+def hash_of(prm, c):
+    c = np.ascontiguousarray(c, np.int32)
+    return int(load().orc_hash_of(byref(prm), P(c)))
+
+
+def hash_and_index(prm, pos4, n=None, off=0, semantics=ORC_GPU, hash_=None, index=None):
+    pos4 = abi.as_f64(pos4)
+    n = pos4.shape[0] if n is None else n
+    if hash_ is None:
+        hash_ = np.zeros(off + n, np.uint32)
+        index = np.zeros(off + n, np.uint32)
+    load().orc_hash_and_index(byref(prm), semantics, P(pos4), n, off, P(hash_), P(index))
+    return hash_, index
+
+
+def reorder(hash_, index, ncell):
+    hash_ = np.ascontiguousarray(hash_, np.uint32).copy()
+    index = np.ascontiguousarray(index, np.uint32).copy()
+    n = hash_.size
+    cstart = np.zeros(ncell, np.uint32)
+    cend = np.zeros(ncell, np.uint32)
+    sorted_ = np.zeros(n, np.uint32)
+    load().orc_reorder_and_find_cell_start(P(hash_), P(index), n, ncell, P(cstart), P(cend),
+                                           P(sorted_))
+    return hash_, index, cstart, cend, sorted_
+
+
+def force_constant(tcm, ir, jr, im, jm, cp):
+    out = abi.ForceConstant()
+    load().orc_force_constant(tcm, ir, jr, im, jm, byref(cp), byref(out))
+    return out
+
+
+def p2p(prm, tcm, pos4, vel, omega, force, moment, mass, sorted_, cstart, cend, cp,
+        semantics=ORC_GPU, want_pairs=False, max_pairs=None):
+    np_ = pos4.shape[0]
+    pairs = None
+    if want_pairs:
+        max_pairs = max_pairs or 64 * np_
+        pairs = np.zeros((max_pairs, 2), np.uint32)
+    n = load().orc_p2p(byref(prm), tcm, semantics, P(pos4), P(vel), P(omega), P(force), P(moment),
+                       P(mass), P(sorted_), P(cstart), P(cend), byref(cp), np_, P(pairs),
+                       max_pairs or 0)
+    if want_pairs:
+        assert n <= max_pairs
+        return pairs[:n].copy()
+    return n
+
+
+def pairs_bruteforce(pos4, max_pairs=None):
+    pos4 = abi.as_f64(pos4)
+    max_pairs = max_pairs or 64 * pos4.shape[0]
+    pairs = np.zeros((max_pairs, 2), np.uint32)
+    n = load().orc_pairs_bruteforce(P(pos4), pos4.shape[0], P(pairs), max_pairs)
+    assert n <= max_pairs
+    return pairs[:n].copy()
+
+
+def plane_detect(plane, xp, wp, r, semantics=ORC_GPU):
+    xp = abi.as_f64(xp)
+    wp = abi.as_f64(wp)
+    u = np.zeros(3)
+    c = load().orc_plane_detect(byref(plane), semantics, P(xp), P(wp), P(u), r)
+    return c, u
+
+
+def plane(prm, tcm, pl, pos4, vel, omega, force, moment, mass, cp):
+    load().orc_plane(byref(prm), tcm, byref(pl), P(pos4), P(vel), P(omega), P(force), P(moment),
+                     P(mass), pos4.shape[0], byref(cp))
+
+
+def cylinder_detect(cy, pt):
+    pt = abi.as_f64(pt)
+    u = np.zeros(3)
+    cp = np.zeros(3)
+    c = load().orc_cylinder_detect(byref(cy), P(pt), P(u), P(cp))
+    return c, u, cp
+
+
+def cylinder(prm, tcm, cy, pos4, vel, omega, force, moment, mass, cp, mpos=None):
+    n = pos4.shape[0]
+    mf = np.zeros((n, 3))
+    mm = np.zeros((n, 3))
+    mpos = abi.as_f64(mpos if mpos is not None else np.zeros(3))
+    load().orc_cylinder(byref(prm), tcm, byref(cy), P(pos4), P(vel), P(omega), P(force),
+                        P(moment), P(mass), n, byref(cp), P(mpos), P(mf), P(mm))
+    return mf, mm
+
+
+def closest_pt_triangle(tri, p):
+    p = abi.as_f64(p)
+    out = np.zeros(3)
+    load().orc_closest_pt_triangle(byref(tri), P(p), P(out))
+    return out
+
+
+def polygon(prm, tcm, dpi_arr, dpmi_arr, pos4, vel, omega, force, moment, mass, sorted_, cstart,
+            cend, cp_arr, obj_reaction=True):
+    load().orc_polygon(byref(prm), tcm, C.cast(dpi_arr, C.c_void_p),
+                       C.cast(dpmi_arr, C.c_void_p), len(dpmi_arr), int(obj_reaction), P(pos4),
+                       P(vel), P(omega), P(force), P(moment), P(mass), P(sorted_), P(cstart),
+                       P(cend), C.cast(cp_arr, C.c_void_p), pos4.shape[0])
+
+
+def vv_position(prm, pos4, vel, acc, semantics=ORC_GPU):
+    load().orc_vv_position(byref(prm), semantics, P(pos4), P(vel), P(acc), pos4.shape[0])
+
+
+def vv_velocity(prm, vel, acc, omega, alpha, force, moment, mass, iner, semantics=ORC_GPU):
+    load().orc_vv_velocity(byref(prm), semantics, P(vel), P(acc), P(omega), P(alpha), P(force),
+                           P(moment), P(mass), P(iner), vel.shape[0])
+
+
+def reduction_d3(a):
+    a = abi.as_f64(a)
+    out = np.zeros(3)
+    load().orc_reduction_d3(P(a), a.shape[0], P(out))
+    return out
+
+
+def circumcenter(P_, Q_, R_):
+    out = np.zeros(3)
+    a, b, c = (abi.as_f64(x) for x in (P_, Q_, R_))
+    load().orc_circumcenter(P(a), P(b), P(c), P(out))
+    return out
+
+
+def update_polygon(dpmi_arr, vlist, sphere4, dpi_arr):
+    vlist = abi.as_f64(vlist)
+    load().orc_update_polygon(C.cast(dpmi_arr, C.c_void_p), P(vlist), P(sphere4),
+                              C.cast(dpi_arr, C.c_void_p), len(dpi_arr))
+
+
+class World:
+    """Owns the host arrays of one oracle scene (reference layout)."""
+
+    def __init__(self, scene, semantics=ORC_GPU, mass=None, iner=None, tri_info=None):
+        self.scene = scene
+        n = scene.np
+        self.pos = abi.as_f64(scene.pos).copy()
+        self.vel = np.zeros((n, 3))
+        self.omega = np.zeros((n, 3))
+        self.acc = np.zeros((n, 3))
+        self.alpha = np.zeros((n, 3))
+        if mass is None:
+            mass, iner = scene.mass_inertia()
+        self.mass = abi.as_f64(mass).copy()
+        self.iner = abi.as_f64(iner).copy()
+        g = np.asarray(scene.gravity)
+        self.force = self.mass[:, None] * g[None, :]     # dem_simulation.cpp:246-248
+        self.moment = np.zeros((n, 3))
+        self.force = np.ascontiguousarray(self.force)
+        w = OrcWorld()
+        w.prm = scene.params()
+        w.tcm = scene.tcm
+        w.semantics = semantics
+        w.np = n
+        for k in ("pos", "vel", "omega", "acc", "alpha", "force", "moment", "mass", "iner"):
+            setattr(w, k, getattr(self, k).ctypes.data)
+        w.cp_pp = scene.contact_pp()
+        cpw = scene.contact_pp()
+        walls = []
+        for i, _ in enumerate(scene.planes):
+            walls.append(OrcWall(ORC_WALL_PLANE, i, 0, 0, cpw))
+        for i, _ in enumerate(scene.cylinders):
+            walls.append(OrcWall(ORC_WALL_CYLINDER, i, 0, 0, cpw))
+        self.walls = abi.struct_array(OrcWall, walls) if walls else None
+        self.planes = abi.struct_array(abi.PlaneInfo, scene.planes) if scene.planes else None
+        self.cyls = abi.struct_array(abi.CylinderInfo, scene.cylinders) if scene.cylinders else None
+        w.nwall = len(walls)
+        w.walls = C.cast(self.walls, C.c_void_p) if walls else None
+        w.planes = C.cast(self.planes, C.c_void_p) if self.planes else None
+        w.cylinders = C.cast(self.cyls, C.c_void_p) if self.cyls else None
+        nt = 0
+        if tri_info is not None:
+            dpi_arr, sph, dpmi_arr, cpo_arr = tri_info
+            nt = len(dpi_arr)
+            self.dpi, self.sph, self.dpmi, self.cpo = dpi_arr, abi.as_f64(sph), dpmi_arr, cpo_arr
+            w.ntri = nt
+            w.tri = C.cast(dpi_arr, C.c_void_p)
+            w.tri_sphere = self.sph.ctypes.data
+            w.nobj = len(dpmi_arr)
+            w.obj = C.cast(dpmi_arr, C.c_void_p)
+            w.cp_obj = C.cast(cpo_arr, C.c_void_p)
+        ncell = w.prm.ncell
+        self.hash = np.zeros(n + nt, np.uint32)
+        self.index = np.zeros(n + nt, np.uint32)
+        self.sorted = np.zeros(n + nt, np.uint32)
+        self.cstart = np.zeros(ncell, np.uint32)
+        self.cend = np.zeros(ncell, np.uint32)
+        for k in ("hash", "index", "sorted", "cstart", "cend"):
+            setattr(w, k, getattr(self, k).ctypes.data)
+        self.w = w
+
+    def step(self, nsteps=1, threads=1):
+        if threads > 1:
+            load().orc_step_omp(byref(self.w), nsteps, threads)
+        else:
+            load().orc_step(byref(self.w), nsteps)

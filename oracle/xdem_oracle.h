@@ -1,0 +1,144 @@
+/*
+ * xdem_oracle.h — CPU restatement (FP64) of the reference DEM contact step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or the timed CPU baseline).  The product (libxdem.so) never calls it.
+ *
+ * Two semantics (the reference's CPU path and its GPU kernels differ):
+ *   ORC_GPU  (default) follows include/mphysics_cuda_impl.cuh:65-1021,1898-1956
+ *   ORC_HOST follows the serial CPU path: source/neighborhood_cell.cpp:31-84,
+ *            source/contactManager.cpp:208-277, source/collision/contact.cpp:20-46,
+ *            source/collision/contact_particles_{particles,plane,cube}.cpp,
+ *            source/velocity_verlet.cpp:18-67.
+ * Differences between the two are listed per function in xdem_oracle.cpp.
+ *
+ * Parity status per function (what pins it) is recorded in the header comment
+ * of xdem_oracle.cpp and checked by tests/test_oracle_*.py.
+ */
+#ifndef XDEM_ORACLE_H
+#define XDEM_ORACLE_H
+#include "../include/xdem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_GPU = 0, ORC_HOST = 1 };
+enum { ORC_WALL_PLANE = 0, ORC_WALL_CYLINDER = 1 };
+
+/* One wall contact in application order. */
+typedef struct orc_wall {
+    int32_t kind;          /* ORC_WALL_PLANE / ORC_WALL_CYLINDER */
+    int32_t index;         /* into planes[] or cylinders[] */
+    int32_t cube_member;   /* 1: plane of a cube (CPU path accumulates; a lone plane assigns) */
+    int32_t pad_;
+    xdem_device_contact_property cp;
+} orc_wall;
+
+/* A whole single-scene state, host memory, reference layout. */
+typedef struct orc_world {
+    xdem_device_parameters prm;  /* np, grid, dt, gravity, rolling ... */
+    int32_t tcm;
+    int32_t semantics;           /* ORC_GPU / ORC_HOST */
+    uint32_t np;
+    double* pos;    /* 4*np */
+    double* vel;    /* 3*np */
+    double* omega;
+    double* acc;
+    double* alpha;
+    double* force;
+    double* moment;
+    double* mass;   /* np */
+    double* iner;
+    xdem_device_contact_property cp_pp;
+    uint32_t nwall;
+    const orc_wall* walls;
+    const xdem_device_plane_info* planes;
+    const xdem_device_cylinder_info* cylinders;
+    /* triangles (polygon objects) */
+    uint32_t ntri;
+    const xdem_device_polygon_info* tri;
+    const double* tri_sphere;    /* 4*ntri circumspheres */
+    uint32_t nobj;
+    xdem_device_polygon_mass_info* obj;
+    const xdem_device_contact_property* cp_obj;
+    /* scratch owned by the caller: n = np + ntri entries / ncell entries */
+    uint32_t* hash;
+    uint32_t* index;
+    uint32_t* sorted;
+    uint32_t* cstart;
+    uint32_t* cend;
+} orc_world;
+
+/* grid_base::getCellNumber / calcGridPos + calcGridHash */
+void orc_cell_of(const xdem_device_parameters* p, int semantics, double x, double y, double z,
+                 int32_t out[3]);
+uint32_t orc_hash_of(const xdem_device_parameters* p, const int32_t c[3]);
+/* K3 / K4: hash[off+i], index[off+i] = off+i for i < n */
+void orc_hash_and_index(const xdem_device_parameters* p, int semantics, const double* pos4,
+                        uint32_t n, uint32_t off, uint32_t* hash, uint32_t* index);
+/* K5-K7: stable sort by hash, cell ranges over ncell, sorted_index */
+void orc_reorder_and_find_cell_start(uint32_t* hash, uint32_t* index, uint32_t n, uint32_t ncell,
+                                     uint32_t* cstart, uint32_t* cend, uint32_t* sorted);
+/* getConstant (mphysics_cuda_impl.cuh:214-256) */
+void orc_force_constant(int32_t tcm, double ir, double jr, double im, double jm,
+                        const xdem_device_contact_property* cp, xdem_device_force_constant* out);
+/* K8 (GPU) / hostCollision p2p part (HOST, adds into force/moment).  If pairs
+ * is non-null, the (i,k) contact pairs are appended in traversal order
+ * (at most max_pairs); the total count is returned. */
+uint64_t orc_p2p(const xdem_device_parameters* p, int32_t tcm, int semantics, const double* pos4,
+                 const double* vel, const double* omega, double* force, double* moment,
+                 const double* mass, const uint32_t* sorted, const uint32_t* cstart,
+                 const uint32_t* cend, const xdem_device_contact_property* cp, uint32_t np,
+                 uint32_t* pairs, uint64_t max_pairs);
+/* Brute-force O(N^2) pair predicate, pairs (i,k) i!=k, ordered by i then k. */
+uint64_t orc_pairs_bruteforce(const double* pos4, uint32_t np, uint32_t* pairs, uint64_t max_pairs);
+/* particle_plane_contact_detection (returns cdist, writes unit) */
+double orc_plane_detect(const xdem_device_plane_info* pe, int semantics, const double xp[3],
+                        const double wp[3], double u[3], double r);
+/* K9 on one plane (GPU: force += ...) */
+void orc_plane(const xdem_device_parameters* p, int32_t tcm, const xdem_device_plane_info* pl,
+               const double* pos4, const double* vel, const double* omega, double* force,
+               double* moment, const double* mass, uint32_t np,
+               const xdem_device_contact_property* cp);
+/* cylinder detection (returns float-truncated cdist as the reference) */
+double orc_cylinder_detect(const xdem_device_cylinder_info* cy, const double pt[4], double u[3],
+                           double cpt[3]);
+void orc_cylinder(const xdem_device_parameters* p, int32_t tcm, const xdem_device_cylinder_info* cy,
+                  const double* pos4, const double* vel, const double* omega, double* force,
+                  double* moment, const double* mass, uint32_t np,
+                  const xdem_device_contact_property* cp, const double* mpos3, double* mf3,
+                  double* mm3);
+/* closestPtPointTriangle (mphysics_cuda_impl.cuh:837-899) */
+void orc_closest_pt_triangle(const xdem_device_polygon_info* t, const double p[3], double out[3]);
+/* K11; reaction sums (per object, zeroed first) go to obj[].force/moment when obj_reaction != 0 */
+void orc_polygon(const xdem_device_parameters* p, int32_t tcm, const xdem_device_polygon_info* dpi,
+                 xdem_device_polygon_mass_info* dpmi, uint32_t nobj, int obj_reaction,
+                 const double* pos4, const double* vel, const double* omega, double* force,
+                 double* moment, const double* mass, const uint32_t* sorted,
+                 const uint32_t* cstart, const uint32_t* cend,
+                 const xdem_device_contact_property* cp, uint32_t np);
+/* K1 / K2 (GPU) or velocity_verlet CPU branch (HOST) */
+void orc_vv_position(const xdem_device_parameters* p, int semantics, double* pos4, const double* vel,
+                     const double* acc, uint32_t np);
+void orc_vv_velocity(const xdem_device_parameters* p, int semantics, double* vel, double* acc,
+                     double* omega, double* alpha, double* force, double* moment,
+                     const double* mass, const double* iner, uint32_t np);
+/* reductionD3: same block-tree order as reduce6<double3,512> + host block sum */
+void orc_reduction_d3(const double* in3, uint32_t n, double out[3]);
+/* updatePolygonObjectData_kernel (mphysics_cuda_impl.cuh:1911-1956) */
+void orc_update_polygon(const xdem_device_polygon_mass_info* dpmi, const double* vList,
+                        double* sphere4, xdem_device_polygon_info* dpi, uint32_t ntri);
+/* numeric::utility::calculate_center_of_triangle (source/numeric_utility.cpp:20-45) */
+void orc_circumcenter(const double P[3], const double Q[3], const double R[3], double out[3]);
+/* dem_simulation::oneStepAnalysis (dem_simulation.cpp:447-469), nsteps times */
+void orc_step(orc_world* w, uint32_t nsteps);
+/* Parallel (OpenMP over particles) variant of orc_step for the multi-core
+ * CPU baseline; identical results (each particle is independent). */
+void orc_step_omp(orc_world* w, uint32_t nsteps, int32_t nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

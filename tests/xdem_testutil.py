@@ -1,0 +1,78 @@
+"""Helpers shared by the tests: device buffers through the C ABI, contact-rich
+states, and the cancellation-safe force error metric."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from metakonga_amd import abi, scenes
+
+
+class DBuf:
+    """A device buffer allocated through xdem_malloc (no torch)."""
+
+    def __init__(self, lib, host: np.ndarray | None = None, nbytes: int | None = None):
+        self.lib = lib
+        if host is not None:
+            host = np.ascontiguousarray(host)
+            nbytes = host.nbytes
+        self.nbytes = max(int(nbytes), 1)
+        p = C.c_void_p()
+        abi.check(lib, lib.xdem_malloc(C.byref(p), self.nbytes))
+        self.ptr = p
+        if host is not None and host.nbytes:
+            abi.check(lib, lib.xdem_memcpy_htod(self.ptr, C.c_void_p(host.ctypes.data), host.nbytes))
+
+    def get(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes:
+            abi.check(self.lib, self.lib.xdem_memcpy_dtoh(C.c_void_p(out.ctypes.data), self.ptr,
+                                                           out.nbytes))
+        return out
+
+    def set(self, host):
+        host = np.ascontiguousarray(host)
+        abi.check(self.lib, self.lib.xdem_memcpy_htod(self.ptr, C.c_void_p(host.ctypes.data),
+                                                       host.nbytes))
+
+    def __del__(self):
+        try:
+            self.lib.xdem_free(self.ptr)
+        except Exception:
+            pass
+
+
+def struct_dev(lib, ctype, items):
+    arr = abi.struct_array(ctype, items)
+    raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
+    return DBuf(lib, raw), arr
+
+
+def dense_state(n_side=12, radius=5e-3, pitch_factor=1.96, jitter=0.03, seed=11, vel_scale=0.05,
+                omega_scale=5.0, f32_exact=True):
+    """A compressed jittered lattice with random velocities and spins: every
+    particle has several overlapping neighbours (contact-rich state)."""
+    r = float(scenes.as_f32_exact(radius))
+    n = n_side ** 3
+    idx = np.arange(n)
+    ix, iy, iz = idx % n_side, (idx // n_side) % n_side, idx // (n_side * n_side)
+    pitch = pitch_factor * r
+    u = scenes.uniform(seed, 9 * n, 3).reshape(n, 9) * 2 - 1
+    pos = np.stack([0.05 + ix * pitch + u[:, 0] * jitter * r,
+                    -0.5 + iy * pitch + u[:, 1] * jitter * r,
+                    0.05 + iz * pitch + u[:, 2] * jitter * r, np.full(n, r)], 1)
+    vel = u[:, 3:6] * vel_scale
+    om = u[:, 6:9] * omega_scale
+    if f32_exact:
+        vel = scenes.as_f32_exact(vel)
+        om = scenes.as_f32_exact(om)
+    return np.ascontiguousarray(pos), np.ascontiguousarray(vel), np.ascontiguousarray(om)
+
+
+def force_error(f_test, f_ref, f_scale):
+    """max_i |f_test - f_ref|_inf / scale_i, with scale_i = sum of the magnitudes
+    of the terms that were summed into particle i (the raw net force of a
+    settled particle cancels)."""
+    d = np.abs(np.asarray(f_test) - np.asarray(f_ref)).max(axis=1)
+    return float((d / np.maximum(f_scale, 1e-300)).max()) if d.size else 0.0
