@@ -35,6 +35,12 @@ PHASES = ["vv_position", "hash", "sort", "cell_ranges", "p2p", "walls", "triangl
 P2P_F64_BYTES = 188.0
 # Algorithmic bytes per particle-step of the whole FP64 step (reference data model).
 STEP_F64_BYTES = 300.0
+# Algorithmic bytes per particle of the fused contact + Verlet kernel (k_step):
+# own state read 84 (pos 24, v+r 16, omega+m 16, a+alpha 24, id 4) + written
+# 84 + sorted->storage index 4 + next cell key 4 (neighbour re-reads excluded).
+KSTEP_BYTES = 176.0
+# Algorithmic bytes per particle-step of the whole FP32c step.
+STEP_F32C_BYTES = 192.0
 
 
 def log(*a):
@@ -133,8 +139,13 @@ def main():
     if fused is not None:
         head = fused
         dtype = "f32 contact + f64 positions"
-        dom_kernel = fused["dominant"]
-        roof = fused["roofline"]
+        k_ms = fused["phase_ms"]["contact_verlet"]
+        achieved = KSTEP_BYTES * scene.np / (k_ms * 1e-3)
+        roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None,
+                "kernel": "k_step<true,false> (fused contact + Verlet, FP32c)",
+                "bytes_per_particle": KSTEP_BYTES, "kernel_ms": k_ms,
+                "step_frac": STEP_F32C_BYTES * fused["value"] / HBM_PEAK}
     else:
         head = f64
         dtype = "f64"

@@ -1,0 +1,103 @@
+/*
+ * xdem_batch.h — the fused FP32c path: many independent DEM scenes advanced
+ * together on one GPU (a segmented cell grid, scene id in the high part of
+ * the cell key), with device-resident state and hipGraph-captured step blocks.
+ *
+ * Precision contract ("FP32c"):
+ *   - positions are FP64 (accumulated in FP64 every step); the contact
+ *     predicate (ri + rj) - |pj - pi| > 0 and the cell hash are evaluated in
+ *     FP64 exactly as the reference (mphysics_cuda_impl.cuh:75-83, 449-452),
+ *     so hashes, sorted indices, cell ranges and contact pair lists are
+ *     bit-exact against the FP64 path on identical states;
+ *   - radii, velocities, spins, accelerations and masses are FP32 (radii
+ *     given in FP64 are rounded to FP32 on entry); the contact force law
+ *     (Hertz normal, linear damping, Coulomb-capped tangential: the reference
+ *     DHS model, mphysics_cuda_impl.cuh:379-401) runs in FP32.
+ *
+ * Semantics per scene are those of dem_simulation::oneStepAnalysis
+ * (source/simulation/dem_simulation.cpp:447-469) with TCM = 1 (DHS): Verlet
+ * position update, grid, particle-particle contacts in the reference's
+ * traversal order (z, y, x neighbour cells; ascending sorted position), then
+ * planes in the order given, then the Verlet velocity update with F reset to
+ * m g.  No counterpart in the reference for the batching itself.
+ */
+#ifndef XDEM_BATCH_H
+#define XDEM_BATCH_H
+
+#include "xdem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xdem_batch xdem_batch;
+
+typedef struct xdem_batch_scene_desc {
+    uint32_t np;                   /* particles */
+    double dt;
+    xdem_double3 gravity;
+    xdem_double3 world_origin;
+    double cell_size;              /* > 0 */
+    xdem_uint3 grid_size;          /* powers of two, product <= 2^24 */
+    bool rolling;                  /* device_parameters.rollingCondition */
+    xdem_material particle;
+    xdem_contact_pair_desc pp;     /* particle-particle pair (pp.wall = particle material) */
+    xdem_contact_pair_desc wall;   /* particle-plane pair (wall.wall = wall material) */
+    uint32_t nplane;
+    const xdem_device_plane_info* planes;  /* host array of nplane planes, applied in order */
+} xdem_batch_scene_desc;
+
+/* Capacities are fixed at creation (HBM is reserved up front). */
+int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
+                          xdem_batch** out);
+int32_t xdem_batch_destroy(xdem_batch* b);
+/* Add a scene (host arrays, reference layout: pos4 = x,y,z,r; vel3/omega3 may be
+ * NULL for zero).  Takes effect at the next xdem_batch_step.  Fails with
+ * XDEM_ERR_CAPACITY when particles, scenes or cells would exceed the capacity. */
+int32_t xdem_batch_add_scene(xdem_batch* b, const xdem_batch_scene_desc* d, const double* pos4,
+                             const double* vel3, const double* omega3, uint32_t* scene_id);
+/* Remove a scene (its slot and memory are reused).  Takes effect at the next step. */
+int32_t xdem_batch_remove_scene(xdem_batch* b, uint32_t scene_id);
+/* Advance every active scene by nsteps (one step block).  When graphs are on
+ * (default), the block is captured once per (layout, nsteps) and replayed. */
+int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps);
+int32_t xdem_batch_set_graphs(xdem_batch* b, int32_t on);
+/* Block size (64..1024, multiple of 64) of the contact kernel; 0 = default. */
+int32_t xdem_batch_set_block(xdem_batch* b, uint32_t threads);
+int32_t xdem_batch_synchronize(xdem_batch* b);
+/* State of one scene in its original particle order (host arrays, reference
+ * layout; any pointer may be NULL).  Positions are FP64; the others are the
+ * FP32 values widened. */
+int32_t xdem_batch_get_state(xdem_batch* b, uint32_t scene_id, double* pos4, double* vel3,
+                             double* omega3, double* acc3, double* alpha3);
+uint32_t xdem_batch_num_particles(const xdem_batch* b);
+uint32_t xdem_batch_num_scenes(const xdem_batch* b);
+/* Kinetic energy (translational + rotational) and linear momentum of one
+ * scene, reduced on the device in FP64 (deterministic order). */
+int32_t xdem_batch_energy(xdem_batch* b, uint32_t scene_id, double* kinetic, double momentum[3]);
+/* Per-phase device times (HIP events), phases: 0 position update + hash,
+ * 1 contact + Verlet, 2 sort (mover detection), 3 sort (mover ordering),
+ * 4 sort (merge), 5 cell ranges.  Enabling resets the totals. */
+int32_t xdem_batch_enable_timing(xdem_batch* b, int32_t on);
+int32_t xdem_batch_phase_times(xdem_batch* b, double* ms6, uint32_t* count6);
+/* Number of particles that changed cell at the last sort (all scenes). */
+int32_t xdem_batch_last_movers(xdem_batch* b, uint32_t* movers);
+
+/* Parity hooks (no counterpart in the reference).  After xdem_batch_prepare,
+ * the batch holds the sorted grid of the current positions (exactly as the
+ * reference computes it from the positions given to add_scene, i.e. before
+ * any Verlet update) and xdem_batch_probe evaluates the contact step on that
+ * grid without advancing: per particle (original order of `scene_id`) it
+ * returns the particle-particle plus wall force and torque (FP32 widened),
+ * the contact partner ids in traversal order (up to max_per), and the sorted
+ * cell keys / sorted ids / cell ranges of that scene. */
+int32_t xdem_batch_prepare(xdem_batch* b);
+int32_t xdem_batch_probe(xdem_batch* b, uint32_t scene_id, double* force3, double* moment3,
+                         uint32_t max_per, uint32_t* counts, uint32_t* partners,
+                         uint32_t* sorted_keys, uint32_t* sorted_ids, uint32_t* cstart,
+                         uint32_t* cend);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,959 @@
+// xdem_batch.hip — host side of the fused FP32c path (include/xdem_batch.h).
+//
+// Layout.  All scenes share one set of SoA arrays (two buffers, ping-pong).
+// Scene s owns the cell keys [cell_base, cell_base + ncell) of one segmented
+// grid; its particles occupy the sorted positions [p0, p0 + n), with scenes in
+// cell_base order.  Storage between steps is the sorted order of the last
+// sort, so a block starts with "storage sorted by (key, id)" except for newly
+// added scenes, appended at the end with a sentinel key (every one of them is
+// a mover of the first sort and is merged into place).
+//
+// A step block of n steps is
+//   k_pos<adv>  sort  [k_step<adv>  sort] x (n-1)  k_step<end>
+// where the last k_step leaves the end-of-step positions (no next update), so
+// the state between blocks is the reference's end-of-step state.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <vector>
+
+#include "../../include/xdem_batch.h"
+#include "xdem_batch_kernels.hpp"
+#include "xdem_common.hpp"
+
+using namespace xdem;
+using namespace xdem::fused;
+
+namespace {
+
+struct HostScene {
+    uint32_t id;           // user-visible scene id
+    xdem_batch_scene_desc desc;
+    std::vector<xdem_device_plane_info> planes;
+    uint32_t n = 0;
+    uint32_t cell_base = 0, ncell = 0;
+    uint32_t p0 = 0;       // sorted segment
+    bool pending = true;   // not yet uploaded
+    std::vector<double> pos, vel, omega;  // staging until upload
+};
+
+template <class T>
+int32_t dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    XDEM_HIP(hipMalloc((void**)p, sizeof(T) * n));
+    return XDEM_OK;
+}
+
+int32_t alloc_state(BState& s, size_t n) {
+    XDEM_TRY(dalloc(&s.px, n));
+    XDEM_TRY(dalloc(&s.py, n));
+    XDEM_TRY(dalloc(&s.pz, n));
+    XDEM_TRY(dalloc(&s.vr, n));
+    XDEM_TRY(dalloc(&s.wm, n));
+    XDEM_TRY(dalloc(&s.acc, n));
+    XDEM_TRY(dalloc(&s.alyz, n));
+    XDEM_TRY(dalloc(&s.id, n));
+    return XDEM_OK;
+}
+
+void free_state(BState& s) {
+    for (void* p : {(void*)s.px, (void*)s.py, (void*)s.pz, (void*)s.vr, (void*)s.wm, (void*)s.acc,
+                    (void*)s.alyz, (void*)s.id})
+        if (p) (void)hipFree(p);
+    s = BState{};
+}
+
+// E* of a pair (getConstant, mphysics_cuda_impl.cuh:223) and the DHS damping
+// factor 4 / (1 + (pi / ln e)^2) (:239-241).
+void pair_consts(const xdem_material& mi, const xdem_material& mj, const xdem_contact_pair_desc& c,
+                 float& ckn, float& cvn, float& sr, float& mu) {
+    double Eeq = (mi.youngs * mj.youngs) /
+                 (mi.youngs * (1 - mj.poisson * mj.poisson) + mj.youngs * (1 - mi.poisson * mi.poisson));
+    double beta = M_PI / std::log(c.restitution);
+    ckn = (float)((4.0 / 3.0) * Eeq);
+    cvn = (float)(4.0 / (1.0 + beta * beta));
+    sr = (float)c.stiffness_ratio;
+    mu = (float)c.friction;
+}
+
+bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
+
+}  // namespace
+
+struct xdem_batch {
+    uint32_t cap_np = 0, cap_scenes = 0;
+    uint64_t cap_cells = 0;
+    hipStream_t st = nullptr;
+    BState buf[2];
+    int cur = 0;                      // buffer holding the state
+    uint32_t* keynew = nullptr;
+    uint32_t* skey[2] = {nullptr, nullptr};
+    int skcur = 0;                    // skey buffer of the last sort (storage order)
+    uint32_t* perm = nullptr;
+    uint2* crange = nullptr;
+    // sort scratch
+    uint64_t* moverbits = nullptr;
+    uint32_t* blockcnt = nullptr;
+    uint32_t* counters = nullptr;     // [0] mcount, [1] msorted
+    uint64_t* mkey = nullptr;
+    uint32_t* ms = nullptr;
+    uint64_t* mkey2 = nullptr;
+    uint32_t* ms2 = nullptr;
+    // tables
+    BScene* d_scenes = nullptr;
+    BPlane* d_planes = nullptr;
+    BlockEntry* d_blocks_sorted = nullptr;
+    BlockEntry* d_blocks_store = nullptr;
+    uint32_t nblk_sorted = 0, nblk_store = 0;
+    double* d_red = nullptr;          // energy partials
+    // scenes
+    std::vector<HostScene> scenes;    // active + pending (not removed)
+    uint32_t next_id = 0;
+    uint32_t n_total = 0;             // particles in the arrays (incl. appended)
+    bool layout_dirty = true;
+    bool any_stepped = false;
+    bool storage_sorted_all_new = false;
+    uint32_t block_threads = 256;
+    bool use_graphs = true;
+    std::map<std::pair<uint32_t, int>, hipGraphExec_t> graphs;
+    PhaseTimer timer;
+    uint32_t last_movers = 0;
+    std::vector<BScene> h_scenes;
+    // parity probe
+    float* dbg_f = nullptr;
+    uint32_t* dbg_cnt = nullptr;
+    uint32_t* dbg_part = nullptr;
+    uint32_t dbg_cap = 0;
+    bool prepared = false;
+
+    ~xdem_batch() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        free_state(buf[0]);
+        free_state(buf[1]);
+        if (d_blocks_store == d_blocks_sorted) d_blocks_store = nullptr;
+        for (void* p : {(void*)keynew, (void*)skey[0], (void*)skey[1], (void*)perm, (void*)crange,
+                        (void*)moverbits, (void*)blockcnt, (void*)counters, (void*)mkey, (void*)ms,
+                        (void*)mkey2, (void*)ms2, (void*)d_scenes, (void*)d_planes,
+                        (void*)d_blocks_sorted, (void*)d_blocks_store, (void*)d_red, (void*)dbg_f,
+                        (void*)dbg_cnt, (void*)dbg_part})
+            if (p) (void)hipFree(p);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    uint32_t nblk_rows() const { return div_up(std::max<uint32_t>(n_total, 1), ROWS_T); }
+
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+
+    uint32_t key_bits() const {
+        uint64_t top = 1;
+        for (auto& s : scenes) top = std::max<uint64_t>(top, (uint64_t)s.cell_base + s.ncell);
+        uint32_t b = 0;
+        while (b < 32 && (1ull << b) < top) ++b;
+        return b;
+    }
+    uint32_t id_bits() const {
+        uint32_t mx = 1;
+        for (auto& s : scenes) mx = std::max(mx, s.n);
+        uint32_t b = 0;
+        while (b < 32 && (1ull << b) < mx) ++b;
+        return b;
+    }
+
+    SortArgs sort_args(const uint32_t* key_old, const uint32_t* key_new, const uint32_t* id,
+                       uint32_t* skey_out) {
+        SortArgs a{};
+        a.n = n_total;
+        a.key_old = key_old;
+        a.key_new = key_new;
+        a.id = id;
+        a.crange = crange;
+        a.moverbits = moverbits;
+        a.blockcnt = blockcnt;
+        a.mcount = counters;
+        a.msorted = counters + 1;
+        a.mkey = mkey;
+        a.ms = ms;
+        a.mkey2 = mkey2;
+        a.ms2 = ms2;
+        a.perm = perm;
+        a.skey = skey_out;
+        a.nblk = nblk_rows();
+        a.key_bits = key_bits();
+        a.id_bits = id_bits();
+        return a;
+    }
+};
+
+namespace {
+
+// First-fit allocation of a contiguous key range for a scene's cells.
+int32_t alloc_cells(xdem_batch* b, uint32_t ncell, uint32_t* base) {
+    std::vector<std::pair<uint64_t, uint64_t>> used;
+    for (auto& s : b->scenes)
+        if (s.ncell) used.push_back({s.cell_base, (uint64_t)s.cell_base + s.ncell});
+    std::sort(used.begin(), used.end());
+    uint64_t at = 0;
+    for (auto& u : used) {
+        if (u.first >= at + ncell) break;
+        at = std::max(at, u.second);
+    }
+    if (at + ncell > b->cap_cells || at + ncell >= 0xFFFFFFFFull) {
+        set_error("cell capacity exceeded (%llu + %u > %llu)", (unsigned long long)at, ncell,
+                  (unsigned long long)b->cap_cells);
+        return XDEM_ERR_CAPACITY;
+    }
+    *base = (uint32_t)at;
+    return XDEM_OK;
+}
+
+BScene make_bscene(const HostScene& h, uint32_t plane0) {
+    const xdem_batch_scene_desc& d = h.desc;
+    BScene s{};
+    s.wox = d.world_origin.x;
+    s.woy = d.world_origin.y;
+    s.woz = d.world_origin.z;
+    s.cs = d.cell_size;
+    s.dt = d.dt;
+    s.half2dt = 0.5 * d.dt * d.dt;
+    s.gx = d.grid_size.x;
+    s.gy = d.grid_size.y;
+    s.gz = d.grid_size.z;
+    s.cell_base = h.cell_base;
+    s.ncell = h.ncell;
+    s.p0 = h.p0;
+    s.n = h.n;
+    s.plane0 = plane0;
+    s.nplane = (uint32_t)h.planes.size();
+    s.gvx = (float)d.gravity.x;
+    s.gvy = (float)d.gravity.y;
+    s.gvz = (float)d.gravity.z;
+    s.dtf = (float)d.dt;
+    pair_consts(d.particle, d.pp.wall, d.pp, s.ckn, s.cvn, s.sr, s.mu);
+    s.mass_c = (float)(d.particle.density * (4.0 / 3.0) * M_PI);
+    s.rolling = d.rolling ? 1 : 0;
+    return s;
+}
+
+std::vector<BlockEntry> make_blocks(const std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>>& segs,
+                                    uint32_t T) {
+    std::vector<BlockEntry> out;
+    for (auto& sg : segs) {
+        uint32_t sc = sg.first, j0 = sg.second.first, n = sg.second.second;
+        for (uint32_t o = 0; o < n; o += T) out.push_back(BlockEntry{sc, j0 + o, j0 + std::min(n, o + T), 0});
+    }
+    return out;
+}
+
+template <class T>
+int32_t upload_vec(T** d, const std::vector<T>& h, hipStream_t st) {
+    if (*d) (void)hipFree(*d);
+    *d = nullptr;
+    XDEM_TRY(dalloc(d, std::max<size_t>(h.size(), 1)));
+    if (!h.empty())
+        XDEM_HIP(hipMemcpyAsync(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
+    return XDEM_OK;
+}
+
+// Copy a storage segment [src, src+n) of buffer `from` to [dst, dst+n) of `to`.
+int32_t copy_segment(const BState& from, const BState& to, uint32_t src, uint32_t dst, uint32_t n,
+                     hipStream_t st) {
+    if (!n) return XDEM_OK;
+    auto cp = [&](void* d, const void* s, size_t b) -> int32_t {
+        XDEM_HIP(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, st));
+        return XDEM_OK;
+    };
+    XDEM_TRY(cp(to.px + dst, from.px + src, 8ull * n));
+    XDEM_TRY(cp(to.py + dst, from.py + src, 8ull * n));
+    XDEM_TRY(cp(to.pz + dst, from.pz + src, 8ull * n));
+    XDEM_TRY(cp(to.vr + dst, from.vr + src, 16ull * n));
+    XDEM_TRY(cp(to.wm + dst, from.wm + src, 16ull * n));
+    XDEM_TRY(cp(to.acc + dst, from.acc + src, 16ull * n));
+    XDEM_TRY(cp(to.alyz + dst, from.alyz + src, 8ull * n));
+    XDEM_TRY(cp(to.id + dst, from.id + src, 4ull * n));
+    return XDEM_OK;
+}
+
+// Apply pending additions/removals at a block boundary (outside any graph).
+int32_t apply_layout(xdem_batch* b) {
+    if (!b->layout_dirty) return XDEM_OK;
+    hipStream_t st = b->st;
+    b->drop_graphs();
+    // 1. compaction of the scenes already on the device: they sit in the
+    //    storage in cell_base order (sorted layout of the last block).
+    std::vector<HostScene*> old, fresh;
+    for (auto& s : b->scenes) (s.pending ? fresh : old).push_back(&s);
+    std::sort(old.begin(), old.end(), [](HostScene* a, HostScene* c) { return a->p0 < c->p0; });
+    BState& A = b->buf[b->cur];
+    BState& B = b->buf[1 - b->cur];
+    uint32_t at = 0;
+    bool moved = false;
+    for (HostScene* s : old) {
+        if (s->p0 != at) moved = true;
+        at += s->n;
+    }
+    uint32_t n_old = at;
+    if (moved) {
+        at = 0;
+        for (HostScene* s : old) {
+            XDEM_TRY(copy_segment(A, B, s->p0, at, s->n, st));
+            XDEM_HIP(hipMemcpyAsync(b->keynew + at, b->skey[b->skcur] + s->p0, 4ull * s->n,
+                                    hipMemcpyDeviceToDevice, st));
+            s->p0 = at;
+            at += s->n;
+        }
+        XDEM_TRY(copy_segment(B, A, 0, 0, n_old, st));
+        XDEM_HIP(hipMemcpyAsync(b->skey[b->skcur], b->keynew, 4ull * n_old, hipMemcpyDeviceToDevice, st));
+    }
+    // 2. append new scenes (id order) with the sentinel old key
+    XDEM_HIP(hipStreamSynchronize(st));
+    std::vector<double> px, py, pz;
+    std::vector<float4> vr, wm, acc;
+    std::vector<float2> alyz;
+    std::vector<uint32_t> ids;
+    for (HostScene* s : fresh) {
+        uint32_t n = s->n;
+        s->p0 = at;  // storage position of the appended segment
+        px.resize(n);
+        py.resize(n);
+        pz.resize(n);
+        vr.resize(n);
+        wm.resize(n);
+        acc.assign(n, make_float4(0, 0, 0, 0));
+        alyz.assign(n, make_float2(0, 0));
+        ids.resize(n);
+        float mc = (float)(s->desc.particle.density * (4.0 / 3.0) * M_PI);
+        for (uint32_t i = 0; i < n; ++i) {
+            px[i] = s->pos[4 * i + 0];
+            py[i] = s->pos[4 * i + 1];
+            pz[i] = s->pos[4 * i + 2];
+            float r = (float)s->pos[4 * i + 3];
+            float m = mc * (r * r * r);
+            const double* v = s->vel.empty() ? nullptr : &s->vel[3 * i];
+            const double* w = s->omega.empty() ? nullptr : &s->omega[3 * i];
+            vr[i] = make_float4(v ? (float)v[0] : 0.f, v ? (float)v[1] : 0.f, v ? (float)v[2] : 0.f, r);
+            wm[i] = make_float4(w ? (float)w[0] : 0.f, w ? (float)w[1] : 0.f, w ? (float)w[2] : 0.f, m);
+            ids[i] = i;
+        }
+        XDEM_HIP(hipMemcpy(A.px + at, px.data(), 8ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.py + at, py.data(), 8ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.pz + at, pz.data(), 8ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.vr + at, vr.data(), 16ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.wm + at, wm.data(), 16ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.acc + at, acc.data(), 16ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.alyz + at, alyz.data(), 8ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.id + at, ids.data(), 4ull * n, hipMemcpyHostToDevice));
+        XDEM_TRY(fill_u32(b->skey[b->skcur] + at, EMPTY, n, st));
+        // the scene's cells start empty
+        uint2 e = make_uint2(EMPTY, 0);
+        hipLaunchKernelGGL(k_fill_u2, dim3(std::min<uint32_t>(4096, div_up(s->ncell, 256))), dim3(256), 0,
+                           st, b->crange + s->cell_base, e, (size_t)s->ncell);
+        XDEM_LAUNCHED();
+        s->pending = false;
+        std::vector<double>().swap(s->pos);
+        std::vector<double>().swap(s->vel);
+        std::vector<double>().swap(s->omega);
+        at += n;
+    }
+    b->storage_sorted_all_new = (n_old == 0);
+    b->n_total = at;
+    // 3. tables: the storage layout (old scenes, then the appended ones) for the
+    //    first k_pos, and the sorted layout (scenes by cell_base) for everything else.
+    std::vector<HostScene*> all;
+    for (auto& s : b->scenes) all.push_back(&s);
+    std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>> store_segs, sorted_segs;
+    std::vector<uint32_t> order(all.size());
+    for (uint32_t i = 0; i < all.size(); ++i) order[i] = i;
+    for (uint32_t i = 0; i < all.size(); ++i) store_segs.push_back({i, {all[i]->p0, all[i]->n}});
+    std::sort(order.begin(), order.end(),
+              [&](uint32_t x, uint32_t y) { return all[x]->cell_base < all[y]->cell_base; });
+    uint32_t p = 0;
+    for (uint32_t k : order) {
+        all[k]->p0 = p;
+        sorted_segs.push_back({k, {p, all[k]->n}});
+        p += all[k]->n;
+    }
+    std::vector<BScene> hs;
+    std::vector<BPlane> hp;
+    for (HostScene* s : all) {
+        hs.push_back(make_bscene(*s, (uint32_t)hp.size()));
+        float ckn, cvn, sr, mu;
+        pair_consts(s->desc.particle, s->desc.wall.wall, s->desc.wall, ckn, cvn, sr, mu);
+        for (auto& pl : s->planes) hp.push_back(BPlane{pl, ckn, cvn, sr, mu});
+    }
+    b->h_scenes = hs;
+    XDEM_TRY(upload_vec(&b->d_scenes, hs, st));
+    XDEM_TRY(upload_vec(&b->d_planes, hp, st));
+    auto bs = make_blocks(sorted_segs, b->block_threads);
+    auto bt = make_blocks(store_segs, b->block_threads);
+    if (b->d_blocks_store == b->d_blocks_sorted) b->d_blocks_store = nullptr;
+    XDEM_TRY(upload_vec(&b->d_blocks_sorted, bs, st));
+    XDEM_TRY(upload_vec(&b->d_blocks_store, bt, st));
+    b->nblk_sorted = (uint32_t)bs.size();
+    b->nblk_store = (uint32_t)bt.size();
+    XDEM_HIP(hipStreamSynchronize(st));
+    b->layout_dirty = false;
+    return XDEM_OK;
+}
+
+// k_pos over the storage layout: Verlet position update (adv) + keys.
+int32_t launch_pos(xdem_batch* b, bool adv, const BlockEntry* blocks, uint32_t nblk) {
+    if (!nblk) return XDEM_OK;
+    BState& A = b->buf[b->cur];
+    if (adv)
+        hipLaunchKernelGGL(k_pos<true>, dim3(nblk), dim3(b->block_threads), 0, b->st, A, b->keynew,
+                           b->d_scenes, blocks);
+    else
+        hipLaunchKernelGGL(k_pos<false>, dim3(nblk), dim3(b->block_threads), 0, b->st, A, b->keynew,
+                           b->d_scenes, blocks);
+    XDEM_LAUNCHED();
+    return XDEM_OK;
+}
+
+// The incremental (key, id) sort of keynew (storage order) against the keys of
+// the previous sort; outputs perm, the new sorted keys and the cell ranges.
+int32_t run_sort(xdem_batch* b) {
+    const uint32_t n = b->n_total;
+    if (!n) return XDEM_OK;
+    hipStream_t st = b->st;
+    PhaseTimer& T = b->timer;
+    const uint32_t* kold = b->skey[b->skcur];
+    uint32_t* knew_sorted = b->skey[1 - b->skcur];
+    SortArgs a = b->sort_args(kold, b->keynew, b->buf[b->cur].id, knew_sorted);
+    XDEM_TRY(T.begin(2, st));
+    hipLaunchKernelGGL(k_movers, dim3(a.nblk), dim3(ROWS_T), 0, st, a);
+    XDEM_LAUNCHED();
+    XDEM_TRY(T.end(st));
+    XDEM_TRY(T.begin(3, st));
+    hipLaunchKernelGGL(k_sort_movers, dim3(1), dim3(1024), 0, st, a);
+    XDEM_LAUNCHED();
+    XDEM_TRY(T.end(st));
+    XDEM_TRY(T.begin(4, st));
+    hipLaunchKernelGGL(k_merge, dim3(a.nblk + 64), dim3(ROWS_T), 0, st, a);
+    XDEM_LAUNCHED();
+    XDEM_TRY(T.end(st));
+    XDEM_TRY(T.begin(5, st));
+    hipLaunchKernelGGL(k_ranges, dim3(a.nblk), dim3(ROWS_T), 0, st, knew_sorted, b->crange, n);
+    XDEM_LAUNCHED();
+    XDEM_TRY(T.end(st));
+    b->skcur = 1 - b->skcur;
+    return XDEM_OK;
+}
+
+// Full sort when every particle is new (storage in id order per scene): a
+// stable radix sort of the keys with the storage index as value.
+int32_t run_full_sort(xdem_batch* b) {
+    const uint32_t n = b->n_total;
+    hipStream_t st = b->st;
+    uint32_t* out = b->skey[1 - b->skcur];
+    XDEM_HIP(hipMemcpyAsync(out, b->keynew, 4ull * n, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_iota, dim3(div_up(n, 256)), dim3(256), 0, st, b->perm, n);
+    XDEM_LAUNCHED();
+    XDEM_HIP(hipStreamSynchronize(st));  // sort_pairs may grow the library workspace
+    XDEM_TRY(sort_pairs(out, b->perm, n, b->key_bits(), st));
+    hipLaunchKernelGGL(k_ranges, dim3(div_up(n, ROWS_T)), dim3(ROWS_T), 0, st, out, b->crange, n);
+    XDEM_LAUNCHED();
+    b->skcur = 1 - b->skcur;
+    return XDEM_OK;
+}
+
+StepArgs step_args(xdem_batch* b, bool dbg) {
+    StepArgs a{};
+    a.in = b->buf[b->cur];
+    a.out = b->buf[1 - b->cur];
+    a.perm = b->perm;
+    a.crange = b->crange;
+    a.keynew = b->keynew;
+    a.scenes = b->d_scenes;
+    a.planes = b->d_planes;
+    a.blocks = b->d_blocks_sorted;
+    a.nblocks = b->nblk_sorted;
+    if (dbg) {
+        a.dbg_f = b->dbg_f;
+        a.dbg_cnt = b->dbg_cnt;
+        a.dbg_part = b->dbg_part;
+        a.dbg_max = b->dbg_cap;
+    }
+    return a;
+}
+
+int32_t launch_step(xdem_batch* b, bool adv) {
+    if (!b->nblk_sorted) return XDEM_OK;
+    StepArgs a = step_args(b, false);
+    XDEM_TRY(b->timer.begin(1, b->st));
+    if (adv)
+        hipLaunchKernelGGL((k_step<true, false>), dim3(b->nblk_sorted), dim3(b->block_threads), 0,
+                           b->st, a);
+    else
+        hipLaunchKernelGGL((k_step<false, false>), dim3(b->nblk_sorted), dim3(b->block_threads), 0,
+                           b->st, a);
+    XDEM_LAUNCHED();
+    XDEM_TRY(b->timer.end(b->st));
+    b->cur = 1 - b->cur;
+    return XDEM_OK;
+}
+
+// One block of nsteps launches (eager or under capture).
+int32_t enqueue_block(xdem_batch* b, uint32_t nsteps, bool first_full_sort) {
+    XDEM_TRY(b->timer.begin(0, b->st));
+    XDEM_TRY(launch_pos(b, true, b->d_blocks_store, b->nblk_store));
+    XDEM_TRY(b->timer.end(b->st));
+    if (first_full_sort)
+        XDEM_TRY(run_full_sort(b));
+    else
+        XDEM_TRY(run_sort(b));
+    for (uint32_t k = 1; k < nsteps; ++k) {
+        XDEM_TRY(launch_step(b, true));
+        XDEM_TRY(run_sort(b));
+    }
+    XDEM_TRY(launch_step(b, false));
+    return XDEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
+                          xdem_batch** out) {
+    XDEM_REQUIRE(out && max_particles && max_scenes && max_cells, "invalid capacities");
+    XDEM_REQUIRE(max_cells < 0xFFFFFFFFull, "max_cells must be < 2^32 - 1");
+    *out = nullptr;
+    xdem_batch* b = new xdem_batch();
+    b->cap_np = max_particles;
+    b->cap_scenes = max_scenes;
+    b->cap_cells = max_cells;
+    int32_t rc = XDEM_OK;
+    auto fail = [&](int32_t r) {
+        delete b;
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreate failed");
+        return fail(XDEM_ERR_HIP);
+    }
+    size_t n = max_particles;
+    uint32_t nblk = div_up(max_particles, ROWS_T);
+    if ((rc = alloc_state(b->buf[0], n)) || (rc = alloc_state(b->buf[1], n)) ||
+        (rc = dalloc(&b->keynew, n)) || (rc = dalloc(&b->skey[0], n)) ||
+        (rc = dalloc(&b->skey[1], n)) || (rc = dalloc(&b->perm, n)) ||
+        (rc = dalloc(&b->crange, (size_t)max_cells)) ||
+        (rc = dalloc(&b->moverbits, (size_t)nblk * (ROWS_T / 64) + 1)) ||
+        (rc = dalloc(&b->blockcnt, (size_t)nblk + 1)) || (rc = dalloc(&b->counters, 4)) ||
+        (rc = dalloc(&b->mkey, n)) || (rc = dalloc(&b->ms, n)) || (rc = dalloc(&b->mkey2, n)) ||
+        (rc = dalloc(&b->ms2, n)) || (rc = dalloc(&b->d_red, 8 * 1024)))
+        return fail(rc);
+    if (hipMemset(b->counters, 0, 16) != hipSuccess) return fail(XDEM_ERR_HIP);
+    *out = b;
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_destroy(xdem_batch* b) {
+    delete b;
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_add_scene(xdem_batch* b, const xdem_batch_scene_desc* d, const double* pos4,
+                             const double* vel3, const double* omega3, uint32_t* scene_id) {
+    XDEM_REQUIRE(b && d && (d->np == 0 || pos4), "null argument");
+    XDEM_REQUIRE(d->np > 0, "a scene needs particles");
+    XDEM_REQUIRE(is_pow2(d->grid_size.x) && is_pow2(d->grid_size.y) && is_pow2(d->grid_size.z),
+                 "grid_size must be powers of two");
+    uint64_t nc = (uint64_t)d->grid_size.x * d->grid_size.y * d->grid_size.z;
+    XDEM_REQUIRE(nc <= (1ull << 24), "a scene grid is limited to 2^24 cells");
+    XDEM_REQUIRE(d->cell_size > 0 && d->dt > 0, "cell_size and dt must be > 0");
+    XDEM_REQUIRE(d->nplane == 0 || d->planes, "null planes");
+    uint32_t active_np = 0;
+    for (auto& s : b->scenes) active_np += s.n;
+    if (b->scenes.size() + 1 > b->cap_scenes || (uint64_t)active_np + d->np > b->cap_np) {
+        set_error("batch capacity exceeded (scenes %zu/%u, particles %u+%u/%u)", b->scenes.size(),
+                  b->cap_scenes, active_np, d->np, b->cap_np);
+        return XDEM_ERR_CAPACITY;
+    }
+    // the CFL-like requirement of the cell list: a particle may not span more than one cell
+    double rmax = 0;
+    for (uint32_t i = 0; i < d->np; ++i) rmax = std::max(rmax, pos4[4 * (size_t)i + 3]);
+    XDEM_REQUIRE(d->cell_size >= 2.0 * (double)(float)rmax * (1.0 - 1e-7),
+                 "cell_size %.9g below 2 * max radius %.9g", d->cell_size, 2 * rmax);
+    HostScene h;
+    h.id = b->next_id++;
+    h.desc = *d;
+    h.planes.assign(d->planes, d->planes + d->nplane);
+    h.desc.planes = nullptr;
+    h.n = d->np;
+    h.ncell = (uint32_t)nc;
+    int32_t rc = alloc_cells(b, h.ncell, &h.cell_base);
+    if (rc) return rc;
+    h.pos.assign(pos4, pos4 + 4 * (size_t)d->np);
+    if (vel3) h.vel.assign(vel3, vel3 + 3 * (size_t)d->np);
+    if (omega3) h.omega.assign(omega3, omega3 + 3 * (size_t)d->np);
+    b->scenes.push_back(std::move(h));
+    b->layout_dirty = true;
+    b->prepared = false;
+    if (scene_id) *scene_id = b->scenes.back().id;
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_remove_scene(xdem_batch* b, uint32_t scene_id) {
+    XDEM_REQUIRE(b, "null batch");
+    for (size_t i = 0; i < b->scenes.size(); ++i)
+        if (b->scenes[i].id == scene_id) {
+            b->scenes.erase(b->scenes.begin() + i);
+            b->layout_dirty = true;
+            b->prepared = false;
+            return XDEM_OK;
+        }
+    set_error("no scene %u", scene_id);
+    return XDEM_ERR_INVALID;
+}
+
+int32_t xdem_batch_set_graphs(xdem_batch* b, int32_t on) {
+    XDEM_REQUIRE(b, "null batch");
+    b->use_graphs = on != 0;
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_set_block(xdem_batch* b, uint32_t threads) {
+    XDEM_REQUIRE(b, "null batch");
+    if (threads == 0) threads = 256;
+    XDEM_REQUIRE(threads >= 64 && threads <= 256 && threads % 64 == 0,
+                 "block size must be 64..256, a multiple of 64");
+    if (threads != b->block_threads) {
+        b->block_threads = threads;
+        b->layout_dirty = true;  // rebuild block tables
+    }
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps) {
+    XDEM_REQUIRE(b, "null batch");
+    if (nsteps == 0) return XDEM_OK;
+    XDEM_TRY(apply_layout(b));
+    if (!b->n_total) return XDEM_OK;
+    b->prepared = false;
+    bool first = b->storage_sorted_all_new;
+    // The first block after a layout change runs eagerly (it uses the storage
+    // table and may take the full sort); later blocks replay a captured graph.
+    if (first || !b->use_graphs || b->timer.on || b->d_blocks_store != b->d_blocks_sorted) {
+        XDEM_TRY(enqueue_block(b, nsteps, first));
+        b->storage_sorted_all_new = false;
+        // from now on the storage layout is the sorted layout
+        if (b->d_blocks_store != b->d_blocks_sorted) {
+            (void)hipFree(b->d_blocks_store);
+            b->d_blocks_store = b->d_blocks_sorted;
+            b->nblk_store = b->nblk_sorted;
+        }
+        b->any_stepped = true;
+        return XDEM_OK;
+    }
+    auto key = std::make_pair(nsteps, b->cur * 2 + b->skcur);
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+        hipGraph_t g;
+        int cur0 = b->cur, sk0 = b->skcur;
+        XDEM_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        int32_t rc = enqueue_block(b, nsteps, false);
+        hipGraph_t gg = nullptr;
+        hipError_t e2 = hipStreamEndCapture(b->st, &gg);
+        b->cur = cur0;
+        b->skcur = sk0;
+        if (rc) {
+            if (gg) (void)hipGraphDestroy(gg);
+            return rc;
+        }
+        XDEM_HIP(e2);
+        g = gg;
+        hipGraphExec_t ex;
+        XDEM_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        XDEM_HIP(hipGraphDestroy(g));
+        it = b->graphs.emplace(key, ex).first;
+    }
+    XDEM_HIP(hipGraphLaunch(it->second, b->st));
+    // the captured block flips buffers nsteps times (state) and nsteps times (keys)
+    if (nsteps & 1) {
+        b->cur = 1 - b->cur;
+        b->skcur = 1 - b->skcur;
+    }
+    b->any_stepped = true;
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_synchronize(xdem_batch* b) {
+    XDEM_REQUIRE(b, "null batch");
+    XDEM_HIP(hipStreamSynchronize(b->st));
+    return XDEM_OK;
+}
+
+uint32_t xdem_batch_num_particles(const xdem_batch* b) {
+    uint32_t n = 0;
+    if (b)
+        for (auto& s : b->scenes) n += s.n;
+    return n;
+}
+
+uint32_t xdem_batch_num_scenes(const xdem_batch* b) { return b ? (uint32_t)b->scenes.size() : 0; }
+
+static HostScene* find_scene(xdem_batch* b, uint32_t id) {
+    for (auto& s : b->scenes)
+        if (s.id == id) return &s;
+    return nullptr;
+}
+
+int32_t xdem_batch_get_state(xdem_batch* b, uint32_t scene_id, double* pos4, double* vel3,
+                             double* omega3, double* acc3, double* alpha3) {
+    XDEM_REQUIRE(b, "null batch");
+    HostScene* s = find_scene(b, scene_id);
+    XDEM_REQUIRE(s, "no scene %u", scene_id);
+    uint32_t n = s->n;
+    if (s->pending) {  // not on the device yet: the staged inputs
+        for (uint32_t i = 0; i < n; ++i) {
+            if (pos4)
+                for (int k = 0; k < 4; ++k)
+                    pos4[4 * i + k] = k < 3 ? s->pos[4 * i + k] : (double)(float)s->pos[4 * i + 3];
+            for (int k = 0; k < 3; ++k) {
+                if (vel3) vel3[3 * i + k] = s->vel.empty() ? 0.0 : (double)(float)s->vel[3 * i + k];
+                if (omega3) omega3[3 * i + k] = s->omega.empty() ? 0.0 : (double)(float)s->omega[3 * i + k];
+                if (acc3) acc3[3 * i + k] = 0.0;
+                if (alpha3) alpha3[3 * i + k] = 0.0;
+            }
+        }
+        return XDEM_OK;
+    }
+    XDEM_HIP(hipStreamSynchronize(b->st));
+    const BState& A = b->buf[b->cur];
+    uint32_t p0 = s->p0;
+    std::vector<double> px(n), py(n), pz(n);
+    std::vector<float4> vr(n), wm(n), ac(n);
+    std::vector<float2> al(n);
+    std::vector<uint32_t> id(n);
+    XDEM_HIP(hipMemcpy(px.data(), A.px + p0, 8ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(py.data(), A.py + p0, 8ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(pz.data(), A.pz + p0, 8ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(vr.data(), A.vr + p0, 16ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(wm.data(), A.wm + p0, 16ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(ac.data(), A.acc + p0, 16ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(al.data(), A.alyz + p0, 8ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(id.data(), A.id + p0, 4ull * n, hipMemcpyDeviceToHost));
+    for (uint32_t j = 0; j < n; ++j) {
+        uint32_t i = id[j];
+        XDEM_REQUIRE(i < n, "corrupt id %u at %u", i, j);
+        if (pos4) {
+            pos4[4 * i] = px[j];
+            pos4[4 * i + 1] = py[j];
+            pos4[4 * i + 2] = pz[j];
+            pos4[4 * i + 3] = vr[j].w;
+        }
+        if (vel3) {
+            vel3[3 * i] = vr[j].x;
+            vel3[3 * i + 1] = vr[j].y;
+            vel3[3 * i + 2] = vr[j].z;
+        }
+        if (omega3) {
+            omega3[3 * i] = wm[j].x;
+            omega3[3 * i + 1] = wm[j].y;
+            omega3[3 * i + 2] = wm[j].z;
+        }
+        if (acc3) {
+            acc3[3 * i] = ac[j].x;
+            acc3[3 * i + 1] = ac[j].y;
+            acc3[3 * i + 2] = ac[j].z;
+        }
+        if (alpha3) {
+            alpha3[3 * i] = ac[j].w;
+            alpha3[3 * i + 1] = al[j].x;
+            alpha3[3 * i + 2] = al[j].y;
+        }
+    }
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_enable_timing(xdem_batch* b, int32_t on) {
+    XDEM_REQUIRE(b, "null batch");
+    XDEM_TRY(b->timer.flush());
+    b->timer.reset();
+    b->timer.on = on != 0;
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_phase_times(xdem_batch* b, double* ms6, uint32_t* count6) {
+    XDEM_REQUIRE(b, "null batch");
+    XDEM_TRY(b->timer.flush());
+    for (int i = 0; i < 6; ++i) {
+        if (ms6) ms6[i] = b->timer.ms[i];
+        if (count6) count6[i] = b->timer.count[i];
+    }
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_last_movers(xdem_batch* b, uint32_t* movers) {
+    XDEM_REQUIRE(b && movers, "null argument");
+    XDEM_HIP(hipStreamSynchronize(b->st));
+    XDEM_HIP(hipMemcpy(movers, b->counters + 1, 4, hipMemcpyDeviceToHost));
+    return XDEM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ energy
+namespace {
+// Kinetic energy and momentum of one segment, FP64 accumulation, one block,
+// fixed order (deterministic).
+__global__ __launch_bounds__(1024) void k_energy(BState st, uint32_t p0, uint32_t n, double* out) {
+    __shared__ double s[4][1024];
+    double ke = 0, mx = 0, my = 0, mz = 0;
+    for (uint32_t j = threadIdx.x; j < n; j += 1024) {
+        float4 vr = st.vr[p0 + j], wm = st.wm[p0 + j];
+        double m = wm.w, r = vr.w;
+        double vx = vr.x, vy = vr.y, vz = vr.z, wx = wm.x, wy = wm.y, wz = wm.z;
+        double I = 0.4 * m * r * r;
+        ke += 0.5 * m * (vx * vx + vy * vy + vz * vz) + 0.5 * I * (wx * wx + wy * wy + wz * wz);
+        mx += m * vx;
+        my += m * vy;
+        mz += m * vz;
+    }
+    s[0][threadIdx.x] = ke;
+    s[1][threadIdx.x] = mx;
+    s[2][threadIdx.x] = my;
+    s[3][threadIdx.x] = mz;
+    __syncthreads();
+    for (uint32_t h = 512; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < 4; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Gather the state into sorted order (perm) — used after a parity prepare so
+// that storage is sorted again.
+__global__ void k_gather(BState in, BState out, const uint32_t* perm, uint32_t n) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t s = perm[j];
+    out.px[j] = in.px[s];
+    out.py[j] = in.py[s];
+    out.pz[j] = in.pz[s];
+    out.vr[j] = in.vr[s];
+    out.wm[j] = in.wm[s];
+    out.acc[j] = in.acc[s];
+    out.alyz[j] = in.alyz[s];
+    out.id[j] = in.id[s];
+}
+}  // namespace
+
+extern "C" {
+
+int32_t xdem_batch_energy(xdem_batch* b, uint32_t scene_id, double* kinetic, double momentum[3]) {
+    XDEM_REQUIRE(b, "null batch");
+    HostScene* s = find_scene(b, scene_id);
+    XDEM_REQUIRE(s, "no scene %u", scene_id);
+    XDEM_REQUIRE(!s->pending, "scene %u not on the device yet (step first)", scene_id);
+    hipLaunchKernelGGL(k_energy, dim3(1), dim3(1024), 0, b->st, b->buf[b->cur], s->p0, s->n, b->d_red);
+    XDEM_LAUNCHED();
+    double h[4];
+    XDEM_HIP(hipMemcpyAsync(h, b->d_red, sizeof(h), hipMemcpyDeviceToHost, b->st));
+    XDEM_HIP(hipStreamSynchronize(b->st));
+    if (kinetic) *kinetic = h[0];
+    if (momentum) {
+        momentum[0] = h[1];
+        momentum[1] = h[2];
+        momentum[2] = h[3];
+    }
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_prepare(xdem_batch* b) {
+    XDEM_REQUIRE(b, "null batch");
+    XDEM_TRY(apply_layout(b));
+    if (!b->n_total) return XDEM_OK;
+    bool first = b->storage_sorted_all_new;
+    XDEM_TRY(launch_pos(b, false, b->d_blocks_store, b->nblk_store));
+    if (first)
+        XDEM_TRY(run_full_sort(b));
+    else
+        XDEM_TRY(run_sort(b));
+    // storage := sorted order, so the next block starts from sorted storage
+    BState& A = b->buf[b->cur];
+    BState& B = b->buf[1 - b->cur];
+    hipLaunchKernelGGL(k_gather, dim3(div_up(b->n_total, 256)), dim3(256), 0, b->st, A, B, b->perm,
+                       b->n_total);
+    XDEM_LAUNCHED();
+    b->cur = 1 - b->cur;
+    hipLaunchKernelGGL(k_iota, dim3(div_up(b->n_total, 256)), dim3(256), 0, b->st, b->perm, b->n_total);
+    XDEM_LAUNCHED();
+    b->storage_sorted_all_new = false;
+    if (b->d_blocks_store != b->d_blocks_sorted) {
+        (void)hipFree(b->d_blocks_store);
+        b->d_blocks_store = b->d_blocks_sorted;
+        b->nblk_store = b->nblk_sorted;
+    }
+    b->drop_graphs();
+    b->prepared = true;
+    XDEM_HIP(hipStreamSynchronize(b->st));
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_probe(xdem_batch* b, uint32_t scene_id, double* force3, double* moment3,
+                         uint32_t max_per, uint32_t* counts, uint32_t* partners,
+                         uint32_t* sorted_keys, uint32_t* sorted_ids, uint32_t* cstart,
+                         uint32_t* cend) {
+    XDEM_REQUIRE(b, "null batch");
+    XDEM_REQUIRE(b->prepared, "call xdem_batch_prepare first");
+    HostScene* s = find_scene(b, scene_id);
+    XDEM_REQUIRE(s, "no scene %u", scene_id);
+    uint32_t mp = std::max<uint32_t>(max_per, 1);
+    if (b->dbg_cap != mp || !b->dbg_f) {
+        if (b->dbg_f) (void)hipFree(b->dbg_f);
+        if (b->dbg_cnt) (void)hipFree(b->dbg_cnt);
+        if (b->dbg_part) (void)hipFree(b->dbg_part);
+        b->dbg_f = nullptr;
+        b->dbg_cnt = nullptr;
+        b->dbg_part = nullptr;
+        XDEM_TRY(dalloc(&b->dbg_f, 6ull * b->cap_np));
+        XDEM_TRY(dalloc(&b->dbg_cnt, (size_t)b->cap_np));
+        XDEM_TRY(dalloc(&b->dbg_part, (size_t)b->cap_np * mp));
+        b->dbg_cap = mp;
+    }
+    StepArgs a = step_args(b, true);  // no advance: keynew is not written
+    hipLaunchKernelGGL((k_step<false, true>), dim3(b->nblk_sorted), dim3(b->block_threads), 0, b->st,
+                       a);
+    XDEM_LAUNCHED();
+    XDEM_HIP(hipStreamSynchronize(b->st));
+    uint32_t n = s->n, p0 = s->p0;
+    std::vector<float> f(6ull * n);
+    std::vector<uint32_t> cnt(n), part((size_t)n * mp), id(n), sk(n), prm(n);
+    XDEM_HIP(hipMemcpy(f.data(), b->dbg_f + 6ull * p0, 24ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(cnt.data(), b->dbg_cnt + p0, 4ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(part.data(), b->dbg_part + (size_t)p0 * mp, 4ull * n * mp, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(id.data(), b->buf[b->cur].id + p0, 4ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(sk.data(), b->skey[b->skcur] + p0, 4ull * n, hipMemcpyDeviceToHost));
+    for (uint32_t j = 0; j < n; ++j) {
+        uint32_t i = id[j];
+        XDEM_REQUIRE(i < n, "corrupt id");
+        for (int k = 0; k < 3; ++k) {
+            if (force3) force3[3 * i + k] = f[6 * j + k];
+            if (moment3) moment3[3 * i + k] = f[6 * j + 3 + k];
+        }
+        if (counts) counts[i] = cnt[j];
+        if (partners)
+            for (uint32_t c = 0; c < mp; ++c) partners[(size_t)i * mp + c] = part[(size_t)j * mp + c];
+        if (sorted_keys) sorted_keys[j] = sk[j] - s->cell_base;
+        if (sorted_ids) sorted_ids[j] = i;
+    }
+    if (cstart || cend) {
+        std::vector<uint2> cr(s->ncell);
+        XDEM_HIP(hipMemcpy(cr.data(), b->crange + s->cell_base, 8ull * s->ncell, hipMemcpyDeviceToHost));
+        for (uint32_t c = 0; c < s->ncell; ++c) {
+            bool empty = cr[c].x == EMPTY;
+            if (cstart) cstart[c] = empty ? EMPTY : cr[c].x - p0;
+            if (cend) cend[c] = empty ? 0u : cr[c].y - p0;
+        }
+    }
+    return XDEM_OK;
+}
+
+}  // extern "C"

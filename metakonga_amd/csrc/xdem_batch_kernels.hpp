@@ -1,0 +1,606 @@
+// xdem_batch_kernels.hpp — device code of the fused FP32c path.
+//
+// One step of one scene (dem_simulation::oneStepAnalysis,
+// source/simulation/dem_simulation.cpp:447-469, TCM = 1) becomes:
+//   k_step   contacts (particle-particle in the reference traversal order, then
+//            the scene's planes) + Verlet velocity update + next Verlet position
+//            update + next cell key, written in the new sorted order (the
+//            physical reorder rides on the writes);
+//   k_movers / k_sort_movers / k_merge / k_ranges
+//            re-sort by (cell key, particle id): only the particles whose cell
+//            changed ("movers") are sorted; the rest keep their order, and a
+//            merge gives every particle its new position.  The result is the
+//            unique (key, id) order, i.e. what the reference's stable sort of
+//            the id-ordered keys produces (source/mphysics_cuda_def.cu:92-94).
+#pragma once
+#include "xdem_common.hpp"
+#include "xdem_geom.hpp"
+
+namespace xdem {
+namespace fused {
+
+constexpr uint32_t EMPTY = 0xFFFFFFFFu;      // empty cell / sentinel key
+constexpr int CMAX = 16;                     // contact-list slots per particle (LDS)
+constexpr int SORT_LDS_MAX = 4096;           // movers sorted in LDS up to this count
+constexpr int ROWS_T = 256;                  // threads of the row kernels
+
+// Scene constants (FP64 grid/time, FP32 material).
+struct BScene {
+    double wox, woy, woz, cs;
+    double dt, half2dt;
+    uint32_t gx, gy, gz, cell_base;
+    uint32_t p0, n;           // segment in sorted order
+    uint32_t plane0, nplane;  // into the plane table
+    float gvx, gvy, gvz, dtf;
+    float ckn, cvn, sr, mu;   // particle-particle: kn = ckn sqrt(R*), vn = sqrt(cvn M* kn)
+    float mass_c;             // m = mass_c r^3 (rho 4/3 pi)
+    int32_t rolling;
+    uint32_t ncell;
+    uint32_t pad;
+};
+
+struct BPlane {
+    xdem_device_plane_info pl;
+    float ckn, cvn, sr, mu;
+};
+
+// One state buffer (SoA).  vr = (v, r), wm = (omega, m), acc = (a, alpha.x),
+// alyz = (alpha.y, alpha.z).
+struct BState {
+    double* px;
+    double* py;
+    double* pz;
+    float4* vr;
+    float4* wm;
+    float4* acc;
+    float2* alyz;
+    uint32_t* id;
+};
+
+struct BlockEntry {
+    uint32_t scene, j0, j1, pad;
+};
+
+struct StepArgs {
+    BState in, out;
+    const uint32_t* perm;      // sorted position -> storage index of `in`
+    const uint2* crange;       // [key] = (first sorted position, one past last), EMPTY if none
+    uint32_t* keynew;          // next cell key per sorted position
+    const BScene* scenes;
+    const BPlane* planes;
+    const BlockEntry* blocks;
+    uint32_t nblocks;
+    // parity probe outputs (per sorted position) — null in production
+    float* dbg_f;              // 6 floats: force xyz, torque xyz
+    uint32_t* dbg_cnt;
+    uint32_t* dbg_part;        // dbg_max ids per particle
+    uint32_t dbg_max;
+};
+
+__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nblk) {
+    // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous run of logical blocks so neighbouring particle ranges share
+    // one L2.
+    uint32_t q = nblk / 8, r = nblk % 8, x = b % 8, i = b / 8;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+__device__ inline void cell_of(const BScene& S, double x, double y, double z, int& cx, int& cy,
+                               int& cz) {
+    cx = (int)floor((x - S.wox) / S.cs);
+    cy = (int)floor((y - S.woy) / S.cs);
+    cz = (int)floor((z - S.woz) / S.cs);
+}
+
+__device__ inline uint32_t key_of(const BScene& S, int cx, int cy, int cz) {
+    uint32_t x = (uint32_t)cx & (S.gx - 1);
+    uint32_t y = (uint32_t)cy & (S.gy - 1);
+    uint32_t z = (uint32_t)cz & (S.gz - 1);
+    return S.cell_base + (__umul24(__umul24(z, S.gy), S.gx) + __umul24(y, S.gx) + x);
+}
+
+struct f3 {
+    float x, y, z;
+};
+__device__ inline f3 F3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ inline f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+__device__ inline f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+__device__ inline void operator+=(f3& a, f3 b) {
+    a.x += b.x;
+    a.y += b.y;
+    a.z += b.z;
+}
+__device__ inline float dotf(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline f3 crossf(f3 a, f3 b) {
+    return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// DHS force law in FP32 (mphysics_cuda_impl.cuh:379-401).  Ft and M keep the
+// caller's values when the tangential velocity is exactly zero (as the
+// reference kernel, whose Ft/M live outside the neighbour loop).
+__device__ inline void dhs_f32(float kn, float vn, float ks, float vs, float mu, float dt,
+                               float ir, float delta, f3 dv, f3 u, f3& Ft, f3& Fn, f3& M) {
+    float fsn = -kn * (delta * sqrtf(delta));
+    float fdn = vn * dotf(dv, u);
+    Fn = (fsn + fdn) * u;
+    f3 e = dv - dotf(dv, u) * u;
+    float mag_e = sqrtf(dotf(e, e));
+    if (mag_e != 0.0f) {
+        float inv = 1.0f / mag_e;
+        f3 sh = inv * e;
+        float ds = mag_e * dt;
+        float fn_len = sqrtf(dotf(Fn, Fn));
+        Ft = fminf(ks * ds + vs * dotf(dv, sh), mu * fn_len) * sh;
+        M = crossf(ir * u, Ft);
+    }
+}
+
+struct Own {
+    double x, y, z;
+    f3 v, w;
+    float r, m;
+};
+
+// Particle-particle contact of `o` with the particle stored at t of `in`
+// (the predicate was already found true).  Accumulates into sumF/sumM.
+__device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own& o, uint32_t t,
+                                  f3& Ft, f3& Fn, f3& M, f3& sumF, f3& sumM) {
+    double jx = A.in.px[t], jy = A.in.py[t], jz = A.in.pz[t];
+    float4 jvr = A.in.vr[t];
+    float4 jwm = A.in.wm[t];
+    double dx = jx - o.x, dy = jy - o.y, dz = jz - o.z;
+    double d2 = dx * dx + dy * dy + dz * dz;
+    double dist = sqrt(d2);
+    double cdist = ((double)o.r + (double)jvr.w) - dist;
+    float inv = (float)(1.0 / dist);
+    f3 u = F3((float)dx * inv, (float)dy * inv, (float)dz * inv);
+    float jr = jvr.w;
+    f3 jv = F3(jvr.x, jvr.y, jvr.z), jw = F3(jwm.x, jwm.y, jwm.z);
+    f3 dv = S.rolling ? (jv + crossf(jw, (-jr) * u)) - (o.v + crossf(o.w, o.r * u)) : (jv - o.v);
+    float R = (o.r * jr) / (o.r + jr);
+    float Ms = (o.m * jwm.w) / (o.m + jwm.w);
+    float kn = S.ckn * sqrtf(R);
+    float vn = sqrtf(S.cvn * Ms * kn);
+    dhs_f32(kn, vn, kn * S.sr, vn * S.sr, S.mu, S.dtf, o.r, (float)cdist, dv, u, Ft, Fn, M);
+    sumF += Fn + Ft;
+    sumM += M;
+}
+
+template <bool ADV, bool DBG>
+__global__ __launch_bounds__(256) void k_step(StepArgs A) {
+    __shared__ uint32_t clist[256 * CMAX];
+    const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
+    const BlockEntry be = A.blocks[L];
+    const BScene S = A.scenes[be.scene];
+    const uint32_t j = be.j0 + threadIdx.x;
+    if (j >= be.j1) return;
+    uint32_t* mylist = clist + threadIdx.x * CMAX;
+    const uint32_t s = A.perm[j];
+    Own o;
+    o.x = A.in.px[s];
+    o.y = A.in.py[s];
+    o.z = A.in.pz[s];
+    float4 vr = A.in.vr[s], wm = A.in.wm[s], ac = A.in.acc[s];
+    float2 al2 = A.in.alyz[s];
+    uint32_t pid = A.in.id[s];
+    o.v = F3(vr.x, vr.y, vr.z);
+    o.r = vr.w;
+    o.w = F3(wm.x, wm.y, wm.z);
+    o.m = wm.w;
+    int cx, cy, cz;
+    cell_of(S, o.x, o.y, o.z, cx, cy, cz);
+    f3 Ft = F3(0, 0, 0), Fn = F3(0, 0, 0), M = F3(0, 0, 0);
+    f3 sumF = F3(0, 0, 0), sumM = F3(0, 0, 0);
+    uint32_t nlist = 0, ntotal = 0;
+    const double ri = (double)o.r;
+    // phase 1: candidates in the reference order (z, y, x cells; ascending j)
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                uint2 cr = A.crange[key_of(S, cx + dx, cy + dy, cz + dz)];
+                if (cr.x == EMPTY) continue;
+                for (uint32_t jj = cr.x; jj < cr.y; ++jj) {
+                    if (jj == j) continue;
+                    uint32_t t = A.perm[jj];
+                    double ex = A.in.px[t] - o.x, ey = A.in.py[t] - o.y, ez = A.in.pz[t] - o.z;
+                    double d2 = ex * ex + ey * ey + ez * ez;
+                    double Sr = ri + (double)A.in.vr[t].w;
+                    if (d2 > Sr * Sr * 1.000001) continue;
+                    if (!(Sr - sqrt(d2) > 0)) continue;
+                    if (DBG && ntotal < A.dbg_max) A.dbg_part[(size_t)j * A.dbg_max + ntotal] = A.in.id[t];
+                    ++ntotal;
+                    mylist[nlist++] = t;
+                    if (nlist == CMAX) {
+                        for (uint32_t c = 0; c < nlist; ++c)
+                            pp_contact(A, S, o, mylist[c], Ft, Fn, M, sumF, sumM);
+                        nlist = 0;
+                    }
+                }
+            }
+    // phase 2: the listed contacts, in order
+    for (uint32_t c = 0; c < nlist; ++c) pp_contact(A, S, o, mylist[c], Ft, Fn, M, sumF, sumM);
+    // F = m g + particle-particle (vv_update_velocity reset, then cu_calculate_p2p)
+    f3 F = F3(o.m * S.gvx, o.m * S.gvy, o.m * S.gvz) + sumF;
+    f3 T = sumM;
+    // planes, in order (plane_contact_force_kernel: no torque, ir = 0)
+    for (uint32_t p = 0; p < S.nplane; ++p) {
+        const BPlane& bp = A.planes[S.plane0 + p];
+        d3 ip = mk3(o.x, o.y, o.z);
+        d3 dp = ip - of(bp.pl.xw);
+        d3 unit = mk3(0, 0, 0);
+        d3 wp = mk3(dot(dp, of(bp.pl.u1)), dot(dp, of(bp.pl.u2)), dot(dp, of(bp.pl.uw)));
+        double cdist = plane_detect(bp.pl, ip, wp, unit, ri);
+        if (cdist > 0) {
+            f3 u = F3((float)unit.x, (float)unit.y, (float)unit.z);
+            f3 dv = S.rolling ? -(o.v + crossf(o.w, o.r * u)) : -o.v;
+            float kn = bp.ckn * sqrtf(o.r);
+            float vn = sqrtf(bp.cvn * o.m * kn);
+            f3 pFt = F3(0, 0, 0), pFn = F3(0, 0, 0), pM = F3(0, 0, 0);
+            dhs_f32(kn, vn, kn * bp.sr, vn * bp.sr, bp.mu, S.dtf, 0.0f, (float)cdist, dv, u, pFt,
+                    pFn, pM);
+            F += pFn + pFt;
+        }
+    }
+    if (DBG) {
+        float* d = A.dbg_f + 6 * (size_t)j;
+        d[0] = F.x - o.m * S.gvx;
+        d[1] = F.y - o.m * S.gvy;
+        d[2] = F.z - o.m * S.gvz;
+        d[3] = T.x;
+        d[4] = T.y;
+        d[5] = T.z;
+        A.dbg_cnt[j] = ntotal;
+    }
+    // vv_update_velocity_kernel (mphysics_cuda_impl.cuh:98-134) in FP32
+    float inv_m = 1.0f / o.m;
+    float inv_i = 1.0f / (0.4f * o.m * o.r * o.r);
+    f3 a0 = F3(ac.x, ac.y, ac.z), al0 = F3(ac.w, al2.x, al2.y);
+    f3 a = inv_m * F, al = inv_i * T;
+    float hdt = 0.5f * S.dtf;
+    f3 v = o.v + hdt * (a0 + a);
+    f3 w = o.w + hdt * (al0 + al);
+    double nx = o.x, ny = o.y, nz = o.z;
+    if (ADV) {
+        // next vv_update_position (:85-96) in FP64: p += dt v + half2dt a
+        nx += S.dt * (double)v.x + S.half2dt * (double)a.x;
+        ny += S.dt * (double)v.y + S.half2dt * (double)a.y;
+        nz += S.dt * (double)v.z + S.half2dt * (double)a.z;
+        int kx, ky, kz;
+        cell_of(S, nx, ny, nz, kx, ky, kz);
+        A.keynew[j] = key_of(S, kx, ky, kz);
+    }
+    A.out.px[j] = nx;
+    A.out.py[j] = ny;
+    A.out.pz[j] = nz;
+    A.out.vr[j] = make_float4(v.x, v.y, v.z, o.r);
+    A.out.wm[j] = make_float4(w.x, w.y, w.z, o.m);
+    A.out.acc[j] = make_float4(a.x, a.y, a.z, al.x);
+    A.out.alyz[j] = make_float2(al.y, al.z);
+    A.out.id[j] = pid;
+}
+
+// Verlet position update (optional) + cell key of every particle, in place,
+// in storage order (block start / parity prepare).
+template <bool ADV>
+__global__ __launch_bounds__(256) void k_pos(BState st, uint32_t* keynew, const BScene* scenes,
+                                             const BlockEntry* blocks) {
+    const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
+    const BlockEntry be = blocks[L];
+    const BScene S = scenes[be.scene];
+    const uint32_t j = be.j0 + threadIdx.x;
+    if (j >= be.j1) return;
+    double x = st.px[j], y = st.py[j], z = st.pz[j];
+    if (ADV) {
+        float4 vr = st.vr[j], ac = st.acc[j];
+        x += S.dt * (double)vr.x + S.half2dt * (double)ac.x;
+        y += S.dt * (double)vr.y + S.half2dt * (double)ac.y;
+        z += S.dt * (double)vr.z + S.half2dt * (double)ac.z;
+        st.px[j] = x;
+        st.py[j] = y;
+        st.pz[j] = z;
+    }
+    int cx, cy, cz;
+    cell_of(S, x, y, z, cx, cy, cz);
+    keynew[j] = key_of(S, cx, cy, cz);
+}
+
+// ------------------------------------------------------------------ sort
+struct SortArgs {
+    uint32_t n;                 // particles (storage positions)
+    const uint32_t* key_old;    // sorted keys of the previous sort, storage order
+    const uint32_t* key_new;    // keys of the current positions, storage order
+    const uint32_t* id;         // particle ids, storage order
+    uint2* crange;
+    uint64_t* moverbits;        // one bit per storage position
+    uint32_t* blockcnt;         // movers per ROWS_T block; scanned in place (+1 total)
+    uint32_t* mcount;           // device counter of movers (reset by k_sort_movers)
+    uint32_t* msorted;          // number of sorted movers (written by k_sort_movers)
+    uint64_t* mkey;             // mover (key << 32 | id), append order
+    uint32_t* ms;               // mover storage index, append order
+    uint64_t* mkey2;            // sorted movers (and radix ping-pong)
+    uint32_t* ms2;
+    uint32_t* perm;             // out: sorted position -> storage index
+    uint32_t* skey;             // out: sorted keys
+    uint32_t nblk;              // ROWS_T blocks over n
+    uint32_t key_bits, id_bits;
+};
+
+__device__ inline uint64_t lanemask_lt64() {
+    uint32_t lane = __lane_id();
+    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+// Mover detection: clears the previous cell ranges, marks movers, counts them
+// per block and appends them (order irrelevant: they are sorted next).
+__global__ __launch_bounds__(ROWS_T) void k_movers(SortArgs A) {
+    __shared__ uint32_t wc[ROWS_T / 64];
+    const uint32_t s = blockIdx.x * ROWS_T + threadIdx.x;
+    const bool valid = s < A.n;
+    uint32_t old = valid ? A.key_old[s] : 0, nw = valid ? A.key_new[s] : 0;
+    if (valid && old != EMPTY && (s == 0 || A.key_old[s - 1] != old)) A.crange[old] = make_uint2(EMPTY, 0);
+    bool mover = valid && old != nw;
+    uint64_t word = __ballot(mover);
+    uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0 && s < A.n) A.moverbits[s >> 6] = word;
+    uint32_t cnt = __popcll(word);
+    uint32_t base = 0;
+    if (lane == 0 && cnt) base = atomicAdd(A.mcount, cnt);
+    base = __shfl(base, 0, 64);
+    if (mover) {
+        uint32_t p = base + __popcll(word & lanemask_lt64());
+        A.mkey[p] = ((uint64_t)nw << 32) | A.id[s];
+        A.ms[p] = s;
+    }
+    if (lane == 0) wc[w] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int k = 0; k < ROWS_T / 64; ++k) t += wc[k];
+        A.blockcnt[blockIdx.x] = t;
+    }
+}
+
+__device__ inline uint32_t wave_scan_incl(uint32_t v) {
+    uint32_t lane = __lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += u;
+    }
+    return v;
+}
+
+// Single-workgroup: scan of the per-block mover counts, then the movers sorted
+// by (key, id): bitonic in LDS up to SORT_LDS_MAX, else an LSD radix sort in
+// global memory by the one workgroup (rare: large admissions).
+__global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
+    __shared__ uint64_t lk[SORT_LDS_MAX];
+    __shared__ uint32_t lv[SORT_LDS_MAX];
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry_s;
+    __shared__ uint32_t hist[256];
+    uint32_t* wcnt = lv;  // 16 waves x 256 digits, only in the radix path (lv unused there)
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t M = *A.mcount;
+    // exclusive scan of blockcnt[0..nblk) in place, total at [nblk]
+    if (t == 0) carry_s = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < A.nblk; base += 1024) {
+        uint32_t e = base + t;
+        uint32_t v = e < A.nblk ? A.blockcnt[e] : 0;
+        uint32_t inc = wave_scan_incl(v);
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        if (w == 0) {
+            uint32_t x = lane < 16 ? wsum[lane] : 0;
+            uint32_t xi = wave_scan_incl(x);
+            if (lane < 16) wsum[lane] = xi - x;
+        }
+        __syncthreads();
+        uint32_t carry = carry_s;
+        if (e < A.nblk) A.blockcnt[e] = carry + wsum[w] + inc - v;
+        __syncthreads();
+        if (t == 1023) carry_s = carry + wsum[w] + inc;
+        __syncthreads();
+    }
+    if (t == 0) {
+        A.blockcnt[A.nblk] = carry_s;
+        *A.msorted = M;
+        *A.mcount = 0;
+    }
+    if (M == 0) return;
+    if (M <= (uint32_t)SORT_LDS_MAX) {
+        uint32_t P = 2;
+        while (P < M) P <<= 1;
+        for (uint32_t i = t; i < P; i += 1024) {
+            lk[i] = i < M ? A.mkey[i] : ~0ull;
+            lv[i] = i < M ? A.ms[i] : 0;
+        }
+        __syncthreads();
+        for (uint32_t k = 2; k <= P; k <<= 1) {
+            for (uint32_t jb = k >> 1; jb > 0; jb >>= 1) {
+                for (uint32_t i = t; i < P; i += 1024) {
+                    uint32_t ix = i ^ jb;
+                    if (ix > i) {
+                        bool up = (i & k) == 0;
+                        uint64_t a = lk[i], b = lk[ix];
+                        if ((a > b) == up) {
+                            lk[i] = b;
+                            lk[ix] = a;
+                            uint32_t tv = lv[i];
+                            lv[i] = lv[ix];
+                            lv[ix] = tv;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t i = t; i < M; i += 1024) {
+            A.mkey2[i] = lk[i];
+            A.ms2[i] = lv[i];
+        }
+        return;
+    }
+    // LSD radix over the id bits then the key bits; 8-bit digits; stable by
+    // processing 1024-item chunks in order.
+    uint64_t* src_k = A.mkey;
+    uint32_t* src_v = A.ms;
+    uint64_t* dst_k = A.mkey2;
+    uint32_t* dst_v = A.ms2;
+    int shifts[16];
+    int npass = 0;
+    for (uint32_t b = 0; b < A.id_bits; b += 8) shifts[npass++] = (int)b;
+    for (uint32_t b = 0; b < A.key_bits; b += 8) shifts[npass++] = 32 + (int)b;
+    for (int p = 0; p < npass; ++p) {
+        int sh = shifts[p];
+        if (t < 256) hist[t] = 0;
+        __syncthreads();
+        for (uint32_t i = t; i < M; i += 1024) atomicAdd(&hist[(uint32_t)(src_k[i] >> sh) & 255u], 1u);
+        __syncthreads();
+        if (t == 0) {
+            uint32_t run = 0;
+            for (int d = 0; d < 256; ++d) {
+                uint32_t c = hist[d];
+                hist[d] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+        const uint64_t lt = lanemask_lt64();
+        for (uint32_t base = 0; base < M; base += 1024) {
+            for (uint32_t i = t; i < 16 * 256; i += 1024) wcnt[i] = 0;
+            __syncthreads();
+            uint32_t e = base + t;
+            bool valid = e < M;
+            uint64_t kk = valid ? src_k[e] : 0;
+            uint32_t vv = valid ? src_v[e] : 0;
+            uint32_t d = (uint32_t)(kk >> sh) & 255u;
+            uint64_t active = __ballot(valid);
+            uint64_t peers = active;
+            for (int bb = 0; bb < 8; ++bb) {
+                int bit = (d >> bb) & 1;
+                uint64_t m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
+            uint32_t rank = __popcll(peers & lt);
+            if (valid && (peers & lt) == 0) wcnt[w * 256 + d] = __popcll(peers);
+            __syncthreads();
+            if (t < 256) {  // exclusive over waves per digit; chunk total of digit t
+                uint32_t run = 0;
+                for (int k = 0; k < 16; ++k) {
+                    uint32_t c = wcnt[k * 256 + t];
+                    wcnt[k * 256 + t] = run;
+                    run += c;
+                }
+                lk[t] = run;  // lk is free in this path: 256-entry scratch
+            }
+            __syncthreads();
+            if (valid) {
+                uint32_t o = hist[d] + wcnt[w * 256 + d] + rank;
+                dst_k[o] = kk;
+                dst_v[o] = vv;
+            }
+            __syncthreads();
+            if (t < 256) hist[t] += (uint32_t)lk[t];
+            __syncthreads();
+        }
+        uint64_t* tk = src_k;
+        src_k = dst_k;
+        dst_k = tk;
+        uint32_t* tv = src_v;
+        src_v = dst_v;
+        dst_v = tv;
+        __syncthreads();
+    }
+    if (src_k != A.mkey2) {
+        for (uint32_t i = t; i < M; i += 1024) {
+            A.mkey2[i] = src_k[i];
+            A.ms2[i] = src_v[i];
+        }
+    }
+}
+
+// Movers at storage positions < P.
+__device__ inline uint32_t mover_prefix(const SortArgs& A, uint32_t P) {
+    if (P >= A.n) return A.blockcnt[A.nblk];
+    uint32_t b = P / ROWS_T;
+    uint32_t c = A.blockcnt[b];
+    uint32_t w0 = b * (ROWS_T / 64), w = P >> 6;
+    for (uint32_t k = w0; k < w; ++k) c += __popcll(A.moverbits[k]);
+    uint32_t r = P & 63;
+    if (r) c += __popcll(A.moverbits[w] & ((1ull << r) - 1));
+    return c;
+}
+
+// Merge: stayers keep their relative order and shift by the movers placed
+// before them; movers are placed by their rank among the stayers.
+__global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
+    const uint32_t M = *A.msorted;
+    if (blockIdx.x < A.nblk) {
+        const uint32_t s = blockIdx.x * ROWS_T + threadIdx.x;
+        if (s >= A.n) return;
+        uint32_t kn = A.key_new[s];
+        if (M && kn != A.key_old[s]) return;  // a mover
+        uint32_t out = s;
+        if (M) {
+            uint64_t X = ((uint64_t)kn << 32) | A.id[s];
+            uint32_t lo = 0, hi = M;  // count of sorted movers < X
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (A.mkey2[mid] < X)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            out = s - mover_prefix(A, s) + lo;
+        }
+        A.perm[out] = s;
+        A.skey[out] = kn;
+        return;
+    }
+    const uint32_t nmb = gridDim.x - A.nblk;
+    for (uint32_t r = (blockIdx.x - A.nblk) * ROWS_T + threadIdx.x; r < M; r += nmb * ROWS_T) {
+        uint64_t X = A.mkey2[r];
+        uint32_t lo = 0, hi = A.n;  // storage entries with (key_old, id) < X
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            uint64_t Y = ((uint64_t)A.key_old[mid] << 32) | A.id[mid];
+            if (Y < X)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        uint32_t out = r + (lo - mover_prefix(A, lo));
+        uint32_t s = A.ms2[r];
+        A.perm[out] = s;
+        A.skey[out] = (uint32_t)(X >> 32);
+    }
+}
+
+// Cell ranges of the sorted keys (reorderDataAndFindCellStart_kernel, :166-212).
+__global__ __launch_bounds__(ROWS_T) void k_ranges(const uint32_t* skey, uint2* crange, uint32_t n) {
+    const uint32_t j = blockIdx.x * ROWS_T + threadIdx.x;
+    if (j >= n) return;
+    uint32_t h = skey[j];
+    if (j == 0 || skey[j - 1] != h) {
+        crange[h].x = j;
+        if (j > 0) crange[skey[j - 1]].y = j;
+    }
+    if (j == n - 1) crange[h].y = n;
+}
+
+__global__ void k_iota(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+__global__ void k_fill_u2(uint2* p, uint2 v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t st = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) p[i] = v;
+}
+
+}  // namespace fused
+}  // namespace xdem

@@ -13,6 +13,7 @@
 
 #include "xdem_common.hpp"
 #include "xdem_f64.hpp"
+#include "xdem_geom.hpp"
 
 namespace xdem {
 
@@ -254,80 +255,6 @@ __global__ __launch_bounds__(256) void k_pairs(KParams k, const double* pos,
 }
 
 }  // namespace
-
-// particle_plane_contact_detection — mphysics_cuda_impl.cuh:487-570 (branch
-// order kept, including the unreachable seventh branch of :549).
-__device__ double plane_detect(const xdem_device_plane_info& pe, d3 xp, d3 wp, d3& u, double r) {
-    double a_l1 = pow(wp.x - pe.l1, 2.0);
-    double b_l2 = pow(wp.y - pe.l2, 2.0);
-    double sqa = wp.x * wp.x;
-    double sqb = wp.y * wp.y;
-    double sqc = wp.z * wp.z;
-    double sqr = r * r;
-    d3 xw = of(pe.xw), w2 = of(pe.w2), w3 = of(pe.w3), w4 = of(pe.w4), uw = of(pe.uw);
-    if (fabs(wp.z) < r && (wp.x > 0 && wp.x < pe.l1) && (wp.y > 0 && wp.y < pe.l2)) {
-        d3 dp = xp - xw;
-        d3 uu = uw / length(uw);
-        int pp = -(dot(dp, uw) < 0 ? -1 : 1);
-        u = (double)pp * uu;
-        return r - fabs(dot(dp, u));
-    }
-    if (wp.x < 0 && wp.y < 0 && (sqa + sqb + sqc) < sqr) {
-        d3 Xsw = xp - xw;
-        double h = length(Xsw);
-        u = Xsw / h;
-        return r - h;
-    } else if (wp.x > pe.l1 && wp.y < 0 && (a_l1 + sqb + sqc) < sqr) {
-        d3 Xsw = xp - w2;
-        double h = length(Xsw);
-        u = Xsw / h;
-        return r - h;
-    } else if (wp.x > pe.l1 && wp.y > pe.l2 && (a_l1 + b_l2 + sqc) < sqr) {
-        d3 Xsw = xp - w3;
-        double h = length(Xsw);
-        u = Xsw / h;
-        return r - h;
-    } else if (wp.x < 0 && wp.y > pe.l2 && (sqa + b_l2 + sqc) < sqr) {
-        d3 Xsw = xp - w4;
-        double h = length(Xsw);
-        u = Xsw / h;
-        return r - h;
-    }
-    if ((wp.x > 0 && wp.x < pe.l1) && wp.y < 0 && (sqb + sqc) < sqr) {
-        d3 Xsw = xp - xw;
-        d3 wj_wi = w2 - xw;
-        d3 us = wj_wi / length(wj_wi);
-        d3 h_star = Xsw - (dot(Xsw, us)) * us;
-        double h = length(h_star);
-        u = -h_star / h;
-        return r - h;
-    } else if ((wp.x > 0 && wp.x < pe.l1) && wp.y > pe.l2 && (b_l2 + sqc) < sqr) {
-        d3 Xsw = xp - w4;
-        d3 wj_wi = w3 - w4;
-        d3 us = wj_wi / length(wj_wi);
-        d3 h_star = Xsw - (dot(Xsw, us)) * us;
-        double h = length(h_star);
-        u = -h_star / h;
-        return r - h;
-    } else if ((wp.x > 0 && wp.y < pe.l2) && wp.x < 0 && (sqr + sqc) < sqr) {
-        d3 Xsw = xp - xw;
-        d3 wj_wi = w4 - xw;
-        d3 us = wj_wi / length(wj_wi);
-        d3 h_star = Xsw - (dot(Xsw, us)) * us;
-        double h = length(h_star);
-        u = -h_star / h;
-        return r - h;
-    } else if ((wp.x > 0 && wp.y < pe.l2) && wp.x > pe.l1 && (a_l1 + sqc) < sqr) {
-        d3 Xsw = xp - w2;
-        d3 wj_wi = w3 - w2;
-        d3 us = wj_wi / length(wj_wi);
-        d3 h_star = Xsw - (dot(Xsw, us)) * us;
-        double h = length(h_star);
-        u = -h_star / h;
-        return r - h;
-    }
-    return -1.0;
-}
 
 namespace {
 

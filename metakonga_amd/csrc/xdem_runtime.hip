@@ -54,7 +54,7 @@ bool params_set() { return g_params_set; }
 int32_t Workspace::reserve(size_t need) {
     if (need <= bytes) return XDEM_OK;
     if (ptr) {
-        XDEM_HIP(hipStreamSynchronize(g_stream));
+        XDEM_HIP(hipDeviceSynchronize());  // any stream may still use the old buffer
         XDEM_HIP(hipFree(ptr));
         ptr = nullptr;
         bytes = 0;
