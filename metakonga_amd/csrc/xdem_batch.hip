@@ -168,6 +168,7 @@ struct xdem_batch {
                        uint32_t* skey_out) {
         SortArgs a{};
         a.n = n_total;
+        a.ncell_cap = (uint32_t)cap_cells;
         a.key_old = key_old;
         a.key_new = key_new;
         a.id = id;
@@ -437,7 +438,8 @@ int32_t run_sort(xdem_batch* b) {
     XDEM_LAUNCHED();
     XDEM_TRY(T.end(st));
     XDEM_TRY(T.begin(5, st));
-    hipLaunchKernelGGL(k_ranges, dim3(a.nblk), dim3(ROWS_T), 0, st, knew_sorted, b->crange, n);
+    hipLaunchKernelGGL(k_ranges, dim3(a.nblk), dim3(ROWS_T), 0, st, knew_sorted, b->crange, n,
+                       (uint32_t)b->cap_cells);
     XDEM_LAUNCHED();
     XDEM_TRY(T.end(st));
     b->skcur = 1 - b->skcur;
@@ -455,7 +457,8 @@ int32_t run_full_sort(xdem_batch* b) {
     XDEM_LAUNCHED();
     XDEM_HIP(hipStreamSynchronize(st));  // sort_pairs may grow the library workspace
     XDEM_TRY(sort_pairs(out, b->perm, n, b->key_bits(), st));
-    hipLaunchKernelGGL(k_ranges, dim3(div_up(n, ROWS_T)), dim3(ROWS_T), 0, st, out, b->crange, n);
+    hipLaunchKernelGGL(k_ranges, dim3(div_up(n, ROWS_T)), dim3(ROWS_T), 0, st, out, b->crange, n,
+                       (uint32_t)b->cap_cells);
     XDEM_LAUNCHED();
     b->skcur = 1 - b->skcur;
     return XDEM_OK;
@@ -472,6 +475,7 @@ StepArgs step_args(xdem_batch* b, bool dbg) {
     a.planes = b->d_planes;
     a.blocks = b->d_blocks_sorted;
     a.nblocks = b->nblk_sorted;
+    a.n = b->n_total;
     if (dbg) {
         a.dbg_f = b->dbg_f;
         a.dbg_cnt = b->dbg_cnt;

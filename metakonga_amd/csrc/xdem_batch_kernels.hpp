@@ -70,6 +70,7 @@ struct StepArgs {
     const BPlane* planes;
     const BlockEntry* blocks;
     uint32_t nblocks;
+    uint32_t n;                // particles in the arrays (bounds of perm entries)
     // parity probe outputs (per sorted position) — null in production
     float* dbg_f;              // 6 floats: force xyz, torque xyz
     uint32_t* dbg_cnt;
@@ -178,6 +179,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
     if (j >= be.j1) return;
     uint32_t* mylist = clist + threadIdx.x * CMAX;
     const uint32_t s = A.perm[j];
+    if (s >= A.n) return;  // defensive: perm is a permutation of [0, n)
     Own o;
     o.x = A.in.px[s];
     o.y = A.in.py[s];
@@ -201,9 +203,11 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
             for (int dx = -1; dx <= 1; ++dx) {
                 uint2 cr = A.crange[key_of(S, cx + dx, cy + dy, cz + dz)];
                 if (cr.x == EMPTY) continue;
-                for (uint32_t jj = cr.x; jj < cr.y; ++jj) {
+                const uint32_t e1 = min(cr.y, A.n);
+                for (uint32_t jj = cr.x; jj < e1; ++jj) {
                     if (jj == j) continue;
                     uint32_t t = A.perm[jj];
+                    if (t >= A.n) continue;
                     double ex = A.in.px[t] - o.x, ey = A.in.py[t] - o.y, ez = A.in.pz[t] - o.z;
                     double d2 = ex * ex + ey * ey + ez * ez;
                     double Sr = ri + (double)A.in.vr[t].w;
@@ -309,6 +313,7 @@ __global__ __launch_bounds__(256) void k_pos(BState st, uint32_t* keynew, const 
 // ------------------------------------------------------------------ sort
 struct SortArgs {
     uint32_t n;                 // particles (storage positions)
+    uint32_t ncell_cap;         // size of crange
     const uint32_t* key_old;    // sorted keys of the previous sort, storage order
     const uint32_t* key_new;    // keys of the current positions, storage order
     const uint32_t* id;         // particle ids, storage order
@@ -339,7 +344,8 @@ __global__ __launch_bounds__(ROWS_T) void k_movers(SortArgs A) {
     const uint32_t s = blockIdx.x * ROWS_T + threadIdx.x;
     const bool valid = s < A.n;
     uint32_t old = valid ? A.key_old[s] : 0, nw = valid ? A.key_new[s] : 0;
-    if (valid && old != EMPTY && (s == 0 || A.key_old[s - 1] != old)) A.crange[old] = make_uint2(EMPTY, 0);
+    if (valid && old < A.ncell_cap && (s == 0 || A.key_old[s - 1] != old))
+        A.crange[old] = make_uint2(EMPTY, 0);
     bool mover = valid && old != nw;
     uint64_t word = __ballot(mover);
     uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -350,8 +356,10 @@ __global__ __launch_bounds__(ROWS_T) void k_movers(SortArgs A) {
     base = __shfl(base, 0, 64);
     if (mover) {
         uint32_t p = base + __popcll(word & lanemask_lt64());
-        A.mkey[p] = ((uint64_t)nw << 32) | A.id[s];
-        A.ms[p] = s;
+        if (p < A.n) {
+            A.mkey[p] = ((uint64_t)nw << 32) | A.id[s];
+            A.ms[p] = s;
+        }
     }
     if (lane == 0) wc[w] = cnt;
     __syncthreads();
@@ -382,7 +390,7 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
     __shared__ uint32_t hist[256];
     uint32_t* wcnt = lv;  // 16 waves x 256 digits, only in the radix path (lv unused there)
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const uint32_t M = *A.mcount;
+    const uint32_t M = min(*A.mcount, A.n);
     // exclusive scan of blockcnt[0..nblk) in place, total at [nblk]
     if (t == 0) carry_s = 0;
     __syncthreads();
@@ -556,8 +564,10 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
             }
             out = s - mover_prefix(A, s) + lo;
         }
-        A.perm[out] = s;
-        A.skey[out] = kn;
+        if (out < A.n) {
+            A.perm[out] = s;
+            A.skey[out] = kn;
+        }
         return;
     }
     const uint32_t nmb = gridDim.x - A.nblk;
@@ -574,19 +584,23 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
         }
         uint32_t out = r + (lo - mover_prefix(A, lo));
         uint32_t s = A.ms2[r];
-        A.perm[out] = s;
-        A.skey[out] = (uint32_t)(X >> 32);
+        if (out < A.n && s < A.n) {
+            A.perm[out] = s;
+            A.skey[out] = (uint32_t)(X >> 32);
+        }
     }
 }
 
 // Cell ranges of the sorted keys (reorderDataAndFindCellStart_kernel, :166-212).
-__global__ __launch_bounds__(ROWS_T) void k_ranges(const uint32_t* skey, uint2* crange, uint32_t n) {
+__global__ __launch_bounds__(ROWS_T) void k_ranges(const uint32_t* skey, uint2* crange, uint32_t n,
+                                                  uint32_t ncell_cap) {
     const uint32_t j = blockIdx.x * ROWS_T + threadIdx.x;
     if (j >= n) return;
     uint32_t h = skey[j];
+    if (h >= ncell_cap) return;  // defensive: keys are < the cell capacity
     if (j == 0 || skey[j - 1] != h) {
         crange[h].x = j;
-        if (j > 0) crange[skey[j - 1]].y = j;
+        if (j > 0 && skey[j - 1] < ncell_cap) crange[skey[j - 1]].y = j;
     }
     if (j == n - 1) crange[h].y = n;
 }
