@@ -74,6 +74,57 @@ def run_fp64(scene, steps, warmup, lib):
                 phase_launches=dict(zip(PHASES, cnt.tolist())))
 
 
+def run_batched(nscene, steps, lib, block=100, seed=0xD3E3):
+    """Batched box settling (many independent scenes, N_s ~ logU[2k, 200k],
+    one radius per scene), all advanced together on the fused path."""
+    from metakonga_amd import batch
+    t0 = time.perf_counter()
+    scs = scenes.batch_scenes(nscene=nscene, seed=seed, nsteps=steps)
+    gen = time.perf_counter() - t0
+    ntot = sum(s.np for s in scs)
+    cells = sum(int(np.prod(s.grid_size)) for s in scs)
+    b = batch.Batch(ntot, nscene, cells, lib)
+    for s in scs:
+        b.add(s)
+    b.step(block)          # upload + first (full) sort + first block, untimed
+    b.sync()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        b.step(block)
+        done += block
+    b.sync()
+    wall = time.perf_counter() - t0
+    b.close()
+    return dict(scenes=nscene, particles=ntot, steps=done, cells=cells, wall_s=wall,
+                value=ntot * done / wall, generation_s=gen)
+
+
+def run_serving(duration, lib, rate=20.0, block=100, seed=0xD3E4):
+    """Continuous batching of a seeded Poisson trace (drop tests and hopper
+    discharges, 1k-500k spheres, 500-5000 steps); scenes are generated before
+    the trace starts (a request carries its scene)."""
+    from metakonga_amd import batch, serve
+    trace = serve.poisson_trace(seed=seed, rate=rate, duration=duration, block=block)
+    cache = {}
+    for r in trace:
+        cache[r.rid] = r.scene()
+    for r in trace:
+        r.scene = (lambda rr: (lambda: cache[rr.rid]))(r)
+    cap_np = 3_000_000
+    b = batch.Batch(cap_np, 128, 1 << 28, lib)
+    srv = serve.Server(b, max_particles=cap_np, max_scenes=128, max_cells=1 << 28,
+                       hbm_budget=64 << 30, block=block)
+    t0 = time.perf_counter()
+    srv.run(trace, t0=t0)
+    wall = time.perf_counter() - t0
+    rep = srv.report(wall)
+    b.close()
+    rep.update(trace_s=duration, rate_per_s=rate, wall_s=wall,
+               offered_particle_steps_per_s=sum(r.n * r.steps for r in trace) / duration)
+    return rep
+
+
 def cpu_baseline(scene, budget_s=15.0, threads=1):
     """Oracle CPU path (serial, the reference's CPU semantics) on a bounded
     sample of the same workload: all particles, as many steps as fit ~budget."""
@@ -110,6 +161,9 @@ def main():
     ap.add_argument("--np", type=int, default=1_000_000, help="particles of the hopper workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--batched-scenes", type=int, default=256, help="0 to skip the batched run")
+    ap.add_argument("--batched-steps", type=int, default=1000)
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="0 to skip serving")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -157,6 +211,16 @@ def main():
                 "kernel": dom_kernel, "bytes_per_particle": P2P_F64_BYTES,
                 "kernel_ms": p2p_ms}
 
+    batched = None
+    if fused is not None and args.batched_scenes > 0:
+        batched = run_batched(args.batched_scenes, args.batched_steps, lib)
+        log(f"batched: {batched['scenes']} scenes {batched['particles']} particles: "
+            f"{batched['value']:.4g} particle-steps/s")
+    serving = None
+    if fused is not None and args.serve_seconds > 0:
+        serving = run_serving(args.serve_seconds, lib)
+        log(f"serving: {serving}")
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle, CPU-path semantics, 1 thread) ...")
@@ -192,6 +256,10 @@ def main():
     if fused is not None:
         out["fp32c_fused"] = {k: v for k, v in fused.items() if k not in ("roofline",)}
         out["speedup_vs_fp64"] = fused["value"] / f64["value"]
+    if batched is not None:
+        out["batched"] = batched
+    if serving is not None:
+        out["serving"] = serving
     print(json.dumps(out), flush=True)
 
 
