@@ -36,9 +36,10 @@ P2P_F64_BYTES = 188.0
 # Algorithmic bytes per particle-step of the whole FP64 step (reference data model).
 STEP_F64_BYTES = 300.0
 # Algorithmic bytes per particle of the fused contact + Verlet kernel (k_step):
-# own state read 84 (pos 24, v+r 16, omega+m 16, a+alpha 24, id 4) + written
-# 84 + sorted->storage index 4 + next cell key 4 (neighbour re-reads excluded).
-KSTEP_BYTES = 176.0
+# own state record read 88 (pos+r 32, v+m 16, omega+alpha.x 16, a+alpha.y 16,
+# alpha.z+id 8) + written 88 + sorted->storage index 4 + next cell key 4
+# (neighbour re-reads excluded).
+KSTEP_BYTES = 184.0
 # Algorithmic bytes per particle-step of the whole FP32c step.
 STEP_F32C_BYTES = 192.0
 
@@ -164,6 +165,11 @@ def main():
     ap.add_argument("--batched-scenes", type=int, default=256, help="0 to skip the batched run")
     ap.add_argument("--batched-steps", type=int, default=1000)
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="0 to skip serving")
+    ap.add_argument("--skip-fp64", action="store_true", help="skip the plain FP64 leg")
+    ap.add_argument("--no-graphs", action="store_true", help="eager launches (profiler runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run the plain copy kernel on this many MiB, 20 times (profiler "
+                         "byte-counter calibration), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -172,20 +178,34 @@ def main():
     lib = abi.load()
     if lib.xdem_device_count() < 1:
         raise SystemExit("no GPU visible to libxdem.so")
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        src, dst = C.c_void_p(), C.c_void_p()
+        abi.check(lib, lib.xdem_malloc(C.byref(src), nb))
+        abi.check(lib, lib.xdem_malloc(C.byref(dst), nb))
+        abi.check(lib, lib.xdem_memset(src, 1, nb))
+        for _ in range(20):
+            abi.check(lib, lib.xdem_copy_kernel(dst, src, nb))
+        abi.check(lib, lib.xdem_synchronize())
+        print(json.dumps({"copy_calibration_bytes": nb, "launches": 20}))
+        return
     t0 = time.perf_counter()
     scene = scenes.hopper(n=args.np)
     log(f"scene {scene.name} np={scene.np} grid={scene.grid_size} cs={scene.cell_size:.4g} "
         f"built in {time.perf_counter() - t0:.1f}s")
 
-    f64 = run_fp64(scene, args.steps, args.warmup, lib)
-    log(f"fp64: {f64['value']:.4g} particle-steps/s, {f64['ms_per_step']:.3f} ms/step")
-    log("fp64 phases ms/step: " + ", ".join(f"{k}={v:.4f}" for k, v in f64["phase_ms"].items()))
+    f64 = None
+    if not args.skip_fp64:
+        f64 = run_fp64(scene, args.steps, args.warmup, lib)
+        log(f"fp64: {f64['value']:.4g} particle-steps/s, {f64['ms_per_step']:.3f} ms/step")
+        log("fp64 phases ms/step: " + ", ".join(f"{k}={v:.4f}" for k, v in f64["phase_ms"].items()))
 
     fused = None
     try:
         from metakonga_amd import batch
         if batch.available(lib):
-            fused = batch.bench_fused(scene, args.steps, args.warmup, lib)
+            fused = batch.bench_fused(scene, args.steps, args.warmup, lib,
+                                      graphs=not args.no_graphs)
             log(f"fp32c: {fused['value']:.4g} particle-steps/s, {fused['ms_per_step']:.3f} ms/step")
     except ImportError:
         fused = None
@@ -247,15 +267,16 @@ def main():
         "config": {"workload": f"hopper {scene.np} polydisperse spheres, 6 planes, dt 5e-6 s, "
                                f"cell 8 mm, grid {scene.grid_size[0]}^3",
                    "np": scene.np, "path": "fp32c-fused" if fused else "fp64-plain"},
-        "fp64_plain": {"value": f64["value"], "ms_per_step": f64["ms_per_step"],
-                       "phase_ms": f64["phase_ms"],
-                       "step_roofline_frac": STEP_F64_BYTES * f64["value"] / HBM_PEAK},
+        "fp64_plain": None if f64 is None else {
+            "value": f64["value"], "ms_per_step": f64["ms_per_step"], "phase_ms": f64["phase_ms"],
+            "step_roofline_frac": STEP_F64_BYTES * f64["value"] / HBM_PEAK},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     if fused is not None:
         out["fp32c_fused"] = {k: v for k, v in fused.items() if k not in ("roofline",)}
-        out["speedup_vs_fp64"] = fused["value"] / f64["value"]
+        if f64 is not None:
+            out["speedup_vs_fp64"] = fused["value"] / f64["value"]
     if batched is not None:
         out["batched"] = batched
     if serving is not None:

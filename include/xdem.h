@@ -123,6 +123,10 @@ int32_t xdem_memset(void* dptr, int value, size_t bytes);
 int32_t xdem_memcpy_htod(void* dst, const void* src, size_t bytes);
 int32_t xdem_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int32_t xdem_memcpy_dtod(void* dst, const void* src, size_t bytes);
+/* A plain copy kernel of `bytes` (multiple of 16, 16-byte aligned pointers):
+ * the known-size reference for calibrating profiler byte counters.  No
+ * counterpart in the reference. */
+int32_t xdem_copy_kernel(void* dst, const void* src, size_t bytes);
 
 /* ====================================================================
  * Set A — live FP64 API (include/mphysics_cuda_dec.cuh:110-161).

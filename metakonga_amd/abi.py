@@ -177,6 +177,7 @@ def _declare(lib):
         "xdem_memcpy_htod": ([vp, vp, C.c_size_t], i32),
         "xdem_memcpy_dtoh": ([vp, vp, C.c_size_t], i32),
         "xdem_memcpy_dtod": ([vp, vp, C.c_size_t], i32),
+        "xdem_copy_kernel": ([vp, vp, C.c_size_t], i32),
         "xdem_set_symbolic_parameter": ([C.POINTER(DeviceParameters)], i32),
         "xdem_get_symbolic_parameter": ([C.POINTER(DeviceParameters)], i32),
         "xdem_set_stream": ([vp], i32),

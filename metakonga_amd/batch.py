@@ -172,10 +172,11 @@ class Batch:
             pass
 
 
-def bench_fused(scene, steps, warmup, lib, block=100):
+def bench_fused(scene, steps, warmup, lib, block=100, graphs=True):
     """Time `steps` steps of one scene on the fused path (graph-replayed blocks),
     then an eager pass with HIP-event phase timing for the roofline."""
     b = Batch(scene.np, 1, int(np.prod(scene.grid_size)), lib)
+    b.graphs(graphs)
     sid = b.add(scene)
     blk = min(block, max(1, steps))
     b.step(max(1, warmup))

@@ -14,6 +14,7 @@
 // the state between blocks is the reference's end-of-step state.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <vector>
 
@@ -46,20 +47,16 @@ int32_t dalloc(T** p, size_t n) {
 }
 
 int32_t alloc_state(BState& s, size_t n) {
-    XDEM_TRY(dalloc(&s.px, n));
-    XDEM_TRY(dalloc(&s.py, n));
-    XDEM_TRY(dalloc(&s.pz, n));
-    XDEM_TRY(dalloc(&s.vr, n));
-    XDEM_TRY(dalloc(&s.wm, n));
-    XDEM_TRY(dalloc(&s.acc, n));
-    XDEM_TRY(dalloc(&s.alyz, n));
-    XDEM_TRY(dalloc(&s.id, n));
+    XDEM_TRY(dalloc(&s.pr, n));
+    XDEM_TRY(dalloc(&s.vm, n));
+    XDEM_TRY(dalloc(&s.wa, n));
+    XDEM_TRY(dalloc(&s.aa, n));
+    XDEM_TRY(dalloc(&s.zi, n));
     return XDEM_OK;
 }
 
 void free_state(BState& s) {
-    for (void* p : {(void*)s.px, (void*)s.py, (void*)s.pz, (void*)s.vr, (void*)s.wm, (void*)s.acc,
-                    (void*)s.alyz, (void*)s.id})
+    for (void* p : {(void*)s.pr, (void*)s.vm, (void*)s.wa, (void*)s.aa, (void*)s.zi})
         if (p) (void)hipFree(p);
     s = BState{};
 }
@@ -164,14 +161,14 @@ struct xdem_batch {
         return b;
     }
 
-    SortArgs sort_args(const uint32_t* key_old, const uint32_t* key_new, const uint32_t* id,
+    SortArgs sort_args(const uint32_t* key_old, const uint32_t* key_new, const uint2* zi,
                        uint32_t* skey_out) {
         SortArgs a{};
         a.n = n_total;
         a.ncell_cap = (uint32_t)cap_cells;
         a.key_old = key_old;
         a.key_new = key_new;
-        a.id = id;
+        a.zi = zi;
         a.crange = crange;
         a.moverbits = moverbits;
         a.blockcnt = blockcnt;
@@ -268,14 +265,11 @@ int32_t copy_segment(const BState& from, const BState& to, uint32_t src, uint32_
         XDEM_HIP(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, st));
         return XDEM_OK;
     };
-    XDEM_TRY(cp(to.px + dst, from.px + src, 8ull * n));
-    XDEM_TRY(cp(to.py + dst, from.py + src, 8ull * n));
-    XDEM_TRY(cp(to.pz + dst, from.pz + src, 8ull * n));
-    XDEM_TRY(cp(to.vr + dst, from.vr + src, 16ull * n));
-    XDEM_TRY(cp(to.wm + dst, from.wm + src, 16ull * n));
-    XDEM_TRY(cp(to.acc + dst, from.acc + src, 16ull * n));
-    XDEM_TRY(cp(to.alyz + dst, from.alyz + src, 8ull * n));
-    XDEM_TRY(cp(to.id + dst, from.id + src, 4ull * n));
+    XDEM_TRY(cp(to.pr + dst, from.pr + src, 32ull * n));
+    XDEM_TRY(cp(to.vm + dst, from.vm + src, 16ull * n));
+    XDEM_TRY(cp(to.wa + dst, from.wa + src, 16ull * n));
+    XDEM_TRY(cp(to.aa + dst, from.aa + src, 16ull * n));
+    XDEM_TRY(cp(to.zi + dst, from.zi + src, 8ull * n));
     return XDEM_OK;
 }
 
@@ -312,42 +306,33 @@ int32_t apply_layout(xdem_batch* b) {
     }
     // 2. append new scenes (id order) with the sentinel old key
     XDEM_HIP(hipStreamSynchronize(st));
-    std::vector<double> px, py, pz;
-    std::vector<float4> vr, wm, acc;
-    std::vector<float2> alyz;
-    std::vector<uint32_t> ids;
+    std::vector<double4> pr;
+    std::vector<float4> vm, wa, aa;
+    std::vector<uint2> zi;
     for (HostScene* s : fresh) {
         uint32_t n = s->n;
         s->p0 = at;  // storage position of the appended segment
-        px.resize(n);
-        py.resize(n);
-        pz.resize(n);
-        vr.resize(n);
-        wm.resize(n);
-        acc.assign(n, make_float4(0, 0, 0, 0));
-        alyz.assign(n, make_float2(0, 0));
-        ids.resize(n);
+        pr.resize(n);
+        vm.resize(n);
+        wa.resize(n);
+        aa.assign(n, make_float4(0, 0, 0, 0));
+        zi.resize(n);
         float mc = (float)(s->desc.particle.density * (4.0 / 3.0) * M_PI);
         for (uint32_t i = 0; i < n; ++i) {
-            px[i] = s->pos[4 * i + 0];
-            py[i] = s->pos[4 * i + 1];
-            pz[i] = s->pos[4 * i + 2];
             float r = (float)s->pos[4 * i + 3];
+            pr[i] = make_double4(s->pos[4 * i + 0], s->pos[4 * i + 1], s->pos[4 * i + 2], (double)r);
             float m = mc * (r * r * r);
             const double* v = s->vel.empty() ? nullptr : &s->vel[3 * i];
             const double* w = s->omega.empty() ? nullptr : &s->omega[3 * i];
-            vr[i] = make_float4(v ? (float)v[0] : 0.f, v ? (float)v[1] : 0.f, v ? (float)v[2] : 0.f, r);
-            wm[i] = make_float4(w ? (float)w[0] : 0.f, w ? (float)w[1] : 0.f, w ? (float)w[2] : 0.f, m);
-            ids[i] = i;
+            vm[i] = make_float4(v ? (float)v[0] : 0.f, v ? (float)v[1] : 0.f, v ? (float)v[2] : 0.f, m);
+            wa[i] = make_float4(w ? (float)w[0] : 0.f, w ? (float)w[1] : 0.f, w ? (float)w[2] : 0.f, 0.f);
+            zi[i] = make_uint2(0u, i);
         }
-        XDEM_HIP(hipMemcpy(A.px + at, px.data(), 8ull * n, hipMemcpyHostToDevice));
-        XDEM_HIP(hipMemcpy(A.py + at, py.data(), 8ull * n, hipMemcpyHostToDevice));
-        XDEM_HIP(hipMemcpy(A.pz + at, pz.data(), 8ull * n, hipMemcpyHostToDevice));
-        XDEM_HIP(hipMemcpy(A.vr + at, vr.data(), 16ull * n, hipMemcpyHostToDevice));
-        XDEM_HIP(hipMemcpy(A.wm + at, wm.data(), 16ull * n, hipMemcpyHostToDevice));
-        XDEM_HIP(hipMemcpy(A.acc + at, acc.data(), 16ull * n, hipMemcpyHostToDevice));
-        XDEM_HIP(hipMemcpy(A.alyz + at, alyz.data(), 8ull * n, hipMemcpyHostToDevice));
-        XDEM_HIP(hipMemcpy(A.id + at, ids.data(), 4ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.pr + at, pr.data(), 32ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.vm + at, vm.data(), 16ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.wa + at, wa.data(), 16ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.aa + at, aa.data(), 16ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.zi + at, zi.data(), 8ull * n, hipMemcpyHostToDevice));
         XDEM_TRY(fill_u32(b->skey[b->skcur] + at, EMPTY, n, st));
         // the scene's cells start empty
         uint2 e = make_uint2(EMPTY, 0);
@@ -424,7 +409,7 @@ int32_t run_sort(xdem_batch* b) {
     PhaseTimer& T = b->timer;
     const uint32_t* kold = b->skey[b->skcur];
     uint32_t* knew_sorted = b->skey[1 - b->skcur];
-    SortArgs a = b->sort_args(kold, b->keynew, b->buf[b->cur].id, knew_sorted);
+    SortArgs a = b->sort_args(kold, b->keynew, b->buf[b->cur].zi, knew_sorted);
     XDEM_TRY(T.begin(2, st));
     hipLaunchKernelGGL(k_movers, dim3(a.nblk), dim3(ROWS_T), 0, st, a);
     XDEM_LAUNCHED();
@@ -730,46 +715,44 @@ int32_t xdem_batch_get_state(xdem_batch* b, uint32_t scene_id, double* pos4, dou
     XDEM_HIP(hipStreamSynchronize(b->st));
     const BState& A = b->buf[b->cur];
     uint32_t p0 = s->p0;
-    std::vector<double> px(n), py(n), pz(n);
-    std::vector<float4> vr(n), wm(n), ac(n);
-    std::vector<float2> al(n);
-    std::vector<uint32_t> id(n);
-    XDEM_HIP(hipMemcpy(px.data(), A.px + p0, 8ull * n, hipMemcpyDeviceToHost));
-    XDEM_HIP(hipMemcpy(py.data(), A.py + p0, 8ull * n, hipMemcpyDeviceToHost));
-    XDEM_HIP(hipMemcpy(pz.data(), A.pz + p0, 8ull * n, hipMemcpyDeviceToHost));
-    XDEM_HIP(hipMemcpy(vr.data(), A.vr + p0, 16ull * n, hipMemcpyDeviceToHost));
-    XDEM_HIP(hipMemcpy(wm.data(), A.wm + p0, 16ull * n, hipMemcpyDeviceToHost));
-    XDEM_HIP(hipMemcpy(ac.data(), A.acc + p0, 16ull * n, hipMemcpyDeviceToHost));
-    XDEM_HIP(hipMemcpy(al.data(), A.alyz + p0, 8ull * n, hipMemcpyDeviceToHost));
-    XDEM_HIP(hipMemcpy(id.data(), A.id + p0, 4ull * n, hipMemcpyDeviceToHost));
+    std::vector<double4> pr(n);
+    std::vector<float4> vm(n), wa(n), aa(n);
+    std::vector<uint2> zi(n);
+    XDEM_HIP(hipMemcpy(pr.data(), A.pr + p0, 32ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(vm.data(), A.vm + p0, 16ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(wa.data(), A.wa + p0, 16ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(aa.data(), A.aa + p0, 16ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(zi.data(), A.zi + p0, 8ull * n, hipMemcpyDeviceToHost));
     for (uint32_t j = 0; j < n; ++j) {
-        uint32_t i = id[j];
+        uint32_t i = zi[j].y;
         XDEM_REQUIRE(i < n, "corrupt id %u at %u", i, j);
         if (pos4) {
-            pos4[4 * i] = px[j];
-            pos4[4 * i + 1] = py[j];
-            pos4[4 * i + 2] = pz[j];
-            pos4[4 * i + 3] = vr[j].w;
+            pos4[4 * i] = pr[j].x;
+            pos4[4 * i + 1] = pr[j].y;
+            pos4[4 * i + 2] = pr[j].z;
+            pos4[4 * i + 3] = pr[j].w;
         }
         if (vel3) {
-            vel3[3 * i] = vr[j].x;
-            vel3[3 * i + 1] = vr[j].y;
-            vel3[3 * i + 2] = vr[j].z;
+            vel3[3 * i] = vm[j].x;
+            vel3[3 * i + 1] = vm[j].y;
+            vel3[3 * i + 2] = vm[j].z;
         }
         if (omega3) {
-            omega3[3 * i] = wm[j].x;
-            omega3[3 * i + 1] = wm[j].y;
-            omega3[3 * i + 2] = wm[j].z;
+            omega3[3 * i] = wa[j].x;
+            omega3[3 * i + 1] = wa[j].y;
+            omega3[3 * i + 2] = wa[j].z;
         }
         if (acc3) {
-            acc3[3 * i] = ac[j].x;
-            acc3[3 * i + 1] = ac[j].y;
-            acc3[3 * i + 2] = ac[j].z;
+            acc3[3 * i] = aa[j].x;
+            acc3[3 * i + 1] = aa[j].y;
+            acc3[3 * i + 2] = aa[j].z;
         }
         if (alpha3) {
-            alpha3[3 * i] = ac[j].w;
-            alpha3[3 * i + 1] = al[j].x;
-            alpha3[3 * i + 2] = al[j].y;
+            float az;
+            std::memcpy(&az, &zi[j].x, 4);
+            alpha3[3 * i] = wa[j].w;
+            alpha3[3 * i + 1] = aa[j].w;
+            alpha3[3 * i + 2] = az;
         }
     }
     return XDEM_OK;
@@ -810,9 +793,9 @@ __global__ __launch_bounds__(1024) void k_energy(BState st, uint32_t p0, uint32_
     __shared__ double s[4][1024];
     double ke = 0, mx = 0, my = 0, mz = 0;
     for (uint32_t j = threadIdx.x; j < n; j += 1024) {
-        float4 vr = st.vr[p0 + j], wm = st.wm[p0 + j];
-        double m = wm.w, r = vr.w;
-        double vx = vr.x, vy = vr.y, vz = vr.z, wx = wm.x, wy = wm.y, wz = wm.z;
+        float4 vm = st.vm[p0 + j], wa = st.wa[p0 + j];
+        double m = vm.w, r = st.pr[p0 + j].w;
+        double vx = vm.x, vy = vm.y, vz = vm.z, wx = wa.x, wy = wa.y, wz = wa.z;
         double I = 0.4 * m * r * r;
         ke += 0.5 * m * (vx * vx + vy * vy + vz * vz) + 0.5 * I * (wx * wx + wy * wy + wz * wz);
         mx += m * vx;
@@ -838,14 +821,11 @@ __global__ void k_gather(BState in, BState out, const uint32_t* perm, uint32_t n
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     uint32_t s = perm[j];
-    out.px[j] = in.px[s];
-    out.py[j] = in.py[s];
-    out.pz[j] = in.pz[s];
-    out.vr[j] = in.vr[s];
-    out.wm[j] = in.wm[s];
-    out.acc[j] = in.acc[s];
-    out.alyz[j] = in.alyz[s];
-    out.id[j] = in.id[s];
+    out.pr[j] = in.pr[s];
+    out.vm[j] = in.vm[s];
+    out.wa[j] = in.wa[s];
+    out.aa[j] = in.aa[s];
+    out.zi[j] = in.zi[s];
 }
 }  // namespace
 
@@ -929,14 +909,15 @@ int32_t xdem_batch_probe(xdem_batch* b, uint32_t scene_id, double* force3, doubl
     XDEM_HIP(hipStreamSynchronize(b->st));
     uint32_t n = s->n, p0 = s->p0;
     std::vector<float> f(6ull * n);
-    std::vector<uint32_t> cnt(n), part((size_t)n * mp), id(n), sk(n), prm(n);
+    std::vector<uint32_t> cnt(n), part((size_t)n * mp), sk(n);
+    std::vector<uint2> zi(n);
     XDEM_HIP(hipMemcpy(f.data(), b->dbg_f + 6ull * p0, 24ull * n, hipMemcpyDeviceToHost));
     XDEM_HIP(hipMemcpy(cnt.data(), b->dbg_cnt + p0, 4ull * n, hipMemcpyDeviceToHost));
     XDEM_HIP(hipMemcpy(part.data(), b->dbg_part + (size_t)p0 * mp, 4ull * n * mp, hipMemcpyDeviceToHost));
-    XDEM_HIP(hipMemcpy(id.data(), b->buf[b->cur].id + p0, 4ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(zi.data(), b->buf[b->cur].zi + p0, 8ull * n, hipMemcpyDeviceToHost));
     XDEM_HIP(hipMemcpy(sk.data(), b->skey[b->skcur] + p0, 4ull * n, hipMemcpyDeviceToHost));
     for (uint32_t j = 0; j < n; ++j) {
-        uint32_t i = id[j];
+        uint32_t i = zi[j].y;
         XDEM_REQUIRE(i < n, "corrupt id");
         for (int k = 0; k < 3; ++k) {
             if (force3) force3[3 * i + k] = f[6 * j + k];

@@ -44,17 +44,16 @@ struct BPlane {
     float ckn, cvn, sr, mu;
 };
 
-// One state buffer (SoA).  vr = (v, r), wm = (omega, m), acc = (a, alpha.x),
-// alyz = (alpha.y, alpha.z).
+// One state buffer (SoA of packed records).  pr = (x, y, z, r) in FP64 (r is
+// an FP32 value widened), vm = (v, m), wa = (omega, alpha.x), aa = (a, alpha.y),
+// zi = (alpha.z bits, particle id).  88 B per particle; a neighbour candidate
+// needs only its 32-byte pr record.
 struct BState {
-    double* px;
-    double* py;
-    double* pz;
-    float4* vr;
-    float4* wm;
-    float4* acc;
-    float2* alyz;
-    uint32_t* id;
+    double4* pr;
+    float4* vm;
+    float4* wa;
+    float4* aa;
+    uint2* zi;
 };
 
 struct BlockEntry {
@@ -148,20 +147,20 @@ struct Own {
 // (the predicate was already found true).  Accumulates into sumF/sumM.
 __device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own& o, uint32_t t,
                                   f3& Ft, f3& Fn, f3& M, f3& sumF, f3& sumM) {
-    double jx = A.in.px[t], jy = A.in.py[t], jz = A.in.pz[t];
-    float4 jvr = A.in.vr[t];
-    float4 jwm = A.in.wm[t];
-    double dx = jx - o.x, dy = jy - o.y, dz = jz - o.z;
+    const double4 jp = A.in.pr[t];
+    const float4 jvm = A.in.vm[t];
+    const float4 jwa = A.in.wa[t];
+    double dx = jp.x - o.x, dy = jp.y - o.y, dz = jp.z - o.z;
     double d2 = dx * dx + dy * dy + dz * dz;
     double dist = sqrt(d2);
-    double cdist = ((double)o.r + (double)jvr.w) - dist;
+    double cdist = ((double)o.r + jp.w) - dist;
     float inv = (float)(1.0 / dist);
     f3 u = F3((float)dx * inv, (float)dy * inv, (float)dz * inv);
-    float jr = jvr.w;
-    f3 jv = F3(jvr.x, jvr.y, jvr.z), jw = F3(jwm.x, jwm.y, jwm.z);
+    float jr = (float)jp.w;
+    f3 jv = F3(jvm.x, jvm.y, jvm.z), jw = F3(jwa.x, jwa.y, jwa.z);
     f3 dv = S.rolling ? (jv + crossf(jw, (-jr) * u)) - (o.v + crossf(o.w, o.r * u)) : (jv - o.v);
     float R = (o.r * jr) / (o.r + jr);
-    float Ms = (o.m * jwm.w) / (o.m + jwm.w);
+    float Ms = (o.m * jvm.w) / (o.m + jvm.w);
     float kn = S.ckn * sqrtf(R);
     float vn = sqrtf(S.cvn * Ms * kn);
     dhs_f32(kn, vn, kn * S.sr, vn * S.sr, S.mu, S.dtf, o.r, (float)cdist, dv, u, Ft, Fn, M);
@@ -181,39 +180,52 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
     const uint32_t s = A.perm[j];
     if (s >= A.n) return;  // defensive: perm is a permutation of [0, n)
     Own o;
-    o.x = A.in.px[s];
-    o.y = A.in.py[s];
-    o.z = A.in.pz[s];
-    float4 vr = A.in.vr[s], wm = A.in.wm[s], ac = A.in.acc[s];
-    float2 al2 = A.in.alyz[s];
-    uint32_t pid = A.in.id[s];
-    o.v = F3(vr.x, vr.y, vr.z);
-    o.r = vr.w;
-    o.w = F3(wm.x, wm.y, wm.z);
-    o.m = wm.w;
+    const double4 ip = A.in.pr[s];
+    const float4 vm = A.in.vm[s], wa = A.in.wa[s], aa = A.in.aa[s];
+    const uint2 zi = A.in.zi[s];
+    o.x = ip.x;
+    o.y = ip.y;
+    o.z = ip.z;
+    o.v = F3(vm.x, vm.y, vm.z);
+    o.r = (float)ip.w;
+    o.w = F3(wa.x, wa.y, wa.z);
+    o.m = vm.w;
     int cx, cy, cz;
     cell_of(S, o.x, o.y, o.z, cx, cy, cz);
     f3 Ft = F3(0, 0, 0), Fn = F3(0, 0, 0), M = F3(0, 0, 0);
     f3 sumF = F3(0, 0, 0), sumM = F3(0, 0, 0);
     uint32_t nlist = 0, ntotal = 0;
     const double ri = (double)o.r;
-    // phase 1: candidates in the reference order (z, y, x cells; ascending j)
+    // phase 1: candidates in the reference order (z, y, x cells; ascending j).
+    // Away from the x wrap the three cells of an x-row have consecutive keys,
+    // so their particles are one contiguous run of sorted positions.
+    const bool xrow = cx >= 1 && cx + 1 < (int)S.gx;
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
-                uint2 cr = A.crange[key_of(S, cx + dx, cy + dy, cz + dz)];
+                uint2 cr;
+                if (xrow) {
+                    if (dx != -1) break;
+                    const uint32_t k0 = key_of(S, cx - 1, cy + dy, cz + dz);
+                    const uint2 c0 = A.crange[k0], c1 = A.crange[k0 + 1], c2 = A.crange[k0 + 2];
+                    cr.x = c0.x != EMPTY ? c0.x : (c1.x != EMPTY ? c1.x : c2.x);
+                    cr.y = c2.x != EMPTY ? c2.y : (c1.x != EMPTY ? c1.y : c0.y);
+                } else {
+                    cr = A.crange[key_of(S, cx + dx, cy + dy, cz + dz)];
+                }
                 if (cr.x == EMPTY) continue;
                 const uint32_t e1 = min(cr.y, A.n);
                 for (uint32_t jj = cr.x; jj < e1; ++jj) {
                     if (jj == j) continue;
                     uint32_t t = A.perm[jj];
                     if (t >= A.n) continue;
-                    double ex = A.in.px[t] - o.x, ey = A.in.py[t] - o.y, ez = A.in.pz[t] - o.z;
+                    const double4 jp = A.in.pr[t];
+                    double ex = jp.x - o.x, ey = jp.y - o.y, ez = jp.z - o.z;
                     double d2 = ex * ex + ey * ey + ez * ez;
-                    double Sr = ri + (double)A.in.vr[t].w;
+                    double Sr = ri + jp.w;
                     if (d2 > Sr * Sr * 1.000001) continue;
                     if (!(Sr - sqrt(d2) > 0)) continue;
-                    if (DBG && ntotal < A.dbg_max) A.dbg_part[(size_t)j * A.dbg_max + ntotal] = A.in.id[t];
+                    if (DBG && ntotal < A.dbg_max) A.dbg_part[(size_t)j * A.dbg_max + ntotal] = A.in.zi[t].y;
                     ++ntotal;
                     mylist[nlist++] = t;
                     if (nlist == CMAX) {
@@ -231,11 +243,11 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
     // planes, in order (plane_contact_force_kernel: no torque, ir = 0)
     for (uint32_t p = 0; p < S.nplane; ++p) {
         const BPlane& bp = A.planes[S.plane0 + p];
-        d3 ip = mk3(o.x, o.y, o.z);
-        d3 dp = ip - of(bp.pl.xw);
+        d3 xp = mk3(o.x, o.y, o.z);
+        d3 dp = xp - of(bp.pl.xw);
         d3 unit = mk3(0, 0, 0);
         d3 wp = mk3(dot(dp, of(bp.pl.u1)), dot(dp, of(bp.pl.u2)), dot(dp, of(bp.pl.uw)));
-        double cdist = plane_detect(bp.pl, ip, wp, unit, ri);
+        double cdist = plane_detect(bp.pl, xp, wp, unit, ri);
         if (cdist > 0) {
             f3 u = F3((float)unit.x, (float)unit.y, (float)unit.z);
             f3 dv = S.rolling ? -(o.v + crossf(o.w, o.r * u)) : -o.v;
@@ -260,7 +272,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
     // vv_update_velocity_kernel (mphysics_cuda_impl.cuh:98-134) in FP32
     float inv_m = 1.0f / o.m;
     float inv_i = 1.0f / (0.4f * o.m * o.r * o.r);
-    f3 a0 = F3(ac.x, ac.y, ac.z), al0 = F3(ac.w, al2.x, al2.y);
+    f3 a0 = F3(aa.x, aa.y, aa.z), al0 = F3(wa.w, aa.w, __uint_as_float(zi.x));
     f3 a = inv_m * F, al = inv_i * T;
     float hdt = 0.5f * S.dtf;
     f3 v = o.v + hdt * (a0 + a);
@@ -275,14 +287,11 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
         cell_of(S, nx, ny, nz, kx, ky, kz);
         A.keynew[j] = key_of(S, kx, ky, kz);
     }
-    A.out.px[j] = nx;
-    A.out.py[j] = ny;
-    A.out.pz[j] = nz;
-    A.out.vr[j] = make_float4(v.x, v.y, v.z, o.r);
-    A.out.wm[j] = make_float4(w.x, w.y, w.z, o.m);
-    A.out.acc[j] = make_float4(a.x, a.y, a.z, al.x);
-    A.out.alyz[j] = make_float2(al.y, al.z);
-    A.out.id[j] = pid;
+    A.out.pr[j] = make_double4(nx, ny, nz, ip.w);
+    A.out.vm[j] = make_float4(v.x, v.y, v.z, o.m);
+    A.out.wa[j] = make_float4(w.x, w.y, w.z, al.x);
+    A.out.aa[j] = make_float4(a.x, a.y, a.z, al.y);
+    A.out.zi[j] = make_uint2(__float_as_uint(al.z), zi.y);
 }
 
 // Verlet position update (optional) + cell key of every particle, in place,
@@ -295,15 +304,14 @@ __global__ __launch_bounds__(256) void k_pos(BState st, uint32_t* keynew, const 
     const BScene S = scenes[be.scene];
     const uint32_t j = be.j0 + threadIdx.x;
     if (j >= be.j1) return;
-    double x = st.px[j], y = st.py[j], z = st.pz[j];
+    double4 p = st.pr[j];
+    double x = p.x, y = p.y, z = p.z;
     if (ADV) {
-        float4 vr = st.vr[j], ac = st.acc[j];
-        x += S.dt * (double)vr.x + S.half2dt * (double)ac.x;
-        y += S.dt * (double)vr.y + S.half2dt * (double)ac.y;
-        z += S.dt * (double)vr.z + S.half2dt * (double)ac.z;
-        st.px[j] = x;
-        st.py[j] = y;
-        st.pz[j] = z;
+        float4 vm = st.vm[j], aa = st.aa[j];
+        x += S.dt * (double)vm.x + S.half2dt * (double)aa.x;
+        y += S.dt * (double)vm.y + S.half2dt * (double)aa.y;
+        z += S.dt * (double)vm.z + S.half2dt * (double)aa.z;
+        st.pr[j] = make_double4(x, y, z, p.w);
     }
     int cx, cy, cz;
     cell_of(S, x, y, z, cx, cy, cz);
@@ -316,7 +324,7 @@ struct SortArgs {
     uint32_t ncell_cap;         // size of crange
     const uint32_t* key_old;    // sorted keys of the previous sort, storage order
     const uint32_t* key_new;    // keys of the current positions, storage order
-    const uint32_t* id;         // particle ids, storage order
+    const uint2* zi;            // (alpha.z bits, particle id), storage order
     uint2* crange;
     uint64_t* moverbits;        // one bit per storage position
     uint32_t* blockcnt;         // movers per ROWS_T block; scanned in place (+1 total)
@@ -357,7 +365,7 @@ __global__ __launch_bounds__(ROWS_T) void k_movers(SortArgs A) {
     if (mover) {
         uint32_t p = base + __popcll(word & lanemask_lt64());
         if (p < A.n) {
-            A.mkey[p] = ((uint64_t)nw << 32) | A.id[s];
+            A.mkey[p] = ((uint64_t)nw << 32) | A.zi[s].y;
             A.ms[p] = s;
         }
     }
@@ -553,7 +561,7 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
         if (M && kn != A.key_old[s]) return;  // a mover
         uint32_t out = s;
         if (M) {
-            uint64_t X = ((uint64_t)kn << 32) | A.id[s];
+            uint64_t X = ((uint64_t)kn << 32) | A.zi[s].y;
             uint32_t lo = 0, hi = M;  // count of sorted movers < X
             while (lo < hi) {
                 uint32_t mid = (lo + hi) >> 1;
@@ -576,7 +584,7 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
         uint32_t lo = 0, hi = A.n;  // storage entries with (key_old, id) < X
         while (lo < hi) {
             uint32_t mid = (lo + hi) >> 1;
-            uint64_t Y = ((uint64_t)A.key_old[mid] << 32) | A.id[mid];
+            uint64_t Y = ((uint64_t)A.key_old[mid] << 32) | A.zi[mid].y;
             if (Y < X)
                 lo = mid + 1;
             else

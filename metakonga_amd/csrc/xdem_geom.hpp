@@ -7,6 +7,11 @@ namespace xdem {
 // particle_plane_contact_detection — mphysics_cuda_impl.cuh:487-570 (branch
 // order kept, including the unreachable seventh branch of :549).
 __device__ inline double plane_detect(const xdem_device_plane_info& pe, d3 xp, d3 wp, d3& u, double r) {
+    // Every branch below needs |wp.z| < r (the face test) or a sum of
+    // non-negative squares that includes wp.z^2 to stay below r^2; when both
+    // fail no branch can be taken, so leave before the pow calls.  Exact: the
+    // rounded sums are >= their last term.
+    if (!(fabs(wp.z) < r) && !(wp.z * wp.z < r * r)) return -1.0;
     double a_l1 = pow(wp.x - pe.l1, 2.0);
     double b_l2 = pow(wp.y - pe.l2, 2.0);
     double sqa = wp.x * wp.x;

@@ -1,5 +1,6 @@
 // xdem_runtime.hip — library state: last error, stream, symbolic parameters,
 // workspace, and the memory helpers of the C ABI.
+#include <algorithm>
 #include <cstdarg>
 #include <mutex>
 
@@ -210,6 +211,28 @@ int32_t xdem_get_symbolic_parameter(xdem_device_parameters* out) {
         return XDEM_ERR_STATE;
     }
     *out = g_params;
+    return XDEM_OK;
+}
+
+// Plain device copy kernel (16-byte vector loads/stores) of a known size, used
+// to calibrate the HBM byte counters of a profiler against real traffic.
+}  // extern "C"
+namespace {
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t st = (size_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += st) dst[i] = src[i];
+}
+}  // namespace
+extern "C" {
+int32_t xdem_copy_kernel(void* dst, const void* src, size_t bytes) {
+    XDEM_REQUIRE(dst && src && bytes % 16 == 0, "copy: 16-byte multiple required");
+    XDEM_REQUIRE(((uintptr_t)dst & 15) == 0 && ((uintptr_t)src & 15) == 0, "copy: alignment");
+    size_t n16 = bytes / 16;
+    uint32_t blocks = (uint32_t)std::min<size_t>((n16 + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy16, dim3(std::max<uint32_t>(blocks, 1)), dim3(256), 0, g_stream,
+                       (const uint4*)src, (uint4*)dst, n16);
+    XDEM_LAUNCHED();
     return XDEM_OK;
 }
 
