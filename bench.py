@@ -88,6 +88,8 @@ def run_batched(nscene, steps, lib, block=100, seed=0xD3E3):
     for s in scs:
         b.add(s)
     b.step(block)          # upload + first (full) sort + first block, untimed
+    tuned = b.autotune(steps_per_trial=5)
+    b.step(block)
     b.sync()
     t0 = time.perf_counter()
     done = 0
@@ -98,7 +100,8 @@ def run_batched(nscene, steps, lib, block=100, seed=0xD3E3):
     wall = time.perf_counter() - t0
     b.close()
     return dict(scenes=nscene, particles=ntot, steps=done, cells=cells, wall_s=wall,
-                value=ntot * done / wall, generation_s=gen)
+                value=ntot * done / wall, generation_s=gen,
+                autotune={"chosen": tuned[0], "kernel_ms": tuned[1]})
 
 
 def run_serving(duration, lib, rate=20.0, block=100, seed=0xD3E4):
@@ -167,6 +170,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="0 to skip serving")
     ap.add_argument("--skip-fp64", action="store_true", help="skip the plain FP64 leg")
     ap.add_argument("--no-graphs", action="store_true", help="eager launches (profiler runs)")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shape")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB, 20 times (profiler "
                          "byte-counter calibration), then exit")
@@ -205,7 +209,7 @@ def main():
         from metakonga_amd import batch
         if batch.available(lib):
             fused = batch.bench_fused(scene, args.steps, args.warmup, lib,
-                                      graphs=not args.no_graphs)
+                                      graphs=not args.no_graphs, autotune=not args.no_autotune)
             log(f"fp32c: {fused['value']:.4g} particle-steps/s, {fused['ms_per_step']:.3f} ms/step")
     except ImportError:
         fused = None

@@ -56,14 +56,29 @@ int32_t xdem_batch_destroy(xdem_batch* b);
  * XDEM_ERR_CAPACITY when particles, scenes or cells would exceed the capacity. */
 int32_t xdem_batch_add_scene(xdem_batch* b, const xdem_batch_scene_desc* d, const double* pos4,
                              const double* vel3, const double* omega3, uint32_t* scene_id);
+/* As xdem_batch_add_scene, also restoring the accelerations (acc3, alpha3 may
+ * be NULL).  With the arrays read back by xdem_batch_get_state this resumes a
+ * scene bit-exactly (checkpoint/resume; the reference's resume drops the
+ * accelerations, source/simulation/dem_simulation.cpp:585-593). */
+int32_t xdem_batch_add_scene_state(xdem_batch* b, const xdem_batch_scene_desc* d,
+                                   const double* pos4, const double* vel3, const double* omega3,
+                                   const double* acc3, const double* alpha3, uint32_t* scene_id);
 /* Remove a scene (its slot and memory are reused).  Takes effect at the next step. */
 int32_t xdem_batch_remove_scene(xdem_batch* b, uint32_t scene_id);
 /* Advance every active scene by nsteps (one step block).  When graphs are on
  * (default), the block is captured once per (layout, nsteps) and replayed. */
 int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps);
 int32_t xdem_batch_set_graphs(xdem_batch* b, int32_t on);
-/* Block size (64..1024, multiple of 64) of the contact kernel; 0 = default. */
+/* Block size (64, 128, 192 or 256) of the contact kernel; 0 = default (256). */
 int32_t xdem_batch_set_block(xdem_batch* b, uint32_t threads);
+/* Autotuner: times the contact kernel for block sizes 64, 128 and 256, each
+ * with an unconstrained register budget and with an 8-waves-per-SIMD budget,
+ * over steps_per_trial real steps (the scenes advance 6 * (steps_per_trial + 1)
+ * steps), and keeps the fastest.  *chosen = block size (+1000 for the 8-wave
+ * variant); ms_per_candidate (6 entries, may be NULL) receives the measured
+ * kernel time per launch in the order 64/128/256 then 64/128/256 (8 waves). */
+int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* chosen,
+                            double* ms_per_candidate);
 int32_t xdem_batch_synchronize(xdem_batch* b);
 /* State of one scene in its original particle order (host arrays, reference
  * layout; any pointer may be NULL).  Positions are FP64; the others are the

@@ -28,11 +28,14 @@ def declare(lib):
         "xdem_batch_create": ([u32, u32, u64, C.POINTER(vp)], i32),
         "xdem_batch_destroy": ([vp], i32),
         "xdem_batch_add_scene": ([vp, C.POINTER(BatchSceneDesc), vp, vp, vp, C.POINTER(u32)], i32),
+        "xdem_batch_add_scene_state": ([vp, C.POINTER(BatchSceneDesc), vp, vp, vp, vp, vp,
+                                        C.POINTER(u32)], i32),
         "xdem_batch_remove_scene": ([vp, u32], i32),
         "xdem_batch_step": ([vp, u32], i32),
         "xdem_batch_set_graphs": ([vp, i32], i32),
         "xdem_batch_set_block": ([vp, u32], i32),
         "xdem_batch_synchronize": ([vp], i32),
+        "xdem_batch_autotune": ([vp, u32, C.POINTER(u32), vp], i32),
         "xdem_batch_get_state": ([vp, u32, vp, vp, vp, vp, vp], i32),
         "xdem_batch_num_particles": ([vp], u32),
         "xdem_batch_num_scenes": ([vp], u32),
@@ -89,14 +92,15 @@ class Batch:
         self.h = h
         self.sizes = {}
 
-    def add(self, scene, vel=None, omega=None):
+    def add(self, scene, vel=None, omega=None, acc=None, alpha=None, pos=None):
+        """Add a scene; pos/vel/omega/acc/alpha (optional) override the scene's
+        initial state (a checkpoint read back with state() resumes bit-exactly)."""
         d, keep = scene_desc(scene)
         sid = C.c_uint32()
-        pos = abi.as_f64(scene.pos)
-        v = None if vel is None else abi.as_f64(vel)
-        w = None if omega is None else abi.as_f64(omega)
-        abi.check(self.lib, self.lib.xdem_batch_add_scene(self.h, C.byref(d), abi.ptr(pos),
-                                                          abi.ptr(v), abi.ptr(w), C.byref(sid)))
+        pos = abi.as_f64(scene.pos if pos is None else pos)
+        arr = [None if x is None else abi.as_f64(x) for x in (vel, omega, acc, alpha)]
+        abi.check(self.lib, self.lib.xdem_batch_add_scene_state(
+            self.h, C.byref(d), abi.ptr(pos), *(abi.ptr(x) for x in arr), C.byref(sid)))
         del keep
         self.sizes[sid.value] = scene.np
         return sid.value
@@ -116,6 +120,15 @@ class Batch:
 
     def block(self, threads: int):
         abi.check(self.lib, self.lib.xdem_batch_set_block(self.h, int(threads)))
+
+    def autotune(self, steps_per_trial=10):
+        """Pick the contact-kernel block size by timing (advances the scenes)."""
+        chosen = C.c_uint32()
+        ms = np.zeros(6)
+        abi.check(self.lib, self.lib.xdem_batch_autotune(self.h, steps_per_trial, C.byref(chosen),
+                                                         abi.ptr(ms)))
+        keys = [f"block{b}" for b in (64, 128, 256)] + [f"block{b}-8waves" for b in (64, 128, 256)]
+        return chosen.value, dict(zip(keys, ms.tolist()))
 
     def state(self, sid):
         n = self.sizes[sid]
@@ -172,7 +185,7 @@ class Batch:
             pass
 
 
-def bench_fused(scene, steps, warmup, lib, block=100, graphs=True):
+def bench_fused(scene, steps, warmup, lib, block=100, graphs=True, autotune=True):
     """Time `steps` steps of one scene on the fused path (graph-replayed blocks),
     then an eager pass with HIP-event phase timing for the roofline."""
     b = Batch(scene.np, 1, int(np.prod(scene.grid_size)), lib)
@@ -180,6 +193,8 @@ def bench_fused(scene, steps, warmup, lib, block=100, graphs=True):
     sid = b.add(scene)
     blk = min(block, max(1, steps))
     b.step(max(1, warmup))
+    tuned = b.autotune(steps_per_trial=5) if autotune else None
+    b.step(blk)  # settle the chosen launch shape (and capture its graph)
     b.sync()
     t0 = time.perf_counter()
     done = 0
@@ -201,7 +216,8 @@ def bench_fused(scene, steps, warmup, lib, block=100, graphs=True):
     b.close()
     per = dict(zip(PHASES, (ms / nt).tolist()))
     return dict(wall_s=wall, ms_per_step=wall * 1e3 / steps, value=scene.np * steps / wall,
-                phase_ms=per, movers_last=movers, timed_steps_events=nt)
+                phase_ms=per, movers_last=movers, timed_steps_events=nt,
+                autotune=None if tuned is None else {"chosen": tuned[0], "kernel_ms": tuned[1]})
 
 
 def smoke_check(scene, lib, w_pos=None):

@@ -35,7 +35,7 @@ struct HostScene {
     uint32_t cell_base = 0, ncell = 0;
     uint32_t p0 = 0;       // sorted segment
     bool pending = true;   // not yet uploaded
-    std::vector<double> pos, vel, omega;  // staging until upload
+    std::vector<double> pos, vel, omega, acc, alpha;  // staging until upload
 };
 
 template <class T>
@@ -112,6 +112,7 @@ struct xdem_batch {
     bool any_stepped = false;
     bool storage_sorted_all_new = false;
     uint32_t block_threads = 256;
+    int occ8 = 0;                     // k_step variant: 0 = unconstrained registers, 1 = 8 waves/SIMD
     bool use_graphs = true;
     std::map<std::pair<uint32_t, int>, hipGraphExec_t> graphs;
     PhaseTimer timer;
@@ -324,9 +325,17 @@ int32_t apply_layout(xdem_batch* b) {
             float m = mc * (r * r * r);
             const double* v = s->vel.empty() ? nullptr : &s->vel[3 * i];
             const double* w = s->omega.empty() ? nullptr : &s->omega[3 * i];
+            const double* a = s->acc.empty() ? nullptr : &s->acc[3 * i];
+            const double* al = s->alpha.empty() ? nullptr : &s->alpha[3 * i];
+            float alz = al ? (float)al[2] : 0.f;
+            uint32_t alz_bits;
+            std::memcpy(&alz_bits, &alz, 4);
             vm[i] = make_float4(v ? (float)v[0] : 0.f, v ? (float)v[1] : 0.f, v ? (float)v[2] : 0.f, m);
-            wa[i] = make_float4(w ? (float)w[0] : 0.f, w ? (float)w[1] : 0.f, w ? (float)w[2] : 0.f, 0.f);
-            zi[i] = make_uint2(0u, i);
+            wa[i] = make_float4(w ? (float)w[0] : 0.f, w ? (float)w[1] : 0.f, w ? (float)w[2] : 0.f,
+                                al ? (float)al[0] : 0.f);
+            if (a) aa[i] = make_float4((float)a[0], (float)a[1], (float)a[2], al ? (float)al[1] : 0.f);
+            else if (al) aa[i] = make_float4(0.f, 0.f, 0.f, (float)al[1]);
+            zi[i] = make_uint2(alz_bits, i);
         }
         XDEM_HIP(hipMemcpy(A.pr + at, pr.data(), 32ull * n, hipMemcpyHostToDevice));
         XDEM_HIP(hipMemcpy(A.vm + at, vm.data(), 16ull * n, hipMemcpyHostToDevice));
@@ -343,6 +352,8 @@ int32_t apply_layout(xdem_batch* b) {
         std::vector<double>().swap(s->pos);
         std::vector<double>().swap(s->vel);
         std::vector<double>().swap(s->omega);
+        std::vector<double>().swap(s->acc);
+        std::vector<double>().swap(s->alpha);
         at += n;
     }
     b->storage_sorted_all_new = (n_old == 0);
@@ -474,12 +485,15 @@ int32_t launch_step(xdem_batch* b, bool adv) {
     if (!b->nblk_sorted) return XDEM_OK;
     StepArgs a = step_args(b, false);
     XDEM_TRY(b->timer.begin(1, b->st));
-    if (adv)
-        hipLaunchKernelGGL((k_step<true, false>), dim3(b->nblk_sorted), dim3(b->block_threads), 0,
-                           b->st, a);
+    dim3 g(b->nblk_sorted), t(b->block_threads);
+    if (adv && b->occ8)
+        hipLaunchKernelGGL((k_step<true, false, 8>), g, t, 0, b->st, a);
+    else if (adv)
+        hipLaunchKernelGGL((k_step<true, false, 1>), g, t, 0, b->st, a);
+    else if (b->occ8)
+        hipLaunchKernelGGL((k_step<false, false, 8>), g, t, 0, b->st, a);
     else
-        hipLaunchKernelGGL((k_step<false, false>), dim3(b->nblk_sorted), dim3(b->block_threads), 0,
-                           b->st, a);
+        hipLaunchKernelGGL((k_step<false, false, 1>), g, t, 0, b->st, a);
     XDEM_LAUNCHED();
     XDEM_TRY(b->timer.end(b->st));
     b->cur = 1 - b->cur;
@@ -548,6 +562,12 @@ int32_t xdem_batch_destroy(xdem_batch* b) {
 
 int32_t xdem_batch_add_scene(xdem_batch* b, const xdem_batch_scene_desc* d, const double* pos4,
                              const double* vel3, const double* omega3, uint32_t* scene_id) {
+    return xdem_batch_add_scene_state(b, d, pos4, vel3, omega3, nullptr, nullptr, scene_id);
+}
+
+int32_t xdem_batch_add_scene_state(xdem_batch* b, const xdem_batch_scene_desc* d,
+                                   const double* pos4, const double* vel3, const double* omega3,
+                                   const double* acc3, const double* alpha3, uint32_t* scene_id) {
     XDEM_REQUIRE(b && d && (d->np == 0 || pos4), "null argument");
     XDEM_REQUIRE(d->np > 0, "a scene needs particles");
     XDEM_REQUIRE(is_pow2(d->grid_size.x) && is_pow2(d->grid_size.y) && is_pow2(d->grid_size.z),
@@ -580,6 +600,8 @@ int32_t xdem_batch_add_scene(xdem_batch* b, const xdem_batch_scene_desc* d, cons
     h.pos.assign(pos4, pos4 + 4 * (size_t)d->np);
     if (vel3) h.vel.assign(vel3, vel3 + 3 * (size_t)d->np);
     if (omega3) h.omega.assign(omega3, omega3 + 3 * (size_t)d->np);
+    if (acc3) h.acc.assign(acc3, acc3 + 3 * (size_t)d->np);
+    if (alpha3) h.alpha.assign(alpha3, alpha3 + 3 * (size_t)d->np);
     b->scenes.push_back(std::move(h));
     b->layout_dirty = true;
     b->prepared = false;
@@ -671,6 +693,48 @@ int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps) {
     return XDEM_OK;
 }
 
+// Autotuner: times the contact kernel (HIP events) for each candidate launch
+// shape — block size 64/128/256 x register budget (unconstrained / 8 waves per
+// SIMD) — over `steps_per_trial` real steps (the scenes advance) and keeps the
+// fastest.  Returns the choice (block size, +1000 for the 8-wave variant) and
+// the per-candidate kernel time per launch (6 entries).
+int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* chosen,
+                            double* ms_per_candidate) {
+    XDEM_REQUIRE(b, "null batch");
+    const uint32_t cand[3] = {64, 128, 256};
+    if (steps_per_trial == 0) steps_per_trial = 10;
+    bool was_on = b->timer.on;
+    double best = 1e300;
+    uint32_t best_c = b->block_threads;
+    int best_o = b->occ8;
+    for (int o = 0; o < 2; ++o)
+        for (int i = 0; i < 3; ++i) {
+            b->occ8 = o;
+            b->drop_graphs();
+            XDEM_TRY(xdem_batch_set_block(b, cand[i]));
+            XDEM_TRY(xdem_batch_step(b, 1));  // applies the new tables
+            XDEM_TRY(xdem_batch_enable_timing(b, 1));
+            XDEM_TRY(xdem_batch_step(b, steps_per_trial));
+            double ms[6];
+            uint32_t cnt[6];
+            XDEM_TRY(xdem_batch_phase_times(b, ms, cnt));
+            double per = cnt[1] ? ms[1] / cnt[1] : 1e300;
+            if (ms_per_candidate) ms_per_candidate[3 * o + i] = per;
+            if (per < best) {
+                best = per;
+                best_c = cand[i];
+                best_o = o;
+            }
+            XDEM_TRY(xdem_batch_enable_timing(b, 0));
+        }
+    b->occ8 = best_o;
+    b->drop_graphs();
+    XDEM_TRY(xdem_batch_set_block(b, best_c));
+    if (was_on) XDEM_TRY(xdem_batch_enable_timing(b, 1));
+    if (chosen) *chosen = best_c + (best_o ? 1000u : 0u);
+    return XDEM_OK;
+}
+
 int32_t xdem_batch_synchronize(xdem_batch* b) {
     XDEM_REQUIRE(b, "null batch");
     XDEM_HIP(hipStreamSynchronize(b->st));
@@ -706,8 +770,8 @@ int32_t xdem_batch_get_state(xdem_batch* b, uint32_t scene_id, double* pos4, dou
             for (int k = 0; k < 3; ++k) {
                 if (vel3) vel3[3 * i + k] = s->vel.empty() ? 0.0 : (double)(float)s->vel[3 * i + k];
                 if (omega3) omega3[3 * i + k] = s->omega.empty() ? 0.0 : (double)(float)s->omega[3 * i + k];
-                if (acc3) acc3[3 * i + k] = 0.0;
-                if (alpha3) alpha3[3 * i + k] = 0.0;
+                if (acc3) acc3[3 * i + k] = s->acc.empty() ? 0.0 : (double)(float)s->acc[3 * i + k];
+                if (alpha3) alpha3[3 * i + k] = s->alpha.empty() ? 0.0 : (double)(float)s->alpha[3 * i + k];
             }
         }
         return XDEM_OK;
@@ -903,8 +967,8 @@ int32_t xdem_batch_probe(xdem_batch* b, uint32_t scene_id, double* force3, doubl
         b->dbg_cap = mp;
     }
     StepArgs a = step_args(b, true);  // no advance: keynew is not written
-    hipLaunchKernelGGL((k_step<false, true>), dim3(b->nblk_sorted), dim3(b->block_threads), 0, b->st,
-                       a);
+    hipLaunchKernelGGL((k_step<false, true, 1>), dim3(b->nblk_sorted), dim3(b->block_threads), 0,
+                       b->st, a);
     XDEM_LAUNCHED();
     XDEM_HIP(hipStreamSynchronize(b->st));
     uint32_t n = s->n, p0 = s->p0;

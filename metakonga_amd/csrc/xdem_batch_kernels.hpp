@@ -154,7 +154,7 @@ __device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own&
     double d2 = dx * dx + dy * dy + dz * dz;
     double dist = sqrt(d2);
     double cdist = ((double)o.r + jp.w) - dist;
-    float inv = (float)(1.0 / dist);
+    float inv = 1.0f / (float)dist;
     f3 u = F3((float)dx * inv, (float)dy * inv, (float)dz * inv);
     float jr = (float)jp.w;
     f3 jv = F3(jvm.x, jvm.y, jvm.z), jw = F3(jwa.x, jwa.y, jwa.z);
@@ -168,8 +168,11 @@ __device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own&
     sumM += M;
 }
 
-template <bool ADV, bool DBG>
-__global__ __launch_bounds__(256) void k_step(StepArgs A) {
+// OCC: minimum waves per SIMD the register allocation must allow (1 = no
+// constraint, 5 waves at 87 VGPRs; 8 = 64 VGPRs with spills in the cold
+// contact/plane code).  The autotuner picks one.
+template <bool ADV, bool DBG, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_step(StepArgs A) {
     __shared__ uint32_t clist[256 * CMAX];
     const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
     const BlockEntry be = A.blocks[L];
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
         d3 dp = xp - of(bp.pl.xw);
         d3 unit = mk3(0, 0, 0);
         d3 wp = mk3(dot(dp, of(bp.pl.u1)), dot(dp, of(bp.pl.u2)), dot(dp, of(bp.pl.uw)));
-        double cdist = plane_detect(bp.pl, xp, wp, unit, ri);
+        double cdist = plane_detect<false>(bp.pl, xp, wp, unit, ri);
         if (cdist > 0) {
             f3 u = F3((float)unit.x, (float)unit.y, (float)unit.z);
             f3 dv = S.rolling ? -(o.v + crossf(o.w, o.r * u)) : -o.v;

@@ -6,14 +6,19 @@ namespace xdem {
 
 // particle_plane_contact_detection — mphysics_cuda_impl.cuh:487-570 (branch
 // order kept, including the unreachable seventh branch of :549).
+// EXACT_POW: the reference's pow(x, 2.0) (plain path, for expression-level
+// parity); otherwise x * x (fused path: identical up to the last bit of an
+// edge/corner branch test).
+template <bool EXACT_POW = true>
 __device__ inline double plane_detect(const xdem_device_plane_info& pe, d3 xp, d3 wp, d3& u, double r) {
     // Every branch below needs |wp.z| < r (the face test) or a sum of
     // non-negative squares that includes wp.z^2 to stay below r^2; when both
     // fail no branch can be taken, so leave before the pow calls.  Exact: the
     // rounded sums are >= their last term.
     if (!(fabs(wp.z) < r) && !(wp.z * wp.z < r * r)) return -1.0;
-    double a_l1 = pow(wp.x - pe.l1, 2.0);
-    double b_l2 = pow(wp.y - pe.l2, 2.0);
+    const double ex_l1 = wp.x - pe.l1, ey_l2 = wp.y - pe.l2;
+    double a_l1 = EXACT_POW ? pow(ex_l1, 2.0) : ex_l1 * ex_l1;
+    double b_l2 = EXACT_POW ? pow(ey_l2, 2.0) : ey_l2 * ey_l2;
     double sqa = wp.x * wp.x;
     double sqb = wp.y * wp.y;
     double sqc = wp.z * wp.z;

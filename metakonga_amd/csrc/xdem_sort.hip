@@ -80,10 +80,6 @@ __global__ __launch_bounds__(SORT_THREADS) void k_hist(const uint32_t* __restric
     }
 }
 
-// Exclusive scan of `count` uint32 in place with one block of 1024 threads.
-constexpr int SCAN_THREADS = 1024;
-constexpr int SCAN_ITEMS = 4;
-
 __device__ inline uint32_t wave_incl_scan(uint32_t v) {
     uint32_t lane = __lane_id();
     for (int o = 1; o < 64; o <<= 1) {
@@ -93,48 +89,41 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan(uint32_t* __restrict__ data,
-                                                       uint32_t count) {
+// Exclusive scan of one digit's row of tile counts (block d scans
+// hist[d*ntiles .. (d+1)*ntiles) in place) and the digit's total.  One block
+// per digit: 256 rows in parallel instead of one block over all of them.
+constexpr int SCAN_THREADS = 256;
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_digits(uint32_t* __restrict__ hist,
+                                                              uint32_t ntiles,
+                                                              uint32_t* __restrict__ dtot) {
     __shared__ uint32_t wsum[SCAN_THREADS / 64];
     __shared__ uint32_t carry_s;
+    uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) carry_s = 0;
     __syncthreads();
-    for (uint32_t base = 0; base < count; base += SCAN_THREADS * SCAN_ITEMS) {
-        uint32_t v[SCAN_ITEMS];
-        uint32_t local = 0;
-        for (int i = 0; i < SCAN_ITEMS; ++i) {
-            uint32_t e = base + t * SCAN_ITEMS + i;
-            v[i] = e < count ? data[e] : 0;
-            local += v[i];
-        }
-        uint32_t incl = wave_incl_scan(local);
-        if (lane == 63) wsum[w] = incl;
+    for (uint32_t base = 0; base < ntiles; base += SCAN_THREADS) {
+        uint32_t e = base + t;
+        uint32_t v = e < ntiles ? row[e] : 0;
+        uint32_t inc = wave_incl_scan(v);
+        if (lane == 63) wsum[w] = inc;
         __syncthreads();
-        if (w == 0) {
-            uint32_t x = lane < SCAN_THREADS / 64 ? wsum[lane] : 0;
-            uint32_t xi = wave_incl_scan(x);
-            if (lane < SCAN_THREADS / 64) wsum[lane] = xi - x;  // exclusive wave offsets
-        }
+        uint32_t before = carry_s;
+        for (uint32_t k = 0; k < w; ++k) before += wsum[k];
+        if (e < ntiles) row[e] = before + inc - v;
         __syncthreads();
-        uint32_t carry = carry_s;
-        uint32_t run = carry + wsum[w] + incl - local;
-        for (int i = 0; i < SCAN_ITEMS; ++i) {
-            uint32_t e = base + t * SCAN_ITEMS + i;
-            if (e < count) data[e] = run;
-            run += v[i];
-        }
-        __syncthreads();
-        if (t == SCAN_THREADS - 1) carry_s = run;
+        if (t == SCAN_THREADS - 1) carry_s = before + inc;
         __syncthreads();
     }
+    if (t == 0) dtot[blockIdx.x] = carry_s;
 }
 
 __global__ __launch_bounds__(SORT_THREADS) void k_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, uint32_t n, int shift, int nbits,
-    const uint32_t* __restrict__ offsets, uint32_t ntiles) {
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ dtot, uint32_t ntiles) {
     __shared__ uint32_t wcnt[WAVES * RADIX];
+    __shared__ uint32_t dbase[RADIX];
     __shared__ uint32_t lstart[RADIX];
     __shared__ uint32_t gstart[RADIX];
     __shared__ uint32_t wtot[WAVES];
@@ -173,6 +162,19 @@ __global__ __launch_bounds__(SORT_THREADS) void k_scatter(
             total += c;
         }
         gstart[t] = offsets[t * ntiles + blockIdx.x];
+    }
+    // digit bases: exclusive scan of the digit totals (every block redoes it)
+    {
+        uint32_t v = dtot[t];
+        uint32_t inc = wave_incl_scan(v);
+        if (lane == 63) wtot[w] = inc;
+        __syncthreads();
+        uint32_t before = 0;
+        for (uint32_t k = 0; k < w; ++k) before += wtot[k];
+        dbase[t] = before + inc - v;
+        __syncthreads();
+        gstart[t] += dbase[t];
+        __syncthreads();
     }
     // exclusive scan of totals over the 256 digits (SORT_THREADS == RADIX)
     uint32_t incl = wave_incl_scan(total);
@@ -258,11 +260,12 @@ int32_t sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits
     if (n <= 1 || key_bits == 0) return XDEM_OK;
     const uint32_t ntiles = div_up(n, TILE);
     const size_t hist_n = (size_t)RADIX * ntiles;
-    const size_t need = sizeof(uint32_t) * (2 * (size_t)n + hist_n) + 256;
+    const size_t need = sizeof(uint32_t) * (2 * (size_t)n + hist_n + RADIX) + 256;
     XDEM_TRY(workspace(0).reserve(need));
     uint32_t* tk = static_cast<uint32_t*>(workspace(0).ptr);
     uint32_t* tv = tk + n;
     uint32_t* hist = tv + n;
+    uint32_t* dtot = hist + hist_n;
     uint32_t* src_k = keys;
     uint32_t* src_v = vals;
     uint32_t* dst_k = tk;
@@ -274,10 +277,10 @@ int32_t sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits
         hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(SORT_THREADS), 0, s, src_k, n, shift, nbits,
                            hist, ntiles);
         XDEM_LAUNCHED();
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, s, hist, (uint32_t)hist_n);
+        hipLaunchKernelGGL(k_scan_digits, dim3(RADIX), dim3(SCAN_THREADS), 0, s, hist, ntiles, dtot);
         XDEM_LAUNCHED();
         hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, s, src_k, src_v, dst_k,
-                           dst_v, n, shift, nbits, hist, ntiles);
+                           dst_v, n, shift, nbits, hist, dtot, ntiles);
         XDEM_LAUNCHED();
         std::swap(src_k, dst_k);
         std::swap(src_v, dst_v);

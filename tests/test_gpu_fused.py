@@ -219,3 +219,42 @@ def test_energy_reduction(lib):
     assert ke == pytest.approx(ke_ref, rel=1e-6)
     assert np.allclose(mom, (m[:, None] * st["vel"]).sum(0), rtol=1e-6, atol=1e-12)
     b.close()
+
+
+def test_checkpoint_resume_bitexact(lib):
+    """State read back (with accelerations) and re-added resumes bit-exactly."""
+    sc, vel, om = dense_scene(n_side=9, seed=13)
+    ref = _run([sc], 40, 20, vel=[vel], om=[om])[0]
+    b = Batch(sc.np, 1, int(np.prod(sc.grid_size)))
+    sid = b.add(sc, vel, om)
+    b.step(20)
+    ck = b.state(sid)
+    b.close()
+    b2 = Batch(sc.np, 1, int(np.prod(sc.grid_size)))
+    sid2 = b2.add(sc, pos=ck["pos"], vel=ck["vel"], omega=ck["omega"], acc=ck["acc"],
+                  alpha=ck["alpha"])
+    b2.step(20)
+    out = b2.state(sid2)
+    b2.close()
+    for k in ref:
+        assert np.array_equal(ref[k], out[k]), k
+
+
+def test_block_size_does_not_change_results(lib):
+    """The autotuned block size only changes the launch shape, not the arithmetic."""
+    sc, vel, om = dense_scene(n_side=9, seed=17)
+    outs = []
+    for bs in (64, 256):
+        b = Batch(sc.np, 1, int(np.prod(sc.grid_size)))
+        b.block(bs)
+        sid = b.add(sc, vel, om)
+        b.step(30)
+        outs.append(b.state(sid))
+        b.close()
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+    b = Batch(sc.np, 1, int(np.prod(sc.grid_size)))
+    b.add(sc, vel, om)
+    chosen, ms = b.autotune(steps_per_trial=5)
+    assert chosen % 1000 in (64, 128, 256) and len(ms) == 6 and all(v > 0 for v in ms.values())
+    b.close()
