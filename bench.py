@@ -104,6 +104,33 @@ def run_batched(nscene, steps, lib, block=100, seed=0xD3E3):
                 autotune={"chosen": tuned[0], "kernel_ms": tuned[1]})
 
 
+def run_settling_8k(lib, steps=2000):
+    """The 8,000-sphere settling box (r = 5 mm, dt = 1e-5 s, 2,000 steps) on both
+    paths: small enough that launch latency, not bandwidth, sets the rate."""
+    from metakonga_amd import batch
+    from metakonga_amd.xdem import Simulation
+    sc = scenes.settling_box()
+    sim = Simulation(sc, lib)
+    sim.step(10)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    f64 = sc.np * steps / (time.perf_counter() - t0)
+    sim.close()
+    b = batch.Batch(sc.np, 1, int(np.prod(sc.grid_size)), lib)
+    b.add(sc)
+    b.step(100)
+    b.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps // 100):
+        b.step(100)
+    b.sync()
+    f32c = sc.np * steps / (time.perf_counter() - t0)
+    b.close()
+    return {"np": sc.np, "steps": steps, "fp64_plain": f64, "fp32c_fused": f32c}
+
+
 def run_serving(duration, lib, rate=20.0, block=100, seed=0xD3E4):
     """Continuous batching of a seeded Poisson trace (drop tests and hopper
     discharges, 1k-500k spheres, 500-5000 steps); scenes are generated before
@@ -245,7 +272,13 @@ def main():
         serving = run_serving(args.serve_seconds, lib)
         log(f"serving: {serving}")
 
+    small = None
+    if fused is not None and not args.skip_fp64 and args.serve_seconds > 0:
+        small = run_settling_8k(lib)
+        log(f"settling 8k: {small}")
+
     cpu = None
+    cpu_omp = None
     if not args.no_cpu:
         log("cpu baseline (oracle, CPU-path semantics, 1 thread) ...")
         c = cpu_baseline(scene, budget_s=args.cpu_budget)
@@ -254,6 +287,11 @@ def main():
                          f"({c['seconds']:.1f}s), serial CPU-path semantics",
                "cpu": cpu_model()}
         log(f"cpu: {c['value']:.4g} particle-steps/s over {c['steps']} steps")
+        nthr = min(16, os.cpu_count() or 1)
+        co = cpu_baseline(scene, budget_s=args.cpu_budget / 2, threads=nthr)
+        cpu_omp = {"value": co["value"], "unit": "particle-steps/s", "cores": nthr,
+                   "kind": "port", "sample": f"same, OpenMP over particles, {co['steps']} steps"}
+        log(f"cpu omp x{nthr}: {co['value']:.4g} particle-steps/s")
 
     out = {
         "metric": METRIC,
@@ -283,6 +321,10 @@ def main():
             out["speedup_vs_fp64"] = fused["value"] / f64["value"]
     if batched is not None:
         out["batched"] = batched
+    if small is not None:
+        out["settling_8k"] = small
+    if cpu_omp is not None:
+        out["cpu_baseline_multicore"] = cpu_omp
     if serving is not None:
         out["serving"] = serving
     print(json.dumps(out), flush=True)
