@@ -72,11 +72,14 @@ int32_t xdem_batch_set_graphs(xdem_batch* b, int32_t on);
 /* Block size (64, 128, 192 or 256) of the contact kernel; 0 = default (256). */
 int32_t xdem_batch_set_block(xdem_batch* b, uint32_t threads);
 /* Autotuner: times the contact kernel for block sizes 64, 128 and 256, each
- * with an unconstrained register budget and with an 8-waves-per-SIMD budget,
- * over steps_per_trial real steps (the scenes advance 6 * (steps_per_trial + 1)
- * steps), and keeps the fastest.  *chosen = block size (+1000 for the 8-wave
- * variant); ms_per_candidate (6 entries, may be NULL) receives the measured
- * kernel time per launch in the order 64/128/256 then 64/128/256 (8 waves). */
+ * with 1, 2 or 4 neighbour candidates loaded together and with the contact
+ * records loaded one contact ahead or not, over steps_per_trial real steps
+ * (the scenes advance 18 * (steps_per_trial + 1) steps), and keeps the
+ * fastest.  *chosen = block size + 1000 * candidates-per-load-batch + 10000 *
+ * (1 if contact records are loaded ahead); ms_per_candidate (18 entries:
+ * block-major, then candidates-per-batch, then load-ahead off/on; may be
+ * NULL) receives the measured kernel time per launch.  Results do not depend
+ * on the choice (bit-identical). */
 int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* chosen,
                             double* ms_per_candidate);
 int32_t xdem_batch_synchronize(xdem_batch* b);

@@ -122,12 +122,13 @@ class Batch:
         abi.check(self.lib, self.lib.xdem_batch_set_block(self.h, int(threads)))
 
     def autotune(self, steps_per_trial=10):
-        """Pick the contact-kernel block size by timing (advances the scenes)."""
+        """Pick the contact-kernel launch shape by timing (advances the scenes)."""
         chosen = C.c_uint32()
-        ms = np.zeros(6)
+        ms = np.zeros(18)
         abi.check(self.lib, self.lib.xdem_batch_autotune(self.h, steps_per_trial, C.byref(chosen),
                                                          abi.ptr(ms)))
-        keys = [f"block{b}" for b in (64, 128, 256)] + [f"block{b}-8waves" for b in (64, 128, 256)]
+        keys = [f"block{b}-unroll{u}{'-ahead' if p else ''}" for b in (64, 128, 256)
+                for u in (1, 2, 4) for p in (0, 1)]
         return chosen.value, dict(zip(keys, ms.tolist()))
 
     def state(self, sid):

@@ -112,7 +112,8 @@ struct xdem_batch {
     bool any_stepped = false;
     bool storage_sorted_all_new = false;
     uint32_t block_threads = 256;
-    int occ8 = 0;                     // k_step variant: 0 = unconstrained registers, 1 = 8 waves/SIMD
+    int unroll = 2;                   // k_step variant: candidates loaded together (1, 2 or 4)
+    bool pipe = false;                // k_step variant: contact records loaded one ahead
     bool use_graphs = true;
     std::map<std::pair<uint32_t, int>, hipGraphExec_t> graphs;
     PhaseTimer timer;
@@ -486,14 +487,23 @@ int32_t launch_step(xdem_batch* b, bool adv) {
     StepArgs a = step_args(b, false);
     XDEM_TRY(b->timer.begin(1, b->st));
     dim3 g(b->nblk_sorted), t(b->block_threads);
-    if (adv && b->occ8)
-        hipLaunchKernelGGL((k_step<true, false, 8>), g, t, 0, b->st, a);
-    else if (adv)
-        hipLaunchKernelGGL((k_step<true, false, 1>), g, t, 0, b->st, a);
-    else if (b->occ8)
-        hipLaunchKernelGGL((k_step<false, false, 8>), g, t, 0, b->st, a);
-    else
-        hipLaunchKernelGGL((k_step<false, false, 1>), g, t, 0, b->st, a);
+#define XDEM_STEP_LAUNCH(U, P)                                                     \
+    do {                                                                           \
+        if (adv)                                                                   \
+            hipLaunchKernelGGL((k_step<true, false, U, P>), g, t, 0, b->st, a);    \
+        else                                                                       \
+            hipLaunchKernelGGL((k_step<false, false, U, P>), g, t, 0, b->st, a);   \
+    } while (0)
+    const int v = b->unroll * 2 + (b->pipe ? 1 : 0);
+    switch (v) {
+        case 2: XDEM_STEP_LAUNCH(1, false); break;
+        case 3: XDEM_STEP_LAUNCH(1, true); break;
+        case 5: XDEM_STEP_LAUNCH(2, true); break;
+        case 8: XDEM_STEP_LAUNCH(4, false); break;
+        case 9: XDEM_STEP_LAUNCH(4, true); break;
+        default: XDEM_STEP_LAUNCH(2, false); break;
+    }
+#undef XDEM_STEP_LAUNCH
     XDEM_LAUNCHED();
     XDEM_TRY(b->timer.end(b->st));
     b->cur = 1 - b->cur;
@@ -694,44 +704,51 @@ int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps) {
 }
 
 // Autotuner: times the contact kernel (HIP events) for each candidate launch
-// shape — block size 64/128/256 x register budget (unconstrained / 8 waves per
-// SIMD) — over `steps_per_trial` real steps (the scenes advance) and keeps the
-// fastest.  Returns the choice (block size, +1000 for the 8-wave variant) and
-// the per-candidate kernel time per launch (6 entries).
+// shape — block size 64/128/256 x candidates loaded together 1/2/4 x contact
+// records loaded one ahead or not — over `steps_per_trial` real steps (the
+// scenes advance) and keeps the fastest.  Returns the choice (block size +
+// 1000 * unroll + 10000 * pipelined) and the per-candidate kernel time per
+// launch (18 entries: block-major, then unroll, then pipelined).
 int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* chosen,
                             double* ms_per_candidate) {
     XDEM_REQUIRE(b, "null batch");
     const uint32_t cand[3] = {64, 128, 256};
+    const int unr[3] = {1, 2, 4};
     if (steps_per_trial == 0) steps_per_trial = 10;
     bool was_on = b->timer.on;
     double best = 1e300;
     uint32_t best_c = b->block_threads;
-    int best_o = b->occ8;
-    for (int o = 0; o < 2; ++o)
-        for (int i = 0; i < 3; ++i) {
-            b->occ8 = o;
-            b->drop_graphs();
-            XDEM_TRY(xdem_batch_set_block(b, cand[i]));
-            XDEM_TRY(xdem_batch_step(b, 1));  // applies the new tables
-            XDEM_TRY(xdem_batch_enable_timing(b, 1));
-            XDEM_TRY(xdem_batch_step(b, steps_per_trial));
-            double ms[6];
-            uint32_t cnt[6];
-            XDEM_TRY(xdem_batch_phase_times(b, ms, cnt));
-            double per = cnt[1] ? ms[1] / cnt[1] : 1e300;
-            if (ms_per_candidate) ms_per_candidate[3 * o + i] = per;
-            if (per < best) {
-                best = per;
-                best_c = cand[i];
-                best_o = o;
+    int best_u = b->unroll;
+    bool best_p = b->pipe;
+    for (int i = 0; i < 3; ++i)
+        for (int u = 0; u < 3; ++u)
+            for (int pp = 0; pp < 2; ++pp) {
+                b->unroll = unr[u];
+                b->pipe = pp != 0;
+                b->drop_graphs();
+                XDEM_TRY(xdem_batch_set_block(b, cand[i]));
+                XDEM_TRY(xdem_batch_step(b, 1));  // applies the new tables
+                XDEM_TRY(xdem_batch_enable_timing(b, 1));
+                XDEM_TRY(xdem_batch_step(b, steps_per_trial));
+                double ms[6];
+                uint32_t cnt[6];
+                XDEM_TRY(xdem_batch_phase_times(b, ms, cnt));
+                double per = cnt[1] ? ms[1] / cnt[1] : 1e300;
+                if (ms_per_candidate) ms_per_candidate[6 * i + 2 * u + pp] = per;
+                if (per < best) {
+                    best = per;
+                    best_c = cand[i];
+                    best_u = unr[u];
+                    best_p = pp != 0;
+                }
+                XDEM_TRY(xdem_batch_enable_timing(b, 0));
             }
-            XDEM_TRY(xdem_batch_enable_timing(b, 0));
-        }
-    b->occ8 = best_o;
+    b->unroll = best_u;
+    b->pipe = best_p;
     b->drop_graphs();
     XDEM_TRY(xdem_batch_set_block(b, best_c));
     if (was_on) XDEM_TRY(xdem_batch_enable_timing(b, 1));
-    if (chosen) *chosen = best_c + (best_o ? 1000u : 0u);
+    if (chosen) *chosen = best_c + 1000u * (uint32_t)best_u + (best_p ? 10000u : 0u);
     return XDEM_OK;
 }
 
@@ -967,7 +984,7 @@ int32_t xdem_batch_probe(xdem_batch* b, uint32_t scene_id, double* force3, doubl
         b->dbg_cap = mp;
     }
     StepArgs a = step_args(b, true);  // no advance: keynew is not written
-    hipLaunchKernelGGL((k_step<false, true, 1>), dim3(b->nblk_sorted), dim3(b->block_threads), 0,
+    hipLaunchKernelGGL((k_step<false, true, 2, false>), dim3(b->nblk_sorted), dim3(b->block_threads), 0,
                        b->st, a);
     XDEM_LAUNCHED();
     XDEM_HIP(hipStreamSynchronize(b->st));

@@ -145,11 +145,9 @@ struct Own {
 
 // Particle-particle contact of `o` with the particle stored at t of `in`
 // (the predicate was already found true).  Accumulates into sumF/sumM.
-__device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own& o, uint32_t t,
+__device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own& o,
+                                  const double4& jp, const float4& jvm, const float4& jwa,
                                   f3& Ft, f3& Fn, f3& M, f3& sumF, f3& sumM) {
-    const double4 jp = A.in.pr[t];
-    const float4 jvm = A.in.vm[t];
-    const float4 jwa = A.in.wa[t];
     double dx = jp.x - o.x, dy = jp.y - o.y, dz = jp.z - o.z;
     double d2 = dx * dx + dy * dy + dz * dz;
     double dist = sqrt(d2);
@@ -168,24 +166,67 @@ __device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own&
     sumM += M;
 }
 
-// OCC: minimum waves per SIMD the register allocation must allow (1 = no
-// constraint, 5 waves at 87 VGPRs; 8 = 64 VGPRs with spills in the cold
-// contact/plane code).  The autotuner picks one.
-template <bool ADV, bool DBG, int OCC>
-__global__ __launch_bounds__(256, OCC) void k_step(StepArgs A) {
+// The contacts of a list, in order.  PIPE: the records of contact c+1 are
+// loaded before contact c is evaluated (more loads in flight, more registers).
+template <bool PIPE>
+__device__ inline void pp_contacts(const StepArgs& A, const BScene& S, const Own& o,
+                                   const uint32_t* list, uint32_t cnt, f3& Ft, f3& Fn, f3& M,
+                                   f3& sumF, f3& sumM) {
+    if (cnt == 0) return;
+    uint32_t t = list[0];
+    double4 jp = A.in.pr[t];
+    float4 jvm = A.in.vm[t], jwa = A.in.wa[t];
+    for (uint32_t c = 0; c < cnt; ++c) {
+        if (PIPE) {
+            const double4 cp = jp;
+            const float4 cvm = jvm, cwa = jwa;
+            if (c + 1 < cnt) {
+                t = list[c + 1];
+                jp = A.in.pr[t];
+                jvm = A.in.vm[t];
+                jwa = A.in.wa[t];
+            }
+            pp_contact(A, S, o, cp, cvm, cwa, Ft, Fn, M, sumF, sumM);
+        } else {
+            if (c) {
+                t = list[c];
+                jp = A.in.pr[t];
+                jvm = A.in.vm[t];
+                jwa = A.in.wa[t];
+            }
+            pp_contact(A, S, o, jp, jvm, jwa, Ft, Fn, M, sumF, sumM);
+        }
+    }
+}
+
+// UNR: candidates whose loads are issued together in the neighbour walk;
+// PIPE: contact records loaded one contact ahead.  Both trade registers
+// (waves per SIMD) for loads in flight per wave; the autotuner picks them
+// together with the block size.
+template <bool ADV, bool DBG, int UNR, bool PIPE>
+__global__ __launch_bounds__(256) void k_step(StepArgs A) {
     __shared__ uint32_t clist[256 * CMAX];
     const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
     const BlockEntry be = A.blocks[L];
     const BScene S = A.scenes[be.scene];
     const uint32_t j = be.j0 + threadIdx.x;
-    if (j >= be.j1) return;
+    if (j >= be.j1 || j >= A.n) return;
     uint32_t* mylist = clist + threadIdx.x * CMAX;
+    // own records: loaded at storage index j together with perm[j] (equal
+    // unless a mover shifted this sorted position), reloaded otherwise
     const uint32_t s = A.perm[j];
+    double4 ip = A.in.pr[j];
+    float4 vm = A.in.vm[j], wa = A.in.wa[j], aa = A.in.aa[j];
+    uint2 zi = A.in.zi[j];
     if (s >= A.n) return;  // defensive: perm is a permutation of [0, n)
+    if (s != j) {
+        ip = A.in.pr[s];
+        vm = A.in.vm[s];
+        wa = A.in.wa[s];
+        aa = A.in.aa[s];
+        zi = A.in.zi[s];
+    }
     Own o;
-    const double4 ip = A.in.pr[s];
-    const float4 vm = A.in.vm[s], wa = A.in.wa[s], aa = A.in.aa[s];
-    const uint2 zi = A.in.zi[s];
     o.x = ip.x;
     o.y = ip.y;
     o.z = ip.z;
@@ -200,46 +241,86 @@ __global__ __launch_bounds__(256, OCC) void k_step(StepArgs A) {
     uint32_t nlist = 0, ntotal = 0;
     const double ri = (double)o.r;
     // phase 1: candidates in the reference order (z, y, x cells; ascending j).
-    // Away from the x wrap the three cells of an x-row have consecutive keys,
-    // so their particles are one contiguous run of sorted positions.
-    const bool xrow = cx >= 1 && cx + 1 < (int)S.gx;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                uint2 cr;
-                if (xrow) {
-                    if (dx != -1) break;
-                    const uint32_t k0 = key_of(S, cx - 1, cy + dy, cz + dz);
-                    const uint2 c0 = A.crange[k0], c1 = A.crange[k0 + 1], c2 = A.crange[k0 + 2];
-                    cr.x = c0.x != EMPTY ? c0.x : (c1.x != EMPTY ? c1.x : c2.x);
-                    cr.y = c2.x != EMPTY ? c2.y : (c1.x != EMPTY ? c1.y : c0.y);
-                } else {
-                    cr = A.crange[key_of(S, cx + dx, cy + dy, cz + dz)];
+    // A candidate passing the FP64 predicate goes to this thread's LDS list;
+    // the list is evaluated (in order) whenever it could overflow, and at the
+    // end.  Ft and M carry over from contact to contact in list order, so
+    // where the list is split does not change the result.
+    auto consider = [&](uint32_t t, const double4& jp) {
+        double ex = jp.x - o.x, ey = jp.y - o.y, ez = jp.z - o.z;
+        double d2 = ex * ex + ey * ey + ez * ez;
+        double Sr = ri + jp.w;
+        if (d2 > Sr * Sr * 1.000001) return;
+        if (!(Sr - sqrt(d2) > 0)) return;
+        if (DBG && ntotal < A.dbg_max) A.dbg_part[(size_t)j * A.dbg_max + ntotal] = A.in.zi[t].y;
+        ++ntotal;
+        mylist[nlist++] = t;
+    };
+    auto flush_if = [&](uint32_t room) {
+        if (nlist + room > CMAX) {
+            pp_contacts<PIPE>(A, S, o, mylist, nlist, Ft, Fn, M, sumF, sumM);
+            nlist = 0;
+        }
+    };
+    if (cx >= 1 && cx + 1 < (int)S.gx) {
+        // Away from the x wrap the three cells of an x-row have consecutive keys,
+        // so their particles are one contiguous run of sorted positions.  The
+        // range of row r+1 is loaded before row r is walked, and a row is
+        // walked UNR candidates at a time.  The storage order is the previous
+        // step's sorted order, so perm[k] == k except where movers shifted
+        // the order: the record at storage index k is loaded together with
+        // perm[k] and reloaded only where perm[k] != k.
+        auto row_range = [&](int r) {
+            const uint32_t k0 = key_of(S, cx - 1, cy + (r % 3) - 1, cz + (r / 3) - 1);
+            const uint2 c0 = A.crange[k0], c1 = A.crange[k0 + 1], c2 = A.crange[k0 + 2];
+            uint2 rg;
+            rg.x = c0.x != EMPTY ? c0.x : (c1.x != EMPTY ? c1.x : c2.x);
+            rg.y = c2.x != EMPTY ? c2.y : (c1.x != EMPTY ? c1.y : c0.y);
+            return rg;
+        };
+        uint2 next = row_range(0);
+#pragma unroll 1
+        for (int r = 0; r < 9; ++r) {
+            const uint2 row = next;
+            if (r < 8) next = row_range(r + 1);
+            if (row.x == EMPTY) continue;
+            const uint32_t e1 = min(row.y, A.n);
+            for (uint32_t jj = row.x; jj < e1; jj += UNR) {
+                uint32_t t[UNR];
+                double4 q[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    const uint32_t k = jj + u;
+                    const bool ok = k < e1 && k != j;  // e1 <= n
+                    t[u] = ok ? A.perm[k] : A.n;
+                    if (ok) q[u] = A.in.pr[k];
                 }
-                if (cr.x == EMPTY) continue;
-                const uint32_t e1 = min(cr.y, A.n);
-                for (uint32_t jj = cr.x; jj < e1; ++jj) {
-                    if (jj == j) continue;
-                    uint32_t t = A.perm[jj];
-                    if (t >= A.n) continue;
-                    const double4 jp = A.in.pr[t];
-                    double ex = jp.x - o.x, ey = jp.y - o.y, ez = jp.z - o.z;
-                    double d2 = ex * ex + ey * ey + ez * ez;
-                    double Sr = ri + jp.w;
-                    if (d2 > Sr * Sr * 1.000001) continue;
-                    if (!(Sr - sqrt(d2) > 0)) continue;
-                    if (DBG && ntotal < A.dbg_max) A.dbg_part[(size_t)j * A.dbg_max + ntotal] = A.in.zi[t].y;
-                    ++ntotal;
-                    mylist[nlist++] = t;
-                    if (nlist == CMAX) {
-                        for (uint32_t c = 0; c < nlist; ++c)
-                            pp_contact(A, S, o, mylist[c], Ft, Fn, M, sumF, sumM);
-                        nlist = 0;
+#pragma unroll
+                for (int u = 0; u < UNR; ++u)
+                    if (t[u] < A.n && t[u] != jj + u) q[u] = A.in.pr[t[u]];
+#pragma unroll
+                for (int u = 0; u < UNR; ++u)
+                    if (t[u] < A.n) consider(t[u], q[u]);
+                flush_if(UNR);
+            }
+        }
+    } else {
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const uint2 cr = A.crange[key_of(S, cx + dx, cy + dy, cz + dz)];
+                    if (cr.x == EMPTY) continue;
+                    const uint32_t e1 = min(cr.y, A.n);
+                    for (uint32_t jj = cr.x; jj < e1; ++jj) {
+                        if (jj == j) continue;
+                        const uint32_t t = A.perm[jj];
+                        if (t >= A.n) continue;
+                        consider(t, A.in.pr[t]);
+                        flush_if(1);
                     }
                 }
-            }
+    }
     // phase 2: the listed contacts, in order
-    for (uint32_t c = 0; c < nlist; ++c) pp_contact(A, S, o, mylist[c], Ft, Fn, M, sumF, sumM);
+    pp_contacts<PIPE>(A, S, o, mylist, nlist, Ft, Fn, M, sumF, sumM);
     // F = m g + particle-particle (vv_update_velocity reset, then cu_calculate_p2p)
     f3 F = F3(o.m * S.gvx, o.m * S.gvy, o.m * S.gvz) + sumF;
     f3 T = sumM;
