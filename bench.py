@@ -48,6 +48,23 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_hopper1M_fp32c.md")
+
+
+def profiled_traffic(kernel_sig):
+    """HBM bytes per launch of `kernel_sig` (e.g. "k_step<true, false, 2, false>")
+    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (counter values
+    scaled by the copy-kernel calibration); None when the table has no row."""
+    try:
+        for line in open(TRAFFIC_TABLE):
+            cols = [c.strip() for c in line.split("|")]
+            if len(cols) > 7 and cols[1].strip("`").endswith(kernel_sig):
+                return (float(cols[5]) + float(cols[6])) * 1e6
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def run_fp64(scene, steps, warmup, lib):
     from metakonga_amd.xdem import Simulation
     sim = Simulation(scene, lib)
@@ -246,9 +263,14 @@ def main():
         dtype = "f32 contact + f64 positions"
         k_ms = fused["phase_ms"]["contact_verlet"]
         achieved = KSTEP_BYTES * scene.np / (k_ms * 1e-3)
+        ch = (fused.get("autotune") or {}).get("chosen", 2256)
+        sig = f"k_step<true, false, {(ch // 1000) % 10}, {'true' if ch >= 10000 else 'false'}>"
+        traffic = profiled_traffic(sig) if scene.np == 1_000_000 else None
         roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None,
-                "kernel": "k_step<true,false> (fused contact + Verlet, FP32c)",
+                "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "traffic_source": "profiles/hbm_traffic_hopper1M_fp32c.md (rocprofv3 FETCH_SIZE + "
+                                  "WRITE_SIZE, copy-kernel calibrated)" if traffic else None,
+                "kernel": f"{sig} (fused contact + Verlet, FP32c), block {ch % 1000}",
                 "bytes_per_particle": KSTEP_BYTES, "kernel_ms": k_ms,
                 "step_frac": STEP_F32C_BYTES * fused["value"] / HBM_PEAK}
     else:

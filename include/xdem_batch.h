@@ -45,7 +45,22 @@ typedef struct xdem_batch_scene_desc {
     xdem_contact_pair_desc wall;   /* particle-plane pair (wall.wall = wall material) */
     uint32_t nplane;
     const xdem_device_plane_info* planes;  /* host array of nplane planes, applied in order */
+    int32_t tangential_history;    /* 0: the reference DHS law (memoryless tangential force);
+                                      1: DHS with a tangential spring per contact (below) */
 } xdem_batch_scene_desc;
+
+/* Tangential-spring history (tangential_history = 1; no counterpart in the
+ * reference).  Per particle up to XDEM_BATCH_HISTORY_SLOTS contacts, in
+ * evaluation order (particle contacts in the reference traversal order, then
+ * planes in order), each a key (partner particle index within the scene, or
+ * XDEM_BATCH_WALL_KEY | plane index) and a spring xi (FP32).  Per step: xi is
+ * projected onto the current tangent plane and advanced by dt e (e the
+ * tangential relative velocity), Ft = ks xi + vs e; beyond the Coulomb cap
+ * mu |Fn| the force is capped and xi reset to (Ft - vs e) / ks.  A contact
+ * that ends drops its spring.  Contacts beyond the slots are evaluated with a
+ * zero spring and counted (xdem_batch_history_overflow). */
+#define XDEM_BATCH_HISTORY_SLOTS 16
+#define XDEM_BATCH_WALL_KEY 0x80000000u
 
 /* Capacities are fixed at creation (HBM is reserved up front). */
 int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
@@ -63,6 +78,14 @@ int32_t xdem_batch_add_scene(xdem_batch* b, const xdem_batch_scene_desc* d, cons
 int32_t xdem_batch_add_scene_state(xdem_batch* b, const xdem_batch_scene_desc* d,
                                    const double* pos4, const double* vel3, const double* omega3,
                                    const double* acc3, const double* alpha3, uint32_t* scene_id);
+/* As xdem_batch_add_scene_state, also restoring the tangential-spring history
+ * (hist_count[np], hist_key[np * SLOTS], hist_xi[np * SLOTS * 3], original
+ * particle order; all NULL for none) as read back by xdem_batch_get_history. */
+int32_t xdem_batch_add_scene_history(xdem_batch* b, const xdem_batch_scene_desc* d,
+                                     const double* pos4, const double* vel3, const double* omega3,
+                                     const double* acc3, const double* alpha3,
+                                     const uint32_t* hist_count, const uint32_t* hist_key,
+                                     const double* hist_xi, uint32_t* scene_id);
 /* Remove a scene (its slot and memory are reused).  Takes effect at the next step. */
 int32_t xdem_batch_remove_scene(xdem_batch* b, uint32_t scene_id);
 /* Advance every active scene by nsteps (one step block).  When graphs are on
@@ -78,8 +101,9 @@ int32_t xdem_batch_set_block(xdem_batch* b, uint32_t threads);
  * fastest.  *chosen = block size + 1000 * candidates-per-load-batch + 10000 *
  * (1 if contact records are loaded ahead); ms_per_candidate (18 entries:
  * block-major, then candidates-per-batch, then load-ahead off/on; may be
- * NULL) receives the measured kernel time per launch.  Results do not depend
- * on the choice (bit-identical). */
+ * NULL) receives the measured kernel time per launch (-1 for the load-ahead
+ * shapes while a scene has history on: they are not used then).  Results do
+ * not depend on the choice (bit-identical). */
 int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* chosen,
                             double* ms_per_candidate);
 int32_t xdem_batch_synchronize(xdem_batch* b);
@@ -88,6 +112,13 @@ int32_t xdem_batch_synchronize(xdem_batch* b);
  * FP32 values widened. */
 int32_t xdem_batch_get_state(xdem_batch* b, uint32_t scene_id, double* pos4, double* vel3,
                              double* omega3, double* acc3, double* alpha3);
+/* Tangential-spring history of one scene in original particle order (key and
+ * xi may be NULL; unused slots read 0).  Scenes without history read count 0. */
+int32_t xdem_batch_get_history(xdem_batch* b, uint32_t scene_id, uint32_t* count, uint32_t* key,
+                               double* xi);
+/* Contacts found with every history slot of their particle taken, summed over
+ * all steps since creation. */
+int32_t xdem_batch_history_overflow(xdem_batch* b, uint32_t* overflow);
 uint32_t xdem_batch_num_particles(const xdem_batch* b);
 uint32_t xdem_batch_num_scenes(const xdem_batch* b);
 /* Kinetic energy (translational + rotational) and linear momentum of one

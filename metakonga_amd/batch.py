@@ -19,7 +19,12 @@ class BatchSceneDesc(C.Structure):
     _fields_ = [("np", C.c_uint32), ("dt", C.c_double), ("gravity", abi.double3),
                 ("world_origin", abi.double3), ("cell_size", C.c_double), ("grid_size", abi.uint3),
                 ("rolling", C.c_bool), ("particle", abi.Material), ("pp", abi.ContactPairDesc),
-                ("wall", abi.ContactPairDesc), ("nplane", C.c_uint32), ("planes", C.c_void_p)]
+                ("wall", abi.ContactPairDesc), ("nplane", C.c_uint32), ("planes", C.c_void_p),
+                ("tangential_history", C.c_int32)]
+
+
+HISTORY_SLOTS = 16
+WALL_KEY = 0x80000000
 
 
 def declare(lib):
@@ -30,6 +35,10 @@ def declare(lib):
         "xdem_batch_add_scene": ([vp, C.POINTER(BatchSceneDesc), vp, vp, vp, C.POINTER(u32)], i32),
         "xdem_batch_add_scene_state": ([vp, C.POINTER(BatchSceneDesc), vp, vp, vp, vp, vp,
                                         C.POINTER(u32)], i32),
+        "xdem_batch_add_scene_history": ([vp, C.POINTER(BatchSceneDesc), vp, vp, vp, vp, vp, vp,
+                                          vp, vp, C.POINTER(u32)], i32),
+        "xdem_batch_get_history": ([vp, u32, vp, vp, vp], i32),
+        "xdem_batch_history_overflow": ([vp, C.POINTER(u32)], i32),
         "xdem_batch_remove_scene": ([vp, u32], i32),
         "xdem_batch_step": ([vp, u32], i32),
         "xdem_batch_set_graphs": ([vp, i32], i32),
@@ -77,6 +86,7 @@ def scene_desc(scene):
     planes = abi.struct_array(abi.PlaneInfo, scene.planes) if scene.planes else None
     d.nplane = len(scene.planes)
     d.planes = C.cast(planes, C.c_void_p) if planes is not None else None
+    d.tangential_history = int(bool(getattr(scene, "tangential_history", False)))
     return d, planes
 
 
@@ -92,15 +102,23 @@ class Batch:
         self.h = h
         self.sizes = {}
 
-    def add(self, scene, vel=None, omega=None, acc=None, alpha=None, pos=None):
+    def add(self, scene, vel=None, omega=None, acc=None, alpha=None, pos=None, history=None):
         """Add a scene; pos/vel/omega/acc/alpha (optional) override the scene's
-        initial state (a checkpoint read back with state() resumes bit-exactly)."""
+        initial state, `history` = (count, key, xi) as returned by history()
+        restores the tangential springs (a checkpoint read back with state()
+        and history() resumes bit-exactly)."""
         d, keep = scene_desc(scene)
         sid = C.c_uint32()
         pos = abi.as_f64(scene.pos if pos is None else pos)
         arr = [None if x is None else abi.as_f64(x) for x in (vel, omega, acc, alpha)]
-        abi.check(self.lib, self.lib.xdem_batch_add_scene_state(
-            self.h, C.byref(d), abi.ptr(pos), *(abi.ptr(x) for x in arr), C.byref(sid)))
+        if history is None:
+            hist = (None, None, None)
+        else:
+            hist = (np.ascontiguousarray(history[0], np.uint32),
+                    np.ascontiguousarray(history[1], np.uint32), abi.as_f64(history[2]))
+        abi.check(self.lib, self.lib.xdem_batch_add_scene_history(
+            self.h, C.byref(d), abi.ptr(pos), *(abi.ptr(x) for x in arr),
+            *(abi.ptr(x) for x in hist), C.byref(sid)))
         del keep
         self.sizes[sid.value] = scene.np
         return sid.value
@@ -138,6 +156,21 @@ class Batch:
         abi.check(self.lib, self.lib.xdem_batch_get_state(
             self.h, sid, *(abi.ptr(out[k]) for k in ("pos", "vel", "omega", "acc", "alpha"))))
         return out
+
+    def history(self, sid):
+        """(count[n], key[n, slots], xi[n, slots, 3]) in original particle order."""
+        n = self.sizes[sid]
+        count = np.zeros(n, np.uint32)
+        key = np.zeros((n, HISTORY_SLOTS), np.uint32)
+        xi = np.zeros((n, HISTORY_SLOTS, 3))
+        abi.check(self.lib, self.lib.xdem_batch_get_history(self.h, sid, abi.ptr(count),
+                                                            abi.ptr(key), abi.ptr(xi)))
+        return count, key, xi
+
+    def history_overflow(self):
+        v = C.c_uint32()
+        abi.check(self.lib, self.lib.xdem_batch_history_overflow(self.h, C.byref(v)))
+        return v.value
 
     def energy(self, sid):
         ke = C.c_double()
@@ -216,9 +249,26 @@ def bench_fused(scene, steps, warmup, lib, block=100, graphs=True, autotune=True
     movers = b.last_movers()
     b.close()
     per = dict(zip(PHASES, (ms / nt).tolist()))
+    # the same job from host buffers: scene upload (host -> HBM), the steps
+    # (first block captured), state read back (HBM -> host)
+    b = Batch(scene.np, 1, int(np.prod(scene.grid_size)), lib)
+    b.graphs(graphs)
+    b.sync()
+    t0 = time.perf_counter()
+    sid = b.add(scene)
+    done = 0
+    while done < steps:
+        k = min(blk, steps - done)
+        b.step(k)
+        done += k
+    st = b.state(sid)
+    pcie_wall = time.perf_counter() - t0
+    b.close()
     return dict(wall_s=wall, ms_per_step=wall * 1e3 / steps, value=scene.np * steps / wall,
                 phase_ms=per, movers_last=movers, timed_steps_events=nt,
-                autotune=None if tuned is None else {"chosen": tuned[0], "kernel_ms": tuned[1]})
+                autotune=None if tuned is None else {"chosen": tuned[0], "kernel_ms": tuned[1]},
+                pcie_inclusive={"value": scene.np * steps / pcie_wall, "wall_s": pcie_wall,
+                                "steps": steps})
 
 
 def smoke_check(scene, lib, w_pos=None):

@@ -25,6 +25,9 @@
 using namespace xdem;
 using namespace xdem::fused;
 
+static_assert(HSLOTS == XDEM_BATCH_HISTORY_SLOTS, "history slots");
+static_assert(WALL_KEY == XDEM_BATCH_WALL_KEY, "wall key");
+
 namespace {
 
 struct HostScene {
@@ -36,6 +39,8 @@ struct HostScene {
     uint32_t p0 = 0;       // sorted segment
     bool pending = true;   // not yet uploaded
     std::vector<double> pos, vel, omega, acc, alpha;  // staging until upload
+    std::vector<uint32_t> hcount, hkey;                 // staged history (optional)
+    std::vector<double> hxi;
 };
 
 template <class T>
@@ -104,6 +109,11 @@ struct xdem_batch {
     BlockEntry* d_blocks_store = nullptr;
     uint32_t nblk_sorted = 0, nblk_store = 0;
     double* d_red = nullptr;          // energy partials
+    // tangential-spring history (allocated with the first scene that uses it)
+    float4* hxi[2] = {nullptr, nullptr};  // HSLOTS slots per particle, parity = cur
+    uint32_t* hcnt[2] = {nullptr, nullptr};
+    uint32_t* hover = nullptr;            // overflow counter
+    bool hist_active = false;             // some scene has history on
     // scenes
     std::vector<HostScene> scenes;    // active + pending (not removed)
     uint32_t next_id = 0;
@@ -136,7 +146,8 @@ struct xdem_batch {
                         (void*)moverbits, (void*)blockcnt, (void*)counters, (void*)mkey, (void*)ms,
                         (void*)mkey2, (void*)ms2, (void*)d_scenes, (void*)d_planes,
                         (void*)d_blocks_sorted, (void*)d_blocks_store, (void*)d_red, (void*)dbg_f,
-                        (void*)dbg_cnt, (void*)dbg_part})
+                        (void*)dbg_cnt, (void*)dbg_part, (void*)hxi[0], (void*)hxi[1],
+                        (void*)hcnt[0], (void*)hcnt[1], (void*)hover})
             if (p) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -236,6 +247,7 @@ BScene make_bscene(const HostScene& h, uint32_t plane0) {
     pair_consts(d.particle, d.pp.wall, d.pp, s.ckn, s.cvn, s.sr, s.mu);
     s.mass_c = (float)(d.particle.density * (4.0 / 3.0) * M_PI);
     s.rolling = d.rolling ? 1 : 0;
+    s.history = d.tangential_history ? 1 : 0;
     return s;
 }
 
@@ -275,11 +287,38 @@ int32_t copy_segment(const BState& from, const BState& to, uint32_t src, uint32_
     return XDEM_OK;
 }
 
+// History buffers, zeroed (a scene without history keeps count 0).
+int32_t ensure_history(xdem_batch* b) {
+    if (b->hover) return XDEM_OK;
+    const size_t n = b->cap_np;
+    for (int k = 0; k < 2; ++k) {
+        XDEM_TRY(dalloc(&b->hxi[k], n * HSLOTS));
+        XDEM_TRY(dalloc(&b->hcnt[k], n));
+        XDEM_HIP(hipMemsetAsync(b->hcnt[k], 0, 4 * n, b->st));
+    }
+    XDEM_TRY(dalloc(&b->hover, 1));
+    XDEM_HIP(hipMemsetAsync(b->hover, 0, 4, b->st));
+    return XDEM_OK;
+}
+
+// History of a storage segment, parity `from` -> parity `to`.
+int32_t copy_hist(xdem_batch* b, int from, int to, uint32_t src, uint32_t dst, uint32_t n) {
+    if (!b->hover || !n) return XDEM_OK;
+    XDEM_HIP(hipMemcpyAsync(b->hxi[to] + (size_t)dst * HSLOTS, b->hxi[from] + (size_t)src * HSLOTS,
+                            16ull * HSLOTS * n, hipMemcpyDeviceToDevice, b->st));
+    XDEM_HIP(hipMemcpyAsync(b->hcnt[to] + dst, b->hcnt[from] + src, 4ull * n,
+                            hipMemcpyDeviceToDevice, b->st));
+    return XDEM_OK;
+}
+
 // Apply pending additions/removals at a block boundary (outside any graph).
 int32_t apply_layout(xdem_batch* b) {
     if (!b->layout_dirty) return XDEM_OK;
     hipStream_t st = b->st;
     b->drop_graphs();
+    b->hist_active = false;
+    for (auto& s : b->scenes) b->hist_active |= s.desc.tangential_history != 0;
+    if (b->hist_active) XDEM_TRY(ensure_history(b));
     // 1. compaction of the scenes already on the device: they sit in the
     //    storage in cell_base order (sorted layout of the last block).
     std::vector<HostScene*> old, fresh;
@@ -298,12 +337,14 @@ int32_t apply_layout(xdem_batch* b) {
         at = 0;
         for (HostScene* s : old) {
             XDEM_TRY(copy_segment(A, B, s->p0, at, s->n, st));
+            XDEM_TRY(copy_hist(b, b->cur, 1 - b->cur, s->p0, at, s->n));
             XDEM_HIP(hipMemcpyAsync(b->keynew + at, b->skey[b->skcur] + s->p0, 4ull * s->n,
                                     hipMemcpyDeviceToDevice, st));
             s->p0 = at;
             at += s->n;
         }
         XDEM_TRY(copy_segment(B, A, 0, 0, n_old, st));
+        XDEM_TRY(copy_hist(b, 1 - b->cur, b->cur, 0, 0, n_old));
         XDEM_HIP(hipMemcpyAsync(b->skey[b->skcur], b->keynew, 4ull * n_old, hipMemcpyDeviceToDevice, st));
     }
     // 2. append new scenes (id order) with the sentinel old key
@@ -343,6 +384,26 @@ int32_t apply_layout(xdem_batch* b) {
         XDEM_HIP(hipMemcpy(A.wa + at, wa.data(), 16ull * n, hipMemcpyHostToDevice));
         XDEM_HIP(hipMemcpy(A.aa + at, aa.data(), 16ull * n, hipMemcpyHostToDevice));
         XDEM_HIP(hipMemcpy(A.zi + at, zi.data(), 8ull * n, hipMemcpyHostToDevice));
+        if (b->hover) {
+            // history slots in storage order (= original order for a new scene)
+            std::vector<uint32_t> hc(n, 0);
+            std::vector<float4> hx((size_t)n * HSLOTS, make_float4(0, 0, 0, 0));
+            if (!s->hcount.empty())
+                for (uint32_t i = 0; i < n; ++i) {
+                    hc[i] = std::min<uint32_t>(s->hcount[i], HSLOTS);
+                    for (uint32_t k = 0; k < hc[i]; ++k) {
+                        size_t q = (size_t)i * HSLOTS + k;
+                        uint32_t key = s->hkey[q];
+                        float kf;
+                        std::memcpy(&kf, &key, 4);
+                        hx[q] = make_float4((float)s->hxi[3 * q], (float)s->hxi[3 * q + 1],
+                                            (float)s->hxi[3 * q + 2], kf);
+                    }
+                }
+            XDEM_HIP(hipMemcpy(b->hcnt[b->cur] + at, hc.data(), 4ull * n, hipMemcpyHostToDevice));
+            XDEM_HIP(hipMemcpy(b->hxi[b->cur] + (size_t)at * HSLOTS, hx.data(), 16ull * HSLOTS * n,
+                               hipMemcpyHostToDevice));
+        }
         XDEM_TRY(fill_u32(b->skey[b->skcur] + at, EMPTY, n, st));
         // the scene's cells start empty
         uint2 e = make_uint2(EMPTY, 0);
@@ -355,6 +416,9 @@ int32_t apply_layout(xdem_batch* b) {
         std::vector<double>().swap(s->omega);
         std::vector<double>().swap(s->acc);
         std::vector<double>().swap(s->alpha);
+        std::vector<uint32_t>().swap(s->hcount);
+        std::vector<uint32_t>().swap(s->hkey);
+        std::vector<double>().swap(s->hxi);
         at += n;
     }
     b->storage_sorted_all_new = (n_old == 0);
@@ -473,6 +537,13 @@ StepArgs step_args(xdem_batch* b, bool dbg) {
     a.blocks = b->d_blocks_sorted;
     a.nblocks = b->nblk_sorted;
     a.n = b->n_total;
+    if (b->hover) {
+        a.hin = b->hxi[b->cur];
+        a.hcin = b->hcnt[b->cur];
+        a.hout = b->hxi[1 - b->cur];
+        a.hcout = b->hcnt[1 - b->cur];
+        a.hoverflow = b->hover;
+    }
     if (dbg) {
         a.dbg_f = b->dbg_f;
         a.dbg_cnt = b->dbg_cnt;
@@ -487,21 +558,31 @@ int32_t launch_step(xdem_batch* b, bool adv) {
     StepArgs a = step_args(b, false);
     XDEM_TRY(b->timer.begin(1, b->st));
     dim3 g(b->nblk_sorted), t(b->block_threads);
-#define XDEM_STEP_LAUNCH(U, P)                                                     \
+#define XDEM_STEP_LAUNCH(U, P, H)                                                  \
     do {                                                                           \
         if (adv)                                                                   \
-            hipLaunchKernelGGL((k_step<true, false, U, P>), g, t, 0, b->st, a);    \
+            hipLaunchKernelGGL((k_step<true, false, U, P, H>), g, t, 0, b->st, a); \
         else                                                                       \
-            hipLaunchKernelGGL((k_step<false, false, U, P>), g, t, 0, b->st, a);   \
+            hipLaunchKernelGGL((k_step<false, false, U, P, H>), g, t, 0, b->st, a);\
     } while (0)
-    const int v = b->unroll * 2 + (b->pipe ? 1 : 0);
-    switch (v) {
-        case 2: XDEM_STEP_LAUNCH(1, false); break;
-        case 3: XDEM_STEP_LAUNCH(1, true); break;
-        case 5: XDEM_STEP_LAUNCH(2, true); break;
-        case 8: XDEM_STEP_LAUNCH(4, false); break;
-        case 9: XDEM_STEP_LAUNCH(4, true); break;
-        default: XDEM_STEP_LAUNCH(2, false); break;
+    if (b->hist_active) {
+        // with history the contact records are not loaded ahead
+        if (b->unroll == 1)
+            XDEM_STEP_LAUNCH(1, false, true);
+        else if (b->unroll == 4)
+            XDEM_STEP_LAUNCH(4, false, true);
+        else
+            XDEM_STEP_LAUNCH(2, false, true);
+    } else {
+        const int v = b->unroll * 2 + (b->pipe ? 1 : 0);
+        switch (v) {
+            case 2: XDEM_STEP_LAUNCH(1, false, false); break;
+            case 3: XDEM_STEP_LAUNCH(1, true, false); break;
+            case 5: XDEM_STEP_LAUNCH(2, true, false); break;
+            case 8: XDEM_STEP_LAUNCH(4, false, false); break;
+            case 9: XDEM_STEP_LAUNCH(4, true, false); break;
+            default: XDEM_STEP_LAUNCH(2, false, false); break;
+        }
     }
 #undef XDEM_STEP_LAUNCH
     XDEM_LAUNCHED();
@@ -578,6 +659,15 @@ int32_t xdem_batch_add_scene(xdem_batch* b, const xdem_batch_scene_desc* d, cons
 int32_t xdem_batch_add_scene_state(xdem_batch* b, const xdem_batch_scene_desc* d,
                                    const double* pos4, const double* vel3, const double* omega3,
                                    const double* acc3, const double* alpha3, uint32_t* scene_id) {
+    return xdem_batch_add_scene_history(b, d, pos4, vel3, omega3, acc3, alpha3, nullptr, nullptr,
+                                        nullptr, scene_id);
+}
+
+int32_t xdem_batch_add_scene_history(xdem_batch* b, const xdem_batch_scene_desc* d,
+                                     const double* pos4, const double* vel3, const double* omega3,
+                                     const double* acc3, const double* alpha3,
+                                     const uint32_t* hist_count, const uint32_t* hist_key,
+                                     const double* hist_xi, uint32_t* scene_id) {
     XDEM_REQUIRE(b && d && (d->np == 0 || pos4), "null argument");
     XDEM_REQUIRE(d->np > 0, "a scene needs particles");
     XDEM_REQUIRE(is_pow2(d->grid_size.x) && is_pow2(d->grid_size.y) && is_pow2(d->grid_size.z),
@@ -586,6 +676,18 @@ int32_t xdem_batch_add_scene_state(xdem_batch* b, const xdem_batch_scene_desc* d
     XDEM_REQUIRE(nc <= (1ull << 24), "a scene grid is limited to 2^24 cells");
     XDEM_REQUIRE(d->cell_size > 0 && d->dt > 0, "cell_size and dt must be > 0");
     XDEM_REQUIRE(d->nplane == 0 || d->planes, "null planes");
+    XDEM_REQUIRE(!hist_count || (d->tangential_history && hist_key && hist_xi),
+                 "history arrays need tangential_history on and all three arrays");
+    if (hist_count)
+        for (uint32_t i = 0; i < d->np; ++i) {
+            XDEM_REQUIRE(hist_count[i] <= XDEM_BATCH_HISTORY_SLOTS, "history count %u > %d",
+                         hist_count[i], XDEM_BATCH_HISTORY_SLOTS);
+            for (uint32_t k = 0; k < hist_count[i]; ++k) {
+                uint32_t key = hist_key[(size_t)i * XDEM_BATCH_HISTORY_SLOTS + k];
+                XDEM_REQUIRE(key & WALL_KEY ? (key & ~WALL_KEY) < d->nplane : key < d->np,
+                             "history key %#x of particle %u out of range", key, i);
+            }
+        }
     uint32_t active_np = 0;
     for (auto& s : b->scenes) active_np += s.n;
     if (b->scenes.size() + 1 > b->cap_scenes || (uint64_t)active_np + d->np > b->cap_np) {
@@ -612,6 +714,12 @@ int32_t xdem_batch_add_scene_state(xdem_batch* b, const xdem_batch_scene_desc* d
     if (omega3) h.omega.assign(omega3, omega3 + 3 * (size_t)d->np);
     if (acc3) h.acc.assign(acc3, acc3 + 3 * (size_t)d->np);
     if (alpha3) h.alpha.assign(alpha3, alpha3 + 3 * (size_t)d->np);
+    if (hist_count) {
+        const size_t ns = (size_t)d->np * XDEM_BATCH_HISTORY_SLOTS;
+        h.hcount.assign(hist_count, hist_count + d->np);
+        h.hkey.assign(hist_key, hist_key + ns);
+        h.hxi.assign(hist_xi, hist_xi + 3 * ns);
+    }
     b->scenes.push_back(std::move(h));
     b->layout_dirty = true;
     b->prepared = false;
@@ -723,6 +831,10 @@ int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* c
     for (int i = 0; i < 3; ++i)
         for (int u = 0; u < 3; ++u)
             for (int pp = 0; pp < 2; ++pp) {
+                if (pp && b->hist_active) {  // not a launch shape with history
+                    if (ms_per_candidate) ms_per_candidate[6 * i + 2 * u + pp] = -1.0;
+                    continue;
+                }
                 b->unroll = unr[u];
                 b->pipe = pp != 0;
                 b->drop_graphs();
@@ -839,6 +951,59 @@ int32_t xdem_batch_get_state(xdem_batch* b, uint32_t scene_id, double* pos4, dou
     return XDEM_OK;
 }
 
+int32_t xdem_batch_get_history(xdem_batch* b, uint32_t scene_id, uint32_t* count, uint32_t* key,
+                               double* xi) {
+    XDEM_REQUIRE(b && count, "null argument");
+    HostScene* s = find_scene(b, scene_id);
+    XDEM_REQUIRE(s, "no scene %u", scene_id);
+    const uint32_t n = s->n;
+    const size_t ns = (size_t)n * HSLOTS;
+    if (s->pending || !b->hover) {  // the staged history (or none)
+        for (uint32_t i = 0; i < n; ++i) count[i] = s->hcount.empty() ? 0 : s->hcount[i];
+        if (key) for (size_t q = 0; q < ns; ++q) key[q] = s->hkey.empty() ? 0 : s->hkey[q];
+        if (xi) for (size_t q = 0; q < 3 * ns; ++q) xi[q] = s->hxi.empty() ? 0.0 : (double)(float)s->hxi[q];
+        return XDEM_OK;
+    }
+    XDEM_HIP(hipStreamSynchronize(b->st));
+    std::vector<uint32_t> hc(n);
+    std::vector<float4> hx(ns);
+    std::vector<uint2> zi(n);
+    XDEM_HIP(hipMemcpy(hc.data(), b->hcnt[b->cur] + s->p0, 4ull * n, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(hx.data(), b->hxi[b->cur] + (size_t)s->p0 * HSLOTS, 16ull * ns,
+                       hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(zi.data(), b->buf[b->cur].zi + s->p0, 8ull * n, hipMemcpyDeviceToHost));
+    for (uint32_t j = 0; j < n; ++j) {
+        uint32_t i = zi[j].y;
+        XDEM_REQUIRE(i < n, "corrupt id %u at %u", i, j);
+        count[i] = std::min<uint32_t>(hc[j], HSLOTS);
+        for (uint32_t k = 0; k < HSLOTS; ++k) {
+            const float4 v = hx[(size_t)j * HSLOTS + k];
+            const size_t q = (size_t)i * HSLOTS + k;
+            bool live = k < count[i];
+            if (key) {
+                uint32_t kb;
+                std::memcpy(&kb, &v.w, 4);
+                key[q] = live ? kb : 0u;
+            }
+            if (xi) {
+                xi[3 * q] = live ? v.x : 0.0;
+                xi[3 * q + 1] = live ? v.y : 0.0;
+                xi[3 * q + 2] = live ? v.z : 0.0;
+            }
+        }
+    }
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_history_overflow(xdem_batch* b, uint32_t* overflow) {
+    XDEM_REQUIRE(b && overflow, "null argument");
+    *overflow = 0;
+    if (!b->hover) return XDEM_OK;
+    XDEM_HIP(hipStreamSynchronize(b->st));
+    XDEM_HIP(hipMemcpy(overflow, b->hover, 4, hipMemcpyDeviceToHost));
+    return XDEM_OK;
+}
+
 int32_t xdem_batch_enable_timing(xdem_batch* b, int32_t on) {
     XDEM_REQUIRE(b, "null batch");
     XDEM_TRY(b->timer.flush());
@@ -908,6 +1073,15 @@ __global__ void k_gather(BState in, BState out, const uint32_t* perm, uint32_t n
     out.aa[j] = in.aa[s];
     out.zi[j] = in.zi[s];
 }
+
+__global__ void k_gather_hist(const float4* hin, const uint32_t* cin, float4* hout, uint32_t* cout,
+                              const uint32_t* perm, uint32_t n) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t s = perm[j];
+    cout[j] = cin[s];
+    for (int k = 0; k < HSLOTS; ++k) hout[(size_t)j * HSLOTS + k] = hin[(size_t)s * HSLOTS + k];
+}
 }  // namespace
 
 extern "C" {
@@ -947,6 +1121,12 @@ int32_t xdem_batch_prepare(xdem_batch* b) {
     hipLaunchKernelGGL(k_gather, dim3(div_up(b->n_total, 256)), dim3(256), 0, b->st, A, B, b->perm,
                        b->n_total);
     XDEM_LAUNCHED();
+    if (b->hover) {
+        hipLaunchKernelGGL(k_gather_hist, dim3(div_up(b->n_total, 256)), dim3(256), 0, b->st,
+                           b->hxi[b->cur], b->hcnt[b->cur], b->hxi[1 - b->cur],
+                           b->hcnt[1 - b->cur], b->perm, b->n_total);
+        XDEM_LAUNCHED();
+    }
     b->cur = 1 - b->cur;
     hipLaunchKernelGGL(k_iota, dim3(div_up(b->n_total, 256)), dim3(256), 0, b->st, b->perm, b->n_total);
     XDEM_LAUNCHED();
@@ -984,8 +1164,8 @@ int32_t xdem_batch_probe(xdem_batch* b, uint32_t scene_id, double* force3, doubl
         b->dbg_cap = mp;
     }
     StepArgs a = step_args(b, true);  // no advance: keynew is not written
-    hipLaunchKernelGGL((k_step<false, true, 2, false>), dim3(b->nblk_sorted), dim3(b->block_threads), 0,
-                       b->st, a);
+    hipLaunchKernelGGL((k_step<false, true, 2, false, false>), dim3(b->nblk_sorted),
+                       dim3(b->block_threads), 0, b->st, a);
     XDEM_LAUNCHED();
     XDEM_HIP(hipStreamSynchronize(b->st));
     uint32_t n = s->n, p0 = s->p0;

@@ -21,6 +21,9 @@ namespace fused {
 
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;      // empty cell / sentinel key
 constexpr int CMAX = 16;                     // contact-list slots per particle (LDS)
+constexpr int HSLOTS = 16;                   // tangential-history slots per particle
+constexpr uint32_t WALL_KEY = 0x80000000u;   // history key of plane p: WALL_KEY | p
+constexpr int LDS_T = 256;                   // lane stride of the per-thread LDS lists
 constexpr int SORT_LDS_MAX = 4096;           // movers sorted in LDS up to this count
 constexpr int ROWS_T = 256;                  // threads of the row kernels
 
@@ -36,7 +39,7 @@ struct BScene {
     float mass_c;             // m = mass_c r^3 (rho 4/3 pi)
     int32_t rolling;
     uint32_t ncell;
-    uint32_t pad;
+    int32_t history;          // tangential-spring history on (DHS with a spring per contact)
 };
 
 struct BPlane {
@@ -70,6 +73,13 @@ struct StepArgs {
     const BlockEntry* blocks;
     uint32_t nblocks;
     uint32_t n;                // particles in the arrays (bounds of perm entries)
+    // tangential-spring history (per storage index of `in`, per sorted
+    // position of `out`): HSLOTS slots of (xi, key bits) and a count
+    const float4* hin;
+    const uint32_t* hcin;
+    float4* hout;
+    uint32_t* hcout;
+    uint32_t* hoverflow;       // contacts found with every slot taken (not kept)
     // parity probe outputs (per sorted position) — null in production
     float* dbg_f;              // 6 floats: force xyz, torque xyz
     uint32_t* dbg_cnt;
@@ -137,6 +147,61 @@ __device__ inline void dhs_f32(float kn, float vn, float ks, float vs, float mu,
     }
 }
 
+// DHS with a tangential spring (no reference counterpart; the reference's
+// tangential force is memoryless).  xi is in/out: projected onto the current
+// tangent plane, advanced by dt e, Ft = ks xi + vs e; when |Ft| > mu |Fn| the
+// force is capped and the spring reset to (Ft - vs e) / ks.  With xi = 0 on
+// entry this is dhs_f32's force whenever |e| != 0.
+__device__ inline void hist_f32(float kn, float vn, float ks, float vs, float mu, float dt, float ir,
+                                float delta, f3 dv, f3 u, f3& xi, f3& Ft, f3& Fn, f3& M) {
+    float fsn = -kn * (delta * sqrtf(delta));
+    float fdn = vn * dotf(dv, u);
+    Fn = (fsn + fdn) * u;
+    f3 e = dv - dotf(dv, u) * u;
+    xi = xi - dotf(xi, u) * u;
+    xi = xi + dt * e;
+    Ft = ks * xi + vs * e;
+    float ft = sqrtf(dotf(Ft, Ft)), fmax = mu * sqrtf(dotf(Fn, Fn));
+    if (ft > fmax) {
+        Ft = (fmax / ft) * Ft;
+        xi = ks > 0.0f ? (1.0f / ks) * (Ft - vs * e) : F3(0, 0, 0);
+    }
+    M = crossf(ir * u, Ft);
+}
+
+// One particle's history: the previous slots (keys staged in LDS, lane
+// stride LDS_T) and the slots written this step, in evaluation order.
+struct HistCtx {
+    bool on;
+    uint32_t* keys;
+    const float4* in;
+    float4* out;
+    uint32_t* overflow;
+    uint32_t nh, nout, hint;
+    __device__ f3 lookup(uint32_t key) {
+        // contacts come in nearly the same order every step: search from the
+        // slot after the last match
+        for (uint32_t q = 0; q < nh; ++q) {
+            uint32_t k = hint + q;
+            if (k >= nh) k -= nh;
+            if (keys[k * LDS_T] == key) {
+                hint = k + 1;
+                const float4 v = in[k];
+                return F3(v.x, v.y, v.z);
+            }
+        }
+        return F3(0, 0, 0);
+    }
+    __device__ void store(uint32_t key, f3 xi) {
+        if (nout < (uint32_t)HSLOTS) {
+            out[nout] = make_float4(xi.x, xi.y, xi.z, __uint_as_float(key));
+            ++nout;
+        } else {
+            atomicAdd(overflow, 1u);
+        }
+    }
+};
+
 struct Own {
     double x, y, z;
     f3 v, w;
@@ -145,9 +210,11 @@ struct Own {
 
 // Particle-particle contact of `o` with the particle stored at t of `in`
 // (the predicate was already found true).  Accumulates into sumF/sumM.
+template <bool HIST>
 __device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own& o,
                                   const double4& jp, const float4& jvm, const float4& jwa,
-                                  f3& Ft, f3& Fn, f3& M, f3& sumF, f3& sumM) {
+                                  uint32_t jid, HistCtx& hc, f3& Ft, f3& Fn, f3& M, f3& sumF,
+                                  f3& sumM) {
     double dx = jp.x - o.x, dy = jp.y - o.y, dz = jp.z - o.z;
     double d2 = dx * dx + dy * dy + dz * dz;
     double dist = sqrt(d2);
@@ -161,40 +228,54 @@ __device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own&
     float Ms = (o.m * jvm.w) / (o.m + jvm.w);
     float kn = S.ckn * sqrtf(R);
     float vn = sqrtf(S.cvn * Ms * kn);
+    if (HIST && hc.on) {
+        f3 xi = hc.lookup(jid), cFt, cFn, cM;
+        hist_f32(kn, vn, kn * S.sr, vn * S.sr, S.mu, S.dtf, o.r, (float)cdist, dv, u, xi, cFt, cFn,
+                 cM);
+        hc.store(jid, xi);
+        sumF += cFn + cFt;
+        sumM += cM;
+        return;
+    }
     dhs_f32(kn, vn, kn * S.sr, vn * S.sr, S.mu, S.dtf, o.r, (float)cdist, dv, u, Ft, Fn, M);
     sumF += Fn + Ft;
     sumM += M;
 }
 
-// The contacts of a list, in order.  PIPE: the records of contact c+1 are
-// loaded before contact c is evaluated (more loads in flight, more registers).
-template <bool PIPE>
+// The contacts of a list (lane stride LDS_T), in order.  PIPE: the records of
+// contact c+1 are loaded before contact c is evaluated (more loads in flight,
+// more registers).  HIST: the partner ids are loaded for the history keys.
+template <bool PIPE, bool HIST>
 __device__ inline void pp_contacts(const StepArgs& A, const BScene& S, const Own& o,
-                                   const uint32_t* list, uint32_t cnt, f3& Ft, f3& Fn, f3& M,
-                                   f3& sumF, f3& sumM) {
+                                   const uint32_t* list, uint32_t cnt, HistCtx& hc, f3& Ft,
+                                   f3& Fn, f3& M, f3& sumF, f3& sumM) {
     if (cnt == 0) return;
     uint32_t t = list[0];
     double4 jp = A.in.pr[t];
     float4 jvm = A.in.vm[t], jwa = A.in.wa[t];
+    uint32_t jid = HIST ? A.in.zi[t].y : 0u;
     for (uint32_t c = 0; c < cnt; ++c) {
         if (PIPE) {
             const double4 cp = jp;
             const float4 cvm = jvm, cwa = jwa;
+            const uint32_t cid = jid;
             if (c + 1 < cnt) {
-                t = list[c + 1];
+                t = list[(c + 1) * LDS_T];
                 jp = A.in.pr[t];
                 jvm = A.in.vm[t];
                 jwa = A.in.wa[t];
+                if (HIST) jid = A.in.zi[t].y;
             }
-            pp_contact(A, S, o, cp, cvm, cwa, Ft, Fn, M, sumF, sumM);
+            pp_contact<HIST>(A, S, o, cp, cvm, cwa, cid, hc, Ft, Fn, M, sumF, sumM);
         } else {
             if (c) {
-                t = list[c];
+                t = list[c * LDS_T];
                 jp = A.in.pr[t];
                 jvm = A.in.vm[t];
                 jwa = A.in.wa[t];
+                if (HIST) jid = A.in.zi[t].y;
             }
-            pp_contact(A, S, o, jp, jvm, jwa, Ft, Fn, M, sumF, sumM);
+            pp_contact<HIST>(A, S, o, jp, jvm, jwa, jid, hc, Ft, Fn, M, sumF, sumM);
         }
     }
 }
@@ -202,16 +283,20 @@ __device__ inline void pp_contacts(const StepArgs& A, const BScene& S, const Own
 // UNR: candidates whose loads are issued together in the neighbour walk;
 // PIPE: contact records loaded one contact ahead.  Both trade registers
 // (waves per SIMD) for loads in flight per wave; the autotuner picks them
-// together with the block size.
-template <bool ADV, bool DBG, int UNR, bool PIPE>
+// together with the block size.  HIST: scenes with history on use the
+// tangential-spring law (others the reference DHS law, same arithmetic as
+// without HIST).
+template <bool ADV, bool DBG, int UNR, bool PIPE, bool HIST>
 __global__ __launch_bounds__(256) void k_step(StepArgs A) {
-    __shared__ uint32_t clist[256 * CMAX];
+    // per-thread LDS lists, lane-contiguous (entry c of thread t at c * LDS_T + t)
+    __shared__ uint32_t clist[LDS_T * CMAX];
+    __shared__ uint32_t hkeys[HIST ? LDS_T * HSLOTS : 1];
     const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
     const BlockEntry be = A.blocks[L];
     const BScene S = A.scenes[be.scene];
     const uint32_t j = be.j0 + threadIdx.x;
     if (j >= be.j1 || j >= A.n) return;
-    uint32_t* mylist = clist + threadIdx.x * CMAX;
+    uint32_t* mylist = clist + threadIdx.x;
     // own records: loaded at storage index j together with perm[j] (equal
     // unless a mover shifted this sorted position), reloaded otherwise
     const uint32_t s = A.perm[j];
@@ -225,6 +310,16 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
         wa = A.in.wa[s];
         aa = A.in.aa[s];
         zi = A.in.zi[s];
+    }
+    HistCtx hc{};
+    if (HIST) {
+        hc.on = S.history != 0;
+        hc.keys = hkeys + threadIdx.x;
+        hc.in = A.hin + (size_t)s * HSLOTS;
+        hc.out = A.hout + (size_t)j * HSLOTS;
+        hc.overflow = A.hoverflow;
+        hc.nh = hc.on ? min(A.hcin[s], (uint32_t)HSLOTS) : 0u;
+        for (uint32_t k = 0; k < hc.nh; ++k) hc.keys[k * LDS_T] = __float_as_uint(hc.in[k].w);
     }
     Own o;
     o.x = ip.x;
@@ -253,11 +348,11 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
         if (!(Sr - sqrt(d2) > 0)) return;
         if (DBG && ntotal < A.dbg_max) A.dbg_part[(size_t)j * A.dbg_max + ntotal] = A.in.zi[t].y;
         ++ntotal;
-        mylist[nlist++] = t;
+        mylist[(nlist++) * LDS_T] = t;
     };
     auto flush_if = [&](uint32_t room) {
         if (nlist + room > CMAX) {
-            pp_contacts<PIPE>(A, S, o, mylist, nlist, Ft, Fn, M, sumF, sumM);
+            pp_contacts<PIPE, HIST>(A, S, o, mylist, nlist, hc, Ft, Fn, M, sumF, sumM);
             nlist = 0;
         }
     };
@@ -320,7 +415,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
                 }
     }
     // phase 2: the listed contacts, in order
-    pp_contacts<PIPE>(A, S, o, mylist, nlist, Ft, Fn, M, sumF, sumM);
+    pp_contacts<PIPE, HIST>(A, S, o, mylist, nlist, hc, Ft, Fn, M, sumF, sumM);
     // F = m g + particle-particle (vv_update_velocity reset, then cu_calculate_p2p)
     f3 F = F3(o.m * S.gvx, o.m * S.gvy, o.m * S.gvz) + sumF;
     f3 T = sumM;
@@ -338,8 +433,15 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
             float kn = bp.ckn * sqrtf(o.r);
             float vn = sqrtf(bp.cvn * o.m * kn);
             f3 pFt = F3(0, 0, 0), pFn = F3(0, 0, 0), pM = F3(0, 0, 0);
-            dhs_f32(kn, vn, kn * bp.sr, vn * bp.sr, bp.mu, S.dtf, 0.0f, (float)cdist, dv, u, pFt,
-                    pFn, pM);
+            if (HIST && hc.on) {
+                f3 xi = hc.lookup(WALL_KEY | p);
+                hist_f32(kn, vn, kn * bp.sr, vn * bp.sr, bp.mu, S.dtf, 0.0f, (float)cdist, dv, u,
+                         xi, pFt, pFn, pM);
+                hc.store(WALL_KEY | p, xi);
+            } else {
+                dhs_f32(kn, vn, kn * bp.sr, vn * bp.sr, bp.mu, S.dtf, 0.0f, (float)cdist, dv, u,
+                        pFt, pFn, pM);
+            }
             F += pFn + pFt;
         }
     }
@@ -376,6 +478,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
     A.out.wa[j] = make_float4(w.x, w.y, w.z, al.x);
     A.out.aa[j] = make_float4(a.x, a.y, a.z, al.y);
     A.out.zi[j] = make_uint2(__float_as_uint(al.z), zi.y);
+    if (HIST) A.hcout[j] = hc.nout;
 }
 
 // Verlet position update (optional) + cell key of every particle, in place,

@@ -194,6 +194,7 @@ class Scene:
     rolling: bool = True
     tcm: int = 1
     seed: int = 0
+    tangential_history: bool = False   # fused path: DHS with a tangential spring per contact
 
     @property
     def np(self) -> int:

@@ -34,6 +34,35 @@ class OrcWorld(C.Structure):
                                                               "cend")]
 
 
+class OrcHist(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("pad_", C.c_uint32), ("count", C.c_void_p),
+                ("key", C.c_void_p), ("xi", C.c_void_p), ("overflow", C.c_uint64)]
+
+
+ORC_WALL_KEY = 0x80000000
+HIST_SLOTS = 16
+
+
+class History:
+    """Per-particle tangential-spring history (host arrays): count[np],
+    key[np, slots], xi[np, slots, 3]."""
+
+    def __init__(self, n, slots=HIST_SLOTS):
+        self.count = np.zeros(n, np.uint32)
+        self.key = np.zeros((n, slots), np.uint32)
+        self.xi = np.zeros((n, slots, 3))
+        self.s = OrcHist(slots, 0, self.count.ctypes.data, self.key.ctypes.data,
+                         self.xi.ctypes.data, 0)
+
+    @property
+    def overflow(self):
+        return int(self.s.overflow)
+
+    def entries(self, i):
+        c = int(self.count[i])
+        return self.key[i, :c].copy(), self.xi[i, :c].copy()
+
+
 _lib = None
 
 
@@ -74,6 +103,9 @@ def _declare(L):
         "orc_circumcenter": ([vp, vp, vp, vp], None),
         "orc_step": ([vp, u32], None),
         "orc_step_omp": ([vp, u32, i32], None),
+        "orc_hist_contacts": ([vp, vp, vp, vp, vp], None),
+        "orc_detect": ([vp], None),
+        "orc_hist_step": ([vp, vp, vp], None),
     }
     for n, (a, r) in sig.items():
         f = getattr(L, n)
@@ -294,3 +326,17 @@ class World:
             load().orc_step_omp(byref(self.w), nsteps, threads)
         else:
             load().orc_step(byref(self.w), nsteps)
+
+    def detect(self):
+        load().orc_detect(byref(self.w))
+
+    def hist_contacts(self, hin, hout):
+        """Contact forces/moments of the current state with history (gravity-free)."""
+        n = self.scene.np
+        f = np.zeros((n, 3))
+        m = np.zeros((n, 3))
+        load().orc_hist_contacts(byref(self.w), byref(hin.s), byref(hout.s), P(f), P(m))
+        return f, m
+
+    def hist_step(self, hin, hout):
+        load().orc_hist_step(byref(self.w), byref(hin.s), byref(hout.s))

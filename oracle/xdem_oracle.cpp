@@ -1045,3 +1045,161 @@ void orc_step_omp(orc_world* w, uint32_t nsteps, int32_t nthreads) {
 }
 
 }  // extern "C"
+
+namespace {
+
+// Tangential-spring DHS (no reference counterpart; the law is stated in
+// xdem_oracle.h).  xi is in/out.
+void hist_law(const xdem_device_force_constant& c, double dt, double ir, double cdist, V3 dv,
+              V3 unit, V3& xi, V3& Ft, V3& Fn, V3& M) {
+    double fsn = -c.kn * std::pow(cdist, 1.5);
+    double fdn = c.vn * dot(dv, unit);
+    Fn = (fsn + fdn) * unit;
+    V3 e = dv - dot(dv, unit) * unit;
+    xi = xi - dot(xi, unit) * unit;
+    xi = xi + dt * e;
+    Ft = c.ks * xi + c.vs * e;
+    double ft = length(Ft), fmax = c.mu * length(Fn);
+    if (ft > fmax) {
+        // sliding: Ft at the Coulomb cap, the spring reset to the value that
+        // reproduces it (ks xi + vs e = Ft)
+        Ft = (fmax / ft) * Ft;
+        xi = c.ks > 0 ? (1.0 / c.ks) * (Ft - c.vs * e) : mk(0, 0, 0);
+    }
+    M = cross(ir * unit, Ft);
+}
+
+struct HistCursor {
+    const orc_hist* in;
+    orc_hist* out;
+    uint32_t id;
+    uint32_t nout = 0;
+    V3 lookup(uint32_t key) const {
+        uint32_t H = in->slots, n = in->count[id];
+        for (uint32_t k = 0; k < n; ++k)
+            if (in->key[(size_t)id * H + k] == key) {
+                const double* x = in->xi + 3 * ((size_t)id * H + k);
+                return mk(x[0], x[1], x[2]);
+            }
+        return mk(0, 0, 0);
+    }
+    void store(uint32_t key, V3 xi, uint64_t* overflow) {
+        uint32_t H = out->slots;
+        if (nout < H) {
+            out->key[(size_t)id * H + nout] = key;
+            st3(out->xi + 3 * ((size_t)id * H + nout), xi);
+            ++nout;
+        } else {
+            ++*overflow;
+        }
+    }
+};
+
+// Contacts of particle id with history.  F and Mo enter with the values the
+// contacts are added to (the kernels' order: F += particle sum, then F += each
+// plane's force).
+void hist_one(const orc_world* w, const orc_hist* in, orc_hist* out, uint32_t id, V3& F, V3& Mo,
+              uint64_t* overflow) {
+    const xdem_device_parameters* p = &w->prm;
+    const double* pos4 = w->pos;
+    V3 ipos = ld4(pos4 + 4 * id);
+    double ir = pos4[4 * id + 3];
+    V3 ivel = ld3(w->vel + 3 * id), iomega = ld3(w->omega + 3 * id);
+    double im = w->mass[id];
+    int32_t g[3];
+    orc_cell_of(p, ORC_GPU, ipos.x, ipos.y, ipos.z, g);
+    HistCursor hc{in, out, id};
+    V3 sumF = mk(0, 0, 0), sumM = mk(0, 0, 0);
+    // particle contacts: calculate_p2p_kernel traversal (mphysics_cuda_impl.cuh:430-482)
+    for (int z = -1; z <= 1; z++)
+        for (int y = -1; y <= 1; y++)
+            for (int x = -1; x <= 1; x++) {
+                int32_t nb[3] = {g[0] + x, g[1] + y, g[2] + z};
+                uint32_t h = orc_hash_of(p, nb);
+                uint32_t s = w->cstart[h];
+                if (s == 0xFFFFFFFFu) continue;
+                uint32_t e = w->cend[h];
+                for (uint32_t j = s; j < e; ++j) {
+                    uint32_t k = w->sorted[j];
+                    if (id == k || k >= w->np) continue;
+                    V3 jpos = ld4(pos4 + 4 * k);
+                    double jr = pos4[4 * k + 3];
+                    V3 rp = mk(jpos.x - ipos.x, jpos.y - ipos.y, jpos.z - ipos.z);
+                    double dist = length(rp);
+                    double cdist = (ir + jr) - dist;
+                    if (!(cdist > 0)) continue;
+                    V3 unit = rp / dist;
+                    V3 jvel = ld3(w->vel + 3 * k), jomega = ld3(w->omega + 3 * k);
+                    V3 rv = p->rollingCondition
+                                ? jvel + cross(jomega, -jr * unit) - (ivel + cross(iomega, ir * unit))
+                                : (jvel - ivel);
+                    xdem_device_force_constant c = force_constant(1, ir, jr, im, w->mass[k], w->cp_pp);
+                    V3 xi = hc.lookup(k), Ft, Fn, M;
+                    hist_law(c, p->dt, ir, cdist, rv, unit, xi, Ft, Fn, M);
+                    hc.store(k, xi, overflow);
+                    sumF += Fn + Ft;
+                    sumM += M;
+                }
+            }
+    F += sumF;
+    Mo += sumM;
+    // planes in application order (plane_contact_force_kernel, :573-621; ir = 0)
+    for (uint32_t q = 0; q < w->nwall; ++q) {
+        const orc_wall& wl = w->walls[q];
+        if (wl.kind != ORC_WALL_PLANE) continue;
+        const xdem_device_plane_info* pl = &w->planes[wl.index];
+        V3 dp = ipos - of(pl->xw);
+        double unit[3] = {0, 0, 0};
+        double wp[3] = {dot(dp, of(pl->u1)), dot(dp, of(pl->u2)), dot(dp, of(pl->uw))};
+        double xp[3] = {ipos.x, ipos.y, ipos.z};
+        double cdist = orc_plane_detect(pl, ORC_GPU, xp, wp, unit, ir);
+        if (!(cdist > 0)) continue;
+        V3 un = ld3(unit);
+        V3 dv = p->rollingCondition ? -(ivel + cross(iomega, ir * un)) : -ivel;
+        xdem_device_force_constant c = force_constant(1, ir, 0.0, im, 0.0, wl.cp);
+        uint32_t key = ORC_WALL_KEY | q;
+        V3 xi = hc.lookup(key), Ft, Fn, M;
+        hist_law(c, p->dt, 0.0, cdist, dv, un, xi, Ft, Fn, M);
+        hc.store(key, xi, overflow);
+        F += Fn + Ft;
+    }
+    out->count[id] = hc.nout;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_detect(orc_world* w) { detection(w); }
+
+void orc_hist_contacts(const orc_world* w, const orc_hist* in, orc_hist* out, double* force,
+                       double* moment) {
+    uint64_t ov = 0;
+    for (uint32_t id = 0; id < w->np; ++id) {
+        V3 F = mk(0, 0, 0), M = mk(0, 0, 0);
+        hist_one(w, in, out, id, F, M, &ov);
+        st3(force + 3 * id, F);
+        st3(moment + 3 * id, M);
+    }
+    out->overflow = ov;
+}
+
+void orc_hist_step(orc_world* w, const orc_hist* in, orc_hist* out) {
+    const xdem_device_parameters* p = &w->prm;
+    uint32_t np = w->np;
+    orc_vv_position(p, ORC_GPU, w->pos, w->vel, w->acc, np);
+    detection(w);
+    uint64_t ov = 0;
+    for (uint32_t id = 0; id < np; ++id) {
+        // force holds m g (the previous velocity update's reset, :128)
+        V3 F = ld3(w->force + 3 * id), M = ld3(w->moment + 3 * id);
+        hist_one(w, in, out, id, F, M, &ov);
+        st3(w->force + 3 * id, F);
+        st3(w->moment + 3 * id, M);
+    }
+    out->overflow = ov;
+    orc_vv_velocity(p, ORC_GPU, w->vel, w->acc, w->omega, w->alpha, w->force, w->moment, w->mass,
+                    w->iner, np);
+}
+
+}  // extern "C"

@@ -138,6 +138,39 @@ void orc_step(orc_world* w, uint32_t nsteps);
  * CPU baseline; identical results (each particle is independent). */
 void orc_step_omp(orc_world* w, uint32_t nsteps, int32_t nthreads);
 
+/* Tangential-spring history (DHS with a per-contact tangential spring).  No
+ * counterpart in the reference, whose DHS tangential force is memoryless
+ * (mphysics_cuda_impl.cuh:379-401, contact.cpp:20-46).  Per particle, up to
+ * `slots` contacts in evaluation order (particle contacts in the traversal
+ * order of calculate_p2p_kernel, then planes in application order); key =
+ * partner particle id, or ORC_WALL_KEY | wall index; xi = spring (3 doubles).
+ * Law (own definition): Fn as DHS; e = dv - (dv.n)n; xi <- xi - (xi.n)n + dt e;
+ * Ft = ks xi + vs e; if |Ft| > mu |Fn| then Ft <- mu |Fn| Ft / |Ft| and
+ * xi <- (Ft - vs e) / ks (the spring that reproduces the capped force);
+ * M = (ri n) x Ft (walls: ri = 0).  With xi = 0 on entry this
+ * is the reference DHS tangential force whenever |e| != 0. */
+#define ORC_WALL_KEY 0x80000000u
+typedef struct orc_hist {
+    uint32_t slots;
+    uint32_t pad_;
+    uint32_t* count;   /* np */
+    uint32_t* key;     /* np * slots */
+    double* xi;        /* np * slots * 3 */
+    uint64_t overflow; /* contacts found with all slots taken (not kept) */
+} orc_hist;
+
+/* Contacts of the current state with history: force = gravity-free sum of
+ * particle and plane contacts, moment likewise (both overwritten), `out`
+ * receives the new history.  Planes only among the walls (TCM 1, GPU
+ * semantics).  The grid (w->sorted/cstart/cend) must be current. */
+void orc_hist_contacts(const orc_world* w, const orc_hist* in, orc_hist* out, double* force,
+                       double* moment);
+/* Rebuild w's grid from its positions (hash, sort, cell ranges). */
+void orc_detect(orc_world* w);
+/* One step with history (GPU semantics, TCM 1): Verlet position, grid,
+ * contacts with history (in -> out), Verlet velocity with F = m g + contacts. */
+void orc_hist_step(orc_world* w, const orc_hist* in, orc_hist* out);
+
 #ifdef __cplusplus
 }
 #endif

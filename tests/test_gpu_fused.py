@@ -16,28 +16,13 @@ import pytest
 
 from metakonga_amd import scenes
 from metakonga_amd.batch import Batch
-from xdem_testutil import dense_state
+from xdem_testutil import dense_scene
 
 pytestmark = pytest.mark.gpu
 
 MAT = scenes.MaterialSpec()
 CP = scenes.contact_property(MAT, MAT, scenes.ContactSpec())
 FP32C_TOL = 2e-5
-
-
-def dense_scene(n_side=12, seed=11, pitch=1.97, jitter=0.04, vel_scale=0.05, omega_scale=5.0):
-    pos, vel, om = dense_state(n_side=n_side, seed=seed, pitch_factor=pitch, jitter=jitter,
-                               vel_scale=vel_scale, omega_scale=omega_scale)
-    r = pos[0, 3]
-    lo = pos[:, :3].min(0) - r
-    hi = pos[:, :3].max(0) + r
-    g = 1
-    while g * 2 * r < (hi - lo).max() + 8 * r:
-        g *= 2
-    sc = scenes.Scene(name="dense", pos=pos, dt=1e-5, nsteps=10, world_origin=tuple(lo - 2 * r),
-                      cell_size=2 * r, grid_size=(g, g, g),
-                      planes=scenes.open_box_planes(lo + 0.3 * r, hi + [0, 0.05, 0]))
-    return sc, vel, om
 
 
 def oracle_reference(orc, sc, vel, om):

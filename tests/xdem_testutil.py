@@ -76,3 +76,20 @@ def force_error(f_test, f_ref, f_scale):
     settled particle cancels)."""
     d = np.abs(np.asarray(f_test) - np.asarray(f_ref)).max(axis=1)
     return float((d / np.maximum(f_scale, 1e-300)).max()) if d.size else 0.0
+
+
+def dense_scene(n_side=12, seed=11, pitch=1.97, jitter=0.04, vel_scale=0.05, omega_scale=5.0):
+    """A contact-rich dense scene in an open box of planes, with its random
+    velocities and spins (FP32-representable)."""
+    pos, vel, om = dense_state(n_side=n_side, seed=seed, pitch_factor=pitch, jitter=jitter,
+                               vel_scale=vel_scale, omega_scale=omega_scale)
+    r = pos[0, 3]
+    lo = pos[:, :3].min(0) - r
+    hi = pos[:, :3].max(0) + r
+    g = 1
+    while g * 2 * r < (hi - lo).max() + 8 * r:
+        g *= 2
+    sc = scenes.Scene(name="dense", pos=pos, dt=1e-5, nsteps=10, world_origin=tuple(lo - 2 * r),
+                      cell_size=2 * r, grid_size=(g, g, g),
+                      planes=scenes.open_box_planes(lo + 0.3 * r, hi + [0, 0.05, 0]))
+    return sc, vel, om
