@@ -94,6 +94,11 @@ int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps);
 int32_t xdem_batch_set_graphs(xdem_batch* b, int32_t on);
 /* Block size (64, 128, 192 or 256) of the contact kernel; 0 = default (256). */
 int32_t xdem_batch_set_block(xdem_batch* b, uint32_t threads);
+/* The contact kernel's launch shape without autotuning: block size (as
+ * xdem_batch_set_block), neighbour candidates loaded together (1, 2 or 4) and
+ * contact records loaded one contact ahead (0/1; ignored while a scene has
+ * tangential history).  Results do not depend on it. */
+int32_t xdem_batch_set_launch(xdem_batch* b, uint32_t threads, uint32_t unroll, int32_t load_ahead);
 /* Autotuner: times the contact kernel for block sizes 64, 128 and 256, each
  * with 1, 2 or 4 neighbour candidates loaded together and with the contact
  * records loaded one contact ahead or not, over steps_per_trial real steps

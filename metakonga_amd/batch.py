@@ -43,6 +43,7 @@ def declare(lib):
         "xdem_batch_step": ([vp, u32], i32),
         "xdem_batch_set_graphs": ([vp, i32], i32),
         "xdem_batch_set_block": ([vp, u32], i32),
+        "xdem_batch_set_launch": ([vp, u32, u32, i32], i32),
         "xdem_batch_synchronize": ([vp], i32),
         "xdem_batch_autotune": ([vp, u32, C.POINTER(u32), vp], i32),
         "xdem_batch_get_state": ([vp, u32, vp, vp, vp, vp, vp], i32),
@@ -138,6 +139,10 @@ class Batch:
 
     def block(self, threads: int):
         abi.check(self.lib, self.lib.xdem_batch_set_block(self.h, int(threads)))
+
+    def launch(self, threads=256, unroll=2, load_ahead=False):
+        abi.check(self.lib, self.lib.xdem_batch_set_launch(self.h, int(threads), int(unroll),
+                                                           int(bool(load_ahead))))
 
     def autotune(self, steps_per_trial=10):
         """Pick the contact-kernel launch shape by timing (advances the scenes)."""

@@ -758,6 +758,19 @@ int32_t xdem_batch_set_block(xdem_batch* b, uint32_t threads) {
     return XDEM_OK;
 }
 
+int32_t xdem_batch_set_launch(xdem_batch* b, uint32_t threads, uint32_t unroll,
+                              int32_t load_ahead) {
+    XDEM_REQUIRE(b, "null batch");
+    XDEM_REQUIRE(unroll == 1 || unroll == 2 || unroll == 4, "unroll must be 1, 2 or 4");
+    XDEM_TRY(xdem_batch_set_block(b, threads));
+    if ((int)unroll != b->unroll || (load_ahead != 0) != b->pipe) {
+        b->unroll = (int)unroll;
+        b->pipe = load_ahead != 0;
+        b->drop_graphs();
+    }
+    return XDEM_OK;
+}
+
 int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps) {
     XDEM_REQUIRE(b, "null batch");
     if (nsteps == 0) return XDEM_OK;

@@ -127,21 +127,27 @@ __device__ inline f3 crossf(f3 a, f3 b) {
     return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
+// FP32 reciprocal and square root of the force law: the native instructions
+// (v_rcp_f32, v_sqrt_f32; about 1 ulp) instead of the correctly rounded
+// expansions — far inside the FP32c force tolerance, a fraction of the cost.
+__device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // DHS force law in FP32 (mphysics_cuda_impl.cuh:379-401).  Ft and M keep the
 // caller's values when the tangential velocity is exactly zero (as the
 // reference kernel, whose Ft/M live outside the neighbour loop).
 __device__ inline void dhs_f32(float kn, float vn, float ks, float vs, float mu, float dt,
                                float ir, float delta, f3 dv, f3 u, f3& Ft, f3& Fn, f3& M) {
-    float fsn = -kn * (delta * sqrtf(delta));
+    float fsn = -kn * (delta * fsqrt(delta));
     float fdn = vn * dotf(dv, u);
     Fn = (fsn + fdn) * u;
     f3 e = dv - dotf(dv, u) * u;
-    float mag_e = sqrtf(dotf(e, e));
+    float mag_e = fsqrt(dotf(e, e));
     if (mag_e != 0.0f) {
-        float inv = 1.0f / mag_e;
+        float inv = frcp(mag_e);
         f3 sh = inv * e;
         float ds = mag_e * dt;
-        float fn_len = sqrtf(dotf(Fn, Fn));
+        float fn_len = fsqrt(dotf(Fn, Fn));
         Ft = fminf(ks * ds + vs * dotf(dv, sh), mu * fn_len) * sh;
         M = crossf(ir * u, Ft);
     }
@@ -154,17 +160,17 @@ __device__ inline void dhs_f32(float kn, float vn, float ks, float vs, float mu,
 // entry this is dhs_f32's force whenever |e| != 0.
 __device__ inline void hist_f32(float kn, float vn, float ks, float vs, float mu, float dt, float ir,
                                 float delta, f3 dv, f3 u, f3& xi, f3& Ft, f3& Fn, f3& M) {
-    float fsn = -kn * (delta * sqrtf(delta));
+    float fsn = -kn * (delta * fsqrt(delta));
     float fdn = vn * dotf(dv, u);
     Fn = (fsn + fdn) * u;
     f3 e = dv - dotf(dv, u) * u;
     xi = xi - dotf(xi, u) * u;
     xi = xi + dt * e;
     Ft = ks * xi + vs * e;
-    float ft = sqrtf(dotf(Ft, Ft)), fmax = mu * sqrtf(dotf(Fn, Fn));
+    float ft = fsqrt(dotf(Ft, Ft)), fmax = mu * fsqrt(dotf(Fn, Fn));
     if (ft > fmax) {
-        Ft = (fmax / ft) * Ft;
-        xi = ks > 0.0f ? (1.0f / ks) * (Ft - vs * e) : F3(0, 0, 0);
+        Ft = (fmax * frcp(ft)) * Ft;
+        xi = ks > 0.0f ? frcp(ks) * (Ft - vs * e) : F3(0, 0, 0);
     }
     M = crossf(ir * u, Ft);
 }
@@ -217,27 +223,30 @@ __device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own&
                                   f3& sumM) {
     double dx = jp.x - o.x, dy = jp.y - o.y, dz = jp.z - o.z;
     double d2 = dx * dx + dy * dy + dz * dz;
-    double dist = sqrt(d2);
-    double cdist = ((double)o.r + jp.w) - dist;
-    float inv = 1.0f / (float)dist;
+    // overlap (ri + rj) - |d| = ((ri + rj)^2 - |d|^2) / ((ri + rj) + |d|): the
+    // cancelling difference in FP64, the rest in FP32 (no FP64 square root)
+    double Sr = (double)o.r + jp.w;
+    float dist = fsqrt((float)d2);
+    float inv = frcp(dist);
+    float cdist = (float)(Sr * Sr - d2) * frcp((float)Sr + dist);
     f3 u = F3((float)dx * inv, (float)dy * inv, (float)dz * inv);
     float jr = (float)jp.w;
     f3 jv = F3(jvm.x, jvm.y, jvm.z), jw = F3(jwa.x, jwa.y, jwa.z);
     f3 dv = S.rolling ? (jv + crossf(jw, (-jr) * u)) - (o.v + crossf(o.w, o.r * u)) : (jv - o.v);
-    float R = (o.r * jr) / (o.r + jr);
-    float Ms = (o.m * jvm.w) / (o.m + jvm.w);
-    float kn = S.ckn * sqrtf(R);
-    float vn = sqrtf(S.cvn * Ms * kn);
+    float R = (o.r * jr) * frcp(o.r + jr);
+    float Ms = (o.m * jvm.w) * frcp(o.m + jvm.w);
+    float kn = S.ckn * fsqrt(R);
+    float vn = fsqrt(S.cvn * Ms * kn);
     if (HIST && hc.on) {
         f3 xi = hc.lookup(jid), cFt, cFn, cM;
-        hist_f32(kn, vn, kn * S.sr, vn * S.sr, S.mu, S.dtf, o.r, (float)cdist, dv, u, xi, cFt, cFn,
+        hist_f32(kn, vn, kn * S.sr, vn * S.sr, S.mu, S.dtf, o.r, cdist, dv, u, xi, cFt, cFn,
                  cM);
         hc.store(jid, xi);
         sumF += cFn + cFt;
         sumM += cM;
         return;
     }
-    dhs_f32(kn, vn, kn * S.sr, vn * S.sr, S.mu, S.dtf, o.r, (float)cdist, dv, u, Ft, Fn, M);
+    dhs_f32(kn, vn, kn * S.sr, vn * S.sr, S.mu, S.dtf, o.r, cdist, dv, u, Ft, Fn, M);
     sumF += Fn + Ft;
     sumM += M;
 }
@@ -297,20 +306,11 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
     const uint32_t j = be.j0 + threadIdx.x;
     if (j >= be.j1 || j >= A.n) return;
     uint32_t* mylist = clist + threadIdx.x;
-    // own records: loaded at storage index j together with perm[j] (equal
-    // unless a mover shifted this sorted position), reloaded otherwise
     const uint32_t s = A.perm[j];
-    double4 ip = A.in.pr[j];
-    float4 vm = A.in.vm[j], wa = A.in.wa[j], aa = A.in.aa[j];
-    uint2 zi = A.in.zi[j];
     if (s >= A.n) return;  // defensive: perm is a permutation of [0, n)
-    if (s != j) {
-        ip = A.in.pr[s];
-        vm = A.in.vm[s];
-        wa = A.in.wa[s];
-        aa = A.in.aa[s];
-        zi = A.in.zi[s];
-    }
+    const double4 ip = A.in.pr[s];
+    const float4 vm = A.in.vm[s], wa = A.in.wa[s], aa = A.in.aa[s];
+    const uint2 zi = A.in.zi[s];
     HistCtx hc{};
     if (HIST) {
         hc.on = S.history != 0;
@@ -343,9 +343,12 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
     auto consider = [&](uint32_t t, const double4& jp) {
         double ex = jp.x - o.x, ey = jp.y - o.y, ez = jp.z - o.z;
         double d2 = ex * ex + ey * ey + ez * ez;
-        double Sr = ri + jp.w;
-        if (d2 > Sr * Sr * 1.000001) return;
-        if (!(Sr - sqrt(d2) > 0)) return;
+        double Sr = ri + jp.w, S2 = Sr * Sr;
+        if (d2 > S2 * 1.000001) return;
+        // the reference predicate (Sr - fl(sqrt(d2)) > 0) is decided by d2
+        // alone unless d2 is within 1e-9 of Sr^2 (fl(sqrt) is within half an
+        // ulp of the root); only then is the square root taken
+        if (!(d2 < S2 * (1.0 - 1e-9)) && !(Sr - sqrt(d2) > 0)) return;
         if (DBG && ntotal < A.dbg_max) A.dbg_part[(size_t)j * A.dbg_max + ntotal] = A.in.zi[t].y;
         ++ntotal;
         mylist[(nlist++) * LDS_T] = t;
@@ -360,10 +363,8 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
         // Away from the x wrap the three cells of an x-row have consecutive keys,
         // so their particles are one contiguous run of sorted positions.  The
         // range of row r+1 is loaded before row r is walked, and a row is
-        // walked UNR candidates at a time.  The storage order is the previous
-        // step's sorted order, so perm[k] == k except where movers shifted
-        // the order: the record at storage index k is loaded together with
-        // perm[k] and reloaded only where perm[k] != k.
+        // walked UNR candidates at a time: UNR index loads, then UNR record
+        // loads in flight before the tests.
         auto row_range = [&](int r) {
             const uint32_t k0 = key_of(S, cx - 1, cy + (r % 3) - 1, cz + (r / 3) - 1);
             const uint2 c0 = A.crange[k0], c1 = A.crange[k0 + 1], c2 = A.crange[k0 + 2];
@@ -387,11 +388,10 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
                     const uint32_t k = jj + u;
                     const bool ok = k < e1 && k != j;  // e1 <= n
                     t[u] = ok ? A.perm[k] : A.n;
-                    if (ok) q[u] = A.in.pr[k];
                 }
 #pragma unroll
                 for (int u = 0; u < UNR; ++u)
-                    if (t[u] < A.n && t[u] != jj + u) q[u] = A.in.pr[t[u]];
+                    if (t[u] < A.n) q[u] = A.in.pr[t[u]];
 #pragma unroll
                 for (int u = 0; u < UNR; ++u)
                     if (t[u] < A.n) consider(t[u], q[u]);
@@ -430,8 +430,8 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
         if (cdist > 0) {
             f3 u = F3((float)unit.x, (float)unit.y, (float)unit.z);
             f3 dv = S.rolling ? -(o.v + crossf(o.w, o.r * u)) : -o.v;
-            float kn = bp.ckn * sqrtf(o.r);
-            float vn = sqrtf(bp.cvn * o.m * kn);
+            float kn = bp.ckn * fsqrt(o.r);
+            float vn = fsqrt(bp.cvn * o.m * kn);
             f3 pFt = F3(0, 0, 0), pFn = F3(0, 0, 0), pM = F3(0, 0, 0);
             if (HIST && hc.on) {
                 f3 xi = hc.lookup(WALL_KEY | p);
@@ -456,8 +456,8 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
         A.dbg_cnt[j] = ntotal;
     }
     // vv_update_velocity_kernel (mphysics_cuda_impl.cuh:98-134) in FP32
-    float inv_m = 1.0f / o.m;
-    float inv_i = 1.0f / (0.4f * o.m * o.r * o.r);
+    float inv_m = frcp(o.m);
+    float inv_i = frcp(0.4f * o.m * o.r * o.r);
     f3 a0 = F3(aa.x, aa.y, aa.z), al0 = F3(wa.w, aa.w, __uint_as_float(zi.x));
     f3 a = inv_m * F, al = inv_i * T;
     float hdt = 0.5f * S.dtf;
