@@ -110,8 +110,31 @@ class SimDesc(C.Structure):
                 ("pp", ContactPairDesc)]
 
 
+# the older entry points (include/xdem_setb.h, cu_dem_dec.cuh:41-87)
+class BDeviceParameters(C.Structure):
+    _fields_ = [("np", C.c_uint32), ("nsp", C.c_uint32), ("ncell", C.c_uint32),
+                ("grid_size", uint3), ("dt", C.c_double), ("half2dt", C.c_double),
+                ("cell_size", C.c_double), ("gravity", double3), ("world_origin", double3),
+                ("kn", C.c_double), ("vn", C.c_double), ("ks", C.c_double), ("vs", C.c_double),
+                ("mu", C.c_double), ("cohesive", C.c_double)]
+
+
+class BPlaneInfo(C.Structure):
+    _fields_ = [("l1", C.c_double), ("l2", C.c_double)] + [
+        (n, double3) for n in ("u1", "u2", "uw", "xw", "pa", "pb")]
+
+
+class BPolygon(C.Structure):
+    _fields_ = [(n, double3) for n in ("P", "Q", "R", "V", "W", "N")]
+
+
+class BMassInfo(C.Structure):
+    _fields_ = [("pos", double3), ("vel", double3)]
+
+
 _SIZES = {DeviceParameters: 112, PolygonInfo: 152, PlaneInfo: 232, PolygonMassInfo: 160,
-          CylinderInfo: 176, ContactProperty: 88}
+          CylinderInfo: 176, ContactProperty: 88, BDeviceParameters: 144, BPlaneInfo: 160,
+          BPolygon: 144, BMassInfo: 48}
 for _t, _s in _SIZES.items():
     assert C.sizeof(_t) == _s, (_t.__name__, C.sizeof(_t), _s)
 assert DeviceParameters.dt.offset == 32 and DeviceParameters.world_origin.offset == 88
@@ -225,6 +248,19 @@ def _declare(lib):
         "xdem_sim_phase_times": ([vp, vp, vp], i32),
         "xdem_contact_pairs_f64": ([vp] * 4 + [u32, u32, vp, vp], i32),
         "xdem_sim_grid": ([vp], vp),
+        "xdem_b_set_symbolic_parameter": ([C.POINTER(BDeviceParameters)], i32),
+        "xdem_b_get_symbolic_parameter": ([C.POINTER(BDeviceParameters)], i32),
+        "xdem_b_vv_update_position": ([vp, vp, vp, u32], i32),
+        "xdem_b_vv_update_velocity": ([vp] * 6 + [u32], i32),
+        "xdem_b_mergedata": ([vp, vp, u32, vp, u32], i32),
+        "xdem_b_calculate_hash_and_index": ([vp, vp, vp, u32], i32),
+        "xdem_b_reorder_data_and_find_cell_start": ([vp] * 5 + [u32, u32], i32),
+        "xdem_b_calculate_p2p": ([vp] * 10 + [u32, u32], i32),
+        "xdem_b_cube_hertzian_contact_force": ([vp] + [f64] * 5 + [vp] * 6 + [u32], i32),
+        "xdem_b_shape_hertzian_contact_force":
+            ([vp] * 5 + [f64] * 5 + [vp] * 7 + [u32] + [vp] * 4, i32),
+        "xdem_b_update_polygons_position": ([vp, vp, vp, u32, vp], i32),
+        "xdem_b_update_polygons_information": ([vp, vp, vp, u32], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name, None)

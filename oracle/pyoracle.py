@@ -106,6 +106,16 @@ def _declare(L):
         "orc_hist_contacts": ([vp, vp, vp, vp, vp], None),
         "orc_detect": ([vp], None),
         "orc_hist_step": ([vp, vp, vp], None),
+        "orcb_vv_position": ([vp, vp, vp, vp, u32], None),
+        "orcb_vv_velocity": ([vp] * 7 + [u32], None),
+        "orcb_mergedata": ([vp, vp, u32, vp, u32], None),
+        "orcb_hash_and_index": ([vp, vp, vp, vp, u32], None),
+        "orcb_reorder": ([vp] * 6 + [u32, u32], None),
+        "orcb_p2p": ([vp] * 10 + [u32], None),
+        "orcb_cube": ([vp, vp] + [f64] * 5 + [vp] * 5 + [u32], None),
+        "orcb_shape": ([vp] * 6 + [f64] * 5 + [vp] * 6 + [u32] + [vp] * 4, None),
+        "orcb_update_polygons_position": ([vp, vp, vp, u32, vp], None),
+        "orcb_update_polygons_information": ([vp, vp, vp, u32], None),
     }
     for n, (a, r) in sig.items():
         f = getattr(L, n)
@@ -340,3 +350,75 @@ class World:
 
     def hist_step(self, hin, hout):
         load().orc_hist_step(byref(self.w), byref(hin.s), byref(hout.s))
+
+
+# ------------------------------------------------------------ older entry points
+class SetB:
+    """Host-array wrappers of the orcb_* restatements (xdem_setb.h layouts:
+    double4 pos/vel/acc/omega/alpha, double3 force/moment)."""
+
+    def __init__(self, params):
+        self.p = params
+
+    def vv_position(self, pos4, vel4, acc4, np_=None):
+        load().orcb_vv_position(byref(self.p), P(pos4), P(vel4), P(acc4),
+                                pos4.shape[0] if np_ is None else np_)
+
+    def vv_velocity(self, vel4, acc4, omega4, alpha4, force3, moment3, np_=None):
+        load().orcb_vv_velocity(byref(self.p), P(vel4), P(acc4), P(omega4), P(alpha4), P(force3),
+                                P(moment3), vel4.shape[0] if np_ is None else np_)
+
+    @staticmethod
+    def mergedata(pos4, vertice3):
+        n, m = pos4.shape[0], vertice3.shape[0]
+        out = np.zeros((n + m, 4))
+        load().orcb_mergedata(P(out), P(abi.as_f64(pos4)), n, P(abi.as_f64(vertice3)), m)
+        return out
+
+    def hash_and_index(self, pos4, np_=None):
+        n = pos4.shape[0] if np_ is None else np_
+        h = np.zeros(n, np.uint32)
+        i = np.zeros(n, np.uint32)
+        load().orcb_hash_and_index(byref(self.p), P(h), P(i), P(abi.as_f64(pos4)), n)
+        return h, i
+
+    def reorder(self, h, i, nc, np_=None, cend=None):
+        n = h.size if np_ is None else np_
+        cs = np.zeros(nc, np.uint32)
+        ce = np.zeros(nc, np.uint32) if cend is None else cend.copy()
+        srt = np.zeros(h.size, np.uint32)
+        load().orcb_reorder(byref(self.p), P(h), P(i), P(cs), P(ce), P(srt), n, nc)
+        return cs, ce, srt
+
+    def p2p(self, pos4, vel4, acc4, omega4, force3, moment3, srt, cs, ce, np_=None):
+        load().orcb_p2p(byref(self.p), P(pos4), P(vel4), P(acc4), P(omega4), P(force3),
+                        P(moment3), P(srt), P(cs), P(ce), pos4.shape[0] if np_ is None else np_)
+
+    def cube(self, planes, k, pos4, vel4, omega4, force3, moment3):
+        load().orcb_cube(byref(self.p), C.cast(planes, C.c_void_p), *k, P(pos4), P(vel4),
+                         P(omega4), P(force3), P(moment3), pos4.shape[0])
+
+    def shape(self, mi, polys, id_set, pstart, pend, k, pos4, vel4, omega4, force3, moment3, srt,
+              cs, ce, np_):
+        is_line = np.zeros(np_, np.bool_)
+        bf = np.zeros(3)
+        load().orcb_shape(byref(self.p), byref(mi), C.cast(polys, C.c_void_p), P(id_set),
+                          P(pstart), P(pend), *k, P(is_line), P(pos4), P(vel4), P(omega4),
+                          P(force3), P(moment3), np_, P(srt), P(cs), P(ce), P(bf))
+        return is_line, bf
+
+    @staticmethod
+    def update_polygons_position(mpos3, A9, local3):
+        out = np.zeros_like(local3)
+        load().orcb_update_polygons_position(P(abi.as_f64(mpos3)), P(abi.as_f64(A9)),
+                                             P(abi.as_f64(local3)), local3.shape[0], P(out))
+        return out
+
+    @staticmethod
+    def update_polygons_information(indice3, pos3):
+        ni = indice3.shape[0]
+        polys = (abi.BPolygon * ni)()
+        load().orcb_update_polygons_information(C.cast(polys, C.c_void_p),
+                                                P(np.ascontiguousarray(indice3, np.uint32)),
+                                                P(abi.as_f64(pos3)), ni)
+        return polys

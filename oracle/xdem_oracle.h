@@ -19,6 +19,7 @@
 #ifndef XDEM_ORACLE_H
 #define XDEM_ORACLE_H
 #include "../include/xdem.h"
+#include "../include/xdem_setb.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -170,6 +171,37 @@ void orc_detect(orc_world* w);
 /* One step with history (GPU semantics, TCM 1): Verlet position, grid,
  * contacts with history (in -> out), Verlet velocity with F = m g + contacts. */
 void orc_hist_step(orc_world* w, const orc_hist* in, orc_hist* out);
+
+/* The older entry points (include/cu_dem_dec.cuh), restated in
+ * xdem_oracle_b.cpp; arguments as the xdem_b_* functions of xdem_setb.h,
+ * host arrays. */
+void orcb_vv_position(const xdem_b_device_parameters* p, double* pos4, const double* vel4,
+                      const double* acc4, uint32_t np);
+void orcb_vv_velocity(const xdem_b_device_parameters* p, double* vel4, double* acc4,
+                      double* omega4, double* alpha4, const double* force3,
+                      const double* moment3, uint32_t np);
+void orcb_mergedata(double* data4, const double* pos4, uint32_t np, const double* vertice3,
+                    uint32_t snp);
+void orcb_hash_and_index(const xdem_b_device_parameters* p, uint32_t* hash, uint32_t* index,
+                         const double* pos4, uint32_t np);
+void orcb_reorder(const xdem_b_device_parameters* p, const uint32_t* hash, const uint32_t* index,
+                  uint32_t* cstart, uint32_t* cend, uint32_t* sorted, uint32_t np, uint32_t nc);
+void orcb_p2p(const xdem_b_device_parameters* p, const double* pos4, const double* vel4,
+              const double* acc4, const double* omega4, double* force3, double* moment3,
+              const uint32_t* sorted, const uint32_t* cstart, const uint32_t* cend, uint32_t np);
+void orcb_cube(const xdem_b_device_parameters* p, const xdem_b_plane_info* planes, double kn,
+               double vn, double ks, double vs, double mu, const double* pos4, const double* vel4,
+               const double* omega4, double* force3, double* moment3, uint32_t np);
+void orcb_shape(const xdem_b_device_parameters* p, const xdem_b_mass_info* mi,
+                const xdem_b_polygon* polys, const uint32_t* id_set, const uint32_t* poly_start,
+                const uint32_t* poly_end, double kn, double vn, double ks, double vs, double mu,
+                bool* is_line, const double* pos4, const double* vel4, const double* omega4,
+                double* force3, double* moment3, uint32_t np, const uint32_t* sorted,
+                const uint32_t* cstart, const uint32_t* cend, double out_bf[3]);
+void orcb_update_polygons_position(const double* mpos3, const double* A, const double* local3,
+                                   uint32_t nv, double* pos3);
+void orcb_update_polygons_information(xdem_b_polygon* polys, const uint32_t* indice3,
+                                      const double* pos3, uint32_t ni);
 
 #ifdef __cplusplus
 }

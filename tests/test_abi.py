@@ -10,7 +10,7 @@ import pytest
 from metakonga_amd import abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("xdem.h", "xdem_batch.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("xdem.h", "xdem_batch.h", "xdem_setb.h")]
 
 
 def exported():
@@ -41,6 +41,12 @@ def test_struct_layouts():
     assert C.sizeof(abi.PolygonMassInfo) == 160
     assert C.sizeof(abi.CylinderInfo) == 176
     assert C.sizeof(abi.ContactProperty) == 88
+    # include/cu_dem_dec.cuh structs (the older entry points)
+    assert C.sizeof(abi.BDeviceParameters) == 144
+    assert abi.BDeviceParameters.dt.offset == 24 and abi.BDeviceParameters.kn.offset == 96
+    assert C.sizeof(abi.BPlaneInfo) == 160
+    assert C.sizeof(abi.BPolygon) == 144
+    assert C.sizeof(abi.BMassInfo) == 48
 
 
 def test_invalid_parameters_rejected_on_host():
@@ -59,7 +65,8 @@ def test_invalid_parameters_rejected_on_host():
 
 def test_c_header_is_plain_c(tmp_path):
     src = tmp_path / "t.c"
-    src.write_text('#include "xdem.h"\n#include "xdem_batch.h"\nint main(void){return XDEM_OK;}\n')
+    src.write_text('#include "xdem.h"\n#include "xdem_batch.h"\n#include "xdem_setb.h"\n'
+                   'int main(void){return XDEM_OK;}\n')
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only",
                     "-I" + os.path.join(ROOT, "include"), str(src)], check=True)
 
@@ -94,6 +101,47 @@ int main(int argc, char**) {
 }
 ''')
     exe = tmp_path / "caller"
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                    str(src), "-o", str(exe), "-L" + os.path.dirname(abi.LIB_PATH), "-lxdem",
+                    "-Wl,-rpath," + os.path.dirname(abi.LIB_PATH), "-L/opt/rocm/lib",
+                    "-Wl,-rpath-link,/opt/rocm/lib"], check=True)
+    assert exe.exists()
+
+
+def test_compat_setb_header_links(tmp_path):
+    """A caller written against the older reference names (cu_dem_dec.cuh,
+    as the parSIM sources call them) compiles against xdem_compat_setb.hpp and
+    links against libxdem.so."""
+    src = tmp_path / "caller_b.cpp"
+    src.write_text(r'''
+#include "xdem_compat_setb.hpp"
+int main(int argc, char**) {
+    if (argc < 5) return 0;  // never runs the GPU path
+    device_parameters p{};
+    setSymbolicParameter(&p);
+    double *pos = nullptr, *vel = nullptr, *acc = nullptr, *f = nullptr;
+    unsigned int *h = nullptr, *i = nullptr, *cs = nullptr, *ce = nullptr, *s = nullptr;
+    bool* line = nullptr;
+    device_plane_info* pl = nullptr;
+    cu_polygon* po = nullptr;
+    cu_mass_info* mi = nullptr;
+    double3 *v3 = nullptr, *bf = nullptr;
+    uint3* ind = nullptr;
+    vv_update_position(pos, vel, acc, 0);
+    vv_update_velocity(vel, acc, vel, acc, f, f, 0);
+    cu_mergedata(pos, pos, 0, v3, 0);
+    cu_calculateHashAndIndex(h, i, pos, 0);
+    cu_reorderDataAndFindCellStart(h, i, cs, ce, s, 0, 1);
+    cu_calculate_p2p(pos, vel, acc, vel, acc, f, f, s, cs, ce, 0);
+    cu_cube_hertzian_contact_force(pl, 1, 1, 1, 1, 0.3, line, pos, vel, vel, f, f, 0);
+    double3 r = cu_shape_hertzian_contact_force(mi, po, s, s, s, 1, 1, 1, 1, 0.3, line, pos, vel,
+                                                vel, f, f, bf, 0, s, cs, ce);
+    cu_update_polygons_position(v3, f, v3, 0, v3);
+    cu_update_polygons_information(po, ind, v3, 0);
+    return (int)r.x;
+}
+''')
+    exe = tmp_path / "caller_b"
     subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
                     str(src), "-o", str(exe), "-L" + os.path.dirname(abi.LIB_PATH), "-lxdem",
                     "-Wl,-rpath," + os.path.dirname(abi.LIB_PATH), "-L/opt/rocm/lib",

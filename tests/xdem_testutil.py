@@ -93,3 +93,41 @@ def dense_scene(n_side=12, seed=11, pitch=1.97, jitter=0.04, vel_scale=0.05, ome
                       cell_size=2 * r, grid_size=(g, g, g),
                       planes=scenes.open_box_planes(lo + 0.3 * r, hi + [0, 0.05, 0]))
     return sc, vel, om
+
+
+# ------------------------------------------------ the older entry points
+def setb_params(sc, np_, nsp=0, kn=0.0, vn=0.0, ks=0.0, vs=0.0, mu=0.3, cohesive=0.0):
+    """Parameter block of the older entry points (include/xdem_setb.h) for a
+    scene: grid and time from the scene, pair constants as given."""
+    from metakonga_amd import abi
+    p = abi.BDeviceParameters()
+    p.np = np_
+    p.nsp = nsp
+    gs = sc.grid_size
+    p.ncell = gs[0] * gs[1] * gs[2]
+    p.grid_size = abi.uint3(*gs)
+    p.dt = sc.dt
+    p.half2dt = 0.5 * sc.dt * sc.dt
+    p.cell_size = sc.cell_size or 2.0 * float(sc.pos[:, 3].max())
+    p.gravity = abi._d3(sc.gravity)
+    p.world_origin = abi._d3(sc.world_origin)
+    p.kn, p.vn, p.ks, p.vs = kn, vn, ks, vs
+    p.mu = mu
+    p.cohesive = cohesive
+    return p
+
+
+def pair_constants(orc, sc):
+    """DHS constants of two particles of the scene's first radius."""
+    m, _ = sc.mass_inertia()
+    r = float(sc.pos[0, 3])
+    c = orc.force_constant(1, r, r, float(m[0]), float(m[0]), sc.contact_pp())
+    return c.kn, c.vn, c.ks, c.vs
+
+
+def as4(a3, w):
+    """(n, 3) + per-row w -> (n, 4) double4 layout."""
+    out = np.zeros((a3.shape[0], 4))
+    out[:, :3] = a3
+    out[:, 3] = w
+    return out
