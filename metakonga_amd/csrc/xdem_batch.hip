@@ -102,6 +102,7 @@ struct xdem_batch {
     uint32_t* ms = nullptr;
     uint64_t* mkey2 = nullptr;
     uint32_t* ms2 = nullptr;
+    uint32_t* mplo = nullptr;         // 2 n: insertion points, then radix scratch
     // tables
     BScene* d_scenes = nullptr;
     BPlane* d_planes = nullptr;
@@ -129,6 +130,9 @@ struct xdem_batch {
     PhaseTimer timer;
     uint32_t last_movers = 0;
     std::vector<BScene> h_scenes;
+    // cell ranges of removed scenes, cleared at the next layout change (the
+    // incremental sort reads the cell table around a mover's new cell)
+    std::vector<std::pair<uint32_t, uint32_t>> freed_cells;
     // parity probe
     float* dbg_f = nullptr;
     uint32_t* dbg_cnt = nullptr;
@@ -144,7 +148,7 @@ struct xdem_batch {
         if (d_blocks_store == d_blocks_sorted) d_blocks_store = nullptr;
         for (void* p : {(void*)keynew, (void*)skey[0], (void*)skey[1], (void*)perm, (void*)crange,
                         (void*)moverbits, (void*)blockcnt, (void*)counters, (void*)mkey, (void*)ms,
-                        (void*)mkey2, (void*)ms2, (void*)d_scenes, (void*)d_planes,
+                        (void*)mkey2, (void*)ms2, (void*)mplo, (void*)d_scenes, (void*)d_planes,
                         (void*)d_blocks_sorted, (void*)d_blocks_store, (void*)d_red, (void*)dbg_f,
                         (void*)dbg_cnt, (void*)dbg_part, (void*)hxi[0], (void*)hxi[1],
                         (void*)hcnt[0], (void*)hcnt[1], (void*)hover})
@@ -191,6 +195,7 @@ struct xdem_batch {
         a.ms = ms;
         a.mkey2 = mkey2;
         a.ms2 = ms2;
+        a.mplo = mplo;
         a.perm = perm;
         a.skey = skey_out;
         a.nblk = nblk_rows();
@@ -319,6 +324,13 @@ int32_t apply_layout(xdem_batch* b) {
     b->hist_active = false;
     for (auto& s : b->scenes) b->hist_active |= s.desc.tangential_history != 0;
     if (b->hist_active) XDEM_TRY(ensure_history(b));
+    const uint2 empty_range = make_uint2(EMPTY, 0);
+    for (auto& fc : b->freed_cells) {
+        hipLaunchKernelGGL(k_fill_u2, dim3(std::min<uint32_t>(4096, div_up(fc.second, 256))), dim3(256),
+                           0, st, b->crange + fc.first, empty_range, (size_t)fc.second);
+        XDEM_LAUNCHED();
+    }
+    b->freed_cells.clear();
     // 1. compaction of the scenes already on the device: they sit in the
     //    storage in cell_base order (sorted layout of the last block).
     std::vector<HostScene*> old, fresh;
@@ -346,6 +358,18 @@ int32_t apply_layout(xdem_batch* b) {
         XDEM_TRY(copy_segment(B, A, 0, 0, n_old, st));
         XDEM_TRY(copy_hist(b, 1 - b->cur, b->cur, 0, 0, n_old));
         XDEM_HIP(hipMemcpyAsync(b->skey[b->skcur], b->keynew, 4ull * n_old, hipMemcpyDeviceToDevice, st));
+        // the sorted positions of the remaining scenes moved: rebuild their cell ranges
+        for (HostScene* s : old) {
+            hipLaunchKernelGGL(k_fill_u2, dim3(std::min<uint32_t>(4096, div_up(s->ncell, 256))),
+                               dim3(256), 0, st, b->crange + s->cell_base, empty_range,
+                               (size_t)s->ncell);
+            XDEM_LAUNCHED();
+        }
+        if (n_old) {
+            hipLaunchKernelGGL(k_ranges, dim3(div_up(n_old, ROWS_T)), dim3(ROWS_T), 0, st,
+                               b->skey[b->skcur], b->crange, n_old, (uint32_t)b->cap_cells);
+            XDEM_LAUNCHED();
+        }
     }
     // 2. append new scenes (id order) with the sentinel old key
     XDEM_HIP(hipStreamSynchronize(st));
@@ -639,7 +663,8 @@ int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t 
         (rc = dalloc(&b->moverbits, (size_t)nblk * (ROWS_T / 64) + 1)) ||
         (rc = dalloc(&b->blockcnt, (size_t)nblk + 1)) || (rc = dalloc(&b->counters, 4)) ||
         (rc = dalloc(&b->mkey, n)) || (rc = dalloc(&b->ms, n)) || (rc = dalloc(&b->mkey2, n)) ||
-        (rc = dalloc(&b->ms2, n)) || (rc = dalloc(&b->d_red, 8 * 1024)))
+        (rc = dalloc(&b->ms2, n)) || (rc = dalloc(&b->mplo, 2 * n)) ||
+        (rc = dalloc(&b->d_red, 8 * 1024)))
         return fail(rc);
     if (hipMemset(b->counters, 0, 16) != hipSuccess) return fail(XDEM_ERR_HIP);
     *out = b;
@@ -731,6 +756,8 @@ int32_t xdem_batch_remove_scene(xdem_batch* b, uint32_t scene_id) {
     XDEM_REQUIRE(b, "null batch");
     for (size_t i = 0; i < b->scenes.size(); ++i)
         if (b->scenes[i].id == scene_id) {
+            if (!b->scenes[i].pending)
+                b->freed_cells.push_back({b->scenes[i].cell_base, b->scenes[i].ncell});
             b->scenes.erase(b->scenes.begin() + i);
             b->layout_dirty = true;
             b->prepared = false;

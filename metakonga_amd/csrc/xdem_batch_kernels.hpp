@@ -25,6 +25,7 @@ constexpr int HSLOTS = 16;                   // tangential-history slots per par
 constexpr uint32_t WALL_KEY = 0x80000000u;   // history key of plane p: WALL_KEY | p
 constexpr int LDS_T = 256;                   // lane stride of the per-thread LDS lists
 constexpr int SORT_LDS_MAX = 4096;           // movers sorted in LDS up to this count
+constexpr int MERGE_LDS = 2048;              // sorted movers staged in LDS by k_merge up to this count
 constexpr int ROWS_T = 256;                  // threads of the row kernels
 
 // Scene constants (FP64 grid/time, FP32 material).
@@ -519,8 +520,9 @@ struct SortArgs {
     uint32_t* msorted;          // number of sorted movers (written by k_sort_movers)
     uint64_t* mkey;             // mover (key << 32 | id), append order
     uint32_t* ms;               // mover storage index, append order
+    uint32_t* mplo;             // mover insertion point in the previous order, append order
     uint64_t* mkey2;            // sorted movers (and radix ping-pong)
-    uint32_t* ms2;
+    uint32_t* ms2;              // sorted movers' append index
     uint32_t* perm;             // out: sorted position -> storage index
     uint32_t* skey;             // out: sorted keys
     uint32_t nblk;              // ROWS_T blocks over n
@@ -532,15 +534,44 @@ __device__ inline uint64_t lanemask_lt64() {
     return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
 }
 
-// Mover detection: clears the previous cell ranges, marks movers, counts them
-// per block and appends them (order irrelevant: they are sorted next).
+// First position of the previous sorted order whose key is >= kn, from the
+// previous cell ranges (exact for the previous sort): the cell's own start,
+// else the nearest non-empty cell within 8 keys, else a binary search.
+__device__ inline uint32_t insertion_point(const SortArgs& A, uint32_t kn) {
+    if (kn < A.ncell_cap) {
+        const uint2 c = A.crange[kn];
+        if (c.x != EMPTY) return c.x;
+        for (uint32_t d = 1; d <= 8; ++d) {
+            if (kn >= d) {
+                const uint2 a = A.crange[kn - d];
+                if (a.x != EMPTY) return a.y;
+            }
+            if (kn + d < A.ncell_cap) {
+                const uint2 b = A.crange[kn + d];
+                if (b.x != EMPTY) return b.x;
+            }
+        }
+    }
+    uint32_t lo = 0, hi = A.n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (A.key_old[mid] < kn)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Mover detection: marks movers, counts them per block and appends them
+// (order irrelevant: they are sorted next) with their insertion point in the
+// previous order.  The cell ranges are read here and not modified before
+// k_merge.
 __global__ __launch_bounds__(ROWS_T) void k_movers(SortArgs A) {
     __shared__ uint32_t wc[ROWS_T / 64];
     const uint32_t s = blockIdx.x * ROWS_T + threadIdx.x;
     const bool valid = s < A.n;
     uint32_t old = valid ? A.key_old[s] : 0, nw = valid ? A.key_new[s] : 0;
-    if (valid && old < A.ncell_cap && (s == 0 || A.key_old[s - 1] != old))
-        A.crange[old] = make_uint2(EMPTY, 0);
     bool mover = valid && old != nw;
     uint64_t word = __ballot(mover);
     uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -554,6 +585,7 @@ __global__ __launch_bounds__(ROWS_T) void k_movers(SortArgs A) {
         if (p < A.n) {
             A.mkey[p] = ((uint64_t)nw << 32) | A.zi[s].y;
             A.ms[p] = s;
+            A.mplo[p] = insertion_point(A, nw);
         }
     }
     if (lane == 0) wc[w] = cnt;
@@ -618,7 +650,7 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
         while (P < M) P <<= 1;
         for (uint32_t i = t; i < P; i += 1024) {
             lk[i] = i < M ? A.mkey[i] : ~0ull;
-            lv[i] = i < M ? A.ms[i] : 0;
+            lv[i] = i;  // payload: the append index
         }
         __syncthreads();
         for (uint32_t k = 2; k <= P; k <<= 1) {
@@ -647,9 +679,14 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
         return;
     }
     // LSD radix over the id bits then the key bits; 8-bit digits; stable by
-    // processing 1024-item chunks in order.
+    // processing 1024-item chunks in order.  Payload: the append index,
+    // ping-ponged between the second half of the mplo buffer and ms2 in step
+    // with the keys (mkey, mkey2).
+    uint32_t* scratch = A.mplo + A.n;
+    for (uint32_t i = t; i < M; i += 1024) scratch[i] = i;
+    __syncthreads();
     uint64_t* src_k = A.mkey;
-    uint32_t* src_v = A.ms;
+    uint32_t* src_v = scratch;
     uint64_t* dst_k = A.mkey2;
     uint32_t* dst_v = A.ms2;
     int shifts[16];
@@ -738,10 +775,32 @@ __device__ inline uint32_t mover_prefix(const SortArgs& A, uint32_t P) {
 }
 
 // Merge: stayers keep their relative order and shift by the movers placed
-// before them; movers are placed by their rank among the stayers.
+// before them; movers are placed by their rank among the previous order.
+// Stayers count the sorted movers below their (key, id) by a binary search in
+// LDS (global memory when more than MERGE_LDS movers).  A mover starts from
+// its insertion point (k_movers) and steps over the entries of its new cell
+// with a smaller id; it also clears its old cell's range (k_ranges rewrites
+// every non-empty cell).
+__device__ inline uint32_t movers_below(const uint64_t* mk, uint32_t M, uint64_t X) {
+    uint32_t lo = 0, hi = M;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (mk[mid] < X)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
+    __shared__ uint64_t smk[MERGE_LDS];
     const uint32_t M = *A.msorted;
     if (blockIdx.x < A.nblk) {
+        const bool lds = M <= (uint32_t)MERGE_LDS;
+        if (lds)
+            for (uint32_t i = threadIdx.x; i < M; i += ROWS_T) smk[i] = A.mkey2[i];
+        __syncthreads();
         const uint32_t s = blockIdx.x * ROWS_T + threadIdx.x;
         if (s >= A.n) return;
         uint32_t kn = A.key_new[s];
@@ -749,14 +808,7 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
         uint32_t out = s;
         if (M) {
             uint64_t X = ((uint64_t)kn << 32) | A.zi[s].y;
-            uint32_t lo = 0, hi = M;  // count of sorted movers < X
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (A.mkey2[mid] < X)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
+            uint32_t lo = movers_below(lds ? smk : A.mkey2, M, X);
             out = s - mover_prefix(A, s) + lo;
         }
         if (out < A.n) {
@@ -767,21 +819,20 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
     }
     const uint32_t nmb = gridDim.x - A.nblk;
     for (uint32_t r = (blockIdx.x - A.nblk) * ROWS_T + threadIdx.x; r < M; r += nmb * ROWS_T) {
-        uint64_t X = A.mkey2[r];
-        uint32_t lo = 0, hi = A.n;  // storage entries with (key_old, id) < X
-        while (lo < hi) {
-            uint32_t mid = (lo + hi) >> 1;
-            uint64_t Y = ((uint64_t)A.key_old[mid] << 32) | A.zi[mid].y;
-            if (Y < X)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
+        const uint64_t X = A.mkey2[r];
+        const uint32_t p = A.ms2[r];
+        if (p >= A.n) continue;  // defensive: payloads are append indices < M
+        const uint32_t s = A.ms[p];
+        const uint32_t kn = (uint32_t)(X >> 32), id = (uint32_t)X;
+        // entries of the previous order with (key, id) < X
+        uint32_t lo = A.mplo[p];
+        while (lo < A.n && A.key_old[lo] == kn && A.zi[lo].y < id) ++lo;
         uint32_t out = r + (lo - mover_prefix(A, lo));
-        uint32_t s = A.ms2[r];
         if (out < A.n && s < A.n) {
             A.perm[out] = s;
-            A.skey[out] = (uint32_t)(X >> 32);
+            A.skey[out] = kn;
+            const uint32_t old = A.key_old[s];
+            if (old < A.ncell_cap) A.crange[old] = make_uint2(EMPTY, 0);
         }
     }
 }
