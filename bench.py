@@ -37,9 +37,9 @@ P2P_F64_BYTES = 188.0
 STEP_F64_BYTES = 300.0
 # Algorithmic bytes per particle of the fused contact + Verlet kernel (k_step):
 # own state record read 88 (pos+r 32, v+m 16, omega+alpha.x 16, a+alpha.y 16,
-# alpha.z+id 8) + written 88 + sorted->storage index 4 + next cell key 4
-# (neighbour re-reads excluded).
-KSTEP_BYTES = 184.0
+# alpha.z+id 8) + written 88 + FP32 position copy written 16 + sorted->storage
+# index 4 + next cell key 4 (neighbour re-reads excluded).
+KSTEP_BYTES = 200.0
 # Algorithmic bytes per particle-step of the whole FP32c step.
 STEP_F32C_BYTES = 192.0
 

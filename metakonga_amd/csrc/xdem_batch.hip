@@ -35,6 +35,7 @@ struct HostScene {
     xdem_batch_scene_desc desc;
     std::vector<xdem_device_plane_info> planes;
     uint32_t n = 0;
+    double rmax = 0;       // largest radius (FP32 value)
     uint32_t cell_base = 0, ncell = 0;
     uint32_t p0 = 0;       // sorted segment
     bool pending = true;   // not yet uploaded
@@ -57,11 +58,12 @@ int32_t alloc_state(BState& s, size_t n) {
     XDEM_TRY(dalloc(&s.wa, n));
     XDEM_TRY(dalloc(&s.aa, n));
     XDEM_TRY(dalloc(&s.zi, n));
+    XDEM_TRY(dalloc(&s.pf, n));
     return XDEM_OK;
 }
 
 void free_state(BState& s) {
-    for (void* p : {(void*)s.pr, (void*)s.vm, (void*)s.wa, (void*)s.aa, (void*)s.zi})
+    for (void* p : {(void*)s.pr, (void*)s.vm, (void*)s.wa, (void*)s.aa, (void*)s.zi, (void*)s.pf})
         if (p) (void)hipFree(p);
     s = BState{};
 }
@@ -253,6 +255,9 @@ BScene make_bscene(const HostScene& h, uint32_t plane0) {
     s.mass_c = (float)(d.particle.density * (4.0 / 3.0) * M_PI);
     s.rolling = d.rolling ? 1 : 0;
     s.history = d.tangential_history ? 1 : 0;
+    s.rmax = h.rmax;
+    s.pf_range = 2.0 * d.cell_size * std::max(d.grid_size.x, std::max(d.grid_size.y, d.grid_size.z));
+    s.pf_eps = (float)std::ldexp(s.pf_range, -20);
     return s;
 }
 
@@ -289,6 +294,7 @@ int32_t copy_segment(const BState& from, const BState& to, uint32_t src, uint32_
     XDEM_TRY(cp(to.wa + dst, from.wa + src, 16ull * n));
     XDEM_TRY(cp(to.aa + dst, from.aa + src, 16ull * n));
     XDEM_TRY(cp(to.zi + dst, from.zi + src, 8ull * n));
+    XDEM_TRY(cp(to.pf + dst, from.pf + src, 16ull * n));
     return XDEM_OK;
 }
 
@@ -374,12 +380,14 @@ int32_t apply_layout(xdem_batch* b) {
     // 2. append new scenes (id order) with the sentinel old key
     XDEM_HIP(hipStreamSynchronize(st));
     std::vector<double4> pr;
-    std::vector<float4> vm, wa, aa;
+    std::vector<float4> vm, wa, aa, pf;
     std::vector<uint2> zi;
     for (HostScene* s : fresh) {
         uint32_t n = s->n;
         s->p0 = at;  // storage position of the appended segment
+        const BScene bs = make_bscene(*s, 0);
         pr.resize(n);
+        pf.resize(n);
         vm.resize(n);
         wa.resize(n);
         aa.assign(n, make_float4(0, 0, 0, 0));
@@ -388,6 +396,7 @@ int32_t apply_layout(xdem_batch* b) {
         for (uint32_t i = 0; i < n; ++i) {
             float r = (float)s->pos[4 * i + 3];
             pr[i] = make_double4(s->pos[4 * i + 0], s->pos[4 * i + 1], s->pos[4 * i + 2], (double)r);
+            pf[i] = pf_of(bs, pr[i].x, pr[i].y, pr[i].z, r);
             float m = mc * (r * r * r);
             const double* v = s->vel.empty() ? nullptr : &s->vel[3 * i];
             const double* w = s->omega.empty() ? nullptr : &s->omega[3 * i];
@@ -408,6 +417,7 @@ int32_t apply_layout(xdem_batch* b) {
         XDEM_HIP(hipMemcpy(A.wa + at, wa.data(), 16ull * n, hipMemcpyHostToDevice));
         XDEM_HIP(hipMemcpy(A.aa + at, aa.data(), 16ull * n, hipMemcpyHostToDevice));
         XDEM_HIP(hipMemcpy(A.zi + at, zi.data(), 8ull * n, hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(A.pf + at, pf.data(), 16ull * n, hipMemcpyHostToDevice));
         if (b->hover) {
             // history slots in storage order (= original order for a new scene)
             std::vector<uint32_t> hc(n, 0);
@@ -726,6 +736,7 @@ int32_t xdem_batch_add_scene_history(xdem_batch* b, const xdem_batch_scene_desc*
     XDEM_REQUIRE(d->cell_size >= 2.0 * (double)(float)rmax * (1.0 - 1e-7),
                  "cell_size %.9g below 2 * max radius %.9g", d->cell_size, 2 * rmax);
     HostScene h;
+    h.rmax = (double)(float)rmax;
     h.id = b->next_id++;
     h.desc = *d;
     h.planes.assign(d->planes, d->planes + d->nplane);
@@ -1112,6 +1123,7 @@ __global__ void k_gather(BState in, BState out, const uint32_t* perm, uint32_t n
     out.wa[j] = in.wa[s];
     out.aa[j] = in.aa[s];
     out.zi[j] = in.zi[s];
+    out.pf[j] = in.pf[s];
 }
 
 __global__ void k_gather_hist(const float4* hin, const uint32_t* cin, float4* hout, uint32_t* cout,

@@ -41,6 +41,11 @@ struct BScene {
     int32_t rolling;
     uint32_t ncell;
     int32_t history;          // tangential-spring history on (DHS with a spring per contact)
+    uint32_t pad;
+    double rmax;              // largest radius of the scene (cell culling)
+    double pf_range;          // |coordinate - world origin| bound of the FP32 copy
+    float pf_eps;             // distance error bound of the FP32 copy (2^-20 pf_range)
+    uint32_t pad2;
 };
 
 struct BPlane {
@@ -50,15 +55,28 @@ struct BPlane {
 
 // One state buffer (SoA of packed records).  pr = (x, y, z, r) in FP64 (r is
 // an FP32 value widened), vm = (v, m), wa = (omega, alpha.x), aa = (a, alpha.y),
-// zi = (alpha.z bits, particle id).  88 B per particle; a neighbour candidate
-// needs only its 32-byte pr record.
+// zi = (alpha.z bits, particle id), pf = the position relative to the scene's
+// world origin in FP32 and r (negated when a coordinate exceeds pf_range):
+// 104 B per particle.  A neighbour candidate is screened on its 16-byte pf
+// record; only candidates that pass load the 32-byte pr record.
 struct BState {
     double4* pr;
     float4* vm;
     float4* wa;
     float4* aa;
     uint2* zi;
+    float4* pf;
 };
+
+// The FP32 copy of a position.  In range, each coordinate is within
+// 2^-24 pf_range of its FP64 value, so a distance computed from two copies is
+// within 2^-21 pf_range of the true one; pf_eps = 2^-20 pf_range covers it
+// twice.
+__host__ __device__ inline float4 pf_of(const BScene& S, double x, double y, double z, float r) {
+    const double rx = x - S.wox, ry = y - S.woy, rz = z - S.woz;
+    const bool in = fabs(rx) <= S.pf_range && fabs(ry) <= S.pf_range && fabs(rz) <= S.pf_range;
+    return make_float4((float)rx, (float)ry, (float)rz, in ? r : -r);
+}
 
 struct BlockEntry {
     uint32_t scene, j0, j1, pad;
@@ -296,8 +314,14 @@ __device__ inline void pp_contacts(const StepArgs& A, const BScene& S, const Own
 // together with the block size.  HIST: scenes with history on use the
 // tangential-spring law (others the reference DHS law, same arithmetic as
 // without HIST).
+// Waves per SIMD the register allocation targets for each variant (the
+// counts its register footprint nearly reaches without spilling).
+constexpr int step_waves(int unr, bool pipe, bool hist) {
+    return hist || pipe ? 4 : 5;
+}
+
 template <bool ADV, bool DBG, int UNR, bool PIPE, bool HIST>
-__global__ __launch_bounds__(256) void k_step(StepArgs A) {
+__global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST)) void k_step(StepArgs A) {
     // per-thread LDS lists, lane-contiguous (entry c of thread t at c * LDS_T + t)
     __shared__ uint32_t clist[LDS_T * CMAX];
     __shared__ uint32_t hkeys[HIST ? LDS_T * HSLOTS : 1];
@@ -334,13 +358,43 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
     cell_of(S, o.x, o.y, o.z, cx, cy, cz);
     f3 Ft = F3(0, 0, 0), Fn = F3(0, 0, 0), M = F3(0, 0, 0);
     f3 sumF = F3(0, 0, 0), sumM = F3(0, 0, 0);
-    uint32_t nlist = 0, ntotal = 0;
+    uint32_t ntotal = 0;
     const double ri = (double)o.r;
+    // Cell culling: a particle j touching i has |pj - pi| < ri + rj <= ri +
+    // rmax <= cell size, so it lies in a real-space neighbour cell of i, and
+    // every point of that cell is at least the face distances below away from
+    // pi.  A cell whose squared distance bound reaches (ri + rmax)^2 (1e-4
+    // margin, far above the FP32 rounding of the offsets and the rounding of
+    // the cell assignment) holds no contact of i and is not visited; skipping
+    // it changes neither the contacts nor their order.
+    const float csf = (float)S.cs;
+    const float reach2 = (float)((ri + S.rmax) * (ri + S.rmax)) * (1.0f + 1e-4f);
+    const float lox = (float)(o.x - (S.wox + (double)cx * S.cs));
+    const float loy = (float)(o.y - (S.woy + (double)cy * S.cs));
+    const float loz = (float)(o.z - (S.woz + (double)cz * S.cs));
+    auto face2 = [&](int d, float lo) {
+        const float b = d < 0 ? lo : d > 0 ? csf - lo : 0.0f;
+        return b * b;
+    };
+    // FP32 screen of a candidate: rejects only when the FP32 distance exceeds
+    // ri + rj by more than twice its error bound (then the FP64 predicate is
+    // false); out-of-range copies (negative radius) are never rejected.
+    const float4 ownf = pf_of(S, o.x, o.y, o.z, o.r);
+    auto may_touch = [&](const float4& f) {
+        if (!(f.w > 0.0f) || !(ownf.w > 0.0f)) return true;
+        const float dx = f.x - ownf.x, dy = f.y - ownf.y, dz = f.z - ownf.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        const float sr = (ownf.w + f.w) + S.pf_eps;
+        return d2 < sr * sr * (1.0f + 0x1p-19f);
+    };
     // phase 1: candidates in the reference order (z, y, x cells; ascending j).
-    // A candidate passing the FP64 predicate goes to this thread's LDS list;
-    // the list is evaluated (in order) whenever it could overflow, and at the
-    // end.  Ft and M carry over from contact to contact in list order, so
-    // where the list is split does not change the result.
+    // The first CMAX that pass the FP64 predicate go to this thread's LDS list
+    // (phase 2 evaluates them in order); the rest are only counted, and a
+    // particle with more contacts rescans its cells for each further chunk of
+    // CMAX (rare).  Ft and M carry over from contact to contact in list order,
+    // so the chunking does not change the result.  Contact evaluation stays out
+    // of the candidate loop, which keeps the loop's register footprint small.
+    uint32_t skip = 0;
     auto consider = [&](uint32_t t, const double4& jp) {
         double ex = jp.x - o.x, ey = jp.y - o.y, ez = jp.z - o.z;
         double d2 = ex * ex + ey * ey + ez * ez;
@@ -350,25 +404,49 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
         // alone unless d2 is within 1e-9 of Sr^2 (fl(sqrt) is within half an
         // ulp of the root); only then is the square root taken
         if (!(d2 < S2 * (1.0 - 1e-9)) && !(Sr - sqrt(d2) > 0)) return;
-        if (DBG && ntotal < A.dbg_max) A.dbg_part[(size_t)j * A.dbg_max + ntotal] = A.in.zi[t].y;
+        if (DBG && skip == 0 && ntotal < A.dbg_max)
+            A.dbg_part[(size_t)j * A.dbg_max + ntotal] = A.in.zi[t].y;
+        if (ntotal >= skip && ntotal - skip < (uint32_t)CMAX) mylist[(ntotal - skip) * LDS_T] = t;
         ++ntotal;
-        mylist[(nlist++) * LDS_T] = t;
     };
-    auto flush_if = [&](uint32_t room) {
-        if (nlist + room > CMAX) {
-            pp_contacts<PIPE, HIST>(A, S, o, mylist, nlist, hc, Ft, Fn, M, sumF, sumM);
-            nlist = 0;
-        }
+    // the 27 cells one by one (x wrap, and the rescans)
+    auto scan_cells = [&]() {
+#pragma unroll 1
+        for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll 1
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll 1
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if (face2(dx, lox) + face2(dy, loy) + face2(dz, loz) >= reach2) continue;
+                    const uint2 cr = A.crange[key_of(S, cx + dx, cy + dy, cz + dz)];
+                    if (cr.x == EMPTY) continue;
+                    const uint32_t e1 = min(cr.y, A.n);
+                    for (uint32_t jj = cr.x; jj < e1; ++jj) {
+                        if (jj == j) continue;
+                        const uint32_t t = A.perm[jj];
+                        if (t >= A.n) continue;
+                        if (may_touch(A.in.pf[t])) consider(t, A.in.pr[t]);
+                    }
+                }
     };
     if (cx >= 1 && cx + 1 < (int)S.gx) {
         // Away from the x wrap the three cells of an x-row have consecutive keys,
         // so their particles are one contiguous run of sorted positions.  The
         // range of row r+1 is loaded before row r is walked, and a row is
-        // walked UNR candidates at a time: UNR index loads, then UNR record
-        // loads in flight before the tests.
+        // walked UNR candidates at a time: UNR index loads, then UNR FP32
+        // screening records in flight; the FP64 records of the candidates that
+        // pass the screen are loaded for the exact test.
+        // A row is skipped when even its middle cell is out of reach; its
+        // end cells are dropped when out of reach.
         auto row_range = [&](int r) {
-            const uint32_t k0 = key_of(S, cx - 1, cy + (r % 3) - 1, cz + (r / 3) - 1);
-            const uint2 c0 = A.crange[k0], c1 = A.crange[k0 + 1], c2 = A.crange[k0 + 2];
+            const int dy = (r % 3) - 1, dz = (r / 3) - 1;
+            const float b2 = face2(dy, loy) + face2(dz, loz);
+            if (b2 >= reach2) return make_uint2(EMPTY, 0);
+            const bool m1 = face2(-1, lox) + b2 < reach2, p1 = face2(1, lox) + b2 < reach2;
+            const uint32_t k1 = key_of(S, cx, cy + dy, cz + dz);
+            const uint2 c0 = m1 ? A.crange[k1 - 1] : make_uint2(EMPTY, 0);
+            const uint2 c1 = A.crange[k1];
+            const uint2 c2 = p1 ? A.crange[k1 + 1] : make_uint2(EMPTY, 0);
             uint2 rg;
             rg.x = c0.x != EMPTY ? c0.x : (c1.x != EMPTY ? c1.x : c2.x);
             rg.y = c2.x != EMPTY ? c2.y : (c1.x != EMPTY ? c1.y : c0.y);
@@ -383,7 +461,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
             const uint32_t e1 = min(row.y, A.n);
             for (uint32_t jj = row.x; jj < e1; jj += UNR) {
                 uint32_t t[UNR];
-                double4 q[UNR];
+                float4 f[UNR];
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) {
                     const uint32_t k = jj + u;
@@ -392,31 +470,26 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
                 }
 #pragma unroll
                 for (int u = 0; u < UNR; ++u)
-                    if (t[u] < A.n) q[u] = A.in.pr[t[u]];
+                    if (t[u] < A.n) f[u] = A.in.pf[t[u]];
 #pragma unroll
                 for (int u = 0; u < UNR; ++u)
-                    if (t[u] < A.n) consider(t[u], q[u]);
-                flush_if(UNR);
+                    if (t[u] < A.n && may_touch(f[u])) consider(t[u], A.in.pr[t[u]]);
             }
         }
     } else {
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const uint2 cr = A.crange[key_of(S, cx + dx, cy + dy, cz + dz)];
-                    if (cr.x == EMPTY) continue;
-                    const uint32_t e1 = min(cr.y, A.n);
-                    for (uint32_t jj = cr.x; jj < e1; ++jj) {
-                        if (jj == j) continue;
-                        const uint32_t t = A.perm[jj];
-                        if (t >= A.n) continue;
-                        consider(t, A.in.pr[t]);
-                        flush_if(1);
-                    }
-                }
+        scan_cells();
     }
-    // phase 2: the listed contacts, in order
-    pp_contacts<PIPE, HIST>(A, S, o, mylist, nlist, hc, Ft, Fn, M, sumF, sumM);
+    // phase 2: the listed contacts, in order; then the further chunks
+    const uint32_t found = ntotal;
+    pp_contacts<PIPE, HIST>(A, S, o, mylist, min(found, (uint32_t)CMAX), hc, Ft, Fn, M, sumF, sumM);
+#pragma unroll 1
+    for (skip = CMAX; skip < found; skip += CMAX) {
+        ntotal = 0;
+        scan_cells();
+        pp_contacts<PIPE, HIST>(A, S, o, mylist, min(found - skip, (uint32_t)CMAX), hc, Ft, Fn, M,
+                                sumF, sumM);
+    }
+    ntotal = found;
     // F = m g + particle-particle (vv_update_velocity reset, then cu_calculate_p2p)
     f3 F = F3(o.m * S.gvx, o.m * S.gvy, o.m * S.gvz) + sumF;
     f3 T = sumM;
@@ -475,6 +548,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs A) {
         A.keynew[j] = key_of(S, kx, ky, kz);
     }
     A.out.pr[j] = make_double4(nx, ny, nz, ip.w);
+    A.out.pf[j] = pf_of(S, nx, ny, nz, o.r);
     A.out.vm[j] = make_float4(v.x, v.y, v.z, o.m);
     A.out.wa[j] = make_float4(w.x, w.y, w.z, al.x);
     A.out.aa[j] = make_float4(a.x, a.y, a.z, al.y);
@@ -500,6 +574,7 @@ __global__ __launch_bounds__(256) void k_pos(BState st, uint32_t* keynew, const 
         y += S.dt * (double)vm.y + S.half2dt * (double)aa.y;
         z += S.dt * (double)vm.z + S.half2dt * (double)aa.z;
         st.pr[j] = make_double4(x, y, z, p.w);
+        st.pf[j] = pf_of(S, x, y, z, (float)p.w);
     }
     int cx, cy, cz;
     cell_of(S, x, y, z, cx, cy, cz);

@@ -245,3 +245,38 @@ def test_block_size_does_not_change_results(lib):
     assert chosen // 10000 in (0, 1)
     assert len(ms) == 18 and all(v > 0 for v in ms.values())
     b.close()
+
+
+def test_many_contacts_rescan(lib, orc):
+    """A sphere touched by 40 smaller ones (more contacts than the 16-entry
+    list): the further chunks are found by rescanning the cells; the pair list
+    stays bit-exact and in traversal order, the forces within the FP32c bar."""
+    R, r = float(scenes.as_f32_exact(5e-3)), float(scenes.as_f32_exact(1.2e-3))
+    gold = np.pi * (3 - np.sqrt(5))
+    n_small = 40
+    k = np.arange(n_small)
+    z = 1 - 2 * (k + 0.5) / n_small
+    rr = np.sqrt(1 - z * z)
+    dirs = np.c_[rr * np.cos(gold * k), rr * np.sin(gold * k), z]
+    c = np.array([0.05, 0.05, 0.05])
+    pos = np.vstack([np.r_[c, R], np.c_[c + dirs * (R + r - 2e-5), np.full(n_small, r)]])
+    sc = scenes.Scene(name="crowd", pos=pos, dt=1e-5, nsteps=1, world_origin=(0.0, 0.0, 0.0),
+                      cell_size=2 * R, grid_size=(16, 16, 16), gravity=(0.0, 0.0, 0.0))
+    vel = scenes.as_f32_exact(scenes.uniform(7, 3 * sc.np).reshape(-1, 3) * 0.02 - 0.01)
+    om = np.zeros_like(vel)
+    ref = oracle_reference(orc, sc, vel, om)
+    b = Batch(sc.np, 1, int(np.prod(sc.grid_size)), lib)
+    sid = b.add(sc, vel, om)
+    b.prepare()
+    out = b.probe(sid, int(np.prod(sc.grid_size)), max_per=64)
+    cnt = out["counts"]
+    assert cnt[0] == n_small
+    got = np.concatenate([np.stack([np.full(cc, i, np.uint32), out["partners"][i, :cc]], 1)
+                          for i, cc in enumerate(cnt) if cc])
+    assert np.array_equal(got, ref["pairs"])
+    scale = contact_scale(sc.pos, vel, om, ref["m"], ref["pairs"], ref["f"])
+    ferr = (np.abs(out["force"] - ref["f"]).max(1) / scale).max()
+    terr = (np.abs(out["moment"] - ref["t"]).max(1) / (scale * sc.pos[:, 3])).max()
+    assert ferr < FP32C_TOL, ferr
+    assert terr < FP32C_TOL, terr
+    b.close()
