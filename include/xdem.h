@@ -201,6 +201,37 @@ int32_t xdem_cylinder_hertzian_contact_force_f64(
     const double* omega, double* force, double* moment, const double* mass, uint32_t np,
     const xdem_device_contact_property* cp, const xdem_double3* mpos, xdem_double3* mf,
     xdem_double3* mm, xdem_double3* out_mf, xdem_double3* out_mm);
+/* Rotating drum: the inside of a cylindrical shell of radius `radius` around
+ * the axis through `origin` along the unit vector `axis`, between the end
+ * caps at axial coordinates -half_length and +half_length, spinning at
+ * `omega` rad/s about +axis (right-hand rule).  No counterpart in the
+ * reference, whose cylinder is solid (contact on its outside,
+ * mphysics_cuda_impl.cuh:640-700); the contact law, constants and wall
+ * conventions (ir = 0, wall point velocity in dv) are those of
+ * cylinder_hertzian_contact_force_kernel (:720-774). */
+typedef struct xdem_drum_info {
+    xdem_double3 origin;
+    xdem_double3 axis;
+    double radius;
+    double half_length;
+    double omega;
+    int32_t caps;        /* 1: flat end caps spinning with the shell; 0: open ends */
+    int32_t pad_;
+} xdem_drum_info;
+#ifdef __cplusplus
+static_assert(sizeof(xdem_drum_info) == 80, "xdem_drum_info layout");
+#endif
+/* Drum contact for every particle whose centre lies between the caps: one
+ * shell contact (penetration r + rho - radius, rho = distance from the axis)
+ * and, with caps, one cap contact (penetration r - (half_length - |a|)).
+ * drum and cp are device pointers.  mf/mm (device, np entries, may be NULL)
+ * receive each particle's reaction on the drum (normal and tangential force,
+ * moment about `origin`); out_mf/out_mm (host) their sums. */
+int32_t xdem_drum_contact_force_f64(int32_t tcm, const xdem_drum_info* drum, const double* pos,
+                                    const double* vel, const double* omega, double* force,
+                                    double* moment, const double* mass, uint32_t np,
+                                    const xdem_device_contact_property* cp, xdem_double3* mf,
+                                    xdem_double3* mm, xdem_double3* out_mf, xdem_double3* out_mm);
 /* reductionD3 — mphysics_cuda_dec.cuh:157, def mphysics_cuda_def.cu:223-242.
  * Result written to host *out. */
 int32_t xdem_reduction_d3(const xdem_double3* in, uint32_t n, xdem_double3* out);
@@ -281,6 +312,45 @@ int32_t xdem_sim_add_cylinder(xdem_sim* s, const xdem_device_cylinder_info* cyl,
 int32_t xdem_sim_set_polygons(xdem_sim* s, const xdem_device_polygon_info* tris, const double* sphere4,
                               uint32_t ntri, const xdem_device_polygon_mass_info* objs,
                               const xdem_contact_pair_desc* per_object, uint32_t nobj);
+/* A rotating drum wall (xdem_drum_info), applied in order with the other walls. */
+int32_t xdem_sim_add_drum(xdem_sim* s, const xdem_drum_info* drum, const xdem_contact_pair_desc* c);
+
+/* Prescribed rigid motion of a plane or a polygon object, the reference's
+ * geometry motion condition (source/xDynamicsSolver.cpp:176-184: before
+ * every step whose time ct = k*dt satisfies ct >= start_time, the object is
+ * moved by one step of its motion; contact_particles_plane.cpp:33-49 then
+ * re-uploads the moved plane) with an angular velocity added for polygon
+ * objects.  After n active steps the object has moved by velocity*n*dt and
+ * turned by |omega|*n*dt about omega through its origin
+ * (device_polygon_mass_info.origin): origin = origin0 + velocity*tau,
+ * ep = q(omega, tau) * ep0 (Euler parameters, e0 first, Hamilton product),
+ * triangle vertices = origin + A(ep) * local, where local = A(ep0)^T (P - origin0)
+ * is taken when the motion is set.  end_time > start_time stops the motion
+ * after end_time; otherwise it never stops.  While the motion is active the
+ * object's vel/omega (device_polygon_mass_info) are velocity/omega, and zero
+ * otherwise; a moving plane keeps the reference's stationary contact
+ * velocity (mphysics_cuda_impl.cuh:600).  Planes take no rotation (omega must
+ * be zero).  Setting a motion again replaces the previous one. */
+typedef struct xdem_motion {
+    xdem_double3 velocity;
+    xdem_double3 omega;
+    double start_time;
+    double end_time;
+} xdem_motion;
+#ifdef __cplusplus
+static_assert(sizeof(xdem_motion) == 64, "xdem_motion layout");
+#endif
+enum { XDEM_OBJECT_PLANE = 0, XDEM_OBJECT_POLYGON = 1 };
+int32_t xdem_sim_set_motion(xdem_sim* s, int32_t kind, uint32_t index, const xdem_motion* m);
+/* Host copies of the current triangles (ntri entries) and their circumspheres
+ * (4*ntri doubles), either pointer may be NULL. */
+int32_t xdem_sim_get_polygons(const xdem_sim* s, xdem_device_polygon_info* tris, double* sphere4);
+/* Host copies of the polygon objects (nobj entries): pose, velocity and the
+ * reaction force/moment of the last step's triangle contacts. */
+int32_t xdem_sim_get_objects(const xdem_sim* s, xdem_device_polygon_mass_info* objs);
+/* Steps taken since creation (the current time is steps * dt). */
+uint64_t xdem_sim_steps(const xdem_sim* s);
+
 int32_t xdem_sim_step(xdem_sim* s, uint32_t nsteps);
 /* Host copies of the state (any pointer may be NULL). */
 int32_t xdem_sim_get_state(const xdem_sim* s, double* pos4, double* vel3, double* omega3,

@@ -82,6 +82,24 @@ class CylinderInfo(C.Structure):
                 ("omega", double3), ("ep", double4)]
 
 
+class DrumInfo(C.Structure):
+    """xdem_drum_info (80 B): inner cylindrical shell with optional end caps,
+    spinning at omega rad/s about +axis."""
+    _fields_ = [("origin", double3), ("axis", double3), ("radius", C.c_double),
+                ("half_length", C.c_double), ("omega", C.c_double), ("caps", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
+class Motion(C.Structure):
+    """xdem_motion (64 B): constant velocity and angular velocity from
+    start_time (to end_time when end_time > start_time)."""
+    _fields_ = [("velocity", double3), ("omega", double3), ("start_time", C.c_double),
+                ("end_time", C.c_double)]
+
+
+OBJECT_PLANE, OBJECT_POLYGON = 0, 1
+
+
 class ContactProperty(C.Structure):
     """device_contact_property — include/mphysics_cuda_dec.cuh:78-88 (88 B)."""
     _fields_ = [(n, C.c_double) for n in
@@ -134,7 +152,7 @@ class BMassInfo(C.Structure):
 
 _SIZES = {DeviceParameters: 112, PolygonInfo: 152, PlaneInfo: 232, PolygonMassInfo: 160,
           CylinderInfo: 176, ContactProperty: 88, BDeviceParameters: 144, BPlaneInfo: 160,
-          BPolygon: 144, BMassInfo: 48}
+          BPolygon: 144, BMassInfo: 48, DrumInfo: 80, Motion: 64}
 for _t, _s in _SIZES.items():
     assert C.sizeof(_t) == _s, (_t.__name__, C.sizeof(_t), _s)
 assert DeviceParameters.dt.offset == 32 and DeviceParameters.world_origin.offset == 88
@@ -216,6 +234,7 @@ def _declare(lib):
             ([i32, vp, vp, vp, u32, u32] + [vp] * 10 + [u32], i32),
         "xdem_cylinder_hertzian_contact_force_f64":
             ([i32] + [vp] * 7 + [u32] + [vp] * 6, i32),
+        "xdem_drum_contact_force_f64": ([i32] + [vp] * 7 + [u32] + [vp] * 5, i32),
         "xdem_reduction_d3": ([vp, u32, vp], i32),
         "xdem_update_polygon_object_data_f64": ([vp, vp, vp, vp, u32], i32),
         "xdem_grid_create": ([C.POINTER(vp)], i32),
@@ -239,6 +258,11 @@ def _declare(lib):
         "xdem_sim_add_cube": ([vp, vp, vp], i32),
         "xdem_sim_add_cylinder": ([vp, vp, vp], i32),
         "xdem_sim_set_polygons": ([vp, vp, vp, u32, vp, vp, u32], i32),
+        "xdem_sim_add_drum": ([vp, vp, vp], i32),
+        "xdem_sim_set_motion": ([vp, i32, u32, vp], i32),
+        "xdem_sim_get_polygons": ([vp, vp, vp], i32),
+        "xdem_sim_get_objects": ([vp, vp], i32),
+        "xdem_sim_steps": ([vp], C.c_uint64),
         "xdem_sim_step": ([vp, u32], i32),
         "xdem_sim_get_state": ([vp] * 8, i32),
         "xdem_sim_set_state": ([vp] * 8, i32),

@@ -413,6 +413,78 @@ __global__ __launch_bounds__(512) void k_cylinder(KParams k, const xdem_device_c
     }
 }
 
+// Rotating drum (inner shell + optional end caps).  No reference kernel: the
+// per-contact evaluation is that of cylinder_hertzian_contact_force_kernel
+// (:720-774) — getConstant with the wall side empty, ir = 0, dv = wall point
+// velocity - (v + w x (r n)) — applied to the inner surfaces.  The shell
+// contact comes first, then the cap contact; their sum is added once.
+template <int TCM>
+__device__ inline void drum_one(const KParams& k, const xdem_drum_info& dr, double r, double im,
+                                d3 iomega, d3 ivel, d3 unit, d3 cpt, double cdist,
+                                const xdem_device_contact_property& cp, d3& F, d3& M, d3& R,
+                                d3& RM) {
+    d3 o = of(dr.origin);
+    d3 wv = dr.omega * cross(of(dr.axis), cpt - o);
+    d3 dv = wv - (ivel + cross(iomega, r * unit));
+    double rcon = r - 0.5 * cdist;
+    d3 Ft = mk3(0, 0, 0), Fn = mk3(0, 0, 0), Mc = mk3(0, 0, 0);
+    ForceConst c = get_constant<TCM>(r, 0, im, 0, cp);
+    force_law<TCM>(c, k.dt, 0.0, 0.0, rcon, cdist, iomega, dv, unit, Ft, Fn, Mc);
+    d3 f = Fn + Ft;
+    F += f;
+    M += Mc;
+    R += -f;
+    RM += cross(cpt - o, -f);
+}
+
+template <int TCM>
+__global__ __launch_bounds__(256) void k_drum(KParams k, const xdem_drum_info* drp,
+                                              const double* pos, const double* vel,
+                                              const double* omega, double* force, double* moment,
+                                              const xdem_device_contact_property* cpp,
+                                              const double* mass, xdem_double3* mf,
+                                              xdem_double3* mm, uint32_t np) {
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= np) return;
+    const xdem_drum_info dr = *drp;
+    d4 ipos = ld4(pos, id);
+    double r = ipos.w;
+    d3 ax = of(dr.axis);
+    d3 d = mk3(ipos.x, ipos.y, ipos.z) - of(dr.origin);
+    double a = dot(d, ax);
+    d3 q = d - a * ax;
+    double rho = length(q);
+    d3 F = mk3(0, 0, 0), M = mk3(0, 0, 0), R = mk3(0, 0, 0), RM = mk3(0, 0, 0);
+    if (fabs(a) <= dr.half_length) {
+        double im = mass[id];
+        d3 ivel = ld3(vel, id), iomega = ld3(omega, id);
+        double cs = r + rho - dr.radius;
+        if (cs > 0 && rho > 0) {
+            d3 u = q / rho;
+            d3 cpt = of(dr.origin) + a * ax + dr.radius * u;
+            drum_one<TCM>(k, dr, r, im, iomega, ivel, u, cpt, cs, *cpp, F, M, R, RM);
+        }
+        double cc = r - (dr.half_length - fabs(a));
+        if (dr.caps && cc > 0 && rho < dr.radius) {
+            double sg = a >= 0 ? 1.0 : -1.0;
+            d3 u = sg * ax;
+            d3 cpt = of(dr.origin) + (sg * dr.half_length) * ax + q;
+            drum_one<TCM>(k, dr, r, im, iomega, ivel, u, cpt, cc, *cpp, F, M, R, RM);
+        }
+    }
+    add3(force, id, F);
+    add3(moment, id, M);
+    if (mf) mf[id] = xdem_double3{R.x, R.y, R.z};
+    if (mm) mm[id] = xdem_double3{RM.x, RM.y, RM.z};
+}
+
+// One lane stores a struct passed by value (a moved plane or polygon-object
+// pose): no host staging buffer, so consecutive steps never race on it.
+template <class T>
+__global__ void k_put(T* dst, T v) {
+    *dst = v;
+}
+
 // closestPtPointTriangle — mphysics_cuda_impl.cuh:837-899 (Ericson 5.1.5)
 __device__ d3 closest_pt_triangle(const xdem_device_polygon_info& t, d3 p) {
     d3 a = of(t.P), b = of(t.Q), c = of(t.R);
@@ -694,6 +766,35 @@ int32_t launch_cylinder(int tcm, const KParams& k, const xdem_device_cylinder_in
     return XDEM_OK;
 }
 
+int32_t launch_drum(int tcm, const KParams& k, const xdem_drum_info* dr, const double* pos,
+                    const double* vel, const double* omega, double* force, double* moment,
+                    const double* mass, uint32_t np, const xdem_device_contact_property* cp,
+                    xdem_double3* mf, xdem_double3* mm, hipStream_t s) {
+    if (!np) return XDEM_OK;
+    if (tcm == 0)
+        hipLaunchKernelGGL(k_drum<0>, dim3(div_up(np, 256)), dim3(256), 0, s, k, dr, pos, vel,
+                           omega, force, moment, cp, mass, mf, mm, np);
+    else
+        hipLaunchKernelGGL(k_drum<1>, dim3(div_up(np, 256)), dim3(256), 0, s, k, dr, pos, vel,
+                           omega, force, moment, cp, mass, mf, mm, np);
+    XDEM_LAUNCHED();
+    return XDEM_OK;
+}
+
+int32_t launch_put_plane(xdem_device_plane_info* dst, const xdem_device_plane_info& v,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_put<xdem_device_plane_info>, dim3(1), dim3(1), 0, s, dst, v);
+    XDEM_LAUNCHED();
+    return XDEM_OK;
+}
+
+int32_t launch_put_object(xdem_device_polygon_mass_info* dst,
+                          const xdem_device_polygon_mass_info& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_put<xdem_device_polygon_mass_info>, dim3(1), dim3(1), 0, s, dst, v);
+    XDEM_LAUNCHED();
+    return XDEM_OK;
+}
+
 int32_t launch_polygon(int tcm, const KParams& k, const xdem_device_polygon_info* dpi,
                        xdem_device_polygon_mass_info* dpmi, uint32_t nobj, const double* pos,
                        const double* vel, const double* omega, double* force, double* moment,
@@ -881,6 +982,31 @@ int32_t xdem_cylinder_hertzian_contact_force_f64(
     hipStream_t s = stream();
     XDEM_TRY(launch_cylinder(tcm, params_from(host_params()), cyl, pos, vel, omega, force, moment,
                              mass, np, cp, mpos, mf, mm, s));
+    if (out_mf && mf) {
+        d3 r;
+        XDEM_TRY(reduction_d3(reinterpret_cast<const d3*>(mf), np, &r, s));
+        *out_mf = xdem_double3{r.x, r.y, r.z};
+    }
+    if (out_mm && mm) {
+        d3 r;
+        XDEM_TRY(reduction_d3(reinterpret_cast<const d3*>(mm), np, &r, s));
+        *out_mm = xdem_double3{r.x, r.y, r.z};
+    }
+    return XDEM_OK;
+}
+
+int32_t xdem_drum_contact_force_f64(int32_t tcm, const xdem_drum_info* drum, const double* pos,
+                                    const double* vel, const double* omega, double* force,
+                                    double* moment, const double* mass, uint32_t np,
+                                    const xdem_device_contact_property* cp, xdem_double3* mf,
+                                    xdem_double3* mm, xdem_double3* out_mf, xdem_double3* out_mm) {
+    XDEM_TRY(need_params());
+    XDEM_REQUIRE(tcm == 0 || tcm == 1, "tcm must be 0 (HMCM) or 1 (DHS)");
+    XDEM_REQUIRE(np == 0 || (drum && pos && vel && omega && force && moment && mass && cp),
+                 "null array");
+    hipStream_t s = stream();
+    XDEM_TRY(launch_drum(tcm, params_from(host_params()), drum, pos, vel, omega, force, moment,
+                         mass, np, cp, mf, mm, s));
     if (out_mf && mf) {
         d3 r;
         XDEM_TRY(reduction_d3(reinterpret_cast<const d3*>(mf), np, &r, s));

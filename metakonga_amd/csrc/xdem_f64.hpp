@@ -25,6 +25,14 @@ int32_t launch_cylinder(int tcm, const KParams& k, const xdem_device_cylinder_in
                         double* moment, const double* mass, uint32_t np,
                         const xdem_device_contact_property* cp, const xdem_double3* mpos,
                         xdem_double3* mf, xdem_double3* mm, hipStream_t s);
+int32_t launch_drum(int tcm, const KParams& k, const xdem_drum_info* dr, const double* pos,
+                    const double* vel, const double* omega, double* force, double* moment,
+                    const double* mass, uint32_t np, const xdem_device_contact_property* cp,
+                    xdem_double3* mf, xdem_double3* mm, hipStream_t s);
+int32_t launch_put_plane(xdem_device_plane_info* dst, const xdem_device_plane_info& v,
+                         hipStream_t s);
+int32_t launch_put_object(xdem_device_polygon_mass_info* dst,
+                          const xdem_device_polygon_mass_info& v, hipStream_t s);
 int32_t launch_polygon(int tcm, const KParams& k, const xdem_device_polygon_info* dpi,
                        xdem_device_polygon_mass_info* dpmi, uint32_t nobj, const double* pos,
                        const double* vel, const double* omega, double* force, double* moment,

@@ -157,10 +157,52 @@ uint32_t xdem_grid_get_hash(const xdem_grid* g, const int32_t c3[3]) {
 namespace {
 
 struct WallOp {
-    int kind;        // 0 = run of planes, 1 = cylinder
-    uint32_t first;  // into planes / cylinders
+    int kind;        // 0 = run of planes, 1 = cylinder, 2 = drum
+    uint32_t first;  // into planes / cylinders / drums
     uint32_t count;
 };
+
+// A prescribed motion (xdem_motion) of plane or polygon object `index`;
+// n_act counts the steps on which it was active.
+struct Motion {
+    int32_t kind;
+    uint32_t index;
+    xdem_motion m;
+    uint64_t n_act;
+    bool active;
+};
+
+// makeTFM_1..3 (mphysics_cuda_impl.cuh:624-638): rows of A(ep), e0 = ep.x.
+void ep_rows(const xdem_double4& ep, d3& r0, d3& r1, d3& r2) {
+    r0 = mk3(2.0 * (ep.x * ep.x + ep.y * ep.y - 0.5), 2.0 * (ep.y * ep.z - ep.x * ep.w),
+             2.0 * (ep.y * ep.w + ep.x * ep.z));
+    r1 = mk3(2.0 * (ep.y * ep.z + ep.x * ep.w), 2.0 * (ep.x * ep.x + ep.z * ep.z - 0.5),
+             2.0 * (ep.z * ep.w - ep.x * ep.y));
+    r2 = mk3(2.0 * (ep.y * ep.w - ep.x * ep.z), 2.0 * (ep.z * ep.w + ep.x * ep.y),
+             2.0 * (ep.x * ep.x + ep.w * ep.w - 0.5));
+}
+
+// Pose of a moving object after tau seconds of motion: origin0 + v tau and
+// q(omega, tau) * ep0 (Hamilton product, scalar first).
+void motion_pose(const xdem_double3& o0, const xdem_double4& ep0, const xdem_motion& m,
+                 double tau, xdem_double3& o, xdem_double4& ep) {
+    o = xdem_double3{o0.x + tau * m.velocity.x, o0.y + tau * m.velocity.y,
+                     o0.z + tau * m.velocity.z};
+    const xdem_double3& w = m.omega;
+    double wn = std::sqrt(w.x * w.x + w.y * w.y + w.z * w.z);
+    if (!(wn > 0)) {
+        ep = ep0;
+        return;
+    }
+    double h = 0.5 * (wn * tau);
+    double c = std::cos(h), sn = std::sin(h);
+    double q0 = c, q1 = sn * (w.x / wn), q2 = sn * (w.y / wn), q3 = sn * (w.z / wn);
+    const xdem_double4& p = ep0;
+    ep.x = q0 * p.x - (q1 * p.y + q2 * p.z + q3 * p.w);
+    ep.y = q0 * p.y + p.x * q1 + (q2 * p.w - q3 * p.z);
+    ep.z = q0 * p.z + p.x * q2 + (q3 * p.y - q1 * p.w);
+    ep.w = q0 * p.w + p.x * q3 + (q1 * p.z - q2 * p.y);
+}
 
 template <class T>
 int32_t upload(T** d, const std::vector<T>& h) {
@@ -205,7 +247,11 @@ struct xdem_sim {
     std::vector<xdem_device_contact_property> h_plane_cp;
     std::vector<xdem_device_cylinder_info> h_cyl;
     std::vector<xdem_device_contact_property> h_cyl_cp;
+    std::vector<xdem_drum_info> h_drum;
+    std::vector<xdem_device_contact_property> h_drum_cp;
     std::vector<WallOp> ops;
+    xdem_drum_info* d_drum = nullptr;
+    xdem_device_contact_property* d_drum_cp = nullptr;
     xdem_device_plane_info* d_planes = nullptr;
     xdem_device_contact_property* d_plane_cp = nullptr;
     xdem_device_cylinder_info* d_cyl = nullptr;
@@ -216,6 +262,11 @@ struct xdem_sim {
     double* d_sph = nullptr;
     xdem_device_polygon_mass_info* d_obj = nullptr;
     xdem_device_contact_property* d_obj_cp = nullptr;
+    std::vector<xdem_device_polygon_mass_info> h_obj;  // poses at set time
+    std::vector<double> h_vlist;                         // 9 local coordinates per triangle
+    double* d_vlist = nullptr;
+    std::vector<Motion> motions;
+    uint64_t step = 0;
     bool dirty = true;
     PhaseTimer timer;
 
@@ -223,7 +274,8 @@ struct xdem_sim {
         for (void* p : {(void*)pos, (void*)vel, (void*)omega, (void*)acc, (void*)alpha,
                         (void*)force, (void*)moment, (void*)mass, (void*)iner, (void*)d_cp_pp,
                         (void*)d_planes, (void*)d_plane_cp, (void*)d_cyl, (void*)d_cyl_cp,
-                        (void*)d_tri, (void*)d_sph, (void*)d_obj, (void*)d_obj_cp})
+                        (void*)d_tri, (void*)d_sph, (void*)d_obj, (void*)d_obj_cp,
+                        (void*)d_drum, (void*)d_drum_cp, (void*)d_vlist})
             if (p) (void)hipFree(p);
     }
     KParams kparams() const { return params_from(prm); }
@@ -235,6 +287,8 @@ static int32_t sim_sync_walls(xdem_sim* s) {
     XDEM_TRY(upload(&s->d_plane_cp, s->h_plane_cp));
     XDEM_TRY(upload(&s->d_cyl, s->h_cyl));
     XDEM_TRY(upload(&s->d_cyl_cp, s->h_cyl_cp));
+    XDEM_TRY(upload(&s->d_drum, s->h_drum));
+    XDEM_TRY(upload(&s->d_drum_cp, s->h_drum_cp));
     s->dirty = false;
     return XDEM_OK;
 }
@@ -249,12 +303,59 @@ static int32_t sim_regrid(xdem_sim* s) {
     return grid_alloc(&s->grid, s->np + s->ntri);
 }
 
+// The geometry motion of step `s->step` (xDynamicsSolver.cpp:170-184: the
+// time ct = step*dt is advanced, then every object whose motion has started
+// is moved before oneStepAnalysis).  Moved planes and object poses are stored
+// by one-lane kernels taking the struct by value; the triangles of every
+// object are then re-posed from their local coordinates
+// (updatePolygonObjectData_kernel, mphysics_cuda_impl.cuh:1911-1956).
+static int32_t sim_apply_motions(xdem_sim* s, hipStream_t st) {
+    const double dt = s->prm.dt;
+    const double t = (double)s->step * dt;
+    bool repose = false;
+    for (Motion& mo : s->motions) {
+        const xdem_motion& m = mo.m;
+        bool active = t >= m.start_time && (!(m.end_time > m.start_time) || t <= m.end_time);
+        if (active) mo.n_act++;
+        const double tau = (double)mo.n_act * dt;
+        if (mo.kind == XDEM_OBJECT_PLANE) {
+            if (!active) continue;
+            xdem_device_plane_info pl = s->h_planes[mo.index];
+            auto mv = [&](xdem_double3& p) {
+                p = xdem_double3{p.x + tau * m.velocity.x, p.y + tau * m.velocity.y,
+                                 p.z + tau * m.velocity.z};
+            };
+            mv(pl.xw);
+            mv(pl.w2);
+            mv(pl.w3);
+            mv(pl.w4);
+            XDEM_TRY(launch_put_plane(s->d_planes + mo.index, pl, st));
+        } else {
+            if (!active && mo.active == active) continue;
+            const xdem_device_polygon_mass_info& b = s->h_obj[mo.index];
+            xdem_device_polygon_mass_info o = b;
+            motion_pose(b.origin, b.ep, m, tau, o.origin, o.ep);
+            o.vel = active ? m.velocity : xdem_double3{0, 0, 0};
+            o.omega = active ? m.omega : xdem_double3{0, 0, 0};
+            o.force = xdem_double3{0, 0, 0};
+            o.moment = xdem_double3{0, 0, 0};
+            XDEM_TRY(launch_put_object(s->d_obj + mo.index, o, st));
+            repose = true;
+        }
+        mo.active = active;
+    }
+    if (repose) XDEM_TRY(launch_update_polygon(s->d_obj, s->d_vlist, s->d_sph, s->d_tri, s->ntri, st));
+    return XDEM_OK;
+}
+
 // One dem_simulation::oneStepAnalysis (dem_simulation.cpp:447-469) on the GPU.
 static int32_t sim_step_once(xdem_sim* s, hipStream_t st) {
     const KParams k = s->kparams();
     const int tcm = s->desc.tcm;
     uint32_t np = s->np;
     PhaseTimer& T = s->timer;
+    s->step++;
+    if (!s->motions.empty()) XDEM_TRY(sim_apply_motions(s, st));
     XDEM_TRY(T.begin(0, st));
     XDEM_TRY(launch_vv_position(k, s->pos, s->vel, s->acc, np, st));
     XDEM_TRY(T.end(st));
@@ -272,10 +373,14 @@ static int32_t sim_step_once(xdem_sim* s, hipStream_t st) {
             XDEM_TRY(launch_planes(tcm, k, s->d_planes + op.first, s->d_plane_cp + op.first,
                                    op.count, s->pos, s->vel, s->omega, s->force, s->moment,
                                    s->mass, np, st));
-        else
+        else if (op.kind == 1)
             XDEM_TRY(launch_cylinder(tcm, k, s->d_cyl + op.first, s->pos, s->vel, s->omega,
                                      s->force, s->moment, s->mass, np, s->d_cyl_cp + op.first,
                                      nullptr, nullptr, nullptr, st));
+        else
+            XDEM_TRY(launch_drum(tcm, k, s->d_drum + op.first, s->pos, s->vel, s->omega, s->force,
+                                 s->moment, s->mass, np, s->d_drum_cp + op.first, nullptr,
+                                 nullptr, st));
     }
     XDEM_TRY(T.end(st));
     if (s->ntri) {
@@ -404,6 +509,19 @@ int32_t xdem_sim_add_cylinder(xdem_sim* s, const xdem_device_cylinder_info* cyl,
     return XDEM_OK;
 }
 
+int32_t xdem_sim_add_drum(xdem_sim* s, const xdem_drum_info* drum, const xdem_contact_pair_desc* c) {
+    XDEM_REQUIRE(s && drum && c, "null argument");
+    const xdem_double3& a = drum->axis;
+    double an = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    XDEM_REQUIRE(std::fabs(an - 1.0) < 1e-12, "drum axis must be a unit vector (|axis| = %.17g)", an);
+    XDEM_REQUIRE(drum->radius > 0 && drum->half_length > 0, "drum radius and half length must be > 0");
+    s->h_drum.push_back(*drum);
+    s->h_drum_cp.push_back(make_cp(s->desc.particle, *c));
+    push_op(s, 2, (uint32_t)s->h_drum.size() - 1);
+    s->dirty = true;
+    return XDEM_OK;
+}
+
 // contact_particles_polygonObjects::define (contact_particles_polygonObjects.cpp:47-123):
 // triangles with their circumspheres are appended to the hash arrays at sid = np.
 int32_t xdem_sim_set_polygons(xdem_sim* s, const xdem_device_polygon_info* tris,
@@ -424,12 +542,76 @@ int32_t xdem_sim_set_polygons(xdem_sim* s, const xdem_device_polygon_info* tris,
     XDEM_TRY(upload(&s->d_sph, hs));
     XDEM_TRY(upload(&s->d_obj, ho));
     XDEM_TRY(upload(&s->d_obj_cp, hc));
+    // local vertex coordinates, A(ep0)^T (P - origin0) as toLocal
+    // (mphysics_cuda_impl.cuh:640-645), for objects that are later moved
+    s->h_vlist.assign(9 * (size_t)ntri, 0.0);
+    for (uint32_t t = 0; t < ntri; ++t) {
+        const xdem_device_polygon_mass_info& o = ho[tris[t].id];
+        d3 r0, r1, r2;
+        ep_rows(o.ep, r0, r1, r2);
+        const xdem_double3* v[3] = {&tris[t].P, &tris[t].Q, &tris[t].R};
+        for (int j = 0; j < 3; ++j) {
+            d3 d = mk3(v[j]->x - o.origin.x, v[j]->y - o.origin.y, v[j]->z - o.origin.z);
+            double* l = &s->h_vlist[9 * (size_t)t + 3 * j];
+            l[0] = r0.x * d.x + r1.x * d.y + r2.x * d.z;
+            l[1] = r0.y * d.x + r1.y * d.y + r2.y * d.z;
+            l[2] = r0.z * d.x + r1.z * d.y + r2.z * d.z;
+        }
+    }
+    XDEM_TRY(upload(&s->d_vlist, s->h_vlist));
+    s->h_obj = ho;
+    s->motions.erase(std::remove_if(s->motions.begin(), s->motions.end(),
+                                    [](const Motion& m) { return m.kind == XDEM_OBJECT_POLYGON; }),
+                     s->motions.end());
     s->ntri = ntri;
     s->nobj = nobj;
     // dem_simulation.cpp:226-227,249-250: the cell size covers the largest circumsphere.
     for (uint32_t t = 0; t < ntri; ++t) s->max_radius = std::max(s->max_radius, hs[4 * t + 3]);
     return sim_regrid(s);
 }
+
+int32_t xdem_sim_set_motion(xdem_sim* s, int32_t kind, uint32_t index, const xdem_motion* m) {
+    XDEM_REQUIRE(s && m, "null argument");
+    XDEM_REQUIRE(kind == XDEM_OBJECT_PLANE || kind == XDEM_OBJECT_POLYGON,
+                 "kind must be XDEM_OBJECT_PLANE or XDEM_OBJECT_POLYGON");
+    if (kind == XDEM_OBJECT_PLANE) {
+        XDEM_REQUIRE(index < s->h_planes.size(), "plane %u out of range [0,%zu)", index,
+                     s->h_planes.size());
+        XDEM_REQUIRE(m->omega.x == 0 && m->omega.y == 0 && m->omega.z == 0,
+                     "a plane moves by translation only");
+    } else {
+        XDEM_REQUIRE(index < s->nobj, "polygon object %u out of range [0,%u)", index, s->nobj);
+    }
+    for (Motion& mo : s->motions)
+        if (mo.kind == kind && mo.index == index) {
+            mo.m = *m;
+            return XDEM_OK;
+        }
+    s->motions.push_back(Motion{kind, index, *m, 0, true});
+    return XDEM_OK;
+}
+
+int32_t xdem_sim_get_polygons(const xdem_sim* s, xdem_device_polygon_info* tris, double* sphere4) {
+    XDEM_REQUIRE(s, "null sim");
+    XDEM_HIP(hipStreamSynchronize(stream()));
+    if (tris && s->ntri)
+        XDEM_HIP(hipMemcpy(tris, s->d_tri, sizeof(xdem_device_polygon_info) * s->ntri,
+                           hipMemcpyDeviceToHost));
+    if (sphere4 && s->ntri)
+        XDEM_HIP(hipMemcpy(sphere4, s->d_sph, 4 * sizeof(double) * s->ntri, hipMemcpyDeviceToHost));
+    return XDEM_OK;
+}
+
+int32_t xdem_sim_get_objects(const xdem_sim* s, xdem_device_polygon_mass_info* objs) {
+    XDEM_REQUIRE(s && (objs || !s->nobj), "null argument");
+    XDEM_HIP(hipStreamSynchronize(stream()));
+    if (s->nobj)
+        XDEM_HIP(hipMemcpy(objs, s->d_obj, sizeof(xdem_device_polygon_mass_info) * s->nobj,
+                           hipMemcpyDeviceToHost));
+    return XDEM_OK;
+}
+
+uint64_t xdem_sim_steps(const xdem_sim* s) { return s ? s->step : 0; }
 
 int32_t xdem_sim_step(xdem_sim* s, uint32_t nsteps) {
     XDEM_REQUIRE(s, "null sim");

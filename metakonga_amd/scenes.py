@@ -174,6 +174,31 @@ def circumcenter(P, Q, R):
     return M1 + t[..., None] * D1
 
 
+def motion(velocity=(0.0, 0.0, 0.0), omega=(0.0, 0.0, 0.0), start_time=0.0,
+           end_time=0.0) -> abi.Motion:
+    """A prescribed rigid motion (xdem_motion): constant velocity and angular
+    velocity from start_time (until end_time when end_time > start_time)."""
+    return abi.Motion(abi._d3(velocity), abi._d3(omega), float(start_time), float(end_time))
+
+
+def drum_info(origin=(0.0, 0.0, 0.0), axis=(0.0, 0.0, 1.0), radius=0.5, half_length=0.2,
+              omega=0.0, caps=True) -> abi.DrumInfo:
+    """A rotating drum wall (xdem_drum_info); the axis is normalised here."""
+    a = _v3(axis)
+    a = a / math.sqrt(float(a @ a))
+    return abi.DrumInfo(abi._d3(origin), abi._d3(a), float(radius), float(half_length),
+                        float(omega), 1 if caps else 0, 0)
+
+
+@dataclass
+class PolygonObject:
+    """One rigid triangle-mesh object (device_polygon_mass_info): pose of
+    its frame and an optional prescribed motion about its origin."""
+    origin: tuple = (0.0, 0.0, 0.0)
+    ep: tuple = (1.0, 0.0, 0.0, 0.0)       # Euler parameters, e0 first
+    motion: abi.Motion | None = None
+
+
 # ------------------------------------------------------------------ scenes
 @dataclass
 class Scene:
@@ -191,6 +216,10 @@ class Scene:
     planes: list = field(default_factory=list)        # PlaneInfo, applied in order
     cylinders: list = field(default_factory=list)     # CylinderInfo
     tri: np.ndarray | None = None                     # (T,3,3) triangle vertices
+    tri_object: np.ndarray | None = None              # (T,) object of each triangle (default 0)
+    objects: list = field(default_factory=list)       # PolygonObject per object id
+    drums: list = field(default_factory=list)         # DrumInfo, applied after the cylinders
+    plane_motions: dict = field(default_factory=dict)  # plane index -> abi.Motion
     rolling: bool = True
     tcm: int = 1
     seed: int = 0
@@ -216,6 +245,29 @@ class Scene:
         d.gravity = abi._d3(self.gravity)
         d.world_origin = abi._d3(self.world_origin)
         return d
+
+    def polygon_arrays(self):
+        """(triangles, circumspheres (T,4), objects) as the C ABI takes them
+        (contact_particles_polygonObjects::define, source/contact_particles_polygonObjects.cpp:47-123:
+        circumcentre and circumradius per triangle)."""
+        tri = np.ascontiguousarray(self.tri, dtype=np.float64)
+        nt = tri.shape[0]
+        ids = (np.zeros(nt, np.int32) if self.tri_object is None
+               else np.asarray(self.tri_object, np.int32))
+        objs = self.objects or [PolygonObject()]
+        cc = circumcenter(tri[:, 0], tri[:, 1], tri[:, 2])
+        rad = np.sqrt(((cc - tri[:, 0]) ** 2).sum(1))
+        sph = np.ascontiguousarray(np.concatenate([cc, rad[:, None]], 1))
+        dpi = abi.struct_array(abi.PolygonInfo, [
+            abi.PolygonInfo(int(ids[t]), abi._d3(tri[t, 0]), abi._d3(tri[t, 1]), abi._d3(tri[t, 2]))
+            for t in range(nt)])
+        dpmi = []
+        for ob in objs:
+            mi = abi.PolygonMassInfo()
+            mi.origin = abi._d3(ob.origin)
+            mi.ep = abi.double4(*ob.ep)
+            dpmi.append(mi)
+        return dpi, sph, abi.struct_array(abi.PolygonMassInfo, dpmi)
 
     def contact_pp(self) -> abi.ContactProperty:
         return contact_property(self.material, self.material, self.contact)

@@ -50,6 +50,24 @@ class Simulation:
             abi.check(self.lib, self.lib.xdem_sim_add_plane(h, C.byref(pl), C.byref(wall)))
         for cy in scene.cylinders:
             abi.check(self.lib, self.lib.xdem_sim_add_cylinder(h, C.byref(cy), C.byref(wall)))
+        for dr in scene.drums:
+            abi.check(self.lib, self.lib.xdem_sim_add_drum(h, C.byref(dr), C.byref(wall)))
+        self.ntri = self.nobj = 0
+        if scene.tri is not None:
+            dpi, sph, dpmi = scene.polygon_arrays()
+            per = (abi.ContactPairDesc * len(dpmi))(*([wall] * len(dpmi)))
+            abi.check(self.lib, self.lib.xdem_sim_set_polygons(
+                h, C.cast(dpi, C.c_void_p), abi.ptr(sph), len(dpi), C.cast(dpmi, C.c_void_p),
+                C.cast(per, C.c_void_p), len(dpmi)))
+            self.ntri, self.nobj = len(dpi), len(dpmi)
+        # contact_particles_plane::cuda_collision re-uploads a moving plane every
+        # step (source/collision/contact_particles_plane.cpp:33-49)
+        for i, m in sorted(scene.plane_motions.items()):
+            abi.check(self.lib, self.lib.xdem_sim_set_motion(h, abi.OBJECT_PLANE, i, C.byref(m)))
+        for o, ob in enumerate(scene.objects):
+            if ob.motion is not None:
+                abi.check(self.lib, self.lib.xdem_sim_set_motion(h, abi.OBJECT_POLYGON, o,
+                                                                  C.byref(ob.motion)))
 
     @property
     def np(self):
@@ -79,6 +97,24 @@ class Simulation:
             a = kw.get(k)
             arrs.append(None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(n, shp))
         abi.check(self.lib, self.lib.xdem_sim_set_state(self.h, *(abi.ptr(a) for a in arrs)))
+
+    @property
+    def steps(self):
+        return int(self.lib.xdem_sim_steps(self.h))
+
+    def polygons(self):
+        """Current triangles (ctypes array of PolygonInfo) and circumspheres (T,4)."""
+        tris = (abi.PolygonInfo * self.ntri)()
+        sph = np.empty((self.ntri, 4))
+        abi.check(self.lib, self.lib.xdem_sim_get_polygons(self.h, C.cast(tris, C.c_void_p),
+                                                            abi.ptr(sph)))
+        return tris, sph
+
+    def objects(self):
+        """Polygon objects (PolygonMassInfo): pose, velocity, last reaction."""
+        objs = (abi.PolygonMassInfo * self.nobj)()
+        abi.check(self.lib, self.lib.xdem_sim_get_objects(self.h, C.cast(objs, C.c_void_p)))
+        return objs
 
     def mass_inertia(self):
         ptrs = [C.c_void_p() for _ in range(9)]

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 ORC_GPU, ORC_HOST = 0, 1
-ORC_WALL_PLANE, ORC_WALL_CYLINDER = 0, 1
+ORC_WALL_PLANE, ORC_WALL_CYLINDER, ORC_WALL_DRUM = 0, 1, 2
 
 
 class OrcWall(C.Structure):
@@ -31,7 +31,15 @@ class OrcWorld(C.Structure):
         ("planes", C.c_void_p), ("cylinders", C.c_void_p), ("ntri", C.c_uint32),
         ("tri", C.c_void_p), ("tri_sphere", C.c_void_p), ("nobj", C.c_uint32), ("obj", C.c_void_p),
         ("cp_obj", C.c_void_p)] + [(n, C.c_void_p) for n in ("hash", "index", "sorted", "cstart",
-                                                              "cend")]
+                                                              "cend")] + [
+        ("drums", C.c_void_p), ("nmotion", C.c_uint32), ("motions", C.c_void_p),
+        ("planes0", C.c_void_p), ("obj0", C.c_void_p), ("vlist", C.c_void_p),
+        ("step", C.c_uint64)]
+
+
+class OrcMotion(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("index", C.c_uint32), ("m", abi.Motion),
+                ("n_act", C.c_uint64), ("active", C.c_int32), ("pad_", C.c_int32)]
 
 
 class OrcHist(C.Structure):
@@ -100,6 +108,9 @@ def _declare(L):
         "orc_vv_velocity": ([vp, C.c_int] + [vp] * 8 + [u32], None),
         "orc_reduction_d3": ([vp, u32, vp], None),
         "orc_update_polygon": ([vp, vp, vp, vp, u32], None),
+        "orc_drum": ([vp, i32, vp] + [vp] * 6 + [u32, vp, vp, vp], None),
+        "orc_motion_pose": ([vp, vp, vp, f64, vp, vp], None),
+        "orc_local_vertices": ([vp, u32, vp, vp], None),
         "orc_circumcenter": ([vp, vp, vp, vp], None),
         "orc_step": ([vp, u32], None),
         "orc_step_omp": ([vp, u32, i32], None),
@@ -226,6 +237,32 @@ def cylinder(prm, tcm, cy, pos4, vel, omega, force, moment, mass, cp, mpos=None)
     return mf, mm
 
 
+def drum(prm, tcm, dr, pos4, vel, omega, force, moment, mass, cp, reaction=False):
+    """Drum contact; with reaction=True returns the per-particle (mf, mm)."""
+    n = len(mass)
+    mf = np.zeros((n, 3)) if reaction else None
+    mm = np.zeros((n, 3)) if reaction else None
+    load().orc_drum(byref(prm), tcm, byref(dr), P(pos4), P(vel), P(omega), P(force), P(moment),
+                    P(mass), n, byref(cp), P(mf) if reaction else None,
+                    P(mm) if reaction else None)
+    return (mf, mm) if reaction else None
+
+
+def motion_pose(o0, ep0, motion, tau):
+    o = abi.double3()
+    ep = abi.double4()
+    load().orc_motion_pose(byref(abi._d3(o0)), byref(abi.double4(*ep0)), byref(motion), tau,
+                           byref(o), byref(ep))
+    return np.array([o.x, o.y, o.z]), np.array([ep.x, ep.y, ep.z, ep.w])
+
+
+def local_vertices(dpi_arr, dpmi_arr):
+    v = np.zeros((len(dpi_arr), 9))
+    load().orc_local_vertices(C.cast(dpi_arr, C.c_void_p), len(dpi_arr),
+                              C.cast(dpmi_arr, C.c_void_p), P(v))
+    return v
+
+
 def closest_pt_triangle(tri, p):
     p = abi.as_f64(p)
     out = np.zeros(3)
@@ -303,24 +340,45 @@ class World:
             walls.append(OrcWall(ORC_WALL_PLANE, i, 0, 0, cpw))
         for i, _ in enumerate(scene.cylinders):
             walls.append(OrcWall(ORC_WALL_CYLINDER, i, 0, 0, cpw))
+        for i, _ in enumerate(scene.drums):
+            walls.append(OrcWall(ORC_WALL_DRUM, i, 0, 0, cpw))
         self.walls = abi.struct_array(OrcWall, walls) if walls else None
         self.planes = abi.struct_array(abi.PlaneInfo, scene.planes) if scene.planes else None
+        self.planes0 = abi.struct_array(abi.PlaneInfo, scene.planes) if scene.planes else None
+        self.drums = abi.struct_array(abi.DrumInfo, scene.drums) if scene.drums else None
+        w.drums = C.cast(self.drums, C.c_void_p) if self.drums else None
+        w.planes0 = C.cast(self.planes0, C.c_void_p) if self.planes0 else None
         self.cyls = abi.struct_array(abi.CylinderInfo, scene.cylinders) if scene.cylinders else None
         w.nwall = len(walls)
         w.walls = C.cast(self.walls, C.c_void_p) if walls else None
         w.planes = C.cast(self.planes, C.c_void_p) if self.planes else None
         w.cylinders = C.cast(self.cyls, C.c_void_p) if self.cyls else None
         nt = 0
+        if tri_info is None and scene.tri is not None:
+            dpi_arr, sph, dpmi_arr = scene.polygon_arrays()
+            cpo_arr = abi.struct_array(abi.ContactProperty, [cpw] * len(dpmi_arr))
+            tri_info = (dpi_arr, sph, dpmi_arr, cpo_arr)
         if tri_info is not None:
             dpi_arr, sph, dpmi_arr, cpo_arr = tri_info
             nt = len(dpi_arr)
-            self.dpi, self.sph, self.dpmi, self.cpo = dpi_arr, abi.as_f64(sph), dpmi_arr, cpo_arr
+            self.dpi, self.sph, self.dpmi, self.cpo = dpi_arr, abi.as_f64(sph).copy(), dpmi_arr, cpo_arr
+            self.dpmi0 = abi.struct_array(abi.PolygonMassInfo, list(dpmi_arr))
+            self.vlist = local_vertices(dpi_arr, dpmi_arr)
             w.ntri = nt
             w.tri = C.cast(dpi_arr, C.c_void_p)
             w.tri_sphere = self.sph.ctypes.data
             w.nobj = len(dpmi_arr)
             w.obj = C.cast(dpmi_arr, C.c_void_p)
             w.cp_obj = C.cast(cpo_arr, C.c_void_p)
+            w.obj0 = C.cast(self.dpmi0, C.c_void_p)
+            w.vlist = self.vlist.ctypes.data
+        motions = [OrcMotion(abi.OBJECT_PLANE, i, m, 0, 1, 0)
+                   for i, m in sorted(scene.plane_motions.items())]
+        motions += [OrcMotion(abi.OBJECT_POLYGON, o, ob.motion, 0, 1, 0)
+                    for o, ob in enumerate(scene.objects) if ob.motion is not None]
+        self.motions = abi.struct_array(OrcMotion, motions) if motions else None
+        w.nmotion = len(motions)
+        w.motions = C.cast(self.motions, C.c_void_p) if motions else None
         ncell = w.prm.ncell
         self.hash = np.zeros(n + nt, np.uint32)
         self.index = np.zeros(n + nt, np.uint32)

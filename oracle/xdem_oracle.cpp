@@ -34,7 +34,18 @@
 // ranges = invariants; pair list = O(N^2) brute force; forces = two-sphere
 // closed forms, Appendix-C constants, sphere-on-plane static overlap; closest
 // point = brute-force barycentric minimisation; Verlet = exact parabola;
-// reduction = exact integer-valued sums; circumcentre = equidistance.
+// reduction = exact integer-valued sums; circumcentre = equidistance; drum =
+// static shell and cap overlaps, spinning-wall stick and Coulomb-saturated
+// tangential closed forms, drive torque, invariance under a rotated axis;
+// motion pose = rotation/translation closed forms, composition order and the
+// active-step count.
+//
+// Rotating drum and prescribed motion have no reference kernel: orc_drum
+// applies the cylinder kernel's contact evaluation (mphysics_cuda_impl.cuh:
+// 720-774) to the inside of a spinning shell and its end caps; the motion
+// follows the geometry motion condition (xDynamicsSolver.cpp:176-184,
+// contact_particles_plane.cpp:33-49) and updatePolygonObjectData
+// (mphysics_cuda_impl.cuh:1911-1956).
 #include "xdem_oracle.h"
 
 #include <algorithm>
@@ -639,6 +650,69 @@ void orc_cylinder(const xdem_device_parameters* p, int32_t tcm, const xdem_devic
     }
 }
 
+namespace {
+// One drum contact: the cylinder kernel's evaluation (ir = 0, wall point
+// velocity minus the particle's contact-point velocity) on an inner surface.
+void drum_one(const xdem_device_parameters* p, int32_t tcm, const xdem_drum_info* dr, double r,
+              double im, V3 iomega, V3 ivel, V3 unit, V3 cpt, double cdist,
+              const xdem_device_contact_property& cp, V3& F, V3& M, V3& R, V3& RM) {
+    V3 o = of(dr->origin);
+    V3 wv = dr->omega * cross(of(dr->axis), cpt - o);
+    V3 dv = wv - (ivel + cross(iomega, r * unit));
+    double rcon = r - 0.5 * cdist;
+    V3 Ft = mk(0, 0, 0), Fn = mk(0, 0, 0), Mc = mk(0, 0, 0);
+    xdem_device_force_constant c = force_constant(tcm, r, 0, im, 0, cp);
+    gpu_law(tcm, c, p->dt, 0.0, 0.0, rcon, cdist, iomega, dv, unit, Ft, Fn, Mc);
+    V3 f = Fn + Ft;
+    F += f;
+    M += Mc;
+    R += -f;
+    RM += cross(cpt - o, -f);
+}
+}  // namespace
+
+// Rotating drum: shell contact (penetration r + rho - R, rho = distance from
+// the axis) then cap contact (r - (half_length - |a|)), for centres between
+// the caps; their sum is added once.
+void orc_drum(const xdem_device_parameters* p, int32_t tcm, const xdem_drum_info* dr,
+              const double* pos4, const double* vel, const double* omega, double* force,
+              double* moment, const double* mass, uint32_t np,
+              const xdem_device_contact_property* cp, double* mf3, double* mm3) {
+    V3 ax = of(dr->axis);
+    for (uint32_t id = 0; id < np; ++id) {
+        const double* ip = pos4 + 4 * (size_t)id;
+        double r = ip[3];
+        V3 d = ld4(ip) - of(dr->origin);
+        double a = dot(d, ax);
+        V3 q = d - a * ax;
+        double rho = std::sqrt(dot(q, q));
+        V3 F = mk(0, 0, 0), M = mk(0, 0, 0), R = mk(0, 0, 0), RM = mk(0, 0, 0);
+        if (std::fabs(a) <= dr->half_length) {
+            double im = mass[id];
+            V3 ivel = ld3(vel + 3 * (size_t)id), iomega = ld3(omega + 3 * (size_t)id);
+            double cs = r + rho - dr->radius;
+            if (cs > 0 && rho > 0) {
+                V3 u = q / rho;
+                V3 cpt = of(dr->origin) + a * ax + dr->radius * u;
+                drum_one(p, tcm, dr, r, im, iomega, ivel, u, cpt, cs, *cp, F, M, R, RM);
+            }
+            double cc = r - (dr->half_length - std::fabs(a));
+            if (dr->caps && cc > 0 && rho < dr->radius) {
+                double sg = a >= 0 ? 1.0 : -1.0;
+                V3 u = sg * ax;
+                V3 cpt = of(dr->origin) + (sg * dr->half_length) * ax + q;
+                drum_one(p, tcm, dr, r, im, iomega, ivel, u, cpt, cc, *cp, F, M, R, RM);
+            }
+        }
+        double* f = force + 3 * (size_t)id;
+        double* mo = moment + 3 * (size_t)id;
+        f[0] += F.x; f[1] += F.y; f[2] += F.z;
+        mo[0] += M.x; mo[1] += M.y; mo[2] += M.z;
+        if (mf3) st3(mf3 + 3 * (size_t)id, R);
+        if (mm3) st3(mm3 + 3 * (size_t)id, RM);
+    }
+}
+
 // closestPtPointTriangle — mphysics_cuda_impl.cuh:837-899 (Ericson, Real-Time
 // Collision Detection, 5.1.5).
 void orc_closest_pt_triangle(const xdem_device_polygon_info* t, const double p_[3], double out[3]) {
@@ -909,9 +983,87 @@ void orc_update_polygon(const xdem_device_polygon_mass_info* dpmi, const double*
     }
 }
 
+// Pose after tau seconds of motion: origin0 + v tau, q(omega, tau) * ep0
+// (Hamilton product, e0 first), so A(ep) = A(q) A(ep0).
+void orc_motion_pose(const xdem_double3* o0, const xdem_double4* ep0, const xdem_motion* m,
+                     double tau, xdem_double3* o, xdem_double4* ep) {
+    *o = xdem_double3{o0->x + tau * m->velocity.x, o0->y + tau * m->velocity.y,
+                      o0->z + tau * m->velocity.z};
+    const xdem_double3& w = m->omega;
+    double wn = std::sqrt(w.x * w.x + w.y * w.y + w.z * w.z);
+    if (!(wn > 0)) {
+        *ep = *ep0;
+        return;
+    }
+    double h = 0.5 * (wn * tau);
+    double c = std::cos(h), sn = std::sin(h);
+    double q0 = c, q1 = sn * (w.x / wn), q2 = sn * (w.y / wn), q3 = sn * (w.z / wn);
+    const xdem_double4& a = *ep0;
+    xdem_double4 r;
+    r.x = q0 * a.x - (q1 * a.y + q2 * a.z + q3 * a.w);
+    r.y = q0 * a.y + a.x * q1 + (q2 * a.w - q3 * a.z);
+    r.z = q0 * a.z + a.x * q2 + (q3 * a.y - q1 * a.w);
+    r.w = q0 * a.w + a.x * q3 + (q1 * a.z - q2 * a.y);
+    *ep = r;
+}
+
+// toLocal (mphysics_cuda_impl.cuh:640-645) of each vertex relative to its
+// object's origin.
+void orc_local_vertices(const xdem_device_polygon_info* tris, uint32_t ntri,
+                        const xdem_device_polygon_mass_info* objs, double* vlist) {
+    for (uint32_t t = 0; t < ntri; ++t) {
+        const xdem_device_polygon_mass_info& o = objs[tris[t].id];
+        V3 r0 = tfm1(o.ep), r1 = tfm2(o.ep), r2 = tfm3(o.ep);
+        const xdem_double3* v[3] = {&tris[t].P, &tris[t].Q, &tris[t].R};
+        for (int j = 0; j < 3; ++j) {
+            V3 d = mk(v[j]->x - o.origin.x, v[j]->y - o.origin.y, v[j]->z - o.origin.z);
+            st3(vlist + 9 * (size_t)t + 3 * j, to_local(r0, r1, r2, d));
+        }
+    }
+}
+
 }  // extern "C"
 
 namespace {
+
+// Geometry motion of step w->step (xDynamicsSolver.cpp:170-184): time
+// ct = step*dt, every motion whose start has come moves one step; moved planes
+// are rewritten (contact_particles_plane.cpp:33-49), moved objects get their
+// pose and velocity and then every triangle is re-posed from its local
+// coordinates (updatePolygonObjectData_kernel).
+void apply_motions(orc_world* w) {
+    const double dt = w->prm.dt;
+    const double t = (double)w->step * dt;
+    bool repose = false;
+    for (uint32_t i = 0; i < w->nmotion; ++i) {
+        orc_motion& mo = w->motions[i];
+        const xdem_motion& m = mo.m;
+        bool active = t >= m.start_time && (!(m.end_time > m.start_time) || t <= m.end_time);
+        if (active) mo.n_act++;
+        const double tau = (double)mo.n_act * dt;
+        if (mo.kind == XDEM_OBJECT_PLANE) {
+            if (!active) continue;
+            xdem_device_plane_info pl = w->planes0[mo.index];
+            for (xdem_double3* q : {&pl.xw, &pl.w2, &pl.w3, &pl.w4})
+                *q = xdem_double3{q->x + tau * m.velocity.x, q->y + tau * m.velocity.y,
+                                  q->z + tau * m.velocity.z};
+            w->planes[mo.index] = pl;
+        } else {
+            if (!active && (mo.active != 0) == active) continue;
+            const xdem_device_polygon_mass_info& b = w->obj0[mo.index];
+            xdem_device_polygon_mass_info o = b;
+            orc_motion_pose(&b.origin, &b.ep, &m, tau, &o.origin, &o.ep);
+            o.vel = active ? m.velocity : xdem_double3{0, 0, 0};
+            o.omega = active ? m.omega : xdem_double3{0, 0, 0};
+            o.force = xdem_double3{0, 0, 0};
+            o.moment = xdem_double3{0, 0, 0};
+            w->obj[mo.index] = o;
+            repose = true;
+        }
+        mo.active = active ? 1 : 0;
+    }
+    if (repose) orc_update_polygon(w->obj, w->vlist, w->tri_sphere, w->tri, w->ntri);
+}
 
 // Walls of one particle in the CPU path (contactManager.cpp:228-234): a lone
 // plane assigns F/M (contact_particles_plane.cpp:72,233 -> contact.cpp:43,45,
@@ -959,6 +1111,8 @@ void detection(orc_world* w) {
 void step_gpu(orc_world* w, int nthreads) {
     const xdem_device_parameters* p = &w->prm;
     uint32_t np = w->np;
+    w->step++;
+    if (w->nmotion) apply_motions(w);
     orc_vv_position(p, ORC_GPU, w->pos, w->vel, w->acc, np);
     detection(w);
 #pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
@@ -978,9 +1132,12 @@ void step_gpu(orc_world* w, int nthreads) {
         if (wl.kind == ORC_WALL_PLANE)
             orc_plane(p, w->tcm, &w->planes[wl.index], w->pos, w->vel, w->omega, w->force,
                       w->moment, w->mass, np, &wl.cp);
-        else
+        else if (wl.kind == ORC_WALL_CYLINDER)
             orc_cylinder(p, w->tcm, &w->cylinders[wl.index], w->pos, w->vel, w->omega, w->force,
                          w->moment, w->mass, np, &wl.cp, nullptr, nullptr, nullptr);
+        else
+            orc_drum(p, w->tcm, &w->drums[wl.index], w->pos, w->vel, w->omega, w->force,
+                     w->moment, w->mass, np, &wl.cp, nullptr, nullptr);
     }
     if (w->ntri)
         orc_polygon(p, w->tcm, w->tri, w->obj, w->nobj, 1, w->pos, w->vel, w->omega, w->force,
@@ -993,6 +1150,8 @@ void step_gpu(orc_world* w, int nthreads) {
 void step_host(orc_world* w, int nthreads) {
     const xdem_device_parameters* p = &w->prm;
     uint32_t np = w->np;
+    w->step++;
+    if (w->nmotion) apply_motions(w);
     orc_vv_position(p, ORC_HOST, w->pos, w->vel, w->acc, np);
     detection(w);
 #pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)

@@ -26,16 +26,27 @@ extern "C" {
 #endif
 
 enum { ORC_GPU = 0, ORC_HOST = 1 };
-enum { ORC_WALL_PLANE = 0, ORC_WALL_CYLINDER = 1 };
+enum { ORC_WALL_PLANE = 0, ORC_WALL_CYLINDER = 1, ORC_WALL_DRUM = 2 };
 
 /* One wall contact in application order. */
 typedef struct orc_wall {
-    int32_t kind;          /* ORC_WALL_PLANE / ORC_WALL_CYLINDER */
-    int32_t index;         /* into planes[] or cylinders[] */
+    int32_t kind;          /* ORC_WALL_PLANE / ORC_WALL_CYLINDER / ORC_WALL_DRUM */
+    int32_t index;         /* into planes[], cylinders[] or drums[] */
     int32_t cube_member;   /* 1: plane of a cube (CPU path accumulates; a lone plane assigns) */
     int32_t pad_;
     xdem_device_contact_property cp;
 } orc_wall;
+
+/* A prescribed motion (xdem_motion) of plane or polygon object `index`
+ * (kind XDEM_OBJECT_PLANE / XDEM_OBJECT_POLYGON); n_act counts active steps. */
+typedef struct orc_motion {
+    int32_t kind;
+    uint32_t index;
+    xdem_motion m;
+    uint64_t n_act;
+    int32_t active;
+    int32_t pad_;
+} orc_motion;
 
 /* A whole single-scene state, host memory, reference layout. */
 typedef struct orc_world {
@@ -55,12 +66,12 @@ typedef struct orc_world {
     xdem_device_contact_property cp_pp;
     uint32_t nwall;
     const orc_wall* walls;
-    const xdem_device_plane_info* planes;
+    xdem_device_plane_info* planes;   /* moving planes are rewritten every step */
     const xdem_device_cylinder_info* cylinders;
-    /* triangles (polygon objects) */
+    /* triangles (polygon objects); rewritten every step when an object moves */
     uint32_t ntri;
-    const xdem_device_polygon_info* tri;
-    const double* tri_sphere;    /* 4*ntri circumspheres */
+    xdem_device_polygon_info* tri;
+    double* tri_sphere;          /* 4*ntri circumspheres */
     uint32_t nobj;
     xdem_device_polygon_mass_info* obj;
     const xdem_device_contact_property* cp_obj;
@@ -70,6 +81,14 @@ typedef struct orc_world {
     uint32_t* sorted;
     uint32_t* cstart;
     uint32_t* cend;
+    /* drums and prescribed motion */
+    const xdem_drum_info* drums;
+    uint32_t nmotion;
+    orc_motion* motions;
+    const xdem_device_plane_info* planes0;       /* planes as added */
+    const xdem_device_polygon_mass_info* obj0;   /* object poses as set */
+    const double* vlist;                         /* 9*ntri local vertex coordinates */
+    uint64_t step;                               /* steps taken */
 } orc_world;
 
 /* grid_base::getCellNumber / calcGridPos + calcGridHash */
@@ -129,6 +148,17 @@ void orc_vv_velocity(const xdem_device_parameters* p, int semantics, double* vel
 /* reductionD3: same block-tree order as reduce6<double3,512> + host block sum */
 void orc_reduction_d3(const double* in3, uint32_t n, double out[3]);
 /* updatePolygonObjectData_kernel (mphysics_cuda_impl.cuh:1911-1956) */
+/* Rotating drum contact (xdem_drum_contact_force_f64); mf3/mm3 may be NULL. */
+void orc_drum(const xdem_device_parameters* p, int32_t tcm, const xdem_drum_info* dr,
+              const double* pos4, const double* vel, const double* omega, double* force,
+              double* moment, const double* mass, uint32_t np,
+              const xdem_device_contact_property* cp, double* mf3, double* mm3);
+/* Pose after tau seconds of motion m from (o0, ep0). */
+void orc_motion_pose(const xdem_double3* o0, const xdem_double4* ep0, const xdem_motion* m,
+                     double tau, xdem_double3* o, xdem_double4* ep);
+/* Local vertex coordinates A(ep0)^T (P - origin0), 9 per triangle. */
+void orc_local_vertices(const xdem_device_polygon_info* tris, uint32_t ntri,
+                        const xdem_device_polygon_mass_info* objs, double* vlist);
 void orc_update_polygon(const xdem_device_polygon_mass_info* dpmi, const double* vList,
                         double* sphere4, xdem_device_polygon_info* dpi, uint32_t ntri);
 /* numeric::utility::calculate_center_of_triangle (source/numeric_utility.cpp:20-45) */
