@@ -180,6 +180,24 @@ struct xdem_batch {
         return b;
     }
 
+    // Where k_step / k_pos record the movers of the keys they compute, against
+    // the keys of the last sort (on = 0 when a full sort follows instead).
+    MoverOut mover_out(bool on) const {
+        MoverOut m{};
+        m.key_old = skey[skcur];
+        m.crange = crange;
+        m.mcount = counters;
+        m.mkey = mkey;
+        m.ms = ms;
+        m.mplo = mplo;
+        m.moverbits = moverbits;
+        m.blockcnt = blockcnt;
+        m.n = n_total;
+        m.ncell_cap = (uint32_t)cap_cells;
+        m.on = on ? 1u : 0u;
+        return m;
+    }
+
     SortArgs sort_args(const uint32_t* key_old, const uint32_t* key_new, const uint2* zi,
                        uint32_t* skey_out) {
         SortArgs a{};
@@ -373,7 +391,8 @@ int32_t apply_layout(xdem_batch* b) {
         }
         if (n_old) {
             hipLaunchKernelGGL(k_ranges, dim3(div_up(n_old, ROWS_T)), dim3(ROWS_T), 0, st,
-                               b->skey[b->skcur], b->crange, n_old, (uint32_t)b->cap_cells);
+                               b->skey[b->skcur], b->crange, n_old, (uint32_t)b->cap_cells,
+                               (uint64_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr);
             XDEM_LAUNCHED();
         }
     }
@@ -497,21 +516,24 @@ int32_t apply_layout(xdem_batch* b) {
 }
 
 // k_pos over the storage layout: Verlet position update (adv) + keys.
-int32_t launch_pos(xdem_batch* b, bool adv, const BlockEntry* blocks, uint32_t nblk) {
+int32_t launch_pos(xdem_batch* b, bool adv, const BlockEntry* blocks, uint32_t nblk,
+                   bool movers) {
     if (!nblk) return XDEM_OK;
     BState& A = b->buf[b->cur];
+    const MoverOut mv = b->mover_out(movers);
     if (adv)
         hipLaunchKernelGGL(k_pos<true>, dim3(nblk), dim3(b->block_threads), 0, b->st, A, b->keynew,
-                           b->d_scenes, blocks);
+                           b->d_scenes, blocks, mv);
     else
         hipLaunchKernelGGL(k_pos<false>, dim3(nblk), dim3(b->block_threads), 0, b->st, A, b->keynew,
-                           b->d_scenes, blocks);
+                           b->d_scenes, blocks, mv);
     XDEM_LAUNCHED();
     return XDEM_OK;
 }
 
 // The incremental (key, id) sort of keynew (storage order) against the keys of
 // the previous sort; outputs perm, the new sorted keys and the cell ranges.
+// The movers were recorded by the k_step / k_pos launch that wrote keynew.
 int32_t run_sort(xdem_batch* b) {
     const uint32_t n = b->n_total;
     if (!n) return XDEM_OK;
@@ -520,10 +542,6 @@ int32_t run_sort(xdem_batch* b) {
     const uint32_t* kold = b->skey[b->skcur];
     uint32_t* knew_sorted = b->skey[1 - b->skcur];
     SortArgs a = b->sort_args(kold, b->keynew, b->buf[b->cur].zi, knew_sorted);
-    XDEM_TRY(T.begin(2, st));
-    hipLaunchKernelGGL(k_movers, dim3(a.nblk), dim3(ROWS_T), 0, st, a);
-    XDEM_LAUNCHED();
-    XDEM_TRY(T.end(st));
     XDEM_TRY(T.begin(3, st));
     hipLaunchKernelGGL(k_sort_movers, dim3(1), dim3(1024), 0, st, a);
     XDEM_LAUNCHED();
@@ -534,7 +552,7 @@ int32_t run_sort(xdem_batch* b) {
     XDEM_TRY(T.end(st));
     XDEM_TRY(T.begin(5, st));
     hipLaunchKernelGGL(k_ranges, dim3(a.nblk), dim3(ROWS_T), 0, st, knew_sorted, b->crange, n,
-                       (uint32_t)b->cap_cells);
+                       (uint32_t)b->cap_cells, b->moverbits, b->blockcnt, b->counters);
     XDEM_LAUNCHED();
     XDEM_TRY(T.end(st));
     b->skcur = 1 - b->skcur;
@@ -553,7 +571,7 @@ int32_t run_full_sort(xdem_batch* b) {
     XDEM_HIP(hipStreamSynchronize(st));  // sort_pairs may grow the library workspace
     XDEM_TRY(sort_pairs(out, b->perm, n, b->key_bits(), st));
     hipLaunchKernelGGL(k_ranges, dim3(div_up(n, ROWS_T)), dim3(ROWS_T), 0, st, out, b->crange, n,
-                       (uint32_t)b->cap_cells);
+                       (uint32_t)b->cap_cells, b->moverbits, b->blockcnt, b->counters);
     XDEM_LAUNCHED();
     b->skcur = 1 - b->skcur;
     return XDEM_OK;
@@ -566,6 +584,7 @@ StepArgs step_args(xdem_batch* b, bool dbg) {
     a.perm = b->perm;
     a.crange = b->crange;
     a.keynew = b->keynew;
+    a.mv = b->mover_out(true);
     a.scenes = b->d_scenes;
     a.planes = b->d_planes;
     a.blocks = b->d_blocks_sorted;
@@ -628,7 +647,7 @@ int32_t launch_step(xdem_batch* b, bool adv) {
 // One block of nsteps launches (eager or under capture).
 int32_t enqueue_block(xdem_batch* b, uint32_t nsteps, bool first_full_sort) {
     XDEM_TRY(b->timer.begin(0, b->st));
-    XDEM_TRY(launch_pos(b, true, b->d_blocks_store, b->nblk_store));
+    XDEM_TRY(launch_pos(b, true, b->d_blocks_store, b->nblk_store, !first_full_sort));
     XDEM_TRY(b->timer.end(b->st));
     if (first_full_sort)
         XDEM_TRY(run_full_sort(b));
@@ -676,7 +695,10 @@ int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t 
         (rc = dalloc(&b->ms2, n)) || (rc = dalloc(&b->mplo, 2 * n)) ||
         (rc = dalloc(&b->d_red, 8 * 1024)))
         return fail(rc);
-    if (hipMemset(b->counters, 0, 16) != hipSuccess) return fail(XDEM_ERR_HIP);
+    if (hipMemset(b->counters, 0, 16) != hipSuccess ||
+        hipMemset(b->moverbits, 0, 8 * ((size_t)nblk * (ROWS_T / 64) + 1)) != hipSuccess ||
+        hipMemset(b->blockcnt, 0, 4 * ((size_t)nblk + 1)) != hipSuccess)
+        return fail(XDEM_ERR_HIP);
     *out = b;
     return XDEM_OK;
 }
@@ -1162,7 +1184,7 @@ int32_t xdem_batch_prepare(xdem_batch* b) {
     XDEM_TRY(apply_layout(b));
     if (!b->n_total) return XDEM_OK;
     bool first = b->storage_sorted_all_new;
-    XDEM_TRY(launch_pos(b, false, b->d_blocks_store, b->nblk_store));
+    XDEM_TRY(launch_pos(b, false, b->d_blocks_store, b->nblk_store, !first));
     if (first)
         XDEM_TRY(run_full_sort(b));
     else

@@ -82,11 +82,30 @@ struct BlockEntry {
     uint32_t scene, j0, j1, pad;
 };
 
+// Movers (particles whose cell key changed) are recorded by the kernel that
+// computes the new keys (k_step / k_pos), for the incremental sort: appended
+// in arrival order with their insertion point in the previous sorted order,
+// marked in moverbits and counted per ROWS_T block (the many-movers path of
+// k_merge).  k_ranges clears moverbits, blockcnt and the count for the next sort.
+struct MoverOut {
+    const uint32_t* key_old;   // sorted keys of the previous sort, storage order
+    const uint2* crange;       // cell ranges of the previous sort
+    uint32_t* mcount;
+    uint64_t* mkey;            // (key << 32 | id)
+    uint32_t* ms;              // storage index
+    uint32_t* mplo;            // insertion point
+    uint64_t* moverbits;
+    uint32_t* blockcnt;
+    uint32_t n, ncell_cap;
+    uint32_t on;               // 0: keys only (a full sort follows)
+};
+
 struct StepArgs {
     BState in, out;
     const uint32_t* perm;      // sorted position -> storage index of `in`
     const uint2* crange;       // [key] = (first sorted position, one past last), EMPTY if none
     uint32_t* keynew;          // next cell key per sorted position
+    MoverOut mv;               // movers of the next key (ADV)
     const BScene* scenes;
     const BPlane* planes;
     const BlockEntry* blocks;
@@ -126,6 +145,63 @@ __device__ inline uint32_t key_of(const BScene& S, int cx, int cy, int cz) {
     uint32_t y = (uint32_t)cy & (S.gy - 1);
     uint32_t z = (uint32_t)cz & (S.gz - 1);
     return S.cell_base + (__umul24(__umul24(z, S.gy), S.gx) + __umul24(y, S.gx) + x);
+}
+
+__device__ inline uint64_t lanemask_lt64() {
+    uint32_t lane = __lane_id();
+    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+// First position of the previous sorted order whose key is >= kn, from the
+// previous cell ranges (exact for the previous sort): the cell's own start,
+// else the nearest non-empty cell within 8 keys, else a binary search.
+__device__ inline uint32_t insertion_point(const uint2* crange, const uint32_t* key_old,
+                                           uint32_t n, uint32_t ncell_cap, uint32_t kn) {
+    if (kn < ncell_cap) {
+        const uint2 c = crange[kn];
+        if (c.x != EMPTY) return c.x;
+        for (uint32_t d = 1; d <= 8; ++d) {
+            if (kn >= d) {
+                const uint2 a = crange[kn - d];
+                if (a.x != EMPTY) return a.y;
+            }
+            if (kn + d < ncell_cap) {
+                const uint2 b = crange[kn + d];
+                if (b.x != EMPTY) return b.x;
+            }
+        }
+    }
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (key_old[mid] < kn)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Called by every active lane of a wave with its storage index, new key and
+// particle id: one count reservation per wave that has movers.
+__device__ inline void record_mover(const MoverOut& O, uint32_t s, uint32_t nw, uint32_t id) {
+    const bool mover = O.key_old[s] != nw;
+    const uint64_t word = __ballot(mover);
+    if (!word) return;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)word) - 1;
+    uint32_t base = 0;
+    if (__lane_id() == leader) base = atomicAdd(O.mcount, (uint32_t)__popcll(word));
+    base = __shfl(base, (int)leader, 64);
+    if (mover) {
+        const uint32_t p = base + (uint32_t)__popcll(word & lanemask_lt64());
+        if (p < O.n) {
+            O.mkey[p] = ((uint64_t)nw << 32) | id;
+            O.ms[p] = s;
+            O.mplo[p] = insertion_point(O.crange, O.key_old, O.n, O.ncell_cap, nw);
+        }
+        atomicOr((unsigned long long*)&O.moverbits[s >> 6], 1ull << (s & 63));
+        atomicAdd(&O.blockcnt[s / ROWS_T], 1u);
+    }
 }
 
 struct f3 {
@@ -545,7 +621,9 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST)) void k_step(StepA
         nz += S.dt * (double)v.z + S.half2dt * (double)a.z;
         int kx, ky, kz;
         cell_of(S, nx, ny, nz, kx, ky, kz);
-        A.keynew[j] = key_of(S, kx, ky, kz);
+        const uint32_t kn = key_of(S, kx, ky, kz);
+        A.keynew[j] = kn;
+        if (A.mv.on) record_mover(A.mv, j, kn, zi.y);
     }
     A.out.pr[j] = make_double4(nx, ny, nz, ip.w);
     A.out.pf[j] = pf_of(S, nx, ny, nz, o.r);
@@ -560,7 +638,7 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST)) void k_step(StepA
 // in storage order (block start / parity prepare).
 template <bool ADV>
 __global__ __launch_bounds__(256) void k_pos(BState st, uint32_t* keynew, const BScene* scenes,
-                                             const BlockEntry* blocks) {
+                                             const BlockEntry* blocks, MoverOut mv) {
     const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
     const BlockEntry be = blocks[L];
     const BScene S = scenes[be.scene];
@@ -578,7 +656,9 @@ __global__ __launch_bounds__(256) void k_pos(BState st, uint32_t* keynew, const 
     }
     int cx, cy, cz;
     cell_of(S, x, y, z, cx, cy, cz);
-    keynew[j] = key_of(S, cx, cy, cz);
+    const uint32_t kn = key_of(S, cx, cy, cz);
+    keynew[j] = kn;
+    if (mv.on) record_mover(mv, j, kn, st.zi[j].y);
 }
 
 // ------------------------------------------------------------------ sort
@@ -589,13 +669,16 @@ struct SortArgs {
     const uint32_t* key_new;    // keys of the current positions, storage order
     const uint2* zi;            // (alpha.z bits, particle id), storage order
     uint2* crange;
-    uint64_t* moverbits;        // one bit per storage position
+    uint64_t* moverbits;        // one bit per storage position (set by record_mover)
     uint32_t* blockcnt;         // movers per ROWS_T block; scanned in place (+1 total)
+                                // when there are more than MERGE_LDS movers
     uint32_t* mcount;           // device counter of movers (reset by k_sort_movers)
     uint32_t* msorted;          // number of sorted movers (written by k_sort_movers)
     uint64_t* mkey;             // mover (key << 32 | id), append order
     uint32_t* ms;               // mover storage index, append order
-    uint32_t* mplo;             // mover insertion point in the previous order, append order
+    uint32_t* mplo;             // mover insertion point in the previous order, append order;
+                                // second half: sorted mover storage indices (<= MERGE_LDS
+                                // movers) or radix scratch
     uint64_t* mkey2;            // sorted movers (and radix ping-pong)
     uint32_t* ms2;              // sorted movers' append index
     uint32_t* perm;             // out: sorted position -> storage index
@@ -603,74 +686,6 @@ struct SortArgs {
     uint32_t nblk;              // ROWS_T blocks over n
     uint32_t key_bits, id_bits;
 };
-
-__device__ inline uint64_t lanemask_lt64() {
-    uint32_t lane = __lane_id();
-    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
-}
-
-// First position of the previous sorted order whose key is >= kn, from the
-// previous cell ranges (exact for the previous sort): the cell's own start,
-// else the nearest non-empty cell within 8 keys, else a binary search.
-__device__ inline uint32_t insertion_point(const SortArgs& A, uint32_t kn) {
-    if (kn < A.ncell_cap) {
-        const uint2 c = A.crange[kn];
-        if (c.x != EMPTY) return c.x;
-        for (uint32_t d = 1; d <= 8; ++d) {
-            if (kn >= d) {
-                const uint2 a = A.crange[kn - d];
-                if (a.x != EMPTY) return a.y;
-            }
-            if (kn + d < A.ncell_cap) {
-                const uint2 b = A.crange[kn + d];
-                if (b.x != EMPTY) return b.x;
-            }
-        }
-    }
-    uint32_t lo = 0, hi = A.n;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (A.key_old[mid] < kn)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
-// Mover detection: marks movers, counts them per block and appends them
-// (order irrelevant: they are sorted next) with their insertion point in the
-// previous order.  The cell ranges are read here and not modified before
-// k_merge.
-__global__ __launch_bounds__(ROWS_T) void k_movers(SortArgs A) {
-    __shared__ uint32_t wc[ROWS_T / 64];
-    const uint32_t s = blockIdx.x * ROWS_T + threadIdx.x;
-    const bool valid = s < A.n;
-    uint32_t old = valid ? A.key_old[s] : 0, nw = valid ? A.key_new[s] : 0;
-    bool mover = valid && old != nw;
-    uint64_t word = __ballot(mover);
-    uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0 && s < A.n) A.moverbits[s >> 6] = word;
-    uint32_t cnt = __popcll(word);
-    uint32_t base = 0;
-    if (lane == 0 && cnt) base = atomicAdd(A.mcount, cnt);
-    base = __shfl(base, 0, 64);
-    if (mover) {
-        uint32_t p = base + __popcll(word & lanemask_lt64());
-        if (p < A.n) {
-            A.mkey[p] = ((uint64_t)nw << 32) | A.zi[s].y;
-            A.ms[p] = s;
-            A.mplo[p] = insertion_point(A, nw);
-        }
-    }
-    if (lane == 0) wc[w] = cnt;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (int k = 0; k < ROWS_T / 64; ++k) t += wc[k];
-        A.blockcnt[blockIdx.x] = t;
-    }
-}
 
 __device__ inline uint32_t wave_scan_incl(uint32_t v) {
     uint32_t lane = __lane_id();
@@ -681,9 +696,38 @@ __device__ inline uint32_t wave_scan_incl(uint32_t v) {
     return v;
 }
 
-// Single-workgroup: scan of the per-block mover counts, then the movers sorted
-// by (key, id): bitonic in LDS up to SORT_LDS_MAX, else an LSD radix sort in
-// global memory by the one workgroup (rare: large admissions).
+// In-LDS bitonic sort of P (a power of two) entries by the whole-block loop.
+template <class K>
+__device__ inline void bitonic_lds(K* key, uint32_t* val, uint32_t P) {
+    const uint32_t t = threadIdx.x;
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t jb = k >> 1; jb > 0; jb >>= 1) {
+            for (uint32_t i = t; i < P; i += blockDim.x) {
+                uint32_t ix = i ^ jb;
+                if (ix > i) {
+                    bool up = (i & k) == 0;
+                    K a = key[i], b = key[ix];
+                    if ((a > b) == up) {
+                        key[i] = b;
+                        key[ix] = a;
+                        if (val) {
+                            uint32_t tv = val[i];
+                            val[i] = val[ix];
+                            val[ix] = tv;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Single-workgroup: the movers sorted by (key, id) — bitonic in LDS up to
+// SORT_LDS_MAX, else an LSD radix sort in global memory by the one workgroup
+// (large admissions).  Up to MERGE_LDS movers, their storage indices are
+// sorted too (k_merge counts the movers below a position in that list);
+// beyond, the per-block mover counts are scanned for k_merge's bit counts.
 __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
     __shared__ uint64_t lk[SORT_LDS_MAX];
     __shared__ uint32_t lv[SORT_LDS_MAX];
@@ -693,10 +737,11 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
     uint32_t* wcnt = lv;  // 16 waves x 256 digits, only in the radix path (lv unused there)
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     const uint32_t M = min(*A.mcount, A.n);
+    const bool few = M <= (uint32_t)MERGE_LDS;
     // exclusive scan of blockcnt[0..nblk) in place, total at [nblk]
     if (t == 0) carry_s = 0;
     __syncthreads();
-    for (uint32_t base = 0; base < A.nblk; base += 1024) {
+    for (uint32_t base = 0; !few && base < A.nblk; base += 1024) {
         uint32_t e = base + t;
         uint32_t v = e < A.nblk ? A.blockcnt[e] : 0;
         uint32_t inc = wave_scan_incl(v);
@@ -715,7 +760,7 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
         __syncthreads();
     }
     if (t == 0) {
-        A.blockcnt[A.nblk] = carry_s;
+        if (!few) A.blockcnt[A.nblk] = carry_s;
         *A.msorted = M;
         *A.mcount = 0;
     }
@@ -728,28 +773,18 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
             lv[i] = i;  // payload: the append index
         }
         __syncthreads();
-        for (uint32_t k = 2; k <= P; k <<= 1) {
-            for (uint32_t jb = k >> 1; jb > 0; jb >>= 1) {
-                for (uint32_t i = t; i < P; i += 1024) {
-                    uint32_t ix = i ^ jb;
-                    if (ix > i) {
-                        bool up = (i & k) == 0;
-                        uint64_t a = lk[i], b = lk[ix];
-                        if ((a > b) == up) {
-                            lk[i] = b;
-                            lk[ix] = a;
-                            uint32_t tv = lv[i];
-                            lv[i] = lv[ix];
-                            lv[ix] = tv;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
+        bitonic_lds(lk, lv, P);
         for (uint32_t i = t; i < M; i += 1024) {
             A.mkey2[i] = lk[i];
             A.ms2[i] = lv[i];
+        }
+        if (few) {
+            // sorted storage indices (P <= MERGE_LDS fits the free part of lv)
+            __syncthreads();
+            for (uint32_t i = t; i < P; i += 1024) lv[i] = i < M ? A.ms[i] : 0xFFFFFFFFu;
+            __syncthreads();
+            bitonic_lds(lv, (uint32_t*)nullptr, P);
+            for (uint32_t i = t; i < M; i += 1024) A.mplo[A.n + i] = lv[i];
         }
         return;
     }
@@ -868,13 +903,31 @@ __device__ inline uint32_t movers_below(const uint64_t* mk, uint32_t M, uint64_t
     return lo;
 }
 
+// Entries of the sorted list v[0..M) below x.
+__device__ inline uint32_t count_below(const uint32_t* v, uint32_t M, uint32_t x) {
+    uint32_t lo = 0, hi = M;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (v[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
     __shared__ uint64_t smk[MERGE_LDS];
+    __shared__ uint32_t sst[MERGE_LDS];
     const uint32_t M = *A.msorted;
+    const bool few = M <= (uint32_t)MERGE_LDS;
+    const uint32_t* mst = A.mplo + A.n;  // sorted mover storage indices (few movers)
     if (blockIdx.x < A.nblk) {
-        const bool lds = M <= (uint32_t)MERGE_LDS;
-        if (lds)
-            for (uint32_t i = threadIdx.x; i < M; i += ROWS_T) smk[i] = A.mkey2[i];
+        if (few)
+            for (uint32_t i = threadIdx.x; i < M; i += ROWS_T) {
+                smk[i] = A.mkey2[i];
+                sst[i] = mst[i];
+            }
         __syncthreads();
         const uint32_t s = blockIdx.x * ROWS_T + threadIdx.x;
         if (s >= A.n) return;
@@ -883,8 +936,8 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
         uint32_t out = s;
         if (M) {
             uint64_t X = ((uint64_t)kn << 32) | A.zi[s].y;
-            uint32_t lo = movers_below(lds ? smk : A.mkey2, M, X);
-            out = s - mover_prefix(A, s) + lo;
+            uint32_t lo = movers_below(few ? smk : A.mkey2, M, X);
+            out = s - (few ? count_below(sst, M, s) : mover_prefix(A, s)) + lo;
         }
         if (out < A.n) {
             A.perm[out] = s;
@@ -902,7 +955,7 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
         // entries of the previous order with (key, id) < X
         uint32_t lo = A.mplo[p];
         while (lo < A.n && A.key_old[lo] == kn && A.zi[lo].y < id) ++lo;
-        uint32_t out = r + (lo - mover_prefix(A, lo));
+        uint32_t out = r + (lo - (few ? count_below(mst, M, lo) : mover_prefix(A, lo)));
         if (out < A.n && s < A.n) {
             A.perm[out] = s;
             A.skey[out] = kn;
@@ -913,10 +966,18 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
 }
 
 // Cell ranges of the sorted keys (reorderDataAndFindCellStart_kernel, :166-212).
+// Also clears the mover marks of the sort just finished (when given).
 __global__ __launch_bounds__(ROWS_T) void k_ranges(const uint32_t* skey, uint2* crange, uint32_t n,
-                                                  uint32_t ncell_cap) {
+                                                  uint32_t ncell_cap, uint64_t* moverbits,
+                                                  uint32_t* blockcnt, uint32_t* mcount) {
     const uint32_t j = blockIdx.x * ROWS_T + threadIdx.x;
     if (j >= n) return;
+    if (moverbits && (j & 63) == 0) moverbits[j >> 6] = 0;
+    if (blockcnt && j % ROWS_T == 0) blockcnt[j / ROWS_T] = 0;
+    if (j == 0) {
+        if (blockcnt) blockcnt[(n + ROWS_T - 1) / ROWS_T] = 0;
+        if (mcount) *mcount = 0;
+    }
     uint32_t h = skey[j];
     if (h >= ncell_cap) return;  // defensive: keys are < the cell capacity
     if (j == 0 || skey[j - 1] != h) {
