@@ -125,6 +125,8 @@ int32_t xdem_batch_get_history(xdem_batch* b, uint32_t scene_id, uint32_t* count
  * all steps since creation. */
 int32_t xdem_batch_history_overflow(xdem_batch* b, uint32_t* overflow);
 uint32_t xdem_batch_num_particles(const xdem_batch* b);
+/* Particles of one scene (0 when there is no such scene). */
+uint32_t xdem_batch_scene_particles(const xdem_batch* b, uint32_t scene_id);
 uint32_t xdem_batch_num_scenes(const xdem_batch* b);
 /* Kinetic energy (translational + rotational) and linear momentum of one
  * scene, reduced on the device in FP64 (deterministic order). */

@@ -264,6 +264,20 @@ def _declare(lib):
         "xdem_sim_get_objects": ([vp, vp], i32),
         "xdem_sim_steps": ([vp], C.c_uint64),
         "xdem_sim_step": ([vp, u32], i32),
+        "xdem_write_part_f64": ([C.c_char_p, f64, u32, vp], i32),
+        "xdem_write_part_f32": ([C.c_char_p, f64, u32, vp], i32),
+        "xdem_read_part": ([C.c_char_p, vp, vp, vp, vp, u32], i32),
+        "xdem_write_bfr_f64": ([C.c_char_p, f64, u32, vp, vp, vp], i32),
+        "xdem_write_bfr_f32": ([C.c_char_p, f64, u32, vp, vp, vp], i32),
+        "xdem_read_bfr": ([C.c_char_p, u32, vp, vp, vp, vp, vp], i32),
+        "xdem_sim_save_part": ([vp, C.c_char_p, u32], i32),
+        "xdem_sim_save_final": ([vp, C.c_char_p], i32),
+        "xdem_sim_load_bfr": ([vp, C.c_char_p], i32),
+        "xdem_sim_save_checkpoint": ([vp, C.c_char_p], i32),
+        "xdem_sim_load_checkpoint": ([vp, C.c_char_p], i32),
+        "xdem_batch_save_checkpoint": ([vp, C.c_char_p], i32),
+        "xdem_batch_load_checkpoint": ([vp, C.c_char_p, vp, u32, vp], i32),
+        "xdem_checkpoint_verify": ([C.c_char_p, vp], i32),
         "xdem_sim_get_state": ([vp] * 8, i32),
         "xdem_sim_set_state": ([vp] * 8, i32),
         "xdem_sim_device_arrays": ([vp] + [C.POINTER(vp)] * 9, i32),

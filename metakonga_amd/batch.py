@@ -6,6 +6,7 @@ Every computation runs in libxdem.so; this module only marshals scenes
 from __future__ import annotations
 
 import ctypes as C
+import os
 import time
 
 import numpy as np
@@ -38,6 +39,9 @@ def declare(lib):
         "xdem_batch_add_scene_history": ([vp, C.POINTER(BatchSceneDesc), vp, vp, vp, vp, vp, vp,
                                           vp, vp, C.POINTER(u32)], i32),
         "xdem_batch_get_history": ([vp, u32, vp, vp, vp], i32),
+        "xdem_batch_scene_particles": ([vp, u32], u32),
+        "xdem_batch_save_checkpoint": ([vp, C.c_char_p], i32),
+        "xdem_batch_load_checkpoint": ([vp, C.c_char_p, vp, u32, vp], i32),
         "xdem_batch_history_overflow": ([vp, C.POINTER(u32)], i32),
         "xdem_batch_remove_scene": ([vp, u32], i32),
         "xdem_batch_step": ([vp, u32], i32),
@@ -123,6 +127,21 @@ class Batch:
         del keep
         self.sizes[sid.value] = scene.np
         return sid.value
+
+    def save_checkpoint(self, path):
+        """Every scene's description, state and tangential history to a file."""
+        abi.check(self.lib, self.lib.xdem_batch_save_checkpoint(self.h, os.fsencode(path)))
+
+    def load_checkpoint(self, path, max_scenes=1 << 16):
+        """Adds a checkpoint's scenes; returns their new ids in checkpoint order."""
+        ids = np.zeros(max_scenes, np.uint32)
+        n = C.c_uint32()
+        abi.check(self.lib, self.lib.xdem_batch_load_checkpoint(
+            self.h, os.fsencode(path), abi.ptr(ids), max_scenes, C.byref(n)))
+        out = [int(x) for x in ids[:n.value]]
+        for sid in out:
+            self.sizes[sid] = int(self.lib.xdem_batch_scene_particles(self.h, sid))
+        return out
 
     def remove(self, sid):
         abi.check(self.lib, self.lib.xdem_batch_remove_scene(self.h, sid))

@@ -20,7 +20,7 @@ OUT = os.path.join(HERE, "libxdem.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["xdem_runtime.hip", "xdem_sort.hip", "xdem_f64.hip", "xdem_host.hip",
-           "xdem_fused.hip", "xdem_batch.hip", "xdem_setb.hip"]
+           "xdem_fused.hip", "xdem_batch.hip", "xdem_setb.hip", "xdem_io.hip"]
 
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -19,8 +19,10 @@
 #include <vector>
 
 #include "../../include/xdem_batch.h"
+#include "../../include/xdem_io.h"
 #include "xdem_batch_kernels.hpp"
 #include "xdem_common.hpp"
+#include "xdem_io.hpp"
 
 using namespace xdem;
 using namespace xdem::fused;
@@ -952,6 +954,13 @@ uint32_t xdem_batch_num_particles(const xdem_batch* b) {
 
 uint32_t xdem_batch_num_scenes(const xdem_batch* b) { return b ? (uint32_t)b->scenes.size() : 0; }
 
+uint32_t xdem_batch_scene_particles(const xdem_batch* b, uint32_t scene_id) {
+    if (b)
+        for (auto& s : b->scenes)
+            if (s.id == scene_id) return s.n;
+    return 0;
+}
+
 static HostScene* find_scene(xdem_batch* b, uint32_t id) {
     for (auto& s : b->scenes)
         if (s.id == id) return &s;
@@ -1273,6 +1282,87 @@ int32_t xdem_batch_probe(xdem_batch* b, uint32_t scene_id, double* force3, doubl
             if (cend) cend[c] = empty ? 0u : cr[c].y - p0;
         }
     }
+    return XDEM_OK;
+}
+
+// Payload: scene count; per scene its description (planes pointer cleared),
+// planes, the five state arrays in original particle order, and with
+// tangential history the slot counts, keys and springs.
+int32_t xdem_batch_save_checkpoint(xdem_batch* b, const char* path) {
+    XDEM_REQUIRE(b, "null batch");
+    io::Writer w;
+    w.put((uint32_t)b->scenes.size());
+    std::vector<uint32_t> ids;
+    for (auto& s : b->scenes) ids.push_back(s.id);
+    for (uint32_t id : ids) {
+        HostScene* s = find_scene(b, id);
+        const size_t n = s->n, ns = n * XDEM_BATCH_HISTORY_SLOTS;
+        xdem_batch_scene_desc d;
+        std::memset(&d, 0, sizeof(d));
+        d = s->desc;
+        d.planes = nullptr;
+        d.nplane = (uint32_t)s->planes.size();
+        w.put(d);
+        w.arr(s->planes.data(), s->planes.size());
+        std::vector<double> pos(4 * n), vel(3 * n), om(3 * n), acc(3 * n), al(3 * n);
+        XDEM_TRY(xdem_batch_get_state(b, id, pos.data(), vel.data(), om.data(), acc.data(),
+                                      al.data()));
+        w.arr(pos.data(), pos.size());
+        w.arr(vel.data(), vel.size());
+        w.arr(om.data(), om.size());
+        w.arr(acc.data(), acc.size());
+        w.arr(al.data(), al.size());
+        if (d.tangential_history) {
+            std::vector<uint32_t> cnt(n), key(ns);
+            std::vector<double> xi(3 * ns);
+            XDEM_TRY(xdem_batch_get_history(b, id, cnt.data(), key.data(), xi.data()));
+            w.arr(cnt.data(), cnt.size());
+            w.arr(key.data(), key.size());
+            w.arr(xi.data(), xi.size());
+        }
+    }
+    return io::write_checkpoint(path, io::CKPT_BATCH, w);
+}
+
+int32_t xdem_batch_load_checkpoint(xdem_batch* b, const char* path, uint32_t* scene_ids,
+                                   uint32_t cap, uint32_t* nscenes) {
+    XDEM_REQUIRE(b, "null batch");
+    std::vector<uint8_t> file;
+    io::Reader r;
+    XDEM_TRY(io::read_checkpoint(path, io::CKPT_BATCH, file, r, nullptr));
+    uint32_t ns_total;
+    XDEM_REQUIRE(r.get(ns_total), "%s: short header", path);
+    XDEM_REQUIRE(!scene_ids || cap >= ns_total, "%s: %u scenes, room for %u ids", path, ns_total,
+                 cap);
+    for (uint32_t k = 0; k < ns_total; ++k) {
+        xdem_batch_scene_desc d;
+        XDEM_REQUIRE(r.get(d), "%s: truncated scene %u", path, k);
+        XDEM_REQUIRE(d.np > 0 && d.nplane <= (1u << 20), "%s: bad scene %u", path, k);
+        const size_t n = d.np, ns = n * XDEM_BATCH_HISTORY_SLOTS;
+        std::vector<xdem_device_plane_info> planes(d.nplane);
+        std::vector<double> pos(4 * n), vel(3 * n), om(3 * n), acc(3 * n), al(3 * n);
+        bool ok = r.arr(planes.data(), planes.size()) && r.arr(pos.data(), pos.size()) &&
+                  r.arr(vel.data(), vel.size()) && r.arr(om.data(), om.size()) &&
+                  r.arr(acc.data(), acc.size()) && r.arr(al.data(), al.size());
+        std::vector<uint32_t> cnt, key;
+        std::vector<double> xi;
+        if (ok && d.tangential_history) {
+            cnt.resize(n);
+            key.resize(ns);
+            xi.resize(3 * ns);
+            ok = r.arr(cnt.data(), n) && r.arr(key.data(), ns) && r.arr(xi.data(), 3 * ns);
+        }
+        XDEM_REQUIRE(ok, "%s: truncated scene %u", path, k);
+        d.planes = planes.empty() ? nullptr : planes.data();
+        uint32_t id = 0;
+        XDEM_TRY(xdem_batch_add_scene_history(
+            b, &d, pos.data(), vel.data(), om.data(), acc.data(), al.data(),
+            d.tangential_history ? cnt.data() : nullptr, d.tangential_history ? key.data() : nullptr,
+            d.tangential_history ? xi.data() : nullptr, &id));
+        if (scene_ids) scene_ids[k] = id;
+    }
+    XDEM_REQUIRE(r.done(), "%s: trailing bytes", path);
+    if (nscenes) *nscenes = ns_total;
     return XDEM_OK;
 }
 

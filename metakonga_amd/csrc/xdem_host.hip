@@ -10,8 +10,10 @@
 #include <cmath>
 #include <vector>
 
+#include "../../include/xdem_io.h"
 #include "xdem_common.hpp"
 #include "xdem_f64.hpp"
+#include "xdem_io.hpp"
 
 using namespace xdem;
 
@@ -697,5 +699,154 @@ int32_t xdem_sim_phase_times(xdem_sim* s, double* ms8, uint32_t* count8) {
 
 uint32_t xdem_sim_np(const xdem_sim* s) { return s ? s->np : 0; }
 xdem_grid* xdem_sim_grid(xdem_sim* s) { return s ? &s->grid : nullptr; }
+
+// ------------------------------------------------------------ state files
+int32_t xdem_sim_save_part(const xdem_sim* s, const char* dir, uint32_t part) {
+    XDEM_REQUIRE(s && dir, "null argument");
+    char path[4096];
+    int len = std::snprintf(path, sizeof(path), "%s/part%04u.bin", dir, part);
+    XDEM_REQUIRE(len > 0 && len < (int)sizeof(path), "path too long");
+    std::vector<double> pos(4 * (size_t)s->np);
+    XDEM_TRY(xdem_sim_get_state(s, pos.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+    return xdem_write_part_f64(path, (double)s->step * s->prm.dt, s->np, pos.data());
+}
+
+int32_t xdem_sim_save_final(const xdem_sim* s, const char* path) {
+    XDEM_REQUIRE(s, "null sim");
+    size_t n = s->np;
+    std::vector<double> pos(4 * n), vel(3 * n), om(3 * n);
+    XDEM_TRY(xdem_sim_get_state(s, pos.data(), vel.data(), om.data(), nullptr, nullptr, nullptr,
+                                nullptr));
+    return xdem_write_bfr_f64(path, (double)s->step * s->prm.dt, s->np, pos.data(), vel.data(),
+                              om.data());
+}
+
+// dem_simulation::initialize (force = m g, zero accelerations) followed by
+// setStartingData (positions, velocities, angular velocities).
+int32_t xdem_sim_load_bfr(xdem_sim* s, const char* path) {
+    XDEM_REQUIRE(s, "null sim");
+    size_t n = s->np;
+    double ct = 0;
+    int32_t prec = 0;
+    std::vector<double> pos(4 * n), vel(3 * n), om(3 * n), zero(3 * n, 0.0), mass(n), f(3 * n);
+    XDEM_TRY(xdem_read_bfr(path, s->np, &ct, &prec, pos.data(), vel.data(), om.data()));
+    XDEM_HIP(hipStreamSynchronize(stream()));
+    if (n) XDEM_HIP(hipMemcpy(mass.data(), s->mass, 8 * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+        f[3 * i + 0] = mass[i] * s->desc.gravity.x;
+        f[3 * i + 1] = mass[i] * s->desc.gravity.y;
+        f[3 * i + 2] = mass[i] * s->desc.gravity.z;
+    }
+    XDEM_TRY(xdem_sim_set_state(s, pos.data(), vel.data(), om.data(), zero.data(), zero.data(),
+                                f.data(), zero.data()));
+    s->step = (uint64_t)std::llround(ct / s->prm.dt);
+    return XDEM_OK;
+}
+
+// Payload: np, ntri, nobj, nplane, nmotion, step, dt; the seven particle
+// arrays; the current planes, triangles, circumspheres and objects; per
+// motion its kind, index, active-step count, active flag and parameters.
+int32_t xdem_sim_save_checkpoint(const xdem_sim* s, const char* path) {
+    XDEM_REQUIRE(s, "null sim");
+    size_t n = s->np;
+    std::vector<double> a[7];
+    for (int k = 0; k < 7; ++k) a[k].resize((k == 0 ? 4 : 3) * n);
+    XDEM_TRY(xdem_sim_get_state(s, a[0].data(), a[1].data(), a[2].data(), a[3].data(), a[4].data(),
+                                a[5].data(), a[6].data()));
+    const uint32_t nplane = s->dirty ? 0u : (uint32_t)s->h_planes.size();
+    std::vector<xdem_device_plane_info> planes(nplane);
+    std::vector<xdem_device_polygon_info> tris(s->ntri);
+    std::vector<double> sph(4 * (size_t)s->ntri);
+    std::vector<xdem_device_polygon_mass_info> objs(s->nobj);
+    if (nplane)
+        XDEM_HIP(hipMemcpy(planes.data(), s->d_planes, sizeof(xdem_device_plane_info) * nplane,
+                           hipMemcpyDeviceToHost));
+    XDEM_TRY(xdem_sim_get_polygons(s, tris.data(), sph.data()));
+    XDEM_TRY(xdem_sim_get_objects(s, objs.data()));
+    io::Writer w;
+    w.put(s->np);
+    w.put(s->ntri);
+    w.put(s->nobj);
+    w.put(nplane);
+    w.put((uint32_t)s->motions.size());
+    w.put(s->step);
+    w.put(s->prm.dt);
+    for (int k = 0; k < 7; ++k) w.arr(a[k].data(), a[k].size());
+    w.arr(planes.data(), planes.size());
+    w.arr(tris.data(), tris.size());
+    w.arr(sph.data(), sph.size());
+    w.arr(objs.data(), objs.size());
+    for (const Motion& m : s->motions) {
+        w.put(m.kind);
+        w.put(m.index);
+        w.put(m.n_act);
+        w.put((int32_t)(m.active ? 1 : 0));
+        w.put(m.m);
+    }
+    return io::write_checkpoint(path, io::CKPT_SIM, w);
+}
+
+int32_t xdem_sim_load_checkpoint(xdem_sim* s, const char* path) {
+    XDEM_REQUIRE(s, "null sim");
+    std::vector<uint8_t> file;
+    io::Reader r;
+    XDEM_TRY(io::read_checkpoint(path, io::CKPT_SIM, file, r, nullptr));
+    uint32_t np, ntri, nobj, nplane, nmot;
+    uint64_t step;
+    double dt;
+    XDEM_REQUIRE(r.get(np) && r.get(ntri) && r.get(nobj) && r.get(nplane) && r.get(nmot) &&
+                     r.get(step) && r.get(dt),
+                 "%s: short header", path);
+    XDEM_REQUIRE(np == s->np && ntri == s->ntri && nobj == s->nobj &&
+                     nmot == (uint32_t)s->motions.size(),
+                 "%s: built for %u particles, %u triangles, %u objects, %u motions "
+                 "(this simulation: %u, %u, %u, %zu)",
+                 path, np, ntri, nobj, nmot, s->np, s->ntri, s->nobj, s->motions.size());
+    XDEM_REQUIRE(nplane == 0 || nplane == (uint32_t)s->h_planes.size(), "%s: %u planes, not %zu",
+                 path, nplane, s->h_planes.size());
+    XDEM_REQUIRE(dt == s->prm.dt, "%s: dt %.17g differs from %.17g", path, dt, s->prm.dt);
+    size_t n = np;
+    std::vector<double> a[7];
+    for (int k = 0; k < 7; ++k) {
+        a[k].resize((k == 0 ? 4 : 3) * n);
+        XDEM_REQUIRE(r.arr(a[k].data(), a[k].size()), "%s: truncated state", path);
+    }
+    std::vector<xdem_device_plane_info> planes(nplane);
+    std::vector<xdem_device_polygon_info> tris(ntri);
+    std::vector<double> sph(4 * (size_t)ntri);
+    std::vector<xdem_device_polygon_mass_info> objs(nobj);
+    XDEM_REQUIRE(r.arr(planes.data(), nplane) && r.arr(tris.data(), ntri) &&
+                     r.arr(sph.data(), sph.size()) && r.arr(objs.data(), nobj),
+                 "%s: truncated geometry", path);
+    std::vector<Motion> mot(s->motions);
+    for (Motion& m : mot) {
+        int32_t kind, active;
+        uint32_t index;
+        XDEM_REQUIRE(r.get(kind) && r.get(index) && r.get(m.n_act) && r.get(active) && r.get(m.m),
+                     "%s: truncated motions", path);
+        XDEM_REQUIRE(kind == m.kind && index == m.index, "%s: motion (%d, %u) is (%d, %u) here",
+                     path, kind, index, m.kind, m.index);
+        m.active = active != 0;
+    }
+    XDEM_REQUIRE(r.done(), "%s: trailing bytes", path);
+    XDEM_TRY(sim_sync_walls(s));
+    XDEM_TRY(xdem_sim_set_state(s, a[0].data(), a[1].data(), a[2].data(), a[3].data(),
+                                a[4].data(), a[5].data(), a[6].data()));
+    if (nplane)
+        XDEM_HIP(hipMemcpy(s->d_planes, planes.data(), sizeof(xdem_device_plane_info) * nplane,
+                           hipMemcpyHostToDevice));
+    if (ntri) {
+        XDEM_HIP(hipMemcpy(s->d_tri, tris.data(), sizeof(xdem_device_polygon_info) * ntri,
+                           hipMemcpyHostToDevice));
+        XDEM_HIP(hipMemcpy(s->d_sph, sph.data(), sizeof(double) * sph.size(),
+                           hipMemcpyHostToDevice));
+    }
+    if (nobj)
+        XDEM_HIP(hipMemcpy(s->d_obj, objs.data(), sizeof(xdem_device_polygon_mass_info) * nobj,
+                           hipMemcpyHostToDevice));
+    s->motions = mot;
+    s->step = step;
+    return XDEM_OK;
+}
 
 }  // extern "C"

@@ -8,6 +8,7 @@ These are thin ctypes wrappers: every computation runs in libxdem.so.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -115,6 +116,22 @@ class Simulation:
         objs = (abi.PolygonMassInfo * self.nobj)()
         abi.check(self.lib, self.lib.xdem_sim_get_objects(self.h, C.cast(objs, C.c_void_p)))
         return objs
+
+    # state files (include/xdem_io.h)
+    def save_part(self, directory, part):
+        abi.check(self.lib, self.lib.xdem_sim_save_part(self.h, os.fsencode(directory), part))
+
+    def save_final(self, path):
+        abi.check(self.lib, self.lib.xdem_sim_save_final(self.h, os.fsencode(path)))
+
+    def load_bfr(self, path):
+        abi.check(self.lib, self.lib.xdem_sim_load_bfr(self.h, os.fsencode(path)))
+
+    def save_checkpoint(self, path):
+        abi.check(self.lib, self.lib.xdem_sim_save_checkpoint(self.h, os.fsencode(path)))
+
+    def load_checkpoint(self, path):
+        abi.check(self.lib, self.lib.xdem_sim_load_checkpoint(self.h, os.fsencode(path)))
 
     def mass_inertia(self):
         ptrs = [C.c_void_p() for _ in range(9)]

@@ -10,7 +10,7 @@ import pytest
 from metakonga_amd import abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("xdem.h", "xdem_batch.h", "xdem_setb.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("xdem.h", "xdem_batch.h", "xdem_setb.h", "xdem_io.h")]
 
 
 def exported():
