@@ -348,6 +348,16 @@ int32_t xdem_sim_get_polygons(const xdem_sim* s, xdem_device_polygon_info* tris,
 /* Host copies of the polygon objects (nobj entries): pose, velocity and the
  * reaction force/moment of the last step's triangle contacts. */
 int32_t xdem_sim_get_objects(const xdem_sim* s, xdem_device_polygon_mass_info* objs);
+/* Hand-over from a multibody solver (contactManager::update ->
+ * updatePolygonObjectData, contact_particles_polygonObjects.cpp:258-292):
+ * the pose (origin, ep) and velocity (vel, omega) of object obj; its force and
+ * moment fields are ignored.  The object's triangles are re-posed from their
+ * local coordinates (taken when the triangles were set) before the next step.
+ * The hand-over the other way is xdem_sim_get_objects (the reaction force and
+ * moment of the last step, contact_particles_polygonObjects.cpp:469-479).
+ * Refused for an object that has a prescribed motion. */
+int32_t xdem_sim_set_object_state(xdem_sim* s, uint32_t obj,
+                                  const xdem_device_polygon_mass_info* state);
 /* Steps taken since creation (the current time is steps * dt). */
 uint64_t xdem_sim_steps(const xdem_sim* s);
 

@@ -263,6 +263,7 @@ def _declare(lib):
         "xdem_sim_get_polygons": ([vp, vp, vp], i32),
         "xdem_sim_get_objects": ([vp, vp], i32),
         "xdem_sim_steps": ([vp], C.c_uint64),
+        "xdem_sim_set_object_state": ([vp, u32, vp], i32),
         "xdem_sim_step": ([vp, u32], i32),
         "xdem_write_part_f64": ([C.c_char_p, f64, u32, vp], i32),
         "xdem_write_part_f32": ([C.c_char_p, f64, u32, vp], i32),

@@ -613,6 +613,22 @@ int32_t xdem_sim_get_objects(const xdem_sim* s, xdem_device_polygon_mass_info* o
     return XDEM_OK;
 }
 
+int32_t xdem_sim_set_object_state(xdem_sim* s, uint32_t obj,
+                                  const xdem_device_polygon_mass_info* state) {
+    XDEM_REQUIRE(s && state, "null argument");
+    XDEM_REQUIRE(obj < s->nobj, "polygon object %u out of range [0,%u)", obj, s->nobj);
+    for (const Motion& m : s->motions)
+        XDEM_REQUIRE(!(m.kind == XDEM_OBJECT_POLYGON && m.index == obj),
+                     "object %u has a prescribed motion", obj);
+    xdem_device_polygon_mass_info o = *state;
+    o.force = xdem_double3{0, 0, 0};
+    o.moment = xdem_double3{0, 0, 0};
+    hipStream_t st = stream();
+    XDEM_TRY(launch_put_object(s->d_obj + obj, o, st));
+    XDEM_TRY(launch_update_polygon(s->d_obj, s->d_vlist, s->d_sph, s->d_tri, s->ntri, st));
+    return XDEM_OK;
+}
+
 uint64_t xdem_sim_steps(const xdem_sim* s) { return s ? s->step : 0; }
 
 int32_t xdem_sim_step(xdem_sim* s, uint32_t nsteps) {

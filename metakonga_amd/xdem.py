@@ -111,6 +111,16 @@ class Simulation:
                                                             abi.ptr(sph)))
         return tris, sph
 
+    def set_object_state(self, obj, origin, ep, vel, omega):
+        """Pose and velocity of polygon object `obj` from a multibody solver;
+        its triangles are re-posed before the next step."""
+        st = abi.PolygonMassInfo()
+        st.origin = abi._d3(origin)
+        st.ep = abi.double4(*ep)
+        st.vel = abi._d3(vel)
+        st.omega = abi._d3(omega)
+        abi.check(self.lib, self.lib.xdem_sim_set_object_state(self.h, obj, C.byref(st)))
+
     def objects(self):
         """Polygon objects (PolygonMassInfo): pose, velocity, last reaction."""
         objs = (abi.PolygonMassInfo * self.nobj)()
