@@ -62,6 +62,34 @@ typedef struct xdem_batch_scene_desc {
 #define XDEM_BATCH_HISTORY_SLOTS 16
 #define XDEM_BATCH_WALL_KEY 0x80000000u
 
+/* Walls beyond the planes, and triangle meshes, for a scene that has not
+ * stepped yet (no counterpart in the reference's batched form; the contact
+ * rules are those of the FP64 entry points in xdem.h).  Not available with
+ * tangential_history.
+ *
+ * xdem_batch_add_drum: a rotating drum (xdem_drum_info, as
+ * xdem_drum_contact_force_f64), applied after the scene's planes, in the order
+ * added; wall = the particle-drum pair.
+ *
+ * xdem_batch_add_polygon_object: a triangle mesh (ntri triangles in world
+ * coordinates at the scene's step 0; .id ignored) attached to a frame at
+ * origin/ep (Euler parameters, e0 first), moving with motion->velocity and
+ * motion->omega about origin from the scene's first step (motion NULL: static;
+ * start_time and end_time must be 0).  Contacts follow
+ * particle_polygonObject_collision_kernel (mphysics_cuda_impl.cuh:923-1021):
+ * every triangle within reach counts, normal = -(Q-P)x(R-P)/|.|.  The
+ * triangles are binned once in the object's frame; the scene's cell size must
+ * be at least its largest radius plus the largest circumradius (refused
+ * otherwise), which makes the search complete.  No per-object reaction is
+ * returned on this path (xdem_sim_get_objects gives it on the FP64 path). */
+int32_t xdem_batch_add_drum(xdem_batch* b, uint32_t scene_id, const xdem_drum_info* drum,
+                            const xdem_contact_pair_desc* wall);
+int32_t xdem_batch_add_polygon_object(xdem_batch* b, uint32_t scene_id,
+                                      const xdem_device_polygon_info* tris, uint32_t ntri,
+                                      const xdem_double3* origin, const xdem_double4* ep,
+                                      const xdem_motion* motion,
+                                      const xdem_contact_pair_desc* wall);
+
 /* Capacities are fixed at creation (HBM is reserved up front). */
 int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
                           xdem_batch** out);

@@ -40,6 +40,8 @@ def declare(lib):
                                           vp, vp, C.POINTER(u32)], i32),
         "xdem_batch_get_history": ([vp, u32, vp, vp, vp], i32),
         "xdem_batch_scene_particles": ([vp, u32], u32),
+        "xdem_batch_add_drum": ([vp, u32, vp, vp], i32),
+        "xdem_batch_add_polygon_object": ([vp, u32, vp, u32, vp, vp, vp, vp], i32),
         "xdem_batch_save_checkpoint": ([vp, C.c_char_p], i32),
         "xdem_batch_load_checkpoint": ([vp, C.c_char_p, vp, u32, vp], i32),
         "xdem_batch_history_overflow": ([vp, C.POINTER(u32)], i32),
@@ -80,7 +82,7 @@ def scene_desc(scene):
     d.dt = scene.dt
     d.gravity = abi._d3(scene.gravity)
     d.world_origin = abi._d3(scene.world_origin)
-    d.cell_size = scene.cell_size or 2.0 * float(scene.pos[:, 3].max())
+    d.cell_size = scene.cell_size or scene.default_cell_size()
     d.grid_size = abi.uint3(*scene.grid_size)
     d.rolling = scene.rolling
     d.particle = mat
@@ -125,6 +127,23 @@ class Batch:
             self.h, C.byref(d), abi.ptr(pos), *(abi.ptr(x) for x in arr),
             *(abi.ptr(x) for x in hist), C.byref(sid)))
         del keep
+        for dr in scene.drums:
+            abi.check(self.lib, self.lib.xdem_batch_add_drum(self.h, sid.value, C.byref(dr),
+                                                             C.byref(d.wall)))
+        if scene.tri is not None:
+            dpi, _, dpmi = scene.polygon_arrays()
+            ids = np.array([t.id for t in dpi])
+            objs = scene.objects or []
+            for o in range(len(dpmi)):
+                sel = [dpi[k] for k in np.nonzero(ids == o)[0]]
+                if not sel:
+                    continue
+                tris = abi.struct_array(abi.PolygonInfo, sel)
+                mot = objs[o].motion if o < len(objs) else None
+                abi.check(self.lib, self.lib.xdem_batch_add_polygon_object(
+                    self.h, sid.value, C.cast(tris, C.c_void_p), len(sel),
+                    C.byref(dpmi[o].origin), C.byref(dpmi[o].ep),
+                    C.byref(mot) if mot is not None else None, C.byref(d.wall)))
         self.sizes[sid.value] = scene.np
         return sid.value
 

@@ -32,6 +32,14 @@ static_assert(WALL_KEY == XDEM_BATCH_WALL_KEY, "wall key");
 
 namespace {
 
+// A triangle-mesh object of a scene: its triangles in the object frame, sorted
+// by local cell, and the local cell table (offsets filled at layout time).
+struct HostObject {
+    BObject ob;
+    std::vector<BTri> tris;
+    std::vector<uint2> cells;
+};
+
 struct HostScene {
     uint32_t id;           // user-visible scene id
     xdem_batch_scene_desc desc;
@@ -44,6 +52,10 @@ struct HostScene {
     std::vector<double> pos, vel, omega, acc, alpha;  // staging until upload
     std::vector<uint32_t> hcount, hkey;                 // staged history (optional)
     std::vector<double> hxi;
+    std::vector<BDrum> drums;                           // after the planes
+    std::vector<HostObject> objects;
+    uint32_t step0 = 0;    // pose-step count at admission
+    uint32_t steps_before = 0;  // object steps already taken (a scene read from a checkpoint)
 };
 
 template <class T>
@@ -110,6 +122,15 @@ struct xdem_batch {
     // tables
     BScene* d_scenes = nullptr;
     BPlane* d_planes = nullptr;
+    BDrum* d_drums = nullptr;
+    BObject* d_objs = nullptr;
+    BTri* d_tris = nullptr;
+    uint2* d_tcells = nullptr;
+    BPose* d_poses = nullptr;
+    uint32_t* d_pose_ctr = nullptr;   // pose updates so far (k_poses)
+    uint32_t pose_steps = 0;          // host copy of the count
+    uint32_t nobj_total = 0, ndrum_total = 0;
+    bool geom_active() const { return nobj_total || ndrum_total; }
     BlockEntry* d_blocks_sorted = nullptr;
     BlockEntry* d_blocks_store = nullptr;
     uint32_t nblk_sorted = 0, nblk_store = 0;
@@ -155,7 +176,9 @@ struct xdem_batch {
                         (void*)mkey2, (void*)ms2, (void*)mplo, (void*)d_scenes, (void*)d_planes,
                         (void*)d_blocks_sorted, (void*)d_blocks_store, (void*)d_red, (void*)dbg_f,
                         (void*)dbg_cnt, (void*)dbg_part, (void*)hxi[0], (void*)hxi[1],
-                        (void*)hcnt[0], (void*)hcnt[1], (void*)hover})
+                        (void*)hcnt[0], (void*)hcnt[1], (void*)hover, (void*)d_drums,
+                        (void*)d_objs, (void*)d_tris, (void*)d_tcells, (void*)d_poses,
+                        (void*)d_pose_ctr})
             if (p) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -406,6 +429,7 @@ int32_t apply_layout(xdem_batch* b) {
     for (HostScene* s : fresh) {
         uint32_t n = s->n;
         s->p0 = at;  // storage position of the appended segment
+        s->step0 = b->pose_steps - s->steps_before;
         const BScene bs = make_bscene(*s, 0);
         pr.resize(n);
         pf.resize(n);
@@ -496,15 +520,50 @@ int32_t apply_layout(xdem_batch* b) {
     }
     std::vector<BScene> hs;
     std::vector<BPlane> hp;
+    std::vector<BDrum> hd;
+    std::vector<BObject> hob;
+    std::vector<BTri> htr;
+    std::vector<uint2> hcl;
+    std::vector<BPose> hpose;
     for (HostScene* s : all) {
         hs.push_back(make_bscene(*s, (uint32_t)hp.size()));
         float ckn, cvn, sr, mu;
         pair_consts(s->desc.particle, s->desc.wall.wall, s->desc.wall, ckn, cvn, sr, mu);
         for (auto& pl : s->planes) hp.push_back(BPlane{pl, ckn, cvn, sr, mu});
+        BScene& bs = hs.back();
+        bs.drum0 = (uint32_t)hd.size();
+        bs.ndrum = (uint32_t)s->drums.size();
+        hd.insert(hd.end(), s->drums.begin(), s->drums.end());
+        bs.obj0 = (uint32_t)hob.size();
+        bs.nobj = (uint32_t)s->objects.size();
+        for (HostObject& ho : s->objects) {
+            BObject ob = ho.ob;
+            ob.tri0 = (uint32_t)htr.size();
+            ob.cell0 = (uint32_t)hcl.size();
+            ob.step0 = s->step0;
+            htr.insert(htr.end(), ho.tris.begin(), ho.tris.end());
+            hcl.insert(hcl.end(), ho.cells.begin(), ho.cells.end());
+            hob.push_back(ob);
+            // the current pose, for a probe before the next step (every step
+            // recomputes the poses first)
+            hpose.push_back(pose_at(ob, (double)(b->pose_steps - ob.step0) * ob.dt));
+        }
     }
     b->h_scenes = hs;
+    b->nobj_total = (uint32_t)hob.size();
+    b->ndrum_total = (uint32_t)hd.size();
     XDEM_TRY(upload_vec(&b->d_scenes, hs, st));
     XDEM_TRY(upload_vec(&b->d_planes, hp, st));
+    XDEM_TRY(upload_vec(&b->d_drums, hd, st));
+    XDEM_TRY(upload_vec(&b->d_objs, hob, st));
+    XDEM_TRY(upload_vec(&b->d_tris, htr, st));
+    XDEM_TRY(upload_vec(&b->d_tcells, hcl, st));
+    XDEM_TRY(upload_vec(&b->d_poses, hpose, st));
+    if (!b->d_pose_ctr) {
+        XDEM_TRY(dalloc(&b->d_pose_ctr, 1));
+        XDEM_HIP(hipMemsetAsync(b->d_pose_ctr, 0, 4, st));
+    }
+    XDEM_HIP(hipMemcpyAsync(b->d_pose_ctr, &b->pose_steps, 4, hipMemcpyHostToDevice, st));
     auto bs = make_blocks(sorted_segs, b->block_threads);
     auto bt = make_blocks(store_segs, b->block_threads);
     if (b->d_blocks_store == b->d_blocks_sorted) b->d_blocks_store = nullptr;
@@ -589,6 +648,11 @@ StepArgs step_args(xdem_batch* b, bool dbg) {
     a.mv = b->mover_out(true);
     a.scenes = b->d_scenes;
     a.planes = b->d_planes;
+    a.drums = b->d_drums;
+    a.objs = b->d_objs;
+    a.tris = b->d_tris;
+    a.tcells = b->d_tcells;
+    a.poses = b->d_poses;
     a.blocks = b->d_blocks_sorted;
     a.nblocks = b->nblk_sorted;
     a.n = b->n_total;
@@ -610,33 +674,46 @@ StepArgs step_args(xdem_batch* b, bool dbg) {
 
 int32_t launch_step(xdem_batch* b, bool adv) {
     if (!b->nblk_sorted) return XDEM_OK;
+    if (b->nobj_total) {
+        hipLaunchKernelGGL(k_poses, dim3(1), dim3(256), 0, b->st, b->d_objs, b->d_poses,
+                           b->nobj_total, b->d_pose_ctr);
+        XDEM_LAUNCHED();
+    }
     StepArgs a = step_args(b, false);
     XDEM_TRY(b->timer.begin(1, b->st));
     dim3 g(b->nblk_sorted), t(b->block_threads);
-#define XDEM_STEP_LAUNCH(U, P, H)                                                  \
-    do {                                                                           \
-        if (adv)                                                                   \
-            hipLaunchKernelGGL((k_step<true, false, U, P, H>), g, t, 0, b->st, a); \
-        else                                                                       \
-            hipLaunchKernelGGL((k_step<false, false, U, P, H>), g, t, 0, b->st, a);\
+#define XDEM_STEP_LAUNCH(U, P, H, G)                                                   \
+    do {                                                                               \
+        if (adv)                                                                       \
+            hipLaunchKernelGGL((k_step<true, false, U, P, H, G>), g, t, 0, b->st, a);  \
+        else                                                                           \
+            hipLaunchKernelGGL((k_step<false, false, U, P, H, G>), g, t, 0, b->st, a); \
     } while (0)
-    if (b->hist_active) {
+    if (b->geom_active()) {
+        // drums or meshes: the GEOM variants (contact records not loaded ahead)
+        if (b->unroll == 1)
+            XDEM_STEP_LAUNCH(1, false, false, true);
+        else if (b->unroll == 4)
+            XDEM_STEP_LAUNCH(4, false, false, true);
+        else
+            XDEM_STEP_LAUNCH(2, false, false, true);
+    } else if (b->hist_active) {
         // with history the contact records are not loaded ahead
         if (b->unroll == 1)
-            XDEM_STEP_LAUNCH(1, false, true);
+            XDEM_STEP_LAUNCH(1, false, true, false);
         else if (b->unroll == 4)
-            XDEM_STEP_LAUNCH(4, false, true);
+            XDEM_STEP_LAUNCH(4, false, true, false);
         else
-            XDEM_STEP_LAUNCH(2, false, true);
+            XDEM_STEP_LAUNCH(2, false, true, false);
     } else {
         const int v = b->unroll * 2 + (b->pipe ? 1 : 0);
         switch (v) {
-            case 2: XDEM_STEP_LAUNCH(1, false, false); break;
-            case 3: XDEM_STEP_LAUNCH(1, true, false); break;
-            case 5: XDEM_STEP_LAUNCH(2, true, false); break;
-            case 8: XDEM_STEP_LAUNCH(4, false, false); break;
-            case 9: XDEM_STEP_LAUNCH(4, true, false); break;
-            default: XDEM_STEP_LAUNCH(2, false, false); break;
+            case 2: XDEM_STEP_LAUNCH(1, false, false, false); break;
+            case 3: XDEM_STEP_LAUNCH(1, true, false, false); break;
+            case 5: XDEM_STEP_LAUNCH(2, true, false, false); break;
+            case 8: XDEM_STEP_LAUNCH(4, false, false, false); break;
+            case 9: XDEM_STEP_LAUNCH(4, true, false, false); break;
+            default: XDEM_STEP_LAUNCH(2, false, false, false); break;
         }
     }
 #undef XDEM_STEP_LAUNCH
@@ -852,6 +929,7 @@ int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps) {
             b->nblk_store = b->nblk_sorted;
         }
         b->any_stepped = true;
+        if (b->nobj_total) b->pose_steps += nsteps;
         return XDEM_OK;
     }
     auto key = std::make_pair(nsteps, b->cur * 2 + b->skcur);
@@ -883,6 +961,7 @@ int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps) {
         b->skcur = 1 - b->skcur;
     }
     b->any_stepped = true;
+    if (b->nobj_total) b->pose_steps += nsteps;
     return XDEM_OK;
 }
 
@@ -906,7 +985,7 @@ int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* c
     for (int i = 0; i < 3; ++i)
         for (int u = 0; u < 3; ++u)
             for (int pp = 0; pp < 2; ++pp) {
-                if (pp && b->hist_active) {  // not a launch shape with history
+                if (pp && (b->hist_active || b->geom_active())) {  // not a launch shape here
                     if (ms_per_candidate) ms_per_candidate[6 * i + 2 * u + pp] = -1.0;
                     continue;
                 }
@@ -1247,8 +1326,12 @@ int32_t xdem_batch_probe(xdem_batch* b, uint32_t scene_id, double* force3, doubl
         b->dbg_cap = mp;
     }
     StepArgs a = step_args(b, true);  // no advance: keynew is not written
-    hipLaunchKernelGGL((k_step<false, true, 2, false, false>), dim3(b->nblk_sorted),
-                       dim3(b->block_threads), 0, b->st, a);
+    if (b->geom_active())
+        hipLaunchKernelGGL((k_step<false, true, 2, false, false, true>), dim3(b->nblk_sorted),
+                           dim3(b->block_threads), 0, b->st, a);
+    else
+        hipLaunchKernelGGL((k_step<false, true, 2, false, false>), dim3(b->nblk_sorted),
+                           dim3(b->block_threads), 0, b->st, a);
     XDEM_LAUNCHED();
     XDEM_HIP(hipStreamSynchronize(b->st));
     uint32_t n = s->n, p0 = s->p0;
@@ -1282,6 +1365,148 @@ int32_t xdem_batch_probe(xdem_batch* b, uint32_t scene_id, double* force3, doubl
             if (cend) cend[c] = empty ? 0u : cr[c].y - p0;
         }
     }
+    return XDEM_OK;
+}
+
+namespace {
+
+// numeric::utility::calculate_center_of_triangle (source/numeric_utility.cpp:20-45),
+// the circumcentre updatePolygonObjectData also uses.
+d3 circumcenter_host(d3 P, d3 Q, d3 R) {
+    d3 V = Q - P, W = R - P;
+    d3 N = cross(V, W);
+    N = N / std::sqrt(dot(N, N));
+    d3 M1 = 0.5 * (Q + P), M2 = 0.5 * (R + P);
+    d3 D1 = cross(N, V), D2 = cross(N, W);
+    double t = 0.0;
+    if (std::fabs(D1.x * D2.y - D1.y * D2.x) > 1E-13)
+        t = (D2.x * (M1.y - M2.y)) / (D1.x * D2.y - D1.y * D2.x) -
+            (D2.y * (M1.x - M2.x)) / (D1.x * D2.y - D1.y * D2.x);
+    else if (std::fabs(D1.x * D2.z - D1.z * D2.x) > 1E-13)
+        t = (D2.x * (M1.z - M2.z)) / (D1.x * D2.z - D1.z * D2.x) -
+            (D2.z * (M1.x - M2.x)) / (D1.x * D2.z - D1.z * D2.x);
+    else if (std::fabs(D1.y * D2.z - D1.z * D2.y) > 1E-13)
+        t = (D2.y * (M1.z - M2.z)) / (D1.y * D2.z - D1.z * D2.y) -
+            (D2.z * (M1.y - M2.y)) / (D1.y * D2.z - D1.z * D2.y);
+    return M1 + t * D1;
+}
+
+int32_t pending_scene(xdem_batch* b, uint32_t scene_id, HostScene** out) {
+    HostScene* s = find_scene(b, scene_id);
+    XDEM_REQUIRE(s, "no scene %u", scene_id);
+    XDEM_REQUIRE(s->pending, "scene %u has stepped: add its walls before its first step", scene_id);
+    XDEM_REQUIRE(!s->desc.tangential_history,
+                 "scene %u: tangential history is not available with drums or polygon objects",
+                 scene_id);
+    *out = s;
+    return XDEM_OK;
+}
+
+}  // namespace
+
+int32_t xdem_batch_add_drum(xdem_batch* b, uint32_t scene_id, const xdem_drum_info* drum,
+                            const xdem_contact_pair_desc* wall) {
+    XDEM_REQUIRE(b && drum && wall, "null argument");
+    HostScene* s = nullptr;
+    XDEM_TRY(pending_scene(b, scene_id, &s));
+    const xdem_double3& a = drum->axis;
+    double an = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    XDEM_REQUIRE(std::fabs(an - 1.0) < 1e-12, "drum axis must be a unit vector (|axis| = %.17g)", an);
+    XDEM_REQUIRE(drum->radius > 0 && drum->half_length > 0, "drum radius and half length must be > 0");
+    BDrum bd{};
+    bd.d = *drum;
+    pair_consts(s->desc.particle, wall->wall, *wall, bd.ckn, bd.cvn, bd.sr, bd.mu);
+    s->drums.push_back(bd);
+    b->layout_dirty = true;
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_add_polygon_object(xdem_batch* b, uint32_t scene_id,
+                                      const xdem_device_polygon_info* tris, uint32_t ntri,
+                                      const xdem_double3* origin, const xdem_double4* ep,
+                                      const xdem_motion* motion,
+                                      const xdem_contact_pair_desc* wall) {
+    XDEM_REQUIRE(b && tris && ntri && origin && ep && wall, "null argument");
+    HostScene* s = nullptr;
+    XDEM_TRY(pending_scene(b, scene_id, &s));
+    XDEM_REQUIRE(!motion || (motion->start_time == 0 && motion->end_time == 0),
+                 "a polygon object of the batched path moves from the scene's first step "
+                 "(start_time = end_time = 0)");
+    HostObject ho{};
+    BObject& ob = ho.ob;
+    ob.origin0 = *origin;
+    ob.ep0 = *ep;
+    ob.vel = motion ? motion->velocity : xdem_double3{0, 0, 0};
+    ob.omega = motion ? motion->omega : xdem_double3{0, 0, 0};
+    ob.dt = s->desc.dt;
+    ob.cs = s->desc.cell_size;
+    pair_consts(s->desc.particle, wall->wall, *wall, ob.ckn, ob.cvn, ob.sr, ob.mu);
+    // local vertices A(ep0)^T (P - origin0) (toLocal, mphysics_cuda_impl.cuh:640-645)
+    const xdem_double4& e = *ep;
+    const d3 r0 = mk3(2.0 * (e.x * e.x + e.y * e.y - 0.5), 2.0 * (e.y * e.z - e.x * e.w),
+                      2.0 * (e.y * e.w + e.x * e.z));
+    const d3 r1 = mk3(2.0 * (e.y * e.z + e.x * e.w), 2.0 * (e.x * e.x + e.z * e.z - 0.5),
+                      2.0 * (e.z * e.w - e.x * e.y));
+    const d3 r2 = mk3(2.0 * (e.y * e.w - e.x * e.z), 2.0 * (e.z * e.w + e.x * e.y),
+                      2.0 * (e.x * e.x + e.w * e.w - 0.5));
+    auto to_local = [&](const xdem_double3& v) {
+        d3 d = mk3(v.x - origin->x, v.y - origin->y, v.z - origin->z);
+        return mk3(r0.x * d.x + r1.x * d.y + r2.x * d.z, r0.y * d.x + r1.y * d.y + r2.y * d.z,
+                   r0.z * d.x + r1.z * d.y + r2.z * d.z);
+    };
+    std::vector<BTri> lt(ntri);
+    std::vector<d3> cc(ntri);
+    double rc_max = 0.0;
+    d3 lo = mk3(1e300, 1e300, 1e300), hi = mk3(-1e300, -1e300, -1e300);
+    for (uint32_t t = 0; t < ntri; ++t) {
+        d3 P = to_local(tris[t].P), Q = to_local(tris[t].Q), R = to_local(tris[t].R);
+        d3 n = -cross(Q - P, R - P);
+        double nl = std::sqrt(dot(n, n));
+        XDEM_REQUIRE(nl > 0, "triangle %u is degenerate", t);
+        n = n / nl;
+        cc[t] = circumcenter_host(P, Q, R);
+        d3 dp = cc[t] - P;
+        rc_max = std::max(rc_max, std::sqrt(dot(dp, dp)));
+        lo = mk3(std::min(lo.x, cc[t].x), std::min(lo.y, cc[t].y), std::min(lo.z, cc[t].z));
+        hi = mk3(std::max(hi.x, cc[t].x), std::max(hi.y, cc[t].y), std::max(hi.z, cc[t].z));
+        lt[t] = BTri{P.x, P.y, P.z, Q.x, Q.y, Q.z, R.x, R.y, R.z, (float)n.x, (float)n.y,
+                     (float)n.z, 0.0f};
+    }
+    XDEM_REQUIRE(ob.cs >= s->rmax + rc_max,
+                 "cell size %.9g is below the largest radius %.9g plus the largest triangle "
+                 "circumradius %.9g: the triangle search would miss contacts",
+                 ob.cs, s->rmax, rc_max);
+    ob.lox = lo.x - ob.cs;
+    ob.loy = lo.y - ob.cs;
+    ob.loz = lo.z - ob.cs;
+    ob.gx = (uint32_t)std::floor((hi.x - ob.lox) / ob.cs) + 2;
+    ob.gy = (uint32_t)std::floor((hi.y - ob.loy) / ob.cs) + 2;
+    ob.gz = (uint32_t)std::floor((hi.z - ob.loz) / ob.cs) + 2;
+    uint64_t ncell = (uint64_t)ob.gx * ob.gy * ob.gz;
+    XDEM_REQUIRE(ncell <= (1ull << 26), "object grid of %llu cells (extent / cell size too large)",
+                 (unsigned long long)ncell);
+    std::vector<std::pair<uint32_t, uint32_t>> key(ntri);
+    for (uint32_t t = 0; t < ntri; ++t) {
+        uint32_t x = (uint32_t)std::floor((cc[t].x - ob.lox) / ob.cs);
+        uint32_t y = (uint32_t)std::floor((cc[t].y - ob.loy) / ob.cs);
+        uint32_t z = (uint32_t)std::floor((cc[t].z - ob.loz) / ob.cs);
+        key[t] = {(z * ob.gy + y) * ob.gx + x, t};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& c) {
+                         return a.first < c.first;
+                     });
+    ho.cells.assign(ncell, make_uint2(0, 0));
+    ho.tris.resize(ntri);
+    for (uint32_t k = 0; k < ntri; ++k) {
+        ho.tris[k] = lt[key[k].second];
+        uint2& c = ho.cells[key[k].first];
+        if (k == 0 || key[k - 1].first != key[k].first) c.x = k;
+        c.y = k + 1;
+    }
+    ob.ntri = ntri;
+    s->objects.push_back(std::move(ho));
+    b->layout_dirty = true;
     return XDEM_OK;
 }
 
@@ -1320,6 +1545,18 @@ int32_t xdem_batch_save_checkpoint(xdem_batch* b, const char* path) {
             w.arr(key.data(), key.size());
             w.arr(xi.data(), xi.size());
         }
+        // drums, polygon objects and the object steps already taken
+        w.put((uint32_t)s->drums.size());
+        w.arr(s->drums.data(), s->drums.size());
+        w.put((uint32_t)s->objects.size());
+        for (const HostObject& ho : s->objects) {
+            w.put(ho.ob);
+            w.put((uint32_t)ho.tris.size());
+            w.arr(ho.tris.data(), ho.tris.size());
+            w.put((uint32_t)ho.cells.size());
+            w.arr(ho.cells.data(), ho.cells.size());
+        }
+        w.put(s->pending ? s->steps_before : b->pose_steps - s->step0);
     }
     return io::write_checkpoint(path, io::CKPT_BATCH, w);
 }
@@ -1352,13 +1589,43 @@ int32_t xdem_batch_load_checkpoint(xdem_batch* b, const char* path, uint32_t* sc
             xi.resize(3 * ns);
             ok = r.arr(cnt.data(), n) && r.arr(key.data(), ns) && r.arr(xi.data(), 3 * ns);
         }
-        XDEM_REQUIRE(ok, "%s: truncated scene %u", path, k);
+        uint32_t nd = 0, no = 0, before = 0;
+        std::vector<BDrum> drums;
+        std::vector<HostObject> objs;
+        ok = ok && r.get(nd) && nd <= (1u << 16);
+        if (ok) {
+            drums.resize(nd);
+            ok = r.arr(drums.data(), nd) && r.get(no) && no <= (1u << 16);
+        }
+        for (uint32_t q = 0; ok && q < no; ++q) {
+            HostObject ho{};
+            uint32_t nt = 0, nc = 0;
+            ok = r.get(ho.ob) && r.get(nt) && nt <= (1u << 28);
+            if (ok) {
+                ho.tris.resize(nt);
+                ok = r.arr(ho.tris.data(), nt) && r.get(nc) && nc <= (1u << 26);
+            }
+            if (ok) {
+                ho.cells.resize(nc);
+                ok = r.arr(ho.cells.data(), nc) &&
+                     nc == (uint64_t)ho.ob.gx * ho.ob.gy * ho.ob.gz && nt == ho.ob.ntri;
+            }
+            if (ok)
+                for (const uint2& c : ho.cells) ok = ok && c.x <= c.y && c.y <= nt;
+            objs.push_back(std::move(ho));
+        }
+        ok = ok && r.get(before);
+        XDEM_REQUIRE(ok, "%s: truncated or inconsistent scene %u", path, k);
         d.planes = planes.empty() ? nullptr : planes.data();
         uint32_t id = 0;
         XDEM_TRY(xdem_batch_add_scene_history(
             b, &d, pos.data(), vel.data(), om.data(), acc.data(), al.data(),
             d.tangential_history ? cnt.data() : nullptr, d.tangential_history ? key.data() : nullptr,
             d.tangential_history ? xi.data() : nullptr, &id));
+        HostScene* hs = find_scene(b, id);
+        hs->drums = std::move(drums);
+        hs->objects = std::move(objs);
+        hs->steps_before = before;
         if (scene_ids) scene_ids[k] = id;
     }
     XDEM_REQUIRE(r.done(), "%s: trailing bytes", path);

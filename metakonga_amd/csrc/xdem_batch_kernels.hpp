@@ -46,11 +46,53 @@ struct BScene {
     double pf_range;          // |coordinate - world origin| bound of the FP32 copy
     float pf_eps;             // distance error bound of the FP32 copy (2^-20 pf_range)
     uint32_t pad2;
+    uint32_t drum0, ndrum;    // into the drum table (applied after the planes)
+    uint32_t obj0, nobj;      // into the polygon-object and pose tables
 };
 
 struct BPlane {
     xdem_device_plane_info pl;
     float ckn, cvn, sr, mu;
+};
+
+// A rotating drum wall (xdem_drum_info) with its wall-pair constants.
+struct BDrum {
+    xdem_drum_info d;
+    float ckn, cvn, sr, mu;
+};
+
+// A triangle-mesh object.  Its triangles are kept in the object's own frame,
+// binned once by circumcentre into a dense local grid (cell size = the scene's,
+// lo = grid corner), so a rigidly moving object is never re-binned: a particle
+// is carried into the object frame and scans its 27 local cells.  With
+// cs >= r_max + the largest circumradius every triangle within reach of a
+// particle is in those cells, as in the reference's world grid.
+struct BObject {
+    double lox, loy, loz, cs;
+    uint32_t gx, gy, gz, tri0;    // local grid; first triangle in the triangle table
+    uint32_t cell0, ntri;         // first cell in the local cell table
+    uint32_t step0;               // pose-step count when the scene was admitted
+    uint32_t pad;
+    xdem_double3 origin0;         // pose at the scene's step 0
+    xdem_double4 ep0;
+    xdem_double3 vel, omega;      // constant motion (zero: a static object)
+    double dt;
+    float ckn, cvn, sr, mu;
+};
+
+// Local-frame triangle and its face normal -(Q-P)x(R-P)/|.| (the contact
+// normal of particle_polygonObject_collision_kernel, mphysics_cuda_impl.cuh:
+// 923-1021).
+struct BTri {
+    double px, py, pz, qx, qy, qz, rx, ry, rz;
+    float nx, ny, nz, pad;
+};
+
+// Pose of an object for the current step: origin, rows of A(ep), velocity.
+struct BPose {
+    double ox, oy, oz;
+    double a[9];
+    float vx, vy, vz, wx, wy, wz;
 };
 
 // One state buffer (SoA of packed records).  pr = (x, y, z, r) in FP64 (r is
@@ -106,6 +148,11 @@ struct StepArgs {
     const uint2* crange;       // [key] = (first sorted position, one past last), EMPTY if none
     uint32_t* keynew;          // next cell key per sorted position
     MoverOut mv;               // movers of the next key (ADV)
+    const BDrum* drums;
+    const BObject* objs;
+    const BTri* tris;
+    const uint2* tcells;       // local cell table of the objects: (first, end) triangle
+    const BPose* poses;
     const BScene* scenes;
     const BPlane* planes;
     const BlockEntry* blocks;
@@ -390,14 +437,160 @@ __device__ inline void pp_contacts(const StepArgs& A, const BScene& S, const Own
 // together with the block size.  HIST: scenes with history on use the
 // tangential-spring law (others the reference DHS law, same arithmetic as
 // without HIST).
-// Waves per SIMD the register allocation targets for each variant (the
-// counts its register footprint nearly reaches without spilling).
-constexpr int step_waves(int unr, bool pipe, bool hist) {
-    return hist || pipe ? 4 : 5;
+// Pose after tau seconds of an object's constant motion: origin0 + v tau,
+// ep = q(omega, tau) * ep0 (Hamilton product, e0 first), A(ep) as makeTFM_1..3
+// (mphysics_cuda_impl.cuh:624-638).
+__host__ __device__ inline BPose pose_at(const BObject& ob, double tau) {
+    BPose P;
+    P.ox = ob.origin0.x + tau * ob.vel.x;
+    P.oy = ob.origin0.y + tau * ob.vel.y;
+    P.oz = ob.origin0.z + tau * ob.vel.z;
+    const xdem_double3& w = ob.omega;
+    const xdem_double4& p = ob.ep0;
+    double e0 = p.x, e1 = p.y, e2 = p.z, e3 = p.w;
+    const double wn = sqrt(w.x * w.x + w.y * w.y + w.z * w.z);
+    if (wn > 0) {
+        const double h = 0.5 * (wn * tau);
+        const double c = cos(h), sn = sin(h);
+        const double q0 = c, q1 = sn * (w.x / wn), q2 = sn * (w.y / wn), q3 = sn * (w.z / wn);
+        e0 = q0 * p.x - (q1 * p.y + q2 * p.z + q3 * p.w);
+        e1 = q0 * p.y + p.x * q1 + (q2 * p.w - q3 * p.z);
+        e2 = q0 * p.z + p.x * q2 + (q3 * p.y - q1 * p.w);
+        e3 = q0 * p.w + p.x * q3 + (q1 * p.z - q2 * p.y);
+    }
+    P.a[0] = 2.0 * (e0 * e0 + e1 * e1 - 0.5);
+    P.a[1] = 2.0 * (e1 * e2 - e0 * e3);
+    P.a[2] = 2.0 * (e1 * e3 + e0 * e2);
+    P.a[3] = 2.0 * (e1 * e2 + e0 * e3);
+    P.a[4] = 2.0 * (e0 * e0 + e2 * e2 - 0.5);
+    P.a[5] = 2.0 * (e2 * e3 - e0 * e1);
+    P.a[6] = 2.0 * (e1 * e3 - e0 * e2);
+    P.a[7] = 2.0 * (e2 * e3 + e0 * e1);
+    P.a[8] = 2.0 * (e0 * e0 + e3 * e3 - 0.5);
+    P.vx = (float)ob.vel.x;
+    P.vy = (float)ob.vel.y;
+    P.vz = (float)ob.vel.z;
+    P.wx = (float)w.x;
+    P.wy = (float)w.y;
+    P.wz = (float)w.z;
+    return P;
 }
 
-template <bool ADV, bool DBG, int UNR, bool PIPE, bool HIST>
-__global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST)) void k_step(StepArgs A) {
+// Poses of every object for the coming step (one block).  The step of a
+// scene's objects is the number of pose updates since its admission, so the
+// first step after admission is step 1, as the simulation object counts.
+__global__ __launch_bounds__(256) void k_poses(const BObject* objs, BPose* poses, uint32_t nobj,
+                                               uint32_t* counter) {
+    const uint32_t c = *counter;
+    for (uint32_t i = threadIdx.x; i < nobj; i += blockDim.x) {
+        const BObject& ob = objs[i];
+        poses[i] = pose_at(ob, (double)(c + 1 - ob.step0) * ob.dt);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *counter = c + 1;
+}
+
+// Drum walls (after the planes, in order) and triangle-mesh objects of the
+// scene: FP64 geometry, FP32 DHS force with ir = 0 (no torque on the particle,
+// as the reference's wall and triangle kernels).  Drum contacts are those of
+// xdem_drum_contact_force_f64; a triangle contact is the reference's
+// (distance to the closest point, face normal, object point velocity), with
+// Ft carried from triangle contact to triangle contact as in
+// particle_polygonObject_collision_kernel (mphysics_cuda_impl.cuh:923-1021).
+__device__ inline void walls_and_meshes(const StepArgs& A, const BScene& S, const Own& o,
+                                        double ri, f3& F) {
+    const d3 p = mk3(o.x, o.y, o.z);
+    for (uint32_t q = 0; q < S.ndrum; ++q) {
+        const BDrum& bd = A.drums[S.drum0 + q];
+        const d3 ax = of(bd.d.axis), c0 = of(bd.d.origin);
+        const d3 dd = p - c0;
+        const double a = dot(dd, ax);
+        if (!(fabs(a) <= bd.d.half_length)) continue;
+        const d3 qv = dd - a * ax;
+        const double rho = sqrt(dot(qv, qv));
+        const float kn = bd.ckn * fsqrt(o.r);
+        const float vn = fsqrt(bd.cvn * o.m * kn);
+        auto contact = [&](d3 u, d3 cpt, double cdist) {
+            const f3 uf = F3((float)u.x, (float)u.y, (float)u.z);
+            const d3 wv = bd.d.omega * cross(ax, cpt - c0);
+            const f3 dv = F3((float)wv.x, (float)wv.y, (float)wv.z) - (o.v + crossf(o.w, o.r * uf));
+            f3 Ft = F3(0, 0, 0), Fn = F3(0, 0, 0), M = F3(0, 0, 0);
+            dhs_f32(kn, vn, kn * bd.sr, vn * bd.sr, bd.mu, S.dtf, 0.0f, (float)cdist, dv, uf, Ft,
+                    Fn, M);
+            F += Fn + Ft;
+        };
+        const double cs = ri + rho - bd.d.radius;
+        if (cs > 0 && rho > 0) {
+            const d3 u = qv / rho;
+            contact(u, c0 + a * ax + bd.d.radius * u, cs);
+        }
+        const double cc = ri - (bd.d.half_length - fabs(a));
+        if (bd.d.caps && cc > 0 && rho < bd.d.radius) {
+            const double sg = a >= 0 ? 1.0 : -1.0;
+            contact(sg * ax, c0 + (sg * bd.d.half_length) * ax + qv, cc);
+        }
+    }
+    f3 Ft = F3(0, 0, 0), M = F3(0, 0, 0);  // carried across triangle contacts
+    for (uint32_t q = 0; q < S.nobj; ++q) {
+        const BObject& ob = A.objs[S.obj0 + q];
+        const BPose& P = A.poses[S.obj0 + q];
+        const d3 dp = mk3(p.x - P.ox, p.y - P.oy, p.z - P.oz);
+        // the particle in the object frame: A^T (p - origin)
+        const d3 pl = mk3(P.a[0] * dp.x + P.a[3] * dp.y + P.a[6] * dp.z,
+                          P.a[1] * dp.x + P.a[4] * dp.y + P.a[7] * dp.z,
+                          P.a[2] * dp.x + P.a[5] * dp.y + P.a[8] * dp.z);
+        const int cx = (int)floor((pl.x - ob.lox) / ob.cs);
+        const int cy = (int)floor((pl.y - ob.loy) / ob.cs);
+        const int cz = (int)floor((pl.z - ob.loz) / ob.cs);
+        if (cx < -1 || cy < -1 || cz < -1 || cx > (int)ob.gx || cy > (int)ob.gy || cz > (int)ob.gz)
+            continue;
+        const float kn = ob.ckn * fsqrt(o.r);
+        const float vn = fsqrt(ob.cvn * o.m * kn);
+        const f3 vo = F3(P.vx, P.vy, P.vz), wo = F3(P.wx, P.wy, P.wz);
+        for (int z = cz - 1; z <= cz + 1; ++z)
+            for (int y = cy - 1; y <= cy + 1; ++y)
+                for (int x = cx - 1; x <= cx + 1; ++x) {
+                    if (x < 0 || y < 0 || z < 0 || x >= (int)ob.gx || y >= (int)ob.gy ||
+                        z >= (int)ob.gz)
+                        continue;
+                    const uint2 cr = A.tcells[ob.cell0 + ((uint32_t)z * ob.gy + (uint32_t)y) * ob.gx +
+                                              (uint32_t)x];
+                    const uint32_t e1 = min(cr.y, ob.ntri);
+                    for (uint32_t t = cr.x; t < e1; ++t) {
+                        const BTri T = A.tris[ob.tri0 + t];
+                        const d3 cp = closest_pt_tri(pl, mk3(T.px, T.py, T.pz),
+                                                     mk3(T.qx, T.qy, T.qz), mk3(T.rx, T.ry, T.rz));
+                        const d3 dl = pl - cp;
+                        const double cdist = ri - sqrt(dot(dl, dl));
+                        if (!(cdist > 0)) continue;
+                        // world normal A n and lever arm A cp (= contact point - origin)
+                        const f3 u = F3((float)(P.a[0] * T.nx + P.a[1] * T.ny + P.a[2] * T.nz),
+                                        (float)(P.a[3] * T.nx + P.a[4] * T.ny + P.a[5] * T.nz),
+                                        (float)(P.a[6] * T.nx + P.a[7] * T.ny + P.a[8] * T.nz));
+                        const f3 rc = F3((float)(P.a[0] * cp.x + P.a[1] * cp.y + P.a[2] * cp.z),
+                                         (float)(P.a[3] * cp.x + P.a[4] * cp.y + P.a[5] * cp.z),
+                                         (float)(P.a[6] * cp.x + P.a[7] * cp.y + P.a[8] * cp.z));
+                        const f3 dv = S.rolling ? vo + crossf(wo, rc) - (o.v + crossf(o.w, o.r * u))
+                                                : vo - o.v;
+                        f3 Fn = F3(0, 0, 0);
+                        dhs_f32(kn, vn, kn * ob.sr, vn * ob.sr, ob.mu, S.dtf, 0.0f, (float)cdist, dv,
+                                u, Ft, Fn, M);
+                        F += Fn + Ft;
+                    }
+                }
+    }
+}
+
+// Waves per SIMD the register allocation targets for each variant (the
+// counts its register footprint nearly reaches without spilling).
+constexpr int step_waves(int unr, bool pipe, bool hist, bool geom) {
+    return hist || pipe || geom ? 4 : 5;
+}
+
+// GEOM: the variant that also evaluates drum walls and triangle meshes (used
+// when some scene of the batch has them; the others carry no registers for it).
+template <bool ADV, bool DBG, int UNR, bool PIPE, bool HIST, bool GEOM = false>
+__global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step(StepArgs A) {
     // per-thread LDS lists, lane-contiguous (entry c of thread t at c * LDS_T + t)
     __shared__ uint32_t clist[LDS_T * CMAX];
     __shared__ uint32_t hkeys[HIST ? LDS_T * HSLOTS : 1];
@@ -595,6 +788,7 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST)) void k_step(StepA
             F += pFn + pFt;
         }
     }
+    if (GEOM && !HIST && (S.ndrum | S.nobj)) walls_and_meshes(A, S, o, ri, F);
     if (DBG) {
         float* d = A.dbg_f + 6 * (size_t)j;
         d[0] = F.x - o.m * S.gvx;

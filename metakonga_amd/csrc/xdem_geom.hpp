@@ -88,4 +88,28 @@ __device__ inline double plane_detect(const xdem_device_plane_info& pe, d3 xp, d
     return -1.0;
 }
 
+// Closest point of triangle (a, b, c) to p — Ericson, Real-Time Collision
+// Detection, 5.1.5 (the routine of closestPtPointTriangle,
+// mphysics_cuda_impl.cuh:837-899), on plain vertices.
+__host__ __device__ inline d3 closest_pt_tri(d3 p, d3 a, d3 b, d3 c) {
+    d3 ab = b - a, ac = c - a, ap = p - a;
+    double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) return a;
+    d3 bp = p - b;
+    double d3_ = dot(ab, bp), d4_ = dot(ac, bp);
+    if (d3_ >= 0.0 && d4_ <= d3_) return b;
+    double vc = d1 * d4_ - d3_ * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3_ <= 0.0) return a + (d1 / (d1 - d3_)) * ab;
+    d3 cp = p - c;
+    double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) return c;
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) return a + (d2 / (d2 - d6)) * ac;
+    double va = d3_ * d6 - d5 * d4_;
+    if (va <= 0.0 && (d4_ - d3_) >= 0.0 && (d5 - d6) >= 0.0)
+        return b + ((d4_ - d3_) / ((d4_ - d3_) + (d5 - d6))) * (c - b);
+    double denom = 1.0 / (va + vb + vc);
+    return a + (vb * denom) * ab + (vc * denom) * ac;
+}
+
 }  // namespace xdem

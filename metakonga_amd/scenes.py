@@ -229,8 +229,15 @@ class Scene:
     def np(self) -> int:
         return int(self.pos.shape[0])
 
+    def default_cell_size(self) -> float:
+        """2 max(radius, triangle circumradius) (dem_simulation.cpp:226-227,249-250,261)."""
+        rmax = float(self.pos[:, 3].max())
+        if self.tri is not None:
+            rmax = max(rmax, float(self.polygon_arrays()[1][:, 3].max()))
+        return 2.0 * rmax
+
     def params(self) -> abi.DeviceParameters:
-        cs = self.cell_size or 2.0 * float(self.pos[:, 3].max())
+        cs = self.cell_size or self.default_cell_size()
         gs = self.grid_size
         d = abi.DeviceParameters()
         d.rollingCondition = self.rolling

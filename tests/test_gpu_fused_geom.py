@@ -1,0 +1,130 @@
+"""Drum walls and triangle meshes on the fused FP32c path against the FP64
+oracle (the same scene the FP64 GPU path is tested on: a spinning drum with a
+fin turning with it, particles against the shell, the caps and both faces of
+the fin).  Bars: accelerations of a step within the FP32c bound of the other
+fused tests (2e-5 of the largest); runs alone vs in a mix, graph vs eager and
+through a checkpoint bit-identical."""
+import numpy as np
+import pytest
+
+from metakonga_amd import abi, scenes
+from metakonga_amd.batch import Batch
+from test_gpu_drum import _drum_scene
+
+pytestmark = pytest.mark.gpu
+FP32C_TOL = 2e-5
+
+
+def _batch_for(scs):
+    return Batch(sum(s.np for s in scs), len(scs), sum(int(np.prod(s.grid_size)) for s in scs))
+
+
+def _acc_err(st, w):
+    scale = np.abs(w.acc).max()
+    return np.abs(st["acc"] - w.acc).max() / scale
+
+
+def test_first_step_vs_oracle(lib, orc):
+    sc, vel = _drum_scene()
+    b = _batch_for([sc])
+    sid = b.add(sc, vel)
+    b.graphs(False)
+    b.step(1)
+    st = b.state(sid)
+    w = orc.World(sc)
+    w.vel[:] = vel
+    w.step(1)
+    assert np.abs(st["pos"] - w.pos).max() < 1e-9
+    assert _acc_err(st, w) < FP32C_TOL
+    b.close()
+
+
+@pytest.mark.parametrize("start_steps", [15, 40])
+def test_later_step_vs_oracle(lib, orc, start_steps):
+    """From the fused state after some steps, one more step on both paths:
+    the fin's pose at that step, the spinning walls and the particle contacts."""
+    sc, vel = _drum_scene()
+    b = _batch_for([sc])
+    sid = b.add(sc, vel)
+    b.step(start_steps)
+    st = b.state(sid)
+    w = orc.World(sc)
+    for k in ("pos", "vel", "omega", "acc", "alpha"):
+        getattr(w, k)[:] = st[k]
+    w.w.step = start_steps
+    for m in w.motions:
+        m.n_act, m.active = start_steps, 1
+    b.step(1)
+    w.step(1)
+    st2 = b.state(sid)
+    assert np.abs(st2["pos"] - w.pos).max() < 1e-9
+    assert _acc_err(st2, w) < FP32C_TOL
+    b.close()
+
+
+def _run(scs, vels, steps, block, graphs=True):
+    b = _batch_for(scs)
+    b.graphs(graphs)
+    ids = [b.add(s, v) for s, v in zip(scs, vels)]
+    done = 0
+    while done < steps:
+        b.step(min(block, steps - done))
+        done += block
+    out = [b.state(i) for i in ids]
+    b.close()
+    return out
+
+
+def _same(a, b):
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_alone_vs_mix_and_graphs(lib):
+    sc, vel = _drum_scene()
+    box = scenes.box_scene(1500, 3e-3, 91, 1e-5, 100)
+    alone = _run([sc], [vel], 30, 10)[0]
+    _same(alone, _run([box, sc], [None, vel], 30, 10)[1])
+    _same(alone, _run([sc], [vel], 30, 10, graphs=False)[0])
+
+
+def test_checkpoint_resume(lib, tmp_path):
+    sc, vel = _drum_scene()
+    ref = _run([sc], [vel], 30, 15)[0]
+    b = _batch_for([sc])
+    b.add(sc, vel)
+    b.step(15)
+    b.save_checkpoint(tmp_path / "g.ckpt")
+    b.close()
+    b2 = _batch_for([sc])
+    (sid,) = b2.load_checkpoint(tmp_path / "g.ckpt")
+    b2.step(15)
+    _same(ref, b2.state(sid))
+    b2.close()
+
+
+def test_refusals(lib):
+    sc, vel = _drum_scene()
+    b = _batch_for([sc])
+    sid = b.add(sc, vel)
+    b.step(1)
+    dr = sc.drums[0]
+    wall = abi.ContactPairDesc(0.8, 0.3, 0.0, 0.0, 0.8, abi.Material(2500, 1e7, 0.3, 0))
+    import ctypes as C
+    with pytest.raises(RuntimeError):       # the scene has stepped
+        abi.check(lib, lib.xdem_batch_add_drum(b.h, sid, C.byref(dr), C.byref(wall)))
+    b.close()
+    small = scenes.Scene(name="s", pos=sc.pos, dt=sc.dt, nsteps=1, world_origin=sc.world_origin,
+                         cell_size=4.1e-3, grid_size=sc.grid_size, tri=sc.tri,
+                         objects=sc.objects)
+    b = _batch_for([small])
+    with pytest.raises(RuntimeError):       # 4.1 mm < r_max + circumradius = 4.12 mm
+        b.add(small)
+    b.close()
+    hist = scenes.Scene(name="h", pos=sc.pos, dt=sc.dt, nsteps=1, world_origin=sc.world_origin,
+                        cell_size=0.0, grid_size=sc.grid_size, drums=sc.drums,
+                        tangential_history=True)
+    b = _batch_for([hist])
+    with pytest.raises(RuntimeError):
+        b.add(hist)
+    b.close()
