@@ -148,6 +148,52 @@ def run_settling_8k(lib, steps=2000):
     return {"np": sc.np, "steps": steps, "fp64_plain": f64, "fp32c_fused": f32c}
 
 
+def run_c2_drum(lib, steps=200, f64_steps=20):
+    """4e6 spheres (r = 3 mm) in the 30 rpm drum with its turning
+    20,000-triangle baffle, dt = 5e-6 s, on the fused FP32c path (graph-replayed
+    100-step blocks, autotuned) and on the plain FP64 path.  Informational: a
+    reduced number of steps of the 2,000-step configuration."""
+    from metakonga_amd import batch
+    from metakonga_amd.xdem import Simulation
+    t0 = time.perf_counter()
+    sc = scenes.drum()
+    gen = time.perf_counter() - t0
+    b = batch.Batch(sc.np, 1, int(np.prod(sc.grid_size)), lib)
+    sid = b.add(sc)
+    b.step(20)
+    tuned = b.autotune(steps_per_trial=5)
+    b.step(100)
+    b.sync()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(100, steps - done)
+        b.step(k)
+        done += k
+    b.sync()
+    f32c = sc.np * steps / (time.perf_counter() - t0)
+    b.timing(True)
+    b.step(20)
+    ms, cnt = b.phase_times()
+    b.timing(False)
+    st = b.state(sid)
+    assert np.isfinite(st["pos"]).all()
+    b.close()
+    sim = Simulation(sc, lib)
+    sim.step(3)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(f64_steps)
+    sim.sync()
+    f64 = sc.np * f64_steps / (time.perf_counter() - t0)
+    sim.close()
+    return {"workload": f"drum {sc.np} spheres r 3 mm, R 0.5 m, L 1.6 m, 30 rpm, "
+                        f"{len(sc.tri)}-triangle turning baffle, dt 5e-6 s, cell 6 mm, grid 256^3",
+            "fp32c_fused": f32c, "fp32c_steps": steps, "fp64_plain": f64, "fp64_steps": f64_steps,
+            "speedup_vs_fp64": f32c / f64, "contact_ms": ms[1] / cnt[1],
+            "autotune_chosen": tuned[0], "generation_s": gen}
+
+
 def run_serving(duration, lib, rate=20.0, block=100, seed=0xD3E4):
     """Continuous batching of a seeded Poisson trace (drop tests and hopper
     discharges, 1k-500k spheres, 500-5000 steps); scenes are generated before
@@ -213,6 +259,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=1000)
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="0 to skip serving")
     ap.add_argument("--skip-fp64", action="store_true", help="skip the plain FP64 leg")
+    ap.add_argument("--c2-steps", type=int, default=200,
+                    help="steps of the 4e6-sphere drum leg (0 to skip)")
     ap.add_argument("--no-graphs", action="store_true", help="eager launches (profiler runs)")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shape")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -300,6 +348,11 @@ def main():
         small = run_settling_8k(lib)
         log(f"settling 8k: {small}")
 
+    c2 = None
+    if fused is not None and not args.skip_fp64 and args.c2_steps > 0:
+        c2 = run_c2_drum(lib, args.c2_steps)
+        log(f"C2 drum: {c2}")
+
     cpu = None
     cpu_omp = None
     if not args.no_cpu:
@@ -346,6 +399,8 @@ def main():
         out["batched"] = batched
     if small is not None:
         out["settling_8k"] = small
+    if c2 is not None:
+        out["c2_drum"] = c2
     if cpu_omp is not None:
         out["cpu_baseline_multicore"] = cpu_omp
     if serving is not None:

@@ -372,41 +372,56 @@ def hopper(n=1_000_000, r_min=2e-3, r_max=4e-3, seed=0xD3E1, dt=5e-6, nsteps=500
 
 
 def drum(n=4_000_000, radius=3e-3, R=0.5, L=1.6, rpm=30.0, ntri=20000, seed=0xD3E2, dt=5e-6,
-         nsteps=2000, name="C2-drum") -> Scene:
-    """C2: spheres of radius r in a rotating drum (cylinder contact) with a
-    seeded random triangle baffle.  L = 1.6 m instead of 0.4 m: 4e6 spheres of
-    r = 3 mm (0.452 m^3 solid) cannot fit a 0.314 m^3 drum.  The reference's
-    cylinder is solid (contact on the outside); the drum wall here is modelled
-    by the same reference cylinder routine around an inner core, and the inner
-    wall mode is a separate option of the fused path."""
+         nsteps=2000, name="C2-drum", grid=(256, 256, 256)) -> Scene:
+    """C2: n spheres of radius r in a drum (inner shell and end caps, axis z,
+    spinning at rpm) with a seeded baffle of ntri small random triangles that
+    turns with the drum.  L = 1.6 m instead of 0.4 m: 4e6 spheres of r = 3 mm
+    (0.452 m^3 solid) cannot fit a 0.314 m^3 drum.  The fill is the lowest
+    part of a jittered lattice (pitch 2.1 r) in every cross-section, clear of
+    the baffle; gravity -y.  The default grid (256^3 cells of 2 max(r,
+    circumradius)) wraps once along the axis: hashed cells alias, which the
+    distance test keeps correct."""
     r = float(as_f32_exact(radius))
     pitch = 2.1 * r
-    # fill the lower part of the drum cross-section with a lattice
+    omega = rpm * 2.0 * math.pi / 60.0
+    # baffle: equilateral triangles of circumradius 0.9 r, randomly oriented,
+    # centred on the line x = 0.8 R, y = 0 (circumradius <= r keeps the cell
+    # size at 2 r and the triangle search complete)
+    u = uniform(seed, ntri * 3, 3).reshape(ntri, 3)
+    base = np.stack([np.full(ntri, R * 0.8), np.zeros(ntri),
+                     -L / 2 + 2 * r + (L - 4 * r) * uniform(seed, ntri, 4)], 1)
+    nrm = np.stack([np.sqrt(1 - (2 * u[:, 0] - 1) ** 2) * np.cos(2 * np.pi * u[:, 1]),
+                    np.sqrt(1 - (2 * u[:, 0] - 1) ** 2) * np.sin(2 * np.pi * u[:, 1]),
+                    2 * u[:, 0] - 1], 1)
+    e1 = np.cross(nrm, np.where(np.abs(nrm[:, :1]) < 0.9, [[1.0, 0, 0]], [[0, 1.0, 0]]))
+    e1 /= np.linalg.norm(e1, axis=1, keepdims=True)
+    e2 = np.cross(nrm, e1)
+    ang = 2 * np.pi * u[:, 2:3] + np.array([[0.0, 2 * np.pi / 3, 4 * np.pi / 3]])
+    tri = base[:, None, :] + 0.9 * r * (np.cos(ang)[..., None] * e1[:, None, :] +
+                                        np.sin(ang)[..., None] * e2[:, None, :])
+    # lattice over the cross-section, inside the shell, clear of the baffle strip
     per = int((2 * R - 4 * r) / pitch)
     xs = (np.arange(per) - (per - 1) / 2) * pitch
     yy, xx = np.meshgrid(xs, xs, indexing="ij")
-    inside = (xx ** 2 + yy ** 2) < (R - 2 * r) ** 2
+    inside = ((xx ** 2 + yy ** 2) < (R - 2 * r) ** 2) & ~(
+        (np.abs(xx - 0.8 * R) < 2.5 * r) & (np.abs(yy) < 2.5 * r))
     yx = np.stack([xx[inside], yy[inside]], 1)
-    yx = yx[np.argsort(yx[:, 1], kind="stable")]
+    yx = yx[np.lexsort((yx[:, 0], yx[:, 1]))]          # lowest first
     nz = int((L - 4 * r) / pitch)
-    per_slice = yx.shape[0]
-    need = min(n, per_slice * nz)
-    idx = np.arange(need)
-    sl, k = idx // per_slice, idx % per_slice
-    j = (uniform(seed, 3 * need, 1).reshape(-1, 3) * 2 - 1) * (pitch / 2 - r) * 0.9
+    m = min(int(math.ceil(n / nz)), yx.shape[0])
+    idx = np.arange(min(n, m * nz))
+    sl, k = idx // m, idx % m
+    j = (uniform(seed, 3 * idx.size, 1).reshape(-1, 3) * 2 - 1) * (pitch / 2 - r) * 0.9
     pos = np.stack([yx[k, 0] + j[:, 0], yx[k, 1] + j[:, 1], -L / 2 + 2 * r + sl * pitch + j[:, 2],
-                    np.full(need, r)], 1)
-    # baffle: a strip of small random triangles near the wall
-    u = uniform(seed, ntri * 9, 3).reshape(ntri, 3, 3)
-    base = np.stack([np.full(ntri, R * 0.8), np.zeros(ntri),
-                     -L / 2 + (L) * uniform(seed, ntri, 4)], 1)
-    tri = base[:, None, :] + (u - 0.5) * (1.8 * r)
-    cs = 2.0 * r
-    g = _pow2_at_least(int(math.ceil(2 * R / cs)) + 4)
-    gz = _pow2_at_least(int(math.ceil(L / cs)) + 4)
-    wo = (-g * cs / 2, -g * cs / 2, -gz * cs / 2)
-    sc = Scene(name=name, pos=np.ascontiguousarray(pos), dt=dt, nsteps=nsteps, world_origin=wo,
-               cell_size=cs, grid_size=(g, g, gz), tri=tri, seed=seed)
+                    np.full(idx.size, r)], 1)
+    sc = Scene(name=name, pos=np.ascontiguousarray(pos), dt=dt, nsteps=nsteps, tri=tri, seed=seed,
+               objects=[PolygonObject(origin=(0.0, 0.0, 0.0),
+                                      motion=motion(omega=(0.0, 0.0, omega)))],
+               drums=[drum_info((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), R, L / 2, omega, True)],
+               grid_size=grid)
+    cs = sc.default_cell_size()
+    sc.cell_size = cs
+    sc.world_origin = (-grid[0] * cs / 2, -grid[1] * cs / 2, -grid[2] * cs / 2)
     return sc
 
 

@@ -128,3 +128,23 @@ def test_refusals(lib):
     with pytest.raises(RuntimeError):
         b.add(hist)
     b.close()
+
+
+def test_c2_full_size_properties(lib):
+    """configs[2] at full size (4e6 spheres, 20,000-triangle baffle): two runs
+    are bit-identical, stay finite and keep every particle inside the drum."""
+    sc = scenes.drum()
+    outs = []
+    for _ in range(2):
+        b = Batch(sc.np, 1, int(np.prod(sc.grid_size)), lib)
+        sid = b.add(sc)
+        b.step(20)
+        b.step(20)
+        outs.append(b.state(sid))
+        b.close()
+    _same(outs[0], outs[1])
+    st = outs[0]
+    assert np.isfinite(st["pos"]).all() and np.isfinite(st["vel"]).all()
+    rho = np.hypot(st["pos"][:, 0], st["pos"][:, 1])
+    assert rho.max() < 0.5 and np.abs(st["pos"][:, 2]).max() < 0.8
+    assert (np.abs(st["vel"]).max(1) > 0).all()   # gravity acts on every sphere
