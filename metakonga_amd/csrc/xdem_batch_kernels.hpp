@@ -507,7 +507,14 @@ __device__ inline void walls_and_meshes(const StepArgs& A, const BScene& S, cons
         const double a = dot(dd, ax);
         if (!(fabs(a) <= bd.d.half_length)) continue;
         const d3 qv = dd - a * ax;
-        const double rho = sqrt(dot(qv, qv));
+        const double q2 = dot(qv, qv);
+        // clear of the shell (rho + r < R, decided on rho^2 with a margin far
+        // above its rounding) and of the caps: no contact, no square root
+        const double rin = bd.d.radius - ri;
+        if (rin > 0 && q2 < rin * rin * (1.0 - 1e-12) &&
+            !(bd.d.caps && ri - (bd.d.half_length - fabs(a)) > 0))
+            continue;
+        const double rho = sqrt(q2);
         const float kn = bd.ckn * fsqrt(o.r);
         const float vn = fsqrt(bd.cvn * o.m * kn);
         auto contact = [&](d3 u, d3 cpt, double cdist) {
@@ -539,6 +546,12 @@ __device__ inline void walls_and_meshes(const StepArgs& A, const BScene& S, cons
         const d3 pl = mk3(P.a[0] * dp.x + P.a[3] * dp.y + P.a[6] * dp.z,
                           P.a[1] * dp.x + P.a[4] * dp.y + P.a[7] * dp.z,
                           P.a[2] * dp.x + P.a[5] * dp.y + P.a[8] * dp.z);
+        // outside the local grid and its one-cell rim: no triangle in reach
+        const double w = 2.0 * ob.cs;
+        if (pl.x < ob.lox - w || pl.y < ob.loy - w || pl.z < ob.loz - w ||
+            pl.x > ob.lox + ob.cs * ob.gx + w || pl.y > ob.loy + ob.cs * ob.gy + w ||
+            pl.z > ob.loz + ob.cs * ob.gz + w)
+            continue;
         const int cx = (int)floor((pl.x - ob.lox) / ob.cs);
         const int cy = (int)floor((pl.y - ob.loy) / ob.cs);
         const int cz = (int)floor((pl.z - ob.loz) / ob.cs);
@@ -917,7 +930,27 @@ __device__ inline void bitonic_lds(K* key, uint32_t* val, uint32_t P) {
     }
 }
 
-// Single-workgroup: the movers sorted by (key, id) — bitonic in LDS up to
+// Bitonic sort of one value per lane across a 64-lane wave (register
+// shuffles, no barrier); val rides with key.  Keys must be distinct except
+// padding (the largest value), whose payloads are never read.
+__device__ inline void bitonic_wave(uint64_t& key, uint32_t& val) {
+    const uint32_t lane = __lane_id();
+    for (uint32_t k = 2; k <= 64; k <<= 1)
+        for (uint32_t jb = k >> 1; jb > 0; jb >>= 1) {
+            const uint32_t lo = __shfl_xor((uint32_t)key, (int)jb, 64);
+            const uint32_t hi = __shfl_xor((uint32_t)(key >> 32), (int)jb, 64);
+            const uint32_t ov = __shfl_xor(val, (int)jb, 64);
+            const uint64_t other = ((uint64_t)hi << 32) | lo;
+            const bool take_min = ((lane & jb) == 0) == ((lane & k) == 0);
+            if (take_min ? other < key : other > key) {
+                key = other;
+                val = ov;
+            }
+        }
+}
+
+// Single-workgroup: the movers sorted by (key, id) — one wave with register
+// shuffles up to 64 movers, bitonic in LDS up to
 // SORT_LDS_MAX, else an LSD radix sort in global memory by the one workgroup
 // (large admissions).  Up to MERGE_LDS movers, their storage indices are
 // sorted too (k_merge counts the movers below a position in that list);
@@ -959,6 +992,22 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
         *A.mcount = 0;
     }
     if (M == 0) return;
+    if (M <= 64) {
+        if (w != 0) return;
+        uint64_t key = lane < M ? A.mkey[lane] : ~0ull;
+        uint32_t val = lane;  // payload: the append index
+        bitonic_wave(key, val);
+        if (lane < M) {
+            A.mkey2[lane] = key;
+            A.ms2[lane] = val;
+        }
+        // sorted storage indices (distinct; padding sorts last)
+        uint64_t sk = lane < M ? (uint64_t)A.ms[lane] : ~0ull;
+        uint32_t unused = 0;
+        bitonic_wave(sk, unused);
+        if (lane < M) A.mplo[A.n + lane] = (uint32_t)sk;
+        return;
+    }
     if (M <= (uint32_t)SORT_LDS_MAX) {
         uint32_t P = 2;
         while (P < M) P <<= 1;
