@@ -38,8 +38,9 @@ STEP_F64_BYTES = 300.0
 # Algorithmic bytes per particle of the fused contact + Verlet kernel (k_step):
 # own state record read 88 (pos+r 32, v+m 16, omega+alpha.x 16, a+alpha.y 16,
 # alpha.z+id 8) + written 88 + FP32 position copy written 16 + sorted->storage
-# index 4 + next cell key 4 (neighbour re-reads excluded).
-KSTEP_BYTES = 200.0
+# index 4 + next cell key written 4 + previous cell key read 4 (mover
+# detection for the incremental sort); neighbour re-reads excluded.
+KSTEP_BYTES = 204.0
 # Algorithmic bytes per particle-step of the whole FP32c step.
 STEP_F32C_BYTES = 192.0
 
@@ -313,7 +314,7 @@ def main():
         achieved = KSTEP_BYTES * scene.np / (k_ms * 1e-3)
         ch = (fused.get("autotune") or {}).get("chosen", 2256)
         sig = (f"k_step<true, false, {(ch // 1000) % 10}, {'true' if ch >= 10000 else 'false'}, "
-               "false>")
+               "false, false>")
         traffic = profiled_traffic(sig) if scene.np == 1_000_000 else None
         roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic,

@@ -6,9 +6,10 @@
 //            the scene's planes) + Verlet velocity update + next Verlet position
 //            update + next cell key, written in the new sorted order (the
 //            physical reorder rides on the writes);
-//   k_movers / k_sort_movers / k_merge / k_ranges
+//   k_sort_movers / k_merge / k_ranges
 //            re-sort by (cell key, particle id): only the particles whose cell
-//            changed ("movers") are sorted; the rest keep their order, and a
+//            changed ("movers", recorded by k_step / k_pos together with their
+//            insertion point) are sorted; the rest keep their order, and a
 //            merge gives every particle its new position.  The result is the
 //            unique (key, id) order, i.e. what the reference's stable sort of
 //            the id-ordered keys produces (source/mphysics_cuda_def.cu:92-94).
@@ -1131,7 +1132,7 @@ __device__ inline uint32_t mover_prefix(const SortArgs& A, uint32_t P) {
 // before them; movers are placed by their rank among the previous order.
 // Stayers count the sorted movers below their (key, id) by a binary search in
 // LDS (global memory when more than MERGE_LDS movers).  A mover starts from
-// its insertion point (k_movers) and steps over the entries of its new cell
+// its insertion point (recorded with it) and steps over the entries of its new cell
 // with a smaller id; it also clears its old cell's range (k_ranges rewrites
 // every non-empty cell).
 __device__ inline uint32_t movers_below(const uint64_t* mk, uint32_t M, uint64_t X) {

@@ -605,6 +605,26 @@ __global__ __launch_bounds__(512) void k_rows_sum(const d3* in, uint32_t n, d3* 
     if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
 
+// First pass of the row sums for long rows: block (b, row) sums the fixed
+// chunk [b chunk, (b + 1) chunk) of its row in a fixed tree (deterministic).
+constexpr uint32_t ROW_PARTS = 256;
+__global__ __launch_bounds__(512) void k_rows_partial(const d3* in, uint32_t n, uint32_t chunk,
+                                                      d3* partial) {
+    __shared__ d3 s[512];
+    const d3* row = in + (size_t)blockIdx.y * n;
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk;
+    const uint64_t hi = lo + chunk < n ? lo + chunk : n;
+    d3 acc = mk3(0, 0, 0);
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += 512) acc += row[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t st = 256; st > 0; st >>= 1) {
+        if (threadIdx.x < st) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.y * ROW_PARTS + blockIdx.x] = s[0];
+}
+
 __global__ void k_store_reactions(const d3* sums, xdem_device_polygon_mass_info* dpmi,
                                   uint32_t nobj) {
     uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
@@ -804,10 +824,13 @@ int32_t launch_polygon(int tcm, const KParams& k, const xdem_device_polygon_info
     if (!np) return XDEM_OK;
     d3* react = nullptr;
     d3* sums = nullptr;
+    d3* partial = nullptr;
     if (nobj) {
-        XDEM_TRY(workspace(1).reserve(sizeof(d3) * (2 * (size_t)nobj * np + 2 * nobj)));
+        XDEM_TRY(workspace(1).reserve(sizeof(d3) * (2 * (size_t)nobj * np + 2 * nobj +
+                                                    2 * (size_t)nobj * ROW_PARTS)));
         react = static_cast<d3*>(workspace(1).ptr);
         sums = react + 2 * (size_t)nobj * np;
+        partial = sums + 2 * nobj;
     }
     if (tcm == 0)
         hipLaunchKernelGGL(k_polygon<0>, dim3(div_up(np, 512)), dim3(512), 0, s, k, dpi, dpmi, pos,
@@ -819,7 +842,11 @@ int32_t launch_polygon(int tcm, const KParams& k, const xdem_device_polygon_info
                            react);
     XDEM_LAUNCHED();
     if (nobj) {
-        hipLaunchKernelGGL(k_rows_sum, dim3(2 * nobj), dim3(512), 0, s, react, np, sums);
+        const uint32_t chunk = div_up(np, ROW_PARTS);
+        hipLaunchKernelGGL(k_rows_partial, dim3(ROW_PARTS, 2 * nobj), dim3(512), 0, s, react, np,
+                           chunk, partial);
+        XDEM_LAUNCHED();
+        hipLaunchKernelGGL(k_rows_sum, dim3(2 * nobj), dim3(512), 0, s, partial, ROW_PARTS, sums);
         XDEM_LAUNCHED();
         hipLaunchKernelGGL(k_store_reactions, dim3(div_up(nobj, 64)), dim3(64), 0, s, sums, dpmi,
                            nobj);
