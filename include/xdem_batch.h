@@ -90,7 +90,9 @@ int32_t xdem_batch_add_polygon_object(xdem_batch* b, uint32_t scene_id,
                                       const xdem_motion* motion,
                                       const xdem_contact_pair_desc* wall);
 
-/* Capacities are fixed at creation (HBM is reserved up front). */
+/* Capacities are fixed at creation (HBM is reserved up front): at most 2^27
+ * particles and 2^29 cells per batch (the contact kernel addresses its arrays
+ * with 32-bit byte offsets); larger runs use several batches. */
 int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
                           xdem_batch** out);
 int32_t xdem_batch_destroy(xdem_batch* b);

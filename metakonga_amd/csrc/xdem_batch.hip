@@ -747,7 +747,9 @@ extern "C" {
 int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
                           xdem_batch** out) {
     XDEM_REQUIRE(out && max_particles && max_scenes && max_cells, "invalid capacities");
-    XDEM_REQUIRE(max_cells < 0xFFFFFFFFull, "max_cells must be < 2^32 - 1");
+    // the contact kernel addresses its arrays with 32-bit byte offsets
+    XDEM_REQUIRE(max_particles <= (1u << 27), "max_particles must be <= 2^27");
+    XDEM_REQUIRE(max_cells <= (1ull << 29), "max_cells must be <= 2^29");
     *out = nullptr;
     xdem_batch* b = new xdem_batch();
     b->cap_np = max_particles;

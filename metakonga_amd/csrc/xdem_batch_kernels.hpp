@@ -99,7 +99,7 @@ struct BPose {
 // One state buffer (SoA of packed records).  pr = (x, y, z, r) in FP64 (r is
 // an FP32 value widened), vm = (v, m), wa = (omega, alpha.x), aa = (a, alpha.y),
 // zi = (alpha.z bits, particle id), pf = the position relative to the scene's
-// world origin in FP32 and r (negated when a coordinate exceeds pf_range):
+// world origin in FP32 and r (x is NaN when a coordinate exceeds pf_range):
 // 104 B per particle.  A neighbour candidate is screened on its 16-byte pf
 // record; only candidates that pass load the 32-byte pr record.
 struct BState {
@@ -118,7 +118,15 @@ struct BState {
 __host__ __device__ inline float4 pf_of(const BScene& S, double x, double y, double z, float r) {
     const double rx = x - S.wox, ry = y - S.woy, rz = z - S.woz;
     const bool in = fabs(rx) <= S.pf_range && fabs(ry) <= S.pf_range && fabs(rz) <= S.pf_range;
-    return make_float4((float)rx, (float)ry, (float)rz, in ? r : -r);
+    return make_float4(in ? (float)rx : __builtin_nanf(""), (float)ry, (float)rz, r);
+}
+
+// Element i of an array addressed with a 32-bit byte offset from its base
+// (the SGPR base + VGPR offset form of the loads); the batch's capacities keep
+// every per-particle and per-cell array below 4 GiB.
+template <typename T>
+__device__ inline const T& at32(const T* base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
 }
 
 struct BlockEntry {
@@ -617,8 +625,6 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     const uint32_t s = A.perm[j];
     if (s >= A.n) return;  // defensive: perm is a permutation of [0, n)
     const double4 ip = A.in.pr[s];
-    const float4 vm = A.in.vm[s], wa = A.in.wa[s], aa = A.in.aa[s];
-    const uint2 zi = A.in.zi[s];
     HistCtx hc{};
     if (HIST) {
         hc.on = S.history != 0;
@@ -633,10 +639,7 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     o.x = ip.x;
     o.y = ip.y;
     o.z = ip.z;
-    o.v = F3(vm.x, vm.y, vm.z);
     o.r = (float)ip.w;
-    o.w = F3(wa.x, wa.y, wa.z);
-    o.m = vm.w;
     int cx, cy, cz;
     cell_of(S, o.x, o.y, o.z, cx, cy, cz);
     f3 Ft = F3(0, 0, 0), Fn = F3(0, 0, 0), M = F3(0, 0, 0);
@@ -661,14 +664,14 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     };
     // FP32 screen of a candidate: rejects only when the FP32 distance exceeds
     // ri + rj by more than twice its error bound (then the FP64 predicate is
-    // false); out-of-range copies (negative radius) are never rejected.
+    // false); out-of-range copies (x NaN, so d2 is NaN) are never rejected.
     const float4 ownf = pf_of(S, o.x, o.y, o.z, o.r);
+    const float sr0 = ownf.w + S.pf_eps;
     auto may_touch = [&](const float4& f) {
-        if (!(f.w > 0.0f) || !(ownf.w > 0.0f)) return true;
         const float dx = f.x - ownf.x, dy = f.y - ownf.y, dz = f.z - ownf.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        const float sr = (ownf.w + f.w) + S.pf_eps;
-        return d2 < sr * sr * (1.0f + 0x1p-19f);
+        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        const float sr = sr0 + f.w;
+        return !(d2 >= sr * sr * (1.0f + 0x1p-19f));
     };
     // phase 1: candidates in the reference order (z, y, x cells; ascending j).
     // The first CMAX that pass the FP64 predicate go to this thread's LDS list
@@ -701,14 +704,14 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
 #pragma unroll 1
                 for (int dx = -1; dx <= 1; ++dx) {
                     if (face2(dx, lox) + face2(dy, loy) + face2(dz, loz) >= reach2) continue;
-                    const uint2 cr = A.crange[key_of(S, cx + dx, cy + dy, cz + dz)];
+                    const uint2 cr = at32(A.crange, key_of(S, cx + dx, cy + dy, cz + dz));
                     if (cr.x == EMPTY) continue;
                     const uint32_t e1 = min(cr.y, A.n);
                     for (uint32_t jj = cr.x; jj < e1; ++jj) {
                         if (jj == j) continue;
-                        const uint32_t t = A.perm[jj];
+                        const uint32_t t = at32(A.perm, jj);
                         if (t >= A.n) continue;
-                        if (may_touch(A.in.pf[t])) consider(t, A.in.pr[t]);
+                        if (may_touch(at32(A.in.pf, t))) consider(t, at32(A.in.pr, t));
                     }
                 }
     };
@@ -727,9 +730,9 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
             if (b2 >= reach2) return make_uint2(EMPTY, 0);
             const bool m1 = face2(-1, lox) + b2 < reach2, p1 = face2(1, lox) + b2 < reach2;
             const uint32_t k1 = key_of(S, cx, cy + dy, cz + dz);
-            const uint2 c0 = m1 ? A.crange[k1 - 1] : make_uint2(EMPTY, 0);
-            const uint2 c1 = A.crange[k1];
-            const uint2 c2 = p1 ? A.crange[k1 + 1] : make_uint2(EMPTY, 0);
+            const uint2 c0 = m1 ? at32(A.crange, k1 - 1) : make_uint2(EMPTY, 0);
+            const uint2 c1 = at32(A.crange, k1);
+            const uint2 c2 = p1 ? at32(A.crange, k1 + 1) : make_uint2(EMPTY, 0);
             uint2 rg;
             rg.x = c0.x != EMPTY ? c0.x : (c1.x != EMPTY ? c1.x : c2.x);
             rg.y = c2.x != EMPTY ? c2.y : (c1.x != EMPTY ? c1.y : c0.y);
@@ -749,19 +752,25 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
                 for (int u = 0; u < UNR; ++u) {
                     const uint32_t k = jj + u;
                     const bool ok = k < e1 && k != j;  // e1 <= n
-                    t[u] = ok ? A.perm[k] : A.n;
+                    t[u] = ok ? at32(A.perm, k) : A.n;
                 }
 #pragma unroll
                 for (int u = 0; u < UNR; ++u)
-                    if (t[u] < A.n) f[u] = A.in.pf[t[u]];
+                    if (t[u] < A.n) f[u] = at32(A.in.pf, t[u]);
 #pragma unroll
                 for (int u = 0; u < UNR; ++u)
-                    if (t[u] < A.n && may_touch(f[u])) consider(t[u], A.in.pr[t[u]]);
+                    if (t[u] < A.n && may_touch(f[u])) consider(t[u], at32(A.in.pr, t[u]));
             }
         }
     } else {
         scan_cells();
     }
+    // velocity and spin (not needed by the neighbour walk, so not held in
+    // registers through it)
+    const float4 vm = A.in.vm[s], wa = A.in.wa[s];
+    o.v = F3(vm.x, vm.y, vm.z);
+    o.w = F3(wa.x, wa.y, wa.z);
+    o.m = vm.w;
     // phase 2: the listed contacts, in order; then the further chunks
     const uint32_t found = ntotal;
     pp_contacts<PIPE, HIST>(A, S, o, mylist, min(found, (uint32_t)CMAX), hc, Ft, Fn, M, sumF, sumM);
@@ -814,6 +823,8 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
         A.dbg_cnt[j] = ntotal;
     }
     // vv_update_velocity_kernel (mphysics_cuda_impl.cuh:98-134) in FP32
+    const float4 aa = A.in.aa[s];
+    const uint2 zi = A.in.zi[s];
     float inv_m = frcp(o.m);
     float inv_i = frcp(0.4f * o.m * o.r * o.r);
     f3 a0 = F3(aa.x, aa.y, aa.z), al0 = F3(wa.w, aa.w, __uint_as_float(zi.x));
