@@ -161,6 +161,20 @@ uint32_t xdem_batch_num_scenes(const xdem_batch* b);
 /* Kinetic energy (translational + rotational) and linear momentum of one
  * scene, reduced on the device in FP64 (deterministic order). */
 int32_t xdem_batch_energy(xdem_batch* b, uint32_t scene_id, double* kinetic, double momentum[3]);
+/* Reactions of the triangle-mesh objects, the batched counterpart of the
+ * per-object force/moment the reference accumulates in dpmi[] during the
+ * triangle contact kernel (particle_polygonObject_collision_kernel,
+ * include/mphysics_cuda_impl.cuh:1008-1009; read back by
+ * contact_particles_polygonObjects.cpp:470-479): the reaction force
+ * -(sum of Fn + Ft over the object's triangle contacts) and its moment about the
+ * object origin.  When on, the last step of every xdem_batch_step evaluates
+ * them, reduced in a fixed order (identical on every run).  Toggling rebuilds
+ * the captured step graphs. */
+int32_t xdem_batch_enable_reactions(xdem_batch* b, int32_t on);
+/* The reactions of the last step of the scene's objects, in the order they were
+ * added; *nobj = the scene's object count (force/moment may be NULL to query it). */
+int32_t xdem_batch_object_reactions(xdem_batch* b, uint32_t scene_id, xdem_double3* force,
+                                    xdem_double3* moment, uint32_t cap, uint32_t* nobj);
 /* Per-phase device times (HIP events), phases: 0 position update + hash,
  * 1 contact + Verlet, 2 sort (mover detection), 3 sort (mover ordering),
  * 4 sort (merge), 5 cell ranges.  Enabling resets the totals. */

@@ -56,6 +56,8 @@ def declare(lib):
         "xdem_batch_num_particles": ([vp], u32),
         "xdem_batch_num_scenes": ([vp], u32),
         "xdem_batch_energy": ([vp, u32, vp, vp], i32),
+        "xdem_batch_enable_reactions": ([vp, i32], i32),
+        "xdem_batch_object_reactions": ([vp, u32, vp, vp, u32, vp], i32),
         "xdem_batch_enable_timing": ([vp, i32], i32),
         "xdem_batch_phase_times": ([vp, vp, vp], i32),
         "xdem_batch_last_movers": ([vp, C.POINTER(u32)], i32),
@@ -220,6 +222,24 @@ class Batch:
         mom = np.zeros(3)
         abi.check(self.lib, self.lib.xdem_batch_energy(self.h, sid, C.byref(ke), abi.ptr(mom)))
         return ke.value, mom
+
+    def enable_reactions(self, on: bool = True):
+        """Evaluate the mesh objects' reactions at the last step of every step call."""
+        abi.check(self.lib, self.lib.xdem_batch_enable_reactions(self.h, int(on)))
+
+    def object_reactions(self, sid):
+        """(force[k, 3], moment[k, 3]) of the scene's k mesh objects at the last step:
+        minus the summed triangle contact forces, and their moment about the
+        object origin."""
+        k = C.c_uint32()
+        abi.check(self.lib, self.lib.xdem_batch_object_reactions(self.h, sid, None, None, 0,
+                                                                 C.byref(k)))
+        f = np.zeros((k.value, 3))
+        m = np.zeros((k.value, 3))
+        if k.value:
+            abi.check(self.lib, self.lib.xdem_batch_object_reactions(
+                self.h, sid, abi.ptr(f), abi.ptr(m), k.value, C.byref(k)))
+        return f, m
 
     def timing(self, on: bool):
         abi.check(self.lib, self.lib.xdem_batch_enable_timing(self.h, int(on)))

@@ -131,6 +131,15 @@ struct xdem_batch {
     uint32_t pose_steps = 0;          // host copy of the count
     uint32_t nobj_total = 0, ndrum_total = 0;
     bool geom_active() const { return nobj_total || ndrum_total; }
+    // mesh-object reactions (evaluated at the last step of each xdem_batch_step)
+    bool react_on = false;
+    bool react_valid = false;         // computed since the last layout change
+    uint32_t react_slots = 0;         // most objects of one scene
+    double* d_rpart = nullptr;        // [block][slot] 6 doubles
+    size_t rpart_cap = 0;             // doubles
+    uint4* d_robj = nullptr;          // per object: first and end block of its scene, slot
+    double* d_react = nullptr;        // per object: force, moment
+    size_t react_cap = 0;             // doubles
     BlockEntry* d_blocks_sorted = nullptr;
     BlockEntry* d_blocks_store = nullptr;
     uint32_t nblk_sorted = 0, nblk_store = 0;
@@ -178,7 +187,7 @@ struct xdem_batch {
                         (void*)dbg_cnt, (void*)dbg_part, (void*)hxi[0], (void*)hxi[1],
                         (void*)hcnt[0], (void*)hcnt[1], (void*)hover, (void*)d_drums,
                         (void*)d_objs, (void*)d_tris, (void*)d_tcells, (void*)d_poses,
-                        (void*)d_pose_ctr})
+                        (void*)d_pose_ctr, (void*)d_rpart, (void*)d_robj, (void*)d_react})
             if (p) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -571,6 +580,42 @@ int32_t apply_layout(xdem_batch* b) {
     XDEM_TRY(upload_vec(&b->d_blocks_store, bt, st));
     b->nblk_sorted = (uint32_t)bs.size();
     b->nblk_store = (uint32_t)bt.size();
+    // reaction tables: per object the block range of its scene (blocks are
+    // contiguous per scene in the sorted layout) and its slot
+    b->react_valid = false;
+    if (b->nobj_total) {
+        std::vector<uint2> range(hs.size(), make_uint2(0, 0));
+        std::vector<bool> seen(hs.size(), false);
+        for (uint32_t L = 0; L < bs.size(); ++L) {
+            const uint32_t k = bs[L].scene;
+            if (!seen[k]) range[k].x = L;
+            seen[k] = true;
+            range[k].y = L + 1;
+        }
+        std::vector<uint4> robj;
+        uint32_t slots = 0;
+        for (uint32_t k = 0; k < hs.size(); ++k) {
+            slots = std::max(slots, hs[k].nobj);
+            for (uint32_t q = 0; q < hs[k].nobj; ++q) robj.push_back(make_uint4(range[k].x, range[k].y, q, 0));
+        }
+        b->react_slots = slots;
+        XDEM_TRY(upload_vec(&b->d_robj, robj, st));
+        const size_t need = (size_t)b->nblk_sorted * slots * 6;
+        if (need > b->rpart_cap) {
+            if (b->d_rpart) (void)hipFree(b->d_rpart);
+            b->d_rpart = nullptr;
+            b->rpart_cap = 0;
+            XDEM_TRY(dalloc(&b->d_rpart, need));
+            b->rpart_cap = need;
+        }
+        if ((size_t)b->nobj_total * 6 > b->react_cap) {
+            if (b->d_react) (void)hipFree(b->d_react);
+            b->d_react = nullptr;
+            b->react_cap = 0;
+            XDEM_TRY(dalloc(&b->d_react, (size_t)b->nobj_total * 6));
+            b->react_cap = (size_t)b->nobj_total * 6;
+        }
+    }
     XDEM_HIP(hipStreamSynchronize(st));
     b->layout_dirty = false;
     return XDEM_OK;
@@ -680,6 +725,16 @@ int32_t launch_step(xdem_batch* b, bool adv) {
         XDEM_LAUNCHED();
     }
     StepArgs a = step_args(b, false);
+    if (!adv && b->react_on && b->nobj_total) {
+        // reactions of this (last) step, on the state and poses k_step uses
+        hipLaunchKernelGGL(k_react, dim3(b->nblk_sorted), dim3(b->block_threads), 0, b->st, a,
+                           b->d_rpart, b->react_slots);
+        XDEM_LAUNCHED();
+        hipLaunchKernelGGL(k_react_sum, dim3(b->nobj_total), dim3(256), 0, b->st, b->d_rpart,
+                           b->react_slots, b->d_robj, b->d_react);
+        XDEM_LAUNCHED();
+        b->react_valid = true;
+    }
     XDEM_TRY(b->timer.begin(1, b->st));
     dim3 g(b->nblk_sorted), t(b->block_threads);
 #define XDEM_STEP_LAUNCH(U, P, H, G)                                                   \
@@ -1249,6 +1304,41 @@ __global__ void k_gather_hist(const float4* hin, const uint32_t* cin, float4* ho
 }  // namespace
 
 extern "C" {
+
+int32_t xdem_batch_enable_reactions(xdem_batch* b, int32_t on) {
+    XDEM_REQUIRE(b, "null batch");
+    if ((on != 0) != b->react_on) {
+        b->react_on = on != 0;
+        b->react_valid = false;
+        b->drop_graphs();
+    }
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_object_reactions(xdem_batch* b, uint32_t scene_id, xdem_double3* force,
+                                    xdem_double3* moment, uint32_t cap, uint32_t* nobj) {
+    XDEM_REQUIRE(b, "null batch");
+    XDEM_REQUIRE(b->react_on, "reactions are off (xdem_batch_enable_reactions)");
+    uint32_t k = 0;
+    while (k < b->scenes.size() && b->scenes[k].id != scene_id) ++k;
+    XDEM_REQUIRE(k < b->scenes.size(), "no scene %u", scene_id);
+    XDEM_REQUIRE(!b->scenes[k].pending && !b->layout_dirty && b->react_valid &&
+                     k < b->h_scenes.size(),
+                 "no reactions of scene %u yet (step first)", scene_id);
+    const BScene& bs = b->h_scenes[k];
+    if (nobj) *nobj = bs.nobj;
+    XDEM_REQUIRE(cap >= bs.nobj || (!force && !moment), "capacity %u < %u objects", cap, bs.nobj);
+    if (!bs.nobj || (!force && !moment)) return XDEM_OK;
+    std::vector<double> h((size_t)bs.nobj * 6);
+    XDEM_HIP(hipMemcpyAsync(h.data(), b->d_react + (size_t)bs.obj0 * 6, 8 * h.size(),
+                            hipMemcpyDeviceToHost, b->st));
+    XDEM_HIP(hipStreamSynchronize(b->st));
+    for (uint32_t q = 0; q < bs.nobj; ++q) {
+        if (force) force[q] = xdem_double3{h[6 * q], h[6 * q + 1], h[6 * q + 2]};
+        if (moment) moment[q] = xdem_double3{h[6 * q + 3], h[6 * q + 4], h[6 * q + 5]};
+    }
+    return XDEM_OK;
+}
 
 int32_t xdem_batch_energy(xdem_batch* b, uint32_t scene_id, double* kinetic, double momentum[3]) {
     XDEM_REQUIRE(b, "null batch");

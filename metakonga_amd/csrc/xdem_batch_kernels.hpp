@@ -499,6 +499,73 @@ __global__ __launch_bounds__(256) void k_poses(const BObject* objs, BPose* poses
     if (threadIdx.x == 0) *counter = c + 1;
 }
 
+// Triangle contacts of one mesh object q of the scene with the particle `o`
+// (Ft and M carried in and out across contacts, as in the step).  REACT: the
+// reaction on the object is accumulated too: -(Fn + Ft) and its moment about
+// the object origin, -(contact point - origin) x (Fn + Ft)
+// (particle_polygonObject_collision_kernel, mphysics_cuda_impl.cuh:1008-1009).
+template <bool REACT>
+__device__ inline void mesh_object(const StepArgs& A, const BScene& S, const Own& o, double ri,
+                                   uint32_t q, f3& Ft, f3& M, f3& F, f3* rf, f3* rm) {
+    const d3 p = mk3(o.x, o.y, o.z);
+    const BObject& ob = A.objs[S.obj0 + q];
+    const BPose& P = A.poses[S.obj0 + q];
+    const d3 dp = mk3(p.x - P.ox, p.y - P.oy, p.z - P.oz);
+    // the particle in the object frame: A^T (p - origin)
+    const d3 pl = mk3(P.a[0] * dp.x + P.a[3] * dp.y + P.a[6] * dp.z,
+                      P.a[1] * dp.x + P.a[4] * dp.y + P.a[7] * dp.z,
+                      P.a[2] * dp.x + P.a[5] * dp.y + P.a[8] * dp.z);
+    // outside the local grid and its one-cell rim: no triangle in reach
+    const double w = 2.0 * ob.cs;
+    if (pl.x < ob.lox - w || pl.y < ob.loy - w || pl.z < ob.loz - w ||
+        pl.x > ob.lox + ob.cs * ob.gx + w || pl.y > ob.loy + ob.cs * ob.gy + w ||
+        pl.z > ob.loz + ob.cs * ob.gz + w)
+        return;
+    const int cx = (int)floor((pl.x - ob.lox) / ob.cs);
+    const int cy = (int)floor((pl.y - ob.loy) / ob.cs);
+    const int cz = (int)floor((pl.z - ob.loz) / ob.cs);
+    if (cx < -1 || cy < -1 || cz < -1 || cx > (int)ob.gx || cy > (int)ob.gy || cz > (int)ob.gz)
+        return;
+    const float kn = ob.ckn * fsqrt(o.r);
+    const float vn = fsqrt(ob.cvn * o.m * kn);
+    const f3 vo = F3(P.vx, P.vy, P.vz), wo = F3(P.wx, P.wy, P.wz);
+    for (int z = cz - 1; z <= cz + 1; ++z)
+        for (int y = cy - 1; y <= cy + 1; ++y)
+            for (int x = cx - 1; x <= cx + 1; ++x) {
+                if (x < 0 || y < 0 || z < 0 || x >= (int)ob.gx || y >= (int)ob.gy ||
+                    z >= (int)ob.gz)
+                    continue;
+                const uint2 cr = A.tcells[ob.cell0 + ((uint32_t)z * ob.gy + (uint32_t)y) * ob.gx +
+                                          (uint32_t)x];
+                const uint32_t e1 = min(cr.y, ob.ntri);
+                for (uint32_t t = cr.x; t < e1; ++t) {
+                    const BTri T = A.tris[ob.tri0 + t];
+                    const d3 cp = closest_pt_tri(pl, mk3(T.px, T.py, T.pz),
+                                                 mk3(T.qx, T.qy, T.qz), mk3(T.rx, T.ry, T.rz));
+                    const d3 dl = pl - cp;
+                    const double cdist = ri - sqrt(dot(dl, dl));
+                    if (!(cdist > 0)) continue;
+                    // world normal A n and lever arm A cp (= contact point - origin)
+                    const f3 u = F3((float)(P.a[0] * T.nx + P.a[1] * T.ny + P.a[2] * T.nz),
+                                    (float)(P.a[3] * T.nx + P.a[4] * T.ny + P.a[5] * T.nz),
+                                    (float)(P.a[6] * T.nx + P.a[7] * T.ny + P.a[8] * T.nz));
+                    const f3 rc = F3((float)(P.a[0] * cp.x + P.a[1] * cp.y + P.a[2] * cp.z),
+                                     (float)(P.a[3] * cp.x + P.a[4] * cp.y + P.a[5] * cp.z),
+                                     (float)(P.a[6] * cp.x + P.a[7] * cp.y + P.a[8] * cp.z));
+                    const f3 dv = S.rolling ? vo + crossf(wo, rc) - (o.v + crossf(o.w, o.r * u))
+                                            : vo - o.v;
+                    f3 Fn = F3(0, 0, 0);
+                    dhs_f32(kn, vn, kn * ob.sr, vn * ob.sr, ob.mu, S.dtf, 0.0f, (float)cdist, dv,
+                            u, Ft, Fn, M);
+                    F += Fn + Ft;
+                    if (REACT) {
+                        *rf += -(Fn + Ft);
+                        *rm += -crossf(rc, Fn + Ft);
+                    }
+                }
+            }
+}
+
 // Drum walls (after the planes, in order) and triangle-mesh objects of the
 // scene: FP64 geometry, FP32 DHS force with ir = 0 (no torque on the particle,
 // as the reference's wall and triangle kernels).  Drum contacts are those of
@@ -547,60 +614,7 @@ __device__ inline void walls_and_meshes(const StepArgs& A, const BScene& S, cons
         }
     }
     f3 Ft = F3(0, 0, 0), M = F3(0, 0, 0);  // carried across triangle contacts
-    for (uint32_t q = 0; q < S.nobj; ++q) {
-        const BObject& ob = A.objs[S.obj0 + q];
-        const BPose& P = A.poses[S.obj0 + q];
-        const d3 dp = mk3(p.x - P.ox, p.y - P.oy, p.z - P.oz);
-        // the particle in the object frame: A^T (p - origin)
-        const d3 pl = mk3(P.a[0] * dp.x + P.a[3] * dp.y + P.a[6] * dp.z,
-                          P.a[1] * dp.x + P.a[4] * dp.y + P.a[7] * dp.z,
-                          P.a[2] * dp.x + P.a[5] * dp.y + P.a[8] * dp.z);
-        // outside the local grid and its one-cell rim: no triangle in reach
-        const double w = 2.0 * ob.cs;
-        if (pl.x < ob.lox - w || pl.y < ob.loy - w || pl.z < ob.loz - w ||
-            pl.x > ob.lox + ob.cs * ob.gx + w || pl.y > ob.loy + ob.cs * ob.gy + w ||
-            pl.z > ob.loz + ob.cs * ob.gz + w)
-            continue;
-        const int cx = (int)floor((pl.x - ob.lox) / ob.cs);
-        const int cy = (int)floor((pl.y - ob.loy) / ob.cs);
-        const int cz = (int)floor((pl.z - ob.loz) / ob.cs);
-        if (cx < -1 || cy < -1 || cz < -1 || cx > (int)ob.gx || cy > (int)ob.gy || cz > (int)ob.gz)
-            continue;
-        const float kn = ob.ckn * fsqrt(o.r);
-        const float vn = fsqrt(ob.cvn * o.m * kn);
-        const f3 vo = F3(P.vx, P.vy, P.vz), wo = F3(P.wx, P.wy, P.wz);
-        for (int z = cz - 1; z <= cz + 1; ++z)
-            for (int y = cy - 1; y <= cy + 1; ++y)
-                for (int x = cx - 1; x <= cx + 1; ++x) {
-                    if (x < 0 || y < 0 || z < 0 || x >= (int)ob.gx || y >= (int)ob.gy ||
-                        z >= (int)ob.gz)
-                        continue;
-                    const uint2 cr = A.tcells[ob.cell0 + ((uint32_t)z * ob.gy + (uint32_t)y) * ob.gx +
-                                              (uint32_t)x];
-                    const uint32_t e1 = min(cr.y, ob.ntri);
-                    for (uint32_t t = cr.x; t < e1; ++t) {
-                        const BTri T = A.tris[ob.tri0 + t];
-                        const d3 cp = closest_pt_tri(pl, mk3(T.px, T.py, T.pz),
-                                                     mk3(T.qx, T.qy, T.qz), mk3(T.rx, T.ry, T.rz));
-                        const d3 dl = pl - cp;
-                        const double cdist = ri - sqrt(dot(dl, dl));
-                        if (!(cdist > 0)) continue;
-                        // world normal A n and lever arm A cp (= contact point - origin)
-                        const f3 u = F3((float)(P.a[0] * T.nx + P.a[1] * T.ny + P.a[2] * T.nz),
-                                        (float)(P.a[3] * T.nx + P.a[4] * T.ny + P.a[5] * T.nz),
-                                        (float)(P.a[6] * T.nx + P.a[7] * T.ny + P.a[8] * T.nz));
-                        const f3 rc = F3((float)(P.a[0] * cp.x + P.a[1] * cp.y + P.a[2] * cp.z),
-                                         (float)(P.a[3] * cp.x + P.a[4] * cp.y + P.a[5] * cp.z),
-                                         (float)(P.a[6] * cp.x + P.a[7] * cp.y + P.a[8] * cp.z));
-                        const f3 dv = S.rolling ? vo + crossf(wo, rc) - (o.v + crossf(o.w, o.r * u))
-                                                : vo - o.v;
-                        f3 Fn = F3(0, 0, 0);
-                        dhs_f32(kn, vn, kn * ob.sr, vn * ob.sr, ob.mu, S.dtf, 0.0f, (float)cdist, dv,
-                                u, Ft, Fn, M);
-                        F += Fn + Ft;
-                    }
-                }
-    }
+    for (uint32_t q = 0; q < S.nobj; ++q) mesh_object<false>(A, S, o, ri, q, Ft, M, F, nullptr, nullptr);
 }
 
 // Waves per SIMD the register allocation targets for each variant (the
@@ -853,6 +867,74 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     if (HIST) A.hcout[j] = hc.nout;
 }
 
+// Reactions of the mesh objects for one step: the triangle contacts of k_step
+// re-evaluated on the same state and poses, in the same order (Ft and M carried
+// across the objects of a particle), so the forces are the step's own.  Per
+// object, the block's particles are reduced in a fixed order in LDS (FP64) into
+// part[(L * slots + q) * 6 ...]; every lane stays to the end.
+__global__ __launch_bounds__(256) void k_react(StepArgs A, double* part, uint32_t slots) {
+    __shared__ double red[6][256];
+    const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
+    const BlockEntry be = A.blocks[L];
+    const BScene S = A.scenes[be.scene];
+    const uint32_t t = threadIdx.x, j = be.j0 + t;
+    bool live = j < be.j1 && j < A.n;
+    const uint32_t s = live ? A.perm[j] : 0u;
+    live = live && s < A.n;
+    Own o{};
+    if (live) {
+        const double4 ip = A.in.pr[s];
+        const float4 vm = A.in.vm[s], wa = A.in.wa[s];
+        o.x = ip.x;
+        o.y = ip.y;
+        o.z = ip.z;
+        o.r = (float)ip.w;
+        o.v = F3(vm.x, vm.y, vm.z);
+        o.w = F3(wa.x, wa.y, wa.z);
+        o.m = vm.w;
+    }
+    const double ri = (double)o.r;
+    f3 Ft = F3(0, 0, 0), M = F3(0, 0, 0), F = F3(0, 0, 0);
+    for (uint32_t q = 0; q < S.nobj && q < slots; ++q) {  // uniform in the block
+        f3 rf = F3(0, 0, 0), rm = F3(0, 0, 0);
+        if (live) mesh_object<true>(A, S, o, ri, q, Ft, M, F, &rf, &rm);
+        red[0][t] = rf.x;
+        red[1][t] = rf.y;
+        red[2][t] = rf.z;
+        red[3][t] = rm.x;
+        red[4][t] = rm.y;
+        red[5][t] = rm.z;
+        __syncthreads();
+        for (uint32_t h = blockDim.x >> 1; h > 0; h >>= 1) {
+            if (t < h)
+                for (int k = 0; k < 6; ++k) red[k][t] += red[k][t + h];
+            __syncthreads();
+        }
+        if (t < 6) part[((size_t)L * slots + q) * 6 + t] = red[t][0];
+        __syncthreads();
+    }
+}
+
+// Per object g (robj[g] = first and end block of its scene, slot q): the block
+// partials summed in a fixed order into out[6 g ...] (force, moment).
+__global__ __launch_bounds__(256) void k_react_sum(const double* part, uint32_t slots,
+                                                   const uint4* robj, double* out) {
+    __shared__ double red[6][256];
+    const uint4 r = robj[blockIdx.x];
+    const uint32_t t = threadIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t L = r.x + t; L < r.y; L += 256)
+        for (int k = 0; k < 6; ++k) acc[k] += part[((size_t)L * slots + r.z) * 6 + k];
+    for (int k = 0; k < 6; ++k) red[k][t] = acc[k];
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (t < h)
+            for (int k = 0; k < 6; ++k) red[k][t] += red[k][t + h];
+        __syncthreads();
+    }
+    if (t < 6) out[(size_t)blockIdx.x * 6 + t] = red[t][0];
+}
+
 // Verlet position update (optional) + cell key of every particle, in place,
 // in storage order (block start / parity prepare).
 template <bool ADV>
@@ -1005,19 +1087,22 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
     }
     if (M == 0) return;
     if (M <= 64) {
-        if (w != 0) return;
-        uint64_t key = lane < M ? A.mkey[lane] : ~0ull;
-        uint32_t val = lane;  // payload: the append index
-        bitonic_wave(key, val);
-        if (lane < M) {
-            A.mkey2[lane] = key;
-            A.ms2[lane] = val;
+        // wave 0 sorts the keys, wave 1 the storage indices (independent)
+        if (w == 0) {
+            uint64_t key = lane < M ? A.mkey[lane] : ~0ull;
+            uint32_t val = lane;  // payload: the append index
+            bitonic_wave(key, val);
+            if (lane < M) {
+                A.mkey2[lane] = key;
+                A.ms2[lane] = val;
+            }
+        } else if (w == 1) {
+            // sorted storage indices (distinct; padding sorts last)
+            uint64_t sk = lane < M ? (uint64_t)A.ms[lane] : ~0ull;
+            uint32_t unused = 0;
+            bitonic_wave(sk, unused);
+            if (lane < M) A.mplo[A.n + lane] = (uint32_t)sk;
         }
-        // sorted storage indices (distinct; padding sorts last)
-        uint64_t sk = lane < M ? (uint64_t)A.ms[lane] : ~0ull;
-        uint32_t unused = 0;
-        bitonic_wave(sk, unused);
-        if (lane < M) A.mplo[A.n + lane] = (uint32_t)sk;
         return;
     }
     if (M <= (uint32_t)SORT_LDS_MAX) {

@@ -2,8 +2,10 @@
 oracle (the same scene the FP64 GPU path is tested on: a spinning drum with a
 fin turning with it, particles against the shell, the caps and both faces of
 the fin).  Bars: accelerations of a step within the FP32c bound of the other
-fused tests (2e-5 of the largest); runs alone vs in a mix, graph vs eager and
-through a checkpoint bit-identical."""
+fused tests (2e-5 of the largest); the fin's reaction force and moment within
+2e-5 of the summed magnitudes of its contact forces (times the largest lever
+for the moment); runs alone vs in a mix, graph vs eager and through a
+checkpoint bit-identical."""
 import numpy as np
 import pytest
 
@@ -101,6 +103,75 @@ def test_checkpoint_resume(lib, tmp_path):
     b2.step(15)
     _same(ref, b2.state(sid))
     b2.close()
+
+
+def _fin_scale(sc, w):
+    """Summed magnitudes of the fin particles' contact forces (those particles
+    touch nothing else: placed against the fin, clear of shell and lattice)."""
+    g = np.asarray(sc.gravity)
+    fc = w.mass * np.linalg.norm(w.acc - g[None, :], axis=1)   # a = (m g + F) / m
+    fin = (np.abs(w.pos[:, 0]) < 1.2 * 2e-3) & (w.pos[:, 1] < -0.026)
+    assert fin.sum() >= 5 and (fc[fin] > 0).all()
+    return fc[fin].sum()
+
+
+@pytest.mark.parametrize("start_steps", [0, 25])
+def test_reactions_vs_oracle(lib, orc, start_steps):
+    """The fin's reaction (minus its summed triangle contact forces, and their
+    moment about the fin origin) against the oracle's dpmi force and moment of
+    the same step."""
+    sc, vel = _drum_scene()
+    b = _batch_for([sc])
+    sid = b.add(sc, vel)
+    b.enable_reactions(True)
+    w = orc.World(sc)
+    w.vel[:] = vel
+    if start_steps:
+        b.step(start_steps)
+        st = b.state(sid)
+        for k in ("pos", "vel", "omega", "acc", "alpha"):
+            getattr(w, k)[:] = st[k]
+        w.w.step = start_steps
+        for m in w.motions:
+            m.n_act, m.active = start_steps, 1
+    b.step(1)
+    w.step(1)
+    f, m = b.object_reactions(sid)
+    assert f.shape == (1, 3) and m.shape == (1, 3)
+    d = w.dpmi[0]
+    of = np.array([d.force.x, d.force.y, d.force.z])
+    om = np.array([d.moment.x, d.moment.y, d.moment.z])
+    scale = _fin_scale(sc, w)
+    assert np.abs(of).max() > 1e-3 * scale            # a net reaction to compare
+    assert np.abs(f[0] - of).max() <= FP32C_TOL * scale, (f[0], of, scale)
+    assert np.abs(m[0] - om).max() <= FP32C_TOL * scale * 0.05, (m[0], om)
+    b.close()
+
+
+def test_reactions_deterministic_and_errors(lib):
+    sc, vel = _drum_scene()
+    box = scenes.box_scene(1500, 3e-3, 91, 1e-5, 100)
+    outs = []
+    for scs, vels, graphs in (([sc], [vel], True), ([box, sc], [None, vel], True),
+                              ([sc], [vel], False)):
+        b = _batch_for(scs)
+        b.graphs(graphs)
+        ids = [b.add(s, v) for s, v in zip(scs, vels)]
+        b.enable_reactions(True)
+        with pytest.raises(RuntimeError):      # nothing computed yet
+            b.object_reactions(ids[-1])
+        b.step(10)
+        b.step(10)
+        outs.append(b.object_reactions(ids[-1]))
+        if len(ids) == 2:
+            f0, m0 = b.object_reactions(ids[0])   # a scene without objects
+            assert f0.shape == (0, 3)
+        b.enable_reactions(False)
+        with pytest.raises(RuntimeError):
+            b.object_reactions(ids[-1])
+        b.close()
+    for f, m in outs[1:]:
+        assert np.array_equal(f, outs[0][0]) and np.array_equal(m, outs[0][1])
 
 
 def test_refusals(lib):
