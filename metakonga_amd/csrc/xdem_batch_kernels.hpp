@@ -639,6 +639,18 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     const uint32_t s = A.perm[j];
     if (s >= A.n) return;  // defensive: perm is a permutation of [0, n)
     const double4 ip = A.in.pr[s];
+    // The 5-wave variants load the rest of the own record after the neighbour
+    // walk (the walk does not hold it in registers); the 4-wave ones have the
+    // registers and load it here, one latency less.
+    constexpr bool EARLY = step_waves(UNR, PIPE, HIST, GEOM) < 5;
+    float4 vm = make_float4(0, 0, 0, 0), wa = vm, aa = vm;
+    uint2 zi = make_uint2(0, 0);
+    if (EARLY) {
+        vm = A.in.vm[s];
+        wa = A.in.wa[s];
+        aa = A.in.aa[s];
+        zi = A.in.zi[s];
+    }
     HistCtx hc{};
     if (HIST) {
         hc.on = S.history != 0;
@@ -779,9 +791,10 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     } else {
         scan_cells();
     }
-    // velocity and spin (not needed by the neighbour walk, so not held in
-    // registers through it)
-    const float4 vm = A.in.vm[s], wa = A.in.wa[s];
+    if (!EARLY) {
+        vm = A.in.vm[s];
+        wa = A.in.wa[s];
+    }
     o.v = F3(vm.x, vm.y, vm.z);
     o.w = F3(wa.x, wa.y, wa.z);
     o.m = vm.w;
@@ -837,8 +850,10 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
         A.dbg_cnt[j] = ntotal;
     }
     // vv_update_velocity_kernel (mphysics_cuda_impl.cuh:98-134) in FP32
-    const float4 aa = A.in.aa[s];
-    const uint2 zi = A.in.zi[s];
+    if (!EARLY) {
+        aa = A.in.aa[s];
+        zi = A.in.zi[s];
+    }
     float inv_m = frcp(o.m);
     float inv_i = frcp(0.4f * o.m * o.r * o.r);
     f3 a0 = F3(aa.x, aa.y, aa.z), al0 = F3(wa.w, aa.w, __uint_as_float(zi.x));
@@ -1263,19 +1278,23 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
     const bool few = M <= (uint32_t)MERGE_LDS;
     const uint32_t* mst = A.mplo + A.n;  // sorted mover storage indices (few movers)
     if (blockIdx.x < A.nblk) {
+        // the particle's own words first: their loads overlap the staging
+        const uint32_t s = blockIdx.x * ROWS_T + threadIdx.x;
+        const bool in = s < A.n;
+        const uint32_t kn = in ? A.key_new[s] : 0u;
+        const uint32_t ko = in ? A.key_old[s] : 0u;
+        const uint32_t id = in ? A.zi[s].y : 0u;
         if (few)
             for (uint32_t i = threadIdx.x; i < M; i += ROWS_T) {
                 smk[i] = A.mkey2[i];
                 sst[i] = mst[i];
             }
         __syncthreads();
-        const uint32_t s = blockIdx.x * ROWS_T + threadIdx.x;
-        if (s >= A.n) return;
-        uint32_t kn = A.key_new[s];
-        if (M && kn != A.key_old[s]) return;  // a mover
+        if (!in) return;
+        if (M && kn != ko) return;  // a mover
         uint32_t out = s;
         if (M) {
-            uint64_t X = ((uint64_t)kn << 32) | A.zi[s].y;
+            uint64_t X = ((uint64_t)kn << 32) | id;
             uint32_t lo = movers_below(few ? smk : A.mkey2, M, X);
             out = s - (few ? count_below(sst, M, s) : mover_prefix(A, s)) + lo;
         }
