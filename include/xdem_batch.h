@@ -74,14 +74,15 @@ typedef struct xdem_batch_scene_desc {
  * xdem_batch_add_polygon_object: a triangle mesh (ntri triangles in world
  * coordinates at the scene's step 0; .id ignored) attached to a frame at
  * origin/ep (Euler parameters, e0 first), moving with motion->velocity and
- * motion->omega about origin from the scene's first step (motion NULL: static;
- * start_time and end_time must be 0).  Contacts follow
+ * motion->omega about origin at the steps whose time k dt (the scene's step k,
+ * from 1) is >= start_time and, when end_time > start_time, <= end_time, as
+ * the simulation object's motion (motion NULL: static).  Contacts follow
  * particle_polygonObject_collision_kernel (mphysics_cuda_impl.cuh:923-1021):
  * every triangle within reach counts, normal = -(Q-P)x(R-P)/|.|.  The
  * triangles are binned once in the object's frame; the scene's cell size must
  * be at least its largest radius plus the largest circumradius (refused
- * otherwise), which makes the search complete.  No per-object reaction is
- * returned on this path (xdem_sim_get_objects gives it on the FP64 path). */
+ * otherwise), which makes the search complete.  Reactions:
+ * xdem_batch_object_reactions. */
 int32_t xdem_batch_add_drum(xdem_batch* b, uint32_t scene_id, const xdem_drum_info* drum,
                             const xdem_contact_pair_desc* wall);
 int32_t xdem_batch_add_polygon_object(xdem_batch* b, uint32_t scene_id,

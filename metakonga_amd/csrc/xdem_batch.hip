@@ -555,7 +555,7 @@ int32_t apply_layout(xdem_batch* b) {
             hob.push_back(ob);
             // the current pose, for a probe before the next step (every step
             // recomputes the poses first)
-            hpose.push_back(pose_at(ob, (double)(b->pose_steps - ob.step0) * ob.dt));
+            hpose.push_back(object_pose(ob, b->pose_steps - ob.step0));
         }
     }
     b->h_scenes = hs;
@@ -1521,9 +1521,6 @@ int32_t xdem_batch_add_polygon_object(xdem_batch* b, uint32_t scene_id,
     XDEM_REQUIRE(b && tris && ntri && origin && ep && wall, "null argument");
     HostScene* s = nullptr;
     XDEM_TRY(pending_scene(b, scene_id, &s));
-    XDEM_REQUIRE(!motion || (motion->start_time == 0 && motion->end_time == 0),
-                 "a polygon object of the batched path moves from the scene's first step "
-                 "(start_time = end_time = 0)");
     HostObject ho{};
     BObject& ob = ho.ob;
     ob.origin0 = *origin;
@@ -1531,6 +1528,26 @@ int32_t xdem_batch_add_polygon_object(xdem_batch* b, uint32_t scene_id,
     ob.vel = motion ? motion->velocity : xdem_double3{0, 0, 0};
     ob.omega = motion ? motion->omega : xdem_double3{0, 0, 0};
     ob.dt = s->desc.dt;
+    // the motion's active steps: k dt >= start_time and, when end_time >
+    // start_time, k dt <= end_time (k = 1, 2, ...; the same products as the
+    // simulation object's step time)
+    ob.k_on = 1;
+    ob.k_off = 0xFFFFFFFFu;
+    if (motion) {
+        const double dt = ob.dt, t0 = motion->start_time, t1 = motion->end_time;
+        XDEM_REQUIRE(std::isfinite(t0) && std::isfinite(t1) && t0 / dt < 4.0e9 && t1 / dt < 4.0e9,
+                     "motion times must be finite and below 4e9 steps");
+        uint64_t k = (uint64_t)std::max(1.0, std::floor(t0 / dt));
+        while ((double)k * dt < t0) ++k;
+        while (k > 1 && (double)(k - 1) * dt >= t0) --k;
+        ob.k_on = (uint32_t)k;
+        if (t1 > t0) {
+            uint64_t e = (uint64_t)std::max(0.0, std::floor(t1 / dt));
+            while (e > 0 && (double)e * dt > t1) --e;
+            while ((double)(e + 1) * dt <= t1) ++e;
+            ob.k_off = (uint32_t)e;  // below k_on: never active
+        }
+    }
     ob.cs = s->desc.cell_size;
     pair_consts(s->desc.particle, wall->wall, *wall, ob.ckn, ob.cvn, ob.sr, ob.mu);
     // local vertices A(ep0)^T (P - origin0) (toLocal, mphysics_cuda_impl.cuh:640-645)

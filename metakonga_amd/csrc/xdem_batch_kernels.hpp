@@ -73,10 +73,12 @@ struct BObject {
     uint32_t gx, gy, gz, tri0;    // local grid; first triangle in the triangle table
     uint32_t cell0, ntri;         // first cell in the local cell table
     uint32_t step0;               // pose-step count when the scene was admitted
+    uint32_t k_on, k_off;         // first and last step of the motion (the scene's steps,
+                                  // from 1; k_off = 0xFFFFFFFF: no end)
     uint32_t pad;
     xdem_double3 origin0;         // pose at the scene's step 0
     xdem_double4 ep0;
-    xdem_double3 vel, omega;      // constant motion (zero: a static object)
+    xdem_double3 vel, omega;      // the motion (zero: a static object)
     double dt;
     float ckn, cvn, sr, mu;
 };
@@ -286,7 +288,11 @@ __device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // DHS force law in FP32 (mphysics_cuda_impl.cuh:379-401).  Ft and M keep the
 // caller's values when the tangential velocity is exactly zero (as the
-// reference kernel, whose Ft/M live outside the neighbour loop).
+// reference kernel, whose Ft/M live outside the neighbour loop).  The
+// reference's dot(dv, sh) is |e| exactly (sh = e/|e| and e is dv without its
+// normal part); it is taken as |e| here: in FP32 the rounding residual of e
+// along u, multiplied by |dv|/|e| in the dot product, would make the friction
+// of a near head-on contact (|e| << |dv|) lose digits the reference keeps.
 __device__ inline void dhs_f32(float kn, float vn, float ks, float vs, float mu, float dt,
                                float ir, float delta, f3 dv, f3 u, f3& Ft, f3& Fn, f3& M) {
     float fsn = -kn * (delta * fsqrt(delta));
@@ -299,7 +305,7 @@ __device__ inline void dhs_f32(float kn, float vn, float ks, float vs, float mu,
         f3 sh = inv * e;
         float ds = mag_e * dt;
         float fn_len = fsqrt(dotf(Fn, Fn));
-        Ft = fminf(ks * ds + vs * dotf(dv, sh), mu * fn_len) * sh;
+        Ft = fminf(ks * ds + vs * mag_e, mu * fn_len) * sh;
         M = crossf(ir * u, Ft);
     }
 }
@@ -485,6 +491,19 @@ __host__ __device__ inline BPose pose_at(const BObject& ob, double tau) {
     return P;
 }
 
+// Pose of an object at step k of its scene (k = 0: before the first step).
+// The motion is active at steps k_on..k_off (the steps whose time k dt lies in
+// [start_time, end_time]); the pose has moved by the active steps so far, and
+// the object's velocity is the motion's while active and zero otherwise — the
+// simulation object's per-step motion update (mo.n_act, xdem_host.hip).
+__host__ __device__ inline BPose object_pose(const BObject& ob, uint32_t k) {
+    const bool active = k >= ob.k_on && k <= ob.k_off;
+    const uint32_t nact = k < ob.k_on ? 0u : (k < ob.k_off ? k : ob.k_off) - ob.k_on + 1u;
+    BPose P = pose_at(ob, (double)nact * ob.dt);
+    if (!active) P.vx = P.vy = P.vz = P.wx = P.wy = P.wz = 0.0f;
+    return P;
+}
+
 // Poses of every object for the coming step (one block).  The step of a
 // scene's objects is the number of pose updates since its admission, so the
 // first step after admission is step 1, as the simulation object counts.
@@ -493,7 +512,7 @@ __global__ __launch_bounds__(256) void k_poses(const BObject* objs, BPose* poses
     const uint32_t c = *counter;
     for (uint32_t i = threadIdx.x; i < nobj; i += blockDim.x) {
         const BObject& ob = objs[i];
-        poses[i] = pose_at(ob, (double)(c + 1 - ob.step0) * ob.dt);
+        poses[i] = object_pose(ob, c + 1 - ob.step0);
     }
     __syncthreads();
     if (threadIdx.x == 0) *counter = c + 1;

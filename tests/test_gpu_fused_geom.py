@@ -64,6 +64,35 @@ def test_later_step_vs_oracle(lib, orc, start_steps):
     b.close()
 
 
+@pytest.mark.parametrize("start_steps", [2, 10, 27])
+def test_motion_window_vs_oracle(lib, orc, start_steps):
+    """A fin that turns only from t = 5e-5 s to 2.5e-4 s (steps 5..25 at dt
+    1e-5): one step from the fused state before, during and after its motion
+    against the oracle (pose advanced by the active steps so far, velocity zero
+    outside them)."""
+    start, end = 5e-5, 2.5e-4
+    sc, vel = _drum_scene(start=start, end=end)
+    b = _batch_for([sc])
+    sid = b.add(sc, vel)
+    b.step(start_steps)
+    st = b.state(sid)
+    w = orc.World(sc)
+    for k in ("pos", "vel", "omega", "acc", "alpha"):
+        getattr(w, k)[:] = st[k]
+    w.w.step = start_steps
+    act = [k * sc.dt >= start and k * sc.dt <= end for k in range(1, start_steps + 1)]
+    for m in w.motions:
+        # active = 1 makes the oracle re-pose the fin at the next step even when
+        # the motion is off then (as after the transition the fused run went through)
+        m.n_act, m.active = sum(act), 1
+    b.step(1)
+    w.step(1)
+    st2 = b.state(sid)
+    assert np.abs(st2["pos"] - w.pos).max() < 1e-9
+    assert _acc_err(st2, w) < FP32C_TOL
+    b.close()
+
+
 def _run(scs, vels, steps, block, graphs=True):
     b = _batch_for(scs)
     b.graphs(graphs)
