@@ -119,8 +119,11 @@ def test_alone_vs_mix_and_graphs(lib):
     _same(alone, _run([sc], [vel], 30, 10, graphs=False)[0])
 
 
-def test_checkpoint_resume(lib, tmp_path):
-    sc, vel = _drum_scene()
+@pytest.mark.parametrize("window", [(0.0, 0.0), (5e-5, 2.5e-4)])
+def test_checkpoint_resume(lib, tmp_path, window):
+    """Bit-identical through a checkpoint at step 15, also with the fin's motion
+    window open then (steps 5..25)."""
+    sc, vel = _drum_scene(start=window[0], end=window[1])
     ref = _run([sc], [vel], 30, 15)[0]
     b = _batch_for([sc])
     b.add(sc, vel)
