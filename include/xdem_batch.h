@@ -176,6 +176,12 @@ int32_t xdem_batch_enable_reactions(xdem_batch* b, int32_t on);
  * added; *nobj = the scene's object count (force/moment may be NULL to query it). */
 int32_t xdem_batch_object_reactions(xdem_batch* b, uint32_t scene_id, xdem_double3* force,
                                     xdem_double3* moment, uint32_t cap, uint32_t* nobj);
+/* The same for the scene's drum walls (order of xdem_batch_add_drum): minus the
+ * summed drum contact forces and their moment about the drum origin (the drive
+ * torque is its component along the axis), as xdem_drum_contact_force_f64's
+ * out_mf/out_mm; evaluated with the object reactions when those are on. */
+int32_t xdem_batch_drum_reactions(xdem_batch* b, uint32_t scene_id, xdem_double3* force,
+                                  xdem_double3* moment, uint32_t cap, uint32_t* ndrum);
 /* Per-phase device times (HIP events), phases: 0 position update + hash,
  * 1 contact + Verlet, 2 sort (mover detection), 3 sort (mover ordering),
  * 4 sort (merge), 5 cell ranges.  Enabling resets the totals. */

@@ -58,6 +58,7 @@ def declare(lib):
         "xdem_batch_energy": ([vp, u32, vp, vp], i32),
         "xdem_batch_enable_reactions": ([vp, i32], i32),
         "xdem_batch_object_reactions": ([vp, u32, vp, vp, u32, vp], i32),
+        "xdem_batch_drum_reactions": ([vp, u32, vp, vp, u32, vp], i32),
         "xdem_batch_enable_timing": ([vp, i32], i32),
         "xdem_batch_phase_times": ([vp, vp, vp], i32),
         "xdem_batch_last_movers": ([vp, C.POINTER(u32)], i32),
@@ -227,19 +228,25 @@ class Batch:
         """Evaluate the mesh objects' reactions at the last step of every step call."""
         abi.check(self.lib, self.lib.xdem_batch_enable_reactions(self.h, int(on)))
 
+    def _reactions(self, fn, sid):
+        k = C.c_uint32()
+        abi.check(self.lib, fn(self.h, sid, None, None, 0, C.byref(k)))
+        f = np.zeros((k.value, 3))
+        m = np.zeros((k.value, 3))
+        if k.value:
+            abi.check(self.lib, fn(self.h, sid, abi.ptr(f), abi.ptr(m), k.value, C.byref(k)))
+        return f, m
+
     def object_reactions(self, sid):
         """(force[k, 3], moment[k, 3]) of the scene's k mesh objects at the last step:
         minus the summed triangle contact forces, and their moment about the
         object origin."""
-        k = C.c_uint32()
-        abi.check(self.lib, self.lib.xdem_batch_object_reactions(self.h, sid, None, None, 0,
-                                                                 C.byref(k)))
-        f = np.zeros((k.value, 3))
-        m = np.zeros((k.value, 3))
-        if k.value:
-            abi.check(self.lib, self.lib.xdem_batch_object_reactions(
-                self.h, sid, abi.ptr(f), abi.ptr(m), k.value, C.byref(k)))
-        return f, m
+        return self._reactions(self.lib.xdem_batch_object_reactions, sid)
+
+    def drum_reactions(self, sid):
+        """(force[k, 3], moment[k, 3]) of the scene's k drum walls at the last step,
+        the moment about each drum's origin."""
+        return self._reactions(self.lib.xdem_batch_drum_reactions, sid)
 
     def timing(self, on: bool):
         abi.check(self.lib, self.lib.xdem_batch_enable_timing(self.h, int(on)))

@@ -580,10 +580,11 @@ int32_t apply_layout(xdem_batch* b) {
     XDEM_TRY(upload_vec(&b->d_blocks_store, bt, st));
     b->nblk_sorted = (uint32_t)bs.size();
     b->nblk_store = (uint32_t)bt.size();
-    // reaction tables: per object the block range of its scene (blocks are
-    // contiguous per scene in the sorted layout) and its slot
+    // reaction tables: per body (every mesh object, then every drum) the block
+    // range of its scene (blocks are contiguous per scene in the sorted layout)
+    // and its slot in the scene (objects 0.., drums after them)
     b->react_valid = false;
-    if (b->nobj_total) {
+    if (b->nobj_total || b->ndrum_total) {
         std::vector<uint2> range(hs.size(), make_uint2(0, 0));
         std::vector<bool> seen(hs.size(), false);
         for (uint32_t L = 0; L < bs.size(); ++L) {
@@ -595,9 +596,12 @@ int32_t apply_layout(xdem_batch* b) {
         std::vector<uint4> robj;
         uint32_t slots = 0;
         for (uint32_t k = 0; k < hs.size(); ++k) {
-            slots = std::max(slots, hs[k].nobj);
+            slots = std::max(slots, hs[k].nobj + hs[k].ndrum);
             for (uint32_t q = 0; q < hs[k].nobj; ++q) robj.push_back(make_uint4(range[k].x, range[k].y, q, 0));
         }
+        for (uint32_t k = 0; k < hs.size(); ++k)
+            for (uint32_t d = 0; d < hs[k].ndrum; ++d)
+                robj.push_back(make_uint4(range[k].x, range[k].y, hs[k].nobj + d, 0));
         b->react_slots = slots;
         XDEM_TRY(upload_vec(&b->d_robj, robj, st));
         const size_t need = (size_t)b->nblk_sorted * slots * 6;
@@ -608,12 +612,13 @@ int32_t apply_layout(xdem_batch* b) {
             XDEM_TRY(dalloc(&b->d_rpart, need));
             b->rpart_cap = need;
         }
-        if ((size_t)b->nobj_total * 6 > b->react_cap) {
+        const size_t nbody = (size_t)b->nobj_total + b->ndrum_total;
+        if (nbody * 6 > b->react_cap) {
             if (b->d_react) (void)hipFree(b->d_react);
             b->d_react = nullptr;
             b->react_cap = 0;
-            XDEM_TRY(dalloc(&b->d_react, (size_t)b->nobj_total * 6));
-            b->react_cap = (size_t)b->nobj_total * 6;
+            XDEM_TRY(dalloc(&b->d_react, nbody * 6));
+            b->react_cap = nbody * 6;
         }
     }
     XDEM_HIP(hipStreamSynchronize(st));
@@ -725,13 +730,13 @@ int32_t launch_step(xdem_batch* b, bool adv) {
         XDEM_LAUNCHED();
     }
     StepArgs a = step_args(b, false);
-    if (!adv && b->react_on && b->nobj_total) {
+    if (!adv && b->react_on && b->geom_active()) {
         // reactions of this (last) step, on the state and poses k_step uses
         hipLaunchKernelGGL(k_react, dim3(b->nblk_sorted), dim3(b->block_threads), 0, b->st, a,
                            b->d_rpart, b->react_slots);
         XDEM_LAUNCHED();
-        hipLaunchKernelGGL(k_react_sum, dim3(b->nobj_total), dim3(256), 0, b->st, b->d_rpart,
-                           b->react_slots, b->d_robj, b->d_react);
+        hipLaunchKernelGGL(k_react_sum, dim3(b->nobj_total + b->ndrum_total), dim3(256), 0, b->st,
+                           b->d_rpart, b->react_slots, b->d_robj, b->d_react);
         XDEM_LAUNCHED();
         b->react_valid = true;
     }
@@ -1315,8 +1320,9 @@ int32_t xdem_batch_enable_reactions(xdem_batch* b, int32_t on) {
     return XDEM_OK;
 }
 
-int32_t xdem_batch_object_reactions(xdem_batch* b, uint32_t scene_id, xdem_double3* force,
-                                    xdem_double3* moment, uint32_t cap, uint32_t* nobj) {
+// Reactions of one scene's bodies of one kind (drums = false: mesh objects).
+static int32_t body_reactions(xdem_batch* b, uint32_t scene_id, bool drums, xdem_double3* force,
+                              xdem_double3* moment, uint32_t cap, uint32_t* count) {
     XDEM_REQUIRE(b, "null batch");
     XDEM_REQUIRE(b->react_on, "reactions are off (xdem_batch_enable_reactions)");
     uint32_t k = 0;
@@ -1326,18 +1332,30 @@ int32_t xdem_batch_object_reactions(xdem_batch* b, uint32_t scene_id, xdem_doubl
                      k < b->h_scenes.size(),
                  "no reactions of scene %u yet (step first)", scene_id);
     const BScene& bs = b->h_scenes[k];
-    if (nobj) *nobj = bs.nobj;
-    XDEM_REQUIRE(cap >= bs.nobj || (!force && !moment), "capacity %u < %u objects", cap, bs.nobj);
-    if (!bs.nobj || (!force && !moment)) return XDEM_OK;
-    std::vector<double> h((size_t)bs.nobj * 6);
-    XDEM_HIP(hipMemcpyAsync(h.data(), b->d_react + (size_t)bs.obj0 * 6, 8 * h.size(),
-                            hipMemcpyDeviceToHost, b->st));
+    const uint32_t n = drums ? bs.ndrum : bs.nobj;
+    const size_t first = drums ? (size_t)b->nobj_total + bs.drum0 : (size_t)bs.obj0;
+    if (count) *count = n;
+    XDEM_REQUIRE(cap >= n || (!force && !moment), "capacity %u < %u bodies", cap, n);
+    if (!n || (!force && !moment)) return XDEM_OK;
+    std::vector<double> h((size_t)n * 6);
+    XDEM_HIP(hipMemcpyAsync(h.data(), b->d_react + first * 6, 8 * h.size(), hipMemcpyDeviceToHost,
+                            b->st));
     XDEM_HIP(hipStreamSynchronize(b->st));
-    for (uint32_t q = 0; q < bs.nobj; ++q) {
+    for (uint32_t q = 0; q < n; ++q) {
         if (force) force[q] = xdem_double3{h[6 * q], h[6 * q + 1], h[6 * q + 2]};
         if (moment) moment[q] = xdem_double3{h[6 * q + 3], h[6 * q + 4], h[6 * q + 5]};
     }
     return XDEM_OK;
+}
+
+int32_t xdem_batch_drum_reactions(xdem_batch* b, uint32_t scene_id, xdem_double3* force,
+                                  xdem_double3* moment, uint32_t cap, uint32_t* ndrum) {
+    return body_reactions(b, scene_id, true, force, moment, cap, ndrum);
+}
+
+int32_t xdem_batch_object_reactions(xdem_batch* b, uint32_t scene_id, xdem_double3* force,
+                                    xdem_double3* moment, uint32_t cap, uint32_t* nobj) {
+    return body_reactions(b, scene_id, false, force, moment, cap, nobj);
 }
 
 int32_t xdem_batch_energy(xdem_batch* b, uint32_t scene_id, double* kinetic, double momentum[3]) {

@@ -592,46 +592,59 @@ __device__ inline void mesh_object(const StepArgs& A, const BScene& S, const Own
 // (distance to the closest point, face normal, object point velocity), with
 // Ft carried from triangle contact to triangle contact as in
 // particle_polygonObject_collision_kernel (mphysics_cuda_impl.cuh:923-1021).
+// Drum wall q of the scene with the particle `o` (FP64 geometry, FP32 DHS with
+// ir = 0).  REACT: the reaction on the drum is accumulated too: -(Fn + Ft) and
+// its moment about the drum origin, (contact point - origin) x -(Fn + Ft), as
+// xdem_drum_contact_force_f64 reports it.
+template <bool REACT>
+__device__ inline void drum_wall(const StepArgs& A, const BScene& S, const Own& o, double ri,
+                                 uint32_t q, f3& F, f3* rf, f3* rm) {
+    const d3 p = mk3(o.x, o.y, o.z);
+    const BDrum& bd = A.drums[S.drum0 + q];
+    const d3 ax = of(bd.d.axis), c0 = of(bd.d.origin);
+    const d3 dd = p - c0;
+    const double a = dot(dd, ax);
+    if (!(fabs(a) <= bd.d.half_length)) return;
+    const d3 qv = dd - a * ax;
+    const double q2 = dot(qv, qv);
+    // clear of the shell (rho + r < R, decided on rho^2 with a margin far
+    // above its rounding) and of the caps: no contact, no square root
+    const double rin = bd.d.radius - ri;
+    if (rin > 0 && q2 < rin * rin * (1.0 - 1e-12) &&
+        !(bd.d.caps && ri - (bd.d.half_length - fabs(a)) > 0))
+        return;
+    const double rho = sqrt(q2);
+    const float kn = bd.ckn * fsqrt(o.r);
+    const float vn = fsqrt(bd.cvn * o.m * kn);
+    auto contact = [&](d3 u, d3 cpt, double cdist) {
+        const f3 uf = F3((float)u.x, (float)u.y, (float)u.z);
+        const d3 lever = cpt - c0;
+        const d3 wv = bd.d.omega * cross(ax, lever);
+        const f3 dv = F3((float)wv.x, (float)wv.y, (float)wv.z) - (o.v + crossf(o.w, o.r * uf));
+        f3 Ft = F3(0, 0, 0), Fn = F3(0, 0, 0), M = F3(0, 0, 0);
+        dhs_f32(kn, vn, kn * bd.sr, vn * bd.sr, bd.mu, S.dtf, 0.0f, (float)cdist, dv, uf, Ft, Fn,
+                M);
+        F += Fn + Ft;
+        if (REACT) {
+            *rf += -(Fn + Ft);
+            *rm += crossf(F3((float)lever.x, (float)lever.y, (float)lever.z), -(Fn + Ft));
+        }
+    };
+    const double cs = ri + rho - bd.d.radius;
+    if (cs > 0 && rho > 0) {
+        const d3 u = qv / rho;
+        contact(u, c0 + a * ax + bd.d.radius * u, cs);
+    }
+    const double cc = ri - (bd.d.half_length - fabs(a));
+    if (bd.d.caps && cc > 0 && rho < bd.d.radius) {
+        const double sg = a >= 0 ? 1.0 : -1.0;
+        contact(sg * ax, c0 + (sg * bd.d.half_length) * ax + qv, cc);
+    }
+}
+
 __device__ inline void walls_and_meshes(const StepArgs& A, const BScene& S, const Own& o,
                                         double ri, f3& F) {
-    const d3 p = mk3(o.x, o.y, o.z);
-    for (uint32_t q = 0; q < S.ndrum; ++q) {
-        const BDrum& bd = A.drums[S.drum0 + q];
-        const d3 ax = of(bd.d.axis), c0 = of(bd.d.origin);
-        const d3 dd = p - c0;
-        const double a = dot(dd, ax);
-        if (!(fabs(a) <= bd.d.half_length)) continue;
-        const d3 qv = dd - a * ax;
-        const double q2 = dot(qv, qv);
-        // clear of the shell (rho + r < R, decided on rho^2 with a margin far
-        // above its rounding) and of the caps: no contact, no square root
-        const double rin = bd.d.radius - ri;
-        if (rin > 0 && q2 < rin * rin * (1.0 - 1e-12) &&
-            !(bd.d.caps && ri - (bd.d.half_length - fabs(a)) > 0))
-            continue;
-        const double rho = sqrt(q2);
-        const float kn = bd.ckn * fsqrt(o.r);
-        const float vn = fsqrt(bd.cvn * o.m * kn);
-        auto contact = [&](d3 u, d3 cpt, double cdist) {
-            const f3 uf = F3((float)u.x, (float)u.y, (float)u.z);
-            const d3 wv = bd.d.omega * cross(ax, cpt - c0);
-            const f3 dv = F3((float)wv.x, (float)wv.y, (float)wv.z) - (o.v + crossf(o.w, o.r * uf));
-            f3 Ft = F3(0, 0, 0), Fn = F3(0, 0, 0), M = F3(0, 0, 0);
-            dhs_f32(kn, vn, kn * bd.sr, vn * bd.sr, bd.mu, S.dtf, 0.0f, (float)cdist, dv, uf, Ft,
-                    Fn, M);
-            F += Fn + Ft;
-        };
-        const double cs = ri + rho - bd.d.radius;
-        if (cs > 0 && rho > 0) {
-            const d3 u = qv / rho;
-            contact(u, c0 + a * ax + bd.d.radius * u, cs);
-        }
-        const double cc = ri - (bd.d.half_length - fabs(a));
-        if (bd.d.caps && cc > 0 && rho < bd.d.radius) {
-            const double sg = a >= 0 ? 1.0 : -1.0;
-            contact(sg * ax, c0 + (sg * bd.d.half_length) * ax + qv, cc);
-        }
-    }
+    for (uint32_t q = 0; q < S.ndrum; ++q) drum_wall<false>(A, S, o, ri, q, F, nullptr, nullptr);
     f3 Ft = F3(0, 0, 0), M = F3(0, 0, 0);  // carried across triangle contacts
     for (uint32_t q = 0; q < S.nobj; ++q) mesh_object<false>(A, S, o, ri, q, Ft, M, F, nullptr, nullptr);
 }
@@ -901,11 +914,11 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     if (HIST) A.hcout[j] = hc.nout;
 }
 
-// Reactions of the mesh objects for one step: the triangle contacts of k_step
-// re-evaluated on the same state and poses, in the same order (Ft and M carried
-// across the objects of a particle), so the forces are the step's own.  Per
-// object, the block's particles are reduced in a fixed order in LDS (FP64) into
-// part[(L * slots + q) * 6 ...]; every lane stays to the end.
+// Reactions of the mesh objects and drums for one step: their contacts of
+// k_step re-evaluated on the same state and poses, in the same order (Ft and M
+// carried across the objects of a particle), so the forces are the step's own.
+// Per body, the block's particles are reduced in a fixed order in LDS (FP64)
+// into part[(L * slots + q) * 6 ...]; every lane stays to the end.
 __global__ __launch_bounds__(256) void k_react(StepArgs A, double* part, uint32_t slots) {
     __shared__ double red[6][256];
     const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
@@ -929,9 +942,15 @@ __global__ __launch_bounds__(256) void k_react(StepArgs A, double* part, uint32_
     }
     const double ri = (double)o.r;
     f3 Ft = F3(0, 0, 0), M = F3(0, 0, 0), F = F3(0, 0, 0);
-    for (uint32_t q = 0; q < S.nobj && q < slots; ++q) {  // uniform in the block
+    // bodies of the scene: its mesh objects, then its drums (slot nobj + d)
+    for (uint32_t q = 0; q < S.nobj + S.ndrum && q < slots; ++q) {  // uniform in the block
         f3 rf = F3(0, 0, 0), rm = F3(0, 0, 0);
-        if (live) mesh_object<true>(A, S, o, ri, q, Ft, M, F, &rf, &rm);
+        if (live) {
+            if (q < S.nobj)
+                mesh_object<true>(A, S, o, ri, q, Ft, M, F, &rf, &rm);
+            else
+                drum_wall<true>(A, S, o, ri, q - S.nobj, F, &rf, &rm);
+        }
         red[0][t] = rf.x;
         red[1][t] = rf.y;
         red[2][t] = rf.z;

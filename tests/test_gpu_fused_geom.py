@@ -151,7 +151,7 @@ def _fin_scale(sc, w):
 def test_reactions_vs_oracle(lib, orc, start_steps):
     """The fin's reaction (minus its summed triangle contact forces, and their
     moment about the fin origin) against the oracle's dpmi force and moment of
-    the same step."""
+    the same step; the drum's reaction against the oracle's drum reactions."""
     sc, vel = _drum_scene()
     b = _batch_for([sc])
     sid = b.add(sc, vel)
@@ -166,8 +166,20 @@ def test_reactions_vs_oracle(lib, orc, start_steps):
         w.w.step = start_steps
         for m in w.motions:
             m.n_act, m.active = start_steps, 1
+    vel_in, om_in = w.vel.copy(), w.omega.copy()
     b.step(1)
     w.step(1)
+    # the drum: the oracle's per-particle drum reactions at contact time
+    # (positions after the step's position update, the step's input velocities)
+    mf, mm = orc.drum(sc.params(), sc.tcm, sc.drums[0], w.pos, vel_in, om_in,
+                      np.zeros((sc.np, 3)), np.zeros((sc.np, 3)), w.mass, sc.contact_pp(),
+                      reaction=True)
+    df, dm = b.drum_reactions(sid)
+    dscale = np.linalg.norm(mf, axis=1).sum()
+    assert df.shape == (1, 3) and dscale > 0
+    assert np.abs(df[0] - mf.sum(0)).max() <= FP32C_TOL * dscale, (df[0], mf.sum(0))
+    assert np.abs(dm[0] - mm.sum(0)).max() <= FP32C_TOL * dscale * 0.05, (dm[0], mm.sum(0))
+    assert abs(mm.sum(0)[2]) > 1e-3 * dscale * 0.04     # a drive torque about the axis
     f, m = b.object_reactions(sid)
     assert f.shape == (1, 3) and m.shape == (1, 3)
     d = w.dpmi[0]
@@ -196,14 +208,16 @@ def test_reactions_deterministic_and_errors(lib):
         b.step(10)
         outs.append(b.object_reactions(ids[-1]))
         if len(ids) == 2:
-            f0, m0 = b.object_reactions(ids[0])   # a scene without objects
-            assert f0.shape == (0, 3)
+            f0, m0 = b.object_reactions(ids[0])   # a scene without objects or drums
+            assert f0.shape == (0, 3) and b.drum_reactions(ids[0])[0].shape == (0, 3)
+        outs.append(b.drum_reactions(ids[-1]))
         b.enable_reactions(False)
         with pytest.raises(RuntimeError):
             b.object_reactions(ids[-1])
         b.close()
-    for f, m in outs[1:]:
-        assert np.array_equal(f, outs[0][0]) and np.array_equal(m, outs[0][1])
+    for kind in (outs[0::2], outs[1::2]):         # objects, drums
+        for f, m in kind[1:]:
+            assert np.array_equal(f, kind[0][0]) and np.array_equal(m, kind[0][1])
 
 
 def test_refusals(lib):
