@@ -333,3 +333,95 @@ def test_cylinder_lateral(orc):
     assert c == float(np.float32(0.2 + r - (0.2 + r - 1e-4)))
     assert np.allclose(u, [-1, 0, 0])
     assert np.allclose(cp, [0.2, 0.1, 0.0], atol=1e-15)
+
+
+def _cyl(axis_ep=(1.0, 0.0, 0.0, 0.0), L=1.0, R=0.2):
+    """Solid cylinder of length L and radius R centred at the origin, its axis
+    the local y axis turned by the Euler parameters (e0 first)."""
+    e0, e1, e2, e3 = axis_ep
+    ax = np.array([2 * (e1 * e2 - e0 * e3), 2 * (e0 * e0 + e2 * e2 - 0.5), 2 * (e2 * e3 + e0 * e1)])
+    cy = abi.CylinderInfo()
+    cy.len = L
+    cy.rbase = cy.rtop = R
+    cy.pbase = abi._d3(-0.5 * L * ax)
+    cy.ptop = abi._d3(0.5 * L * ax)
+    cy.origin = abi.double3(0, 0, 0)
+    cy.ep = abi.double4(*axis_ep)
+    return cy, ax
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.7])
+def test_cylinder_cap_and_rim(orc, theta):
+    """Cap region (beyond an end face, within the radius): cdist = r - h for a
+    particle h above the face, normal along the axis towards the centre.  Rim
+    region (beyond the face and outside the radius): the distance to the rim
+    circle, sqrt((rho - R)^2 + h^2), normal towards the rim point.  Both ends,
+    for an axis along y and one turned by theta about x (the rim is found in
+    the cylinder's own frame).  Values rounded to FP32 as the reference's
+    float return."""
+    ep = (math.cos(theta / 2), math.sin(theta / 2), 0.0, 0.0)
+    cy, ax = _cyl(ep)
+    L, R, r = 1.0, 0.2, 5e-3
+    # a radial direction perpendicular to the axis
+    e = np.cross(ax, [1.0, 0.0, 0.0])
+    e = e / np.linalg.norm(e) if np.linalg.norm(e) > 0 else np.array([0.0, 0.0, 1.0])
+    for side in (1.0, -1.0):
+        face = side * 0.5 * L * ax
+        # cap: lateral offset 0.1 < R, height h above the face
+        h = r - 1e-4
+        p = face + side * h * ax + 0.1 * e
+        c, u, cp = orc.cylinder_detect(cy, np.r_[p, r])
+        assert c == float(np.float32(r - h)), (side, c)
+        assert np.allclose(u, -side * ax, atol=1e-12)
+        # rim: radial offset R + a, height b above the face, distance < r
+        a, b = 2e-3, 3e-3
+        p = face + side * b * ax + (R + a) * e
+        c, u, cp = orc.cylinder_detect(cy, np.r_[p, r])
+        d = math.hypot(a, b)
+        assert abs(c - (r - d)) <= 2.0 ** -23 * (r - d), (side, c, r - d)   # FP32 return
+        rim = face + R * e
+        assert np.allclose(cp, rim, atol=1e-12)
+        assert np.allclose(u, (rim - p) / d, atol=1e-9)
+        # farther than r from the rim: no contact
+        p = face + side * 6e-3 * ax + (R + 6e-3) * e
+        assert orc.cylinder_detect(cy, np.r_[p, r])[0] < 0
+
+
+def _tri_scene(pos):
+    """A static 6 mm triangle in the plane y = 0 (face normal -(Q-P)x(R-P) = -y),
+    object origin (0.01, 0, 0.02)."""
+    P, Q, R = (-3e-3, 0.0, -3e-3), (-3e-3, 0.0, 3e-3), (3e-3, 0.0, 0.0)
+    return scenes.Scene(name="tri", pos=np.ascontiguousarray(pos), dt=1e-5, nsteps=1,
+                        world_origin=(-0.05, -0.05, -0.05), grid_size=(32, 32, 32),
+                        tri=np.array([[P, Q, R]]),
+                        objects=[scenes.PolygonObject(origin=(0.01, 0.0, 0.02))])
+
+
+def test_polygon_reaction_closed_form(orc):
+    """Particles at rest against a static triangle: each contact's force is
+    Hertz kn delta^1.5 along the face normal (no damping, no tangential part);
+    the object's reaction is minus the sum of the particle forces, its moment
+    about the object origin -(cp - origin) x F summed (cp the foot of the
+    perpendicular).  One particle, then two (the sums add up)."""
+    r = 2e-3
+    pts = [(0.5e-3, r - 1e-5, -0.5e-3), (-1.5e-3, r - 3e-5, 1.0e-3)]
+    origin = np.array([0.01, 0.0, 0.02])
+    for npart in (1, 2):
+        pos = np.array([[x, y, z, r] for x, y, z in pts[:npart]])
+        sc = _tri_scene(pos)
+        w = orc.World(sc)
+        w.step(1)      # from rest: the contact positions are the initial ones
+        F = np.zeros(3)
+        Mo = np.zeros(3)
+        for k in range(npart):
+            c = orc.force_constant(1, r, 0.0, w.mass[k], 0.0, sc.contact_pp())
+            delta = r - pos[k, 1]
+            f = c.kn * delta ** 1.5 * np.array([0.0, 1.0, 0.0])   # on the particle, +y
+            cp = np.array([pos[k, 0], 0.0, pos[k, 2]])
+            F += f
+            Mo += np.cross(cp - origin, f)
+        d = w.dpmi[0]
+        got_f = np.array([d.force.x, d.force.y, d.force.z])
+        got_m = np.array([d.moment.x, d.moment.y, d.moment.z])
+        assert np.allclose(got_f, -F, rtol=1e-12, atol=0), (npart, got_f, -F)
+        assert np.allclose(got_m, -Mo, rtol=1e-12, atol=1e-30), (npart, got_m, -Mo)
