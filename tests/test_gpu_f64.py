@@ -437,3 +437,72 @@ def test_errors_are_reported(lib):
     p = make_params(10, (0, 0, 0), 0.01, (100, 128, 128))
     rc = lib.xdem_set_symbolic_parameter(C.byref(p))
     assert rc == 1 and b"power" in lib.xdem_last_error()
+
+
+def test_zero_particles_are_no_ops(lib):
+    """Every FP64 entry point accepts np = 0 with null particle arrays and does
+    nothing (the reference's launch arithmetic divides by zero there, which the
+    library does not reproduce); a reduction of nothing is the zero vector."""
+    set_params(lib, make_params(0, (-1, -1, -1), 0.01, (128, 128, 128)))
+    cp_d, _ = struct_dev(lib, abi.ContactProperty, [scenes.Scene(
+        name="x", pos=np.zeros((1, 4)), dt=1e-5, nsteps=1).contact_pp()])
+    pl = scenes.plane4((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1))
+    pl_d, _ = struct_dev(lib, abi.PlaneInfo, [pl] * 6)
+    cy = abi.CylinderInfo()
+    cy.len, cy.rbase, cy.rtop = 1.0, 0.1, 0.1
+    cy.ptop, cy.ep = abi.double3(0, 1, 0), abi.double4(1, 0, 0, 0)
+    cy_d, _ = struct_dev(lib, abi.CylinderInfo, [cy])
+    dr_d, _ = struct_dev(lib, abi.DrumInfo, [scenes.drum_info((0, 0, 0), (0, 0, 1), 0.1, 0.1, 1.0)])
+    z = None
+    out_f, out_m = abi.double3(1, 1, 1), abi.double3(1, 1, 1)
+    mf_d, mm_d = DBuf(lib, nbytes=24), DBuf(lib, nbytes=24)   # per-particle sums given: summed over none
+    rcs = {
+        "vv_position": lib.xdem_vv_update_position_f64(z, z, z, 0),
+        "vv_velocity": lib.xdem_vv_update_velocity_f64(z, z, z, z, z, z, z, z, 0),
+        "hash": lib.xdem_calculate_hash_and_index_f64(z, z, z, 0),
+        "p2p": lib.xdem_calculate_p2p_f64(1, z, z, z, z, z, z, z, z, z, cp_d.ptr, 0),
+        "plane": lib.xdem_plane_contact_force_f64(1, pl_d.ptr, z, z, z, z, z, z, 0, cp_d.ptr),
+        "cube": lib.xdem_cube_contact_force_f64(1, pl_d.ptr, z, z, z, z, z, z, 0, cp_d.ptr),
+        "cylinder": lib.xdem_cylinder_hertzian_contact_force_f64(
+            1, cy_d.ptr, z, z, z, z, z, z, 0, cp_d.ptr, z, mf_d.ptr, mm_d.ptr, C.byref(out_f),
+            C.byref(out_m)),
+    }
+    assert (out_f.x, out_f.y, out_f.z, out_m.x, out_m.y, out_m.z) == (0.0,) * 6
+    out_f, out_m = abi.double3(1, 1, 1), abi.double3(1, 1, 1)
+    rcs["drum"] = lib.xdem_drum_contact_force_f64(1, dr_d.ptr, z, z, z, z, z, z, 0, cp_d.ptr,
+                                                  mf_d.ptr, mm_d.ptr, C.byref(out_f), C.byref(out_m))
+    assert all(rc == 0 for rc in rcs.values()), (rcs, lib.xdem_last_error())
+    assert (out_f.x, out_f.y, out_f.z, out_m.x, out_m.y, out_m.z) == (0.0,) * 6
+    red = abi.double3(1, 1, 1)
+    abi.check(lib, lib.xdem_reduction_d3(z, 0, C.byref(red)))
+    assert (red.x, red.y, red.z) == (0.0, 0.0, 0.0)
+    abi.check(lib, lib.xdem_synchronize())
+
+
+def test_single_particle(lib, orc):
+    """One particle: no particle-particle contact (its own cell holds only
+    itself), and a resting contact on a floor plane matches the oracle over the
+    first steps on both GPU paths."""
+    from metakonga_amd.batch import Batch
+    from metakonga_amd.xdem import Simulation
+    r = float(np.float32(5e-3))      # FP32-representable: both paths hold the same radius
+    pos = np.array([[0.02, r - 2e-5, 0.03, r]])
+    floor = scenes.plane4((-0.1, 0.0, -0.1), (0.1, 0.0, -0.1), (0.1, 0.0, 0.1), (-0.1, 0.0, 0.1))
+    sc = scenes.Scene(name="one", pos=pos, dt=1e-5, nsteps=5, world_origin=(-0.2, -0.2, -0.2),
+                      cell_size=2 * r, grid_size=(64, 64, 64), planes=[floor])
+    w = orc.World(sc)
+    w.step(5)
+    sim = Simulation(sc, lib)
+    sim.step(5)
+    st = sim.state()
+    assert np.abs(st["pos"] - w.pos).max() < 1e-15
+    assert np.abs(st["vel"] - w.vel).max() <= 1e-12 * np.abs(w.vel).max()
+    assert w.vel[0, 1] > 0                  # pushed up by the floor
+    sim.close()
+    b = Batch(1, 1, int(np.prod(sc.grid_size)), lib)
+    sid = b.add(sc)
+    b.step(5)
+    st = b.state(sid)
+    assert np.abs(st["pos"] - w.pos).max() < 1e-12
+    assert np.abs(st["vel"] - w.vel).max() <= 1e-4 * np.abs(w.vel).max()
+    b.close()
