@@ -237,6 +237,20 @@ def cpu_baseline(scene, budget_s=15.0, threads=1):
     return dict(value=scene.np * k / dt, steps=k, seconds=dt)
 
 
+def cpu_settling_full():
+    """The 8,000-sphere settling box in full (2,000 steps, 1.6e7 particle-steps)
+    on the oracle's serial CPU-path semantics."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.load()
+    sc = scenes.settling_box()
+    w = pyoracle.World(sc, semantics=pyoracle.ORC_HOST)
+    t0 = time.perf_counter()
+    w.step(sc.nsteps)
+    dt = time.perf_counter() - t0
+    return dict(value=sc.np * sc.nsteps / dt, steps=sc.nsteps, seconds=dt, np=sc.np)
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
@@ -356,6 +370,7 @@ def main():
 
     cpu = None
     cpu_omp = None
+    cpu_small = None
     if not args.no_cpu:
         log("cpu baseline (oracle, CPU-path semantics, 1 thread) ...")
         c = cpu_baseline(scene, budget_s=args.cpu_budget)
@@ -369,6 +384,11 @@ def main():
         cpu_omp = {"value": co["value"], "unit": "particle-steps/s", "cores": nthr,
                    "kind": "port", "sample": f"same, OpenMP over particles, {co['steps']} steps"}
         log(f"cpu omp x{nthr}: {co['value']:.4g} particle-steps/s")
+        cs = cpu_settling_full()
+        cpu_small = {"value": cs["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                     "sample": f"settling box {cs['np']} spheres in full, {cs['steps']} steps "
+                               f"({cs['seconds']:.1f}s), serial CPU-path semantics"}
+        log(f"cpu settling 8k in full: {cs['value']:.4g} particle-steps/s")
 
     out = {
         "metric": METRIC,
@@ -404,6 +424,8 @@ def main():
         out["c2_drum"] = c2
     if cpu_omp is not None:
         out["cpu_baseline_multicore"] = cpu_omp
+    if cpu_small is not None:
+        out["cpu_baseline_settling_8k"] = cpu_small
     if serving is not None:
         out["serving"] = serving
     print(json.dumps(out), flush=True)
