@@ -1592,12 +1592,13 @@ int32_t xdem_batch_add_polygon_object(xdem_batch* b, uint32_t scene_id,
         XDEM_REQUIRE(nl > 0, "triangle %u is degenerate", t);
         n = n / nl;
         cc[t] = circumcenter_host(P, Q, R);
-        d3 dp = cc[t] - P;
+        d3 dp = cc[t] - P, dq = cc[t] - Q, dr = cc[t] - R;
+        const double rc = std::sqrt(std::max(dot(dp, dp), std::max(dot(dq, dq), dot(dr, dr))));
         rc_max = std::max(rc_max, std::sqrt(dot(dp, dp)));
         lo = mk3(std::min(lo.x, cc[t].x), std::min(lo.y, cc[t].y), std::min(lo.z, cc[t].z));
         hi = mk3(std::max(hi.x, cc[t].x), std::max(hi.y, cc[t].y), std::max(hi.z, cc[t].z));
-        lt[t] = BTri{P.x, P.y, P.z, Q.x, Q.y, Q.z, R.x, R.y, R.z, (float)n.x, (float)n.y,
-                     (float)n.z, 0.0f};
+        lt[t] = BTri{cc[t].x, cc[t].y, cc[t].z, rc, P.x, P.y, P.z, Q.x, Q.y, Q.z, R.x, R.y, R.z,
+                     (float)n.x, (float)n.y, (float)n.z, 0.0f};
     }
     XDEM_REQUIRE(ob.cs >= s->rmax + rc_max,
                  "cell size %.9g is below the largest radius %.9g plus the largest triangle "

@@ -87,6 +87,7 @@ struct BObject {
 // normal of particle_polygonObject_collision_kernel, mphysics_cuda_impl.cuh:
 // 923-1021).
 struct BTri {
+    double ccx, ccy, ccz, rc;     // circumcentre and circumradius (bounding sphere)
     double px, py, pz, qx, qy, qz, rx, ry, rz;
     float nx, ny, nz, pad;
 };
@@ -558,7 +559,12 @@ __device__ inline void mesh_object(const StepArgs& A, const BScene& S, const Own
                                           (uint32_t)x];
                 const uint32_t e1 = min(cr.y, ob.ntri);
                 for (uint32_t t = cr.x; t < e1; ++t) {
-                    const BTri T = A.tris[ob.tri0 + t];
+                    const BTri& T = A.tris[ob.tri0 + t];
+                    // the triangle lies in its circumsphere: farther than rc + ri
+                    // from the centre, no contact (1e-9 margin over the rounding)
+                    const double sx = pl.x - T.ccx, sy = pl.y - T.ccy, sz = pl.z - T.ccz;
+                    const double reach = T.rc + ri;
+                    if (sx * sx + sy * sy + sz * sz > reach * reach * (1.0 + 1e-9)) continue;
                     const d3 cp = closest_pt_tri(pl, mk3(T.px, T.py, T.pz),
                                                  mk3(T.qx, T.qy, T.qz), mk3(T.rx, T.ry, T.rz));
                     const d3 dl = pl - cp;
