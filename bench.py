@@ -150,8 +150,8 @@ def run_settling_8k(lib, steps=2000):
 
 
 def run_c2_drum(lib, steps=200, f64_steps=20):
-    """4e6 spheres (r = 3 mm) in the 30 rpm drum with its turning
-    20,000-triangle baffle, dt = 5e-6 s, on the fused FP32c path (graph-replayed
+    """The drum workload: 4e6 spheres (r = 3 mm) in the 30 rpm drum with its turning
+    20,000-triangle lifter-plate baffle, dt = 5e-6 s, on the fused FP32c path (graph-replayed
     100-step blocks, autotuned) and on the plain FP64 path.  Informational: a
     reduced number of steps of the 2,000-step configuration."""
     from metakonga_amd import batch
@@ -189,7 +189,7 @@ def run_c2_drum(lib, steps=200, f64_steps=20):
     f64 = sc.np * f64_steps / (time.perf_counter() - t0)
     sim.close()
     return {"workload": f"drum {sc.np} spheres r 3 mm, R 0.5 m, L 1.6 m, 30 rpm, "
-                        f"{len(sc.tri)}-triangle turning baffle, dt 5e-6 s, cell 6 mm, grid 256^3",
+                        f"{len(sc.tri)}-triangle turning lifter plate, dt 5e-6 s, cell 6 mm, grid 256^3",
             "fp32c_fused": f32c, "fp32c_steps": steps, "fp64_plain": f64, "fp64_steps": f64_steps,
             "speedup_vs_fp64": f32c / f64, "contact_ms": ms[1] / cnt[1],
             "autotune_chosen": tuned[0], "generation_s": gen}

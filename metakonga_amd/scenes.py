@@ -374,37 +374,43 @@ def hopper(n=1_000_000, r_min=2e-3, r_max=4e-3, seed=0xD3E1, dt=5e-6, nsteps=500
 def drum(n=4_000_000, radius=3e-3, R=0.5, L=1.6, rpm=30.0, ntri=20000, seed=0xD3E2, dt=5e-6,
          nsteps=2000, name="C2-drum", grid=(256, 256, 256)) -> Scene:
     """C2: n spheres of radius r in a drum (inner shell and end caps, axis z,
-    spinning at rpm) with a seeded baffle of ntri small random triangles that
-    turns with the drum.  L = 1.6 m instead of 0.4 m: 4e6 spheres of r = 3 mm
-    (0.452 m^3 solid) cannot fit a 0.314 m^3 drum.  The fill is the lowest
-    part of a jittered lattice (pitch 2.1 r) in every cross-section, clear of
-    the baffle; gravity -y.  The default grid (256^3 cells of 2 max(r,
-    circumradius)) wraps once along the axis: hashed cells alias, which the
-    distance test keeps correct."""
+    spinning at rpm) with a seeded random triangle-mesh baffle of ntri
+    triangles that turns with the drum: a radial lifter plate in the plane
+    y = 0 from 0.8 R to the shell along the whole length, a grid of square
+    cells of side h (ntri = 2 x cells; h about 4 mm for the defaults) split
+    along a random diagonal each, interior vertices jittered by 0.025 h in the
+    plate (circumradius <= r, which keeps the cell size at 2 r).  L = 1.6 m
+    instead of 0.4 m: 4e6 spheres of r = 3 mm (0.452 m^3 solid) cannot fit a
+    0.314 m^3 drum.  The fill is the lowest part of a jittered lattice (pitch
+    2.1 r) in every cross-section, clear of the plate by 2.5 r; gravity -y.
+    The default grid (256^3 cells of 2 max(r, circumradius)) wraps once along
+    the axis: hashed cells alias, which the distance test keeps correct."""
     r = float(as_f32_exact(radius))
     pitch = 2.1 * r
     omega = rpm * 2.0 * math.pi / 60.0
-    # baffle: equilateral triangles of circumradius 0.9 r, randomly oriented,
-    # centred on the line x = 0.8 R, y = 0 (circumradius <= r keeps the cell
-    # size at 2 r and the triangle search complete)
-    u = uniform(seed, ntri * 3, 3).reshape(ntri, 3)
-    base = np.stack([np.full(ntri, R * 0.8), np.zeros(ntri),
-                     -L / 2 + 2 * r + (L - 4 * r) * uniform(seed, ntri, 4)], 1)
-    nrm = np.stack([np.sqrt(1 - (2 * u[:, 0] - 1) ** 2) * np.cos(2 * np.pi * u[:, 1]),
-                    np.sqrt(1 - (2 * u[:, 0] - 1) ** 2) * np.sin(2 * np.pi * u[:, 1]),
-                    2 * u[:, 0] - 1], 1)
-    e1 = np.cross(nrm, np.where(np.abs(nrm[:, :1]) < 0.9, [[1.0, 0, 0]], [[0, 1.0, 0]]))
-    e1 /= np.linalg.norm(e1, axis=1, keepdims=True)
-    e2 = np.cross(nrm, e1)
-    ang = 2 * np.pi * u[:, 2:3] + np.array([[0.0, 2 * np.pi / 3, 4 * np.pi / 3]])
-    tri = base[:, None, :] + 0.9 * r * (np.cos(ang)[..., None] * e1[:, None, :] +
-                                        np.sin(ang)[..., None] * e2[:, None, :])
-    # lattice over the cross-section, inside the shell, clear of the baffle strip
+    # baffle: nr x nz square cells over [0.8 R, R] x [-L/2, L/2] in the plane y = 0
+    w = 0.2 * R
+    nr = max(1, int(round(math.sqrt(ntri / 2 * w / L))))
+    nzc = max(1, ntri // (2 * nr))
+    hr, hz = w / nr, L / nzc
+    vr = 0.8 * R + hr * np.arange(nr + 1)
+    vz = -L / 2 + hz * np.arange(nzc + 1)
+    VR, VZ = np.meshgrid(vr, vz, indexing="ij")
+    jit = (uniform(seed, 2 * VR.size, 3).reshape(VR.shape + (2,)) * 2 - 1) * 0.025
+    VR[1:-1, :] += jit[1:-1, :, 0] * hr
+    VZ[:, 1:-1] += jit[:, 1:-1, 1] * hz
+    V = np.stack([VR, np.zeros_like(VR), VZ], -1)
+    a, b_, c_, d_ = V[:-1, :-1], V[1:, :-1], V[1:, 1:], V[:-1, 1:]
+    flip = (uniform(seed, nr * nzc, 4).reshape(nr, nzc) < 0.5)[..., None]
+    t1 = np.stack([a, b_, np.where(flip, d_, c_)], -2)
+    t2 = np.stack([np.where(flip, b_, a), c_, d_], -2)
+    tri = np.concatenate([t1.reshape(-1, 3, 3), t2.reshape(-1, 3, 3)], 0)
+    # lattice over the cross-section, inside the shell, clear of the plate
     per = int((2 * R - 4 * r) / pitch)
     xs = (np.arange(per) - (per - 1) / 2) * pitch
     yy, xx = np.meshgrid(xs, xs, indexing="ij")
     inside = ((xx ** 2 + yy ** 2) < (R - 2 * r) ** 2) & ~(
-        (np.abs(xx - 0.8 * R) < 2.5 * r) & (np.abs(yy) < 2.5 * r))
+        (xx > 0.8 * R - 2.5 * r) & (np.abs(yy) < 2.5 * r))
     yx = np.stack([xx[inside], yy[inside]], 1)
     yx = yx[np.lexsort((yx[:, 0], yx[:, 1]))]          # lowest first
     nz = int((L - 4 * r) / pitch)

@@ -248,7 +248,8 @@ def test_refusals(lib):
 
 
 def test_c2_full_size_properties(lib):
-    """configs[2] at full size (4e6 spheres, 20,000-triangle baffle): two runs
+    """The drum workload at full size (4e6 spheres, a 20,000-triangle lifter
+    plate turning with the drum): two runs
     are bit-identical, stay finite and keep every particle inside the drum."""
     sc = scenes.drum()
     outs = []
