@@ -184,6 +184,63 @@ def read_stl_ascii(path_or_text):
     return 0.001 * np.array(vals, float).reshape(-1, 3, 3)
 
 
+def _tri_rad(p, q, r):
+    """Circumradius as the reference measures it: distance from the first
+    vertex to calculate_center_of_triangle (source/vpolygon.cpp:433-434)."""
+    c = scenes.circumcenter(p, q, r)
+    return float(np.linalg.norm(c - np.asarray(p)))
+
+
+def _split_one(t, to):
+    """vpolygon::_splitTriangle (source/vpolygon.cpp:312-424): rounds over the
+    pieces still too large; a piece whose circumradius exceeds `to` is cut at
+    the midpoint of its longest edge (strict comparisons as the reference) into
+    two pieces with the reference's vertex orders, the others are final, in the
+    order they are reached (level by level)."""
+    done, pending = [], [t]
+    while pending:
+        nxt = []
+        for p, q, r in pending:
+            if _tri_rad(p, q, r) > to:
+                s_pq, s_qr, s_pr = (np.linalg.norm(q - p), np.linalg.norm(r - q),
+                                    np.linalg.norm(r - p))
+                if s_pq > s_qr:
+                    tid = 3 if s_pq > s_pr else 2
+                else:
+                    tid = 1 if s_qr > s_pr else 2
+                if tid == 1:
+                    m = 0.5 * (r + q)
+                    nxt += [(p, q, m), (p, m, r)]
+                elif tid == 2:
+                    m = 0.5 * (r + p)
+                    nxt += [(q, r, m), (q, m, p)]
+                else:
+                    m = 0.5 * (q + p)
+                    nxt += [(r, p, m), (r, m, q)]
+            else:
+                done.append((p, q, r))
+        pending = nxt
+    return done
+
+
+def split_triangles(tris, to):
+    """vpolygon::splitTriangle (source/vpolygon.cpp:426-470), the .xdm tool
+    `poly_refinement`: every triangle whose circumradius exceeds `to` is
+    replaced by its refinement (longest-edge bisection until each piece's
+    circumradius is at most `to`), in place in the triangle order.  Orientation
+    and area are kept.  tris: (T, 3, 3); returns (T', 3, 3)."""
+    to = float(to)
+    if not to > 0:
+        raise ValueError("refinement size must be > 0")
+    out = []
+    for P, Q, R in np.asarray(tris, dtype=np.float64):
+        if _tri_rad(P, Q, R) > to:
+            out += _split_one((P, Q, R), to)
+        else:
+            out.append((P, Q, R))
+    return np.array(out, dtype=np.float64).reshape(-1, 3, 3)
+
+
 def _text(path_or_text):
     if isinstance(path_or_text, (str, os.PathLike)) and os.path.exists(path_or_text):
         with open(path_or_text, "r", encoding="utf-8", errors="replace") as f:
@@ -288,8 +345,6 @@ class Model:
             elif o.shape == "cube":
                 planes += scenes.cube_planes(p["START_POINT"], p["END_POINT"])
             elif o.shape == "polygon":
-                if p["TOOL"][0] == "poly_refinement" and p["TOOL"][2]:
-                    raise ValueError("polygon refinement (vpolygon::splitTriangle) is not supported")
                 f = os.path.join(self.base_dir, p["FILE"])
                 kind = int(p["IMPORT_SHAPE_TYPE"])   # import_shape_type: 1 MilkShape, 2 STL
                 if kind == 1:
@@ -298,6 +353,9 @@ class Model:
                     P, origin = read_stl_ascii(f), tuple(p["CM"])
                 else:
                     raise ValueError(f"unsupported mesh format {kind}")
+                if p["TOOL"][0] == "poly_refinement" and p["TOOL"][2]:
+                    # geometryObjects.cpp:88-89: refine before the object is made
+                    P = split_triangles(P, float(p["TOOL"][2]))
                 tri.append(P)
                 tri_obj.append(np.full(len(P), len(objs), np.int32))
                 objs.append(scenes.PolygonObject(origin=tuple(origin)))

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from metakonga_amd import model_io as mio
+from metakonga_amd import scenes
 
 REF_BOX = "/root/reference/resource/box.txt"
 
@@ -214,3 +215,69 @@ def test_xdm_rejects_wrong_tags():
     bad = XDM.replace("START_POINT", "BEGIN_POINT")
     with pytest.raises(ValueError):
         mio.parse_xdm(bad)
+
+
+def _area(t):
+    return 0.5 * np.linalg.norm(np.cross(t[..., 1, :] - t[..., 0, :], t[..., 2, :] - t[..., 0, :]), axis=-1)
+
+
+def test_split_right_triangle_exact():
+    """vpolygon::splitTriangle on a right triangle (legs 2, circumradius sqrt 2)
+    at size 1: the hypotenuse QR is the longest edge, cut at its midpoint into
+    (P, Q, M) and (P, M, R), each of circumradius exactly 1 (final)."""
+    P, Q, R = np.array([0.0, 0, 0]), np.array([2.0, 0, 0]), np.array([0.0, 2, 0])
+    out = mio.split_triangles(np.array([[P, Q, R]]), 1.0)
+    M = np.array([1.0, 1.0, 0.0])
+    assert np.array_equal(out, np.array([[P, Q, M], [P, M, R]]))
+    # already small enough: unchanged
+    assert np.array_equal(mio.split_triangles(np.array([[P, Q, R]]), 1.5), np.array([[P, Q, R]]))
+
+
+def test_split_level_order():
+    """Pieces are final in the order the reference's rounds reach them: the
+    first cut of this obtuse triangle leaves (P, M, R) small enough and (P, Q,
+    M) not, so (P, M, R) comes before the pieces of (P, Q, M)."""
+    P, Q, R = np.array([0.0, 0, 0]), np.array([4.0, 0, 0]), np.array([0.0, 1, 0])
+    M = 0.5 * (R + Q)
+    out = mio.split_triangles(np.array([[P, Q, R]]), 1.2)
+    assert np.array_equal(out[0], np.array([P, M, R])) and len(out) == 3
+    assert np.isclose(_area(out[1:]).sum(), _area(np.array([P, Q, M])), rtol=1e-14)
+
+
+def test_split_properties():
+    """Random triangles: every piece within the size (the reference's measure),
+    pieces in the parent's plane with its orientation, areas adding up."""
+    u = scenes.uniform(31, 60 * 9, 5).reshape(60, 3, 3) * 0.02
+    rad = np.array([mio._tri_rad(*t) for t in u])
+    to = 0.35 * rad.max()
+    out = mio.split_triangles(u, to)
+    assert len(out) > len(u)
+    assert max(mio._tri_rad(*t) for t in out) <= to
+    assert np.isclose(_area(out).sum(), _area(u).sum(), rtol=1e-12)
+    # in place: the pieces of each parent follow each other, in its plane, with
+    # its orientation and its area
+    k = 0
+    for t in u:
+        pieces = mio.split_triangles(t[None], to)
+        assert np.array_equal(out[k:k + len(pieces)], pieces)
+        k += len(pieces)
+        n = np.cross(t[1] - t[0], t[2] - t[0])
+        n /= np.linalg.norm(n)
+        pn = np.cross(pieces[:, 1] - pieces[:, 0], pieces[:, 2] - pieces[:, 0])
+        assert (pn @ n > 0).all()
+        assert np.abs((pieces - t[0]) @ n).max() < 1e-15
+        assert np.isclose(_area(pieces).sum(), _area(t[None])[0], rtol=1e-12)
+    assert k == len(out)
+
+
+def test_xdm_poly_refinement(tmp_path):
+    """The .xdm tool `poly_refinement name size` refines the mesh before the
+    object is made (geometryObjects.cpp:88-89)."""
+    (tmp_path / "fin mesh.stl").write_text(STL)
+    (tmp_path / "m.xdm").write_text(XDM.replace("TOOL none none 0", "TOOL poly_refinement fin 0.004"))
+    m = mio.read_xdm(tmp_path / "m.xdm")
+    sc = m.to_scene(dt=1e-5, nsteps=10, grid_size=(32, 32, 32), world_origin=(-0.01, -0.01, -0.01))
+    base = mio.read_stl_ascii(STL)
+    assert sc.tri.shape[0] > len(base) and (sc.tri_object == 0).all()
+    assert max(mio._tri_rad(*t) for t in sc.tri) <= 0.004
+    assert np.isclose(_area(sc.tri).sum(), _area(base).sum(), rtol=1e-12)
