@@ -373,6 +373,22 @@ int32_t xdem_sim_device_arrays(xdem_sim* s, double** pos, double** vel, double**
                                double** alpha, double** force, double** moment, double** mass,
                                double** iner);
 uint32_t xdem_sim_np(const xdem_sim* s);
+/* Real-time particle release (particleManager's real-time creation, run by
+ * dem_simulation::initialize/oneStepAnalysis, dem_simulation.cpp:210-222,321-322,
+ * 449-450): with every_steps > 0 the simulation starts with no particle active, and
+ * at each step k (from 1) with (k - 1) % every_steps == 0 the next `group`
+ * particles (in index order) join the simulated prefix; the others keep their
+ * state untouched until then.  every_steps = 0 turns it off.  Before the first
+ * step only.  xdem_sim_active gives the simulated count. */
+int32_t xdem_sim_set_release(xdem_sim* s, uint32_t every_steps, uint32_t group);
+/* The same release with a list of group sizes (particleManager::np_group, one
+ * entry per generated layer or ring, particleManager.cpp:342-362,477; walked by
+ * NextCreatingPerGroup, :132-139): the g-th release adds groups[g] particles,
+ * and none once the list is exhausted.  every_steps > 0; the sizes sum to at
+ * most xdem_sim_np.  Before the first step only. */
+int32_t xdem_sim_set_release_groups(xdem_sim* s, uint32_t every_steps, const uint32_t* groups,
+                                    uint32_t ngroups);
+uint32_t xdem_sim_active(const xdem_sim* s);
 /* Per-phase device time of xdem_sim_step, from HIP events recorded on the
  * library stream around each phase: 0 Verlet position, 1 hash, 2 sort,
  * 3 cell ranges, 4 particle-particle, 5 walls, 6 triangles, 7 Verlet velocity.

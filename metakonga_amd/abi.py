@@ -283,6 +283,9 @@ def _declare(lib):
         "xdem_sim_set_state": ([vp] * 8, i32),
         "xdem_sim_device_arrays": ([vp] + [C.POINTER(vp)] * 9, i32),
         "xdem_sim_np": ([vp], u32),
+        "xdem_sim_set_release": ([vp, u32, u32], i32),
+        "xdem_sim_set_release_groups": ([vp, u32, vp, u32], i32),
+        "xdem_sim_active": ([vp], u32),
         "xdem_sim_enable_timing": ([vp, i32], i32),
         "xdem_sim_phase_times": ([vp, vp, vp], i32),
         "xdem_contact_pairs_f64": ([vp] * 4 + [u32, u32, vp, vp], i32),

@@ -269,6 +269,17 @@ struct xdem_sim {
     double* d_vlist = nullptr;
     std::vector<Motion> motions;
     uint64_t step = 0;
+    // real-time release: the first np_act particles are simulated; every
+    // release_every steps (from step 1) the next group joins: release_group
+    // particles, or the next entry of release_groups when that list is set
+    // (0: all, always)
+    uint32_t release_every = 0, release_group = 0, np_act = 0, release_next = 0;
+    std::vector<uint32_t> release_groups;
+    uint32_t active() const { return release_every ? np_act : np; }
+    uint32_t next_group() {
+        if (release_groups.empty()) return release_group;
+        return release_next < release_groups.size() ? release_groups[release_next++] : 0;
+    }
     bool dirty = true;
     PhaseTimer timer;
 
@@ -354,9 +365,15 @@ static int32_t sim_apply_motions(xdem_sim* s, hipStream_t st) {
 static int32_t sim_step_once(xdem_sim* s, hipStream_t st) {
     const KParams k = s->kparams();
     const int tcm = s->desc.tcm;
-    uint32_t np = s->np;
     PhaseTimer& T = s->timer;
     s->step++;
+    // dem_simulation::oneStepAnalysis (dem_simulation.cpp:449-450): at step k with
+    // (k - 1) % per_np == 0 the next group joins the simulated prefix (the group
+    // list is walked as particleManager::NextCreatingPerGroup does,
+    // particleManager.cpp:132-139: 0 once it is exhausted)
+    if (s->release_every && (s->step - 1) % s->release_every == 0 && s->np_act < s->np)
+        s->np_act = std::min(s->np, s->np_act + s->next_group());
+    const uint32_t np = s->active();
     if (!s->motions.empty()) XDEM_TRY(sim_apply_motions(s, st));
     XDEM_TRY(T.begin(0, st));
     XDEM_TRY(launch_vv_position(k, s->pos, s->vel, s->acc, np, st));
@@ -714,6 +731,38 @@ int32_t xdem_sim_phase_times(xdem_sim* s, double* ms8, uint32_t* count8) {
 }
 
 uint32_t xdem_sim_np(const xdem_sim* s) { return s ? s->np : 0; }
+
+uint32_t xdem_sim_active(const xdem_sim* s) { return s ? s->active() : 0; }
+
+int32_t xdem_sim_set_release(xdem_sim* s, uint32_t every_steps, uint32_t group) {
+    XDEM_REQUIRE(s, "null sim");
+    XDEM_REQUIRE(s->step == 0, "the release is set before the first step");
+    XDEM_REQUIRE(every_steps == 0 || group > 0, "a release needs a group size > 0");
+    s->release_every = every_steps;
+    s->release_group = every_steps ? group : 0;
+    s->release_groups.clear();
+    s->release_next = 0;
+    s->np_act = 0;
+    return XDEM_OK;
+}
+
+int32_t xdem_sim_set_release_groups(xdem_sim* s, uint32_t every_steps, const uint32_t* groups,
+                                    uint32_t ngroups) {
+    XDEM_REQUIRE(s, "null sim");
+    XDEM_REQUIRE(s->step == 0, "the release is set before the first step");
+    XDEM_REQUIRE(every_steps > 0 && groups && ngroups > 0,
+                 "a group release needs every_steps > 0 and a non-empty group list");
+    uint64_t total = 0;
+    for (uint32_t g = 0; g < ngroups; ++g) total += groups[g];
+    XDEM_REQUIRE(total <= s->np, "the groups hold %llu particles, the simulation %u",
+                 (unsigned long long)total, s->np);
+    s->release_every = every_steps;
+    s->release_group = 0;
+    s->release_groups.assign(groups, groups + ngroups);
+    s->release_next = 0;
+    s->np_act = 0;
+    return XDEM_OK;
+}
 xdem_grid* xdem_sim_grid(xdem_sim* s) { return s ? &s->grid : nullptr; }
 
 // ------------------------------------------------------------ state files
@@ -787,6 +836,12 @@ int32_t xdem_sim_save_checkpoint(const xdem_sim* s, const char* path) {
     w.put((uint32_t)s->motions.size());
     w.put(s->step);
     w.put(s->prm.dt);
+    w.put(s->release_every);
+    w.put(s->release_group);
+    w.put(s->np_act);
+    w.put(s->release_next);
+    w.put((uint32_t)s->release_groups.size());
+    w.arr(s->release_groups.data(), s->release_groups.size());
     for (int k = 0; k < 7; ++k) w.arr(a[k].data(), a[k].size());
     w.arr(planes.data(), planes.size());
     w.arr(tris.data(), tris.size());
@@ -807,12 +862,18 @@ int32_t xdem_sim_load_checkpoint(xdem_sim* s, const char* path) {
     std::vector<uint8_t> file;
     io::Reader r;
     XDEM_TRY(io::read_checkpoint(path, io::CKPT_SIM, file, r, nullptr));
-    uint32_t np, ntri, nobj, nplane, nmot;
+    uint32_t np, ntri, nobj, nplane, nmot, rel_every, rel_group, np_act, rel_next, ngroups;
     uint64_t step;
     double dt;
     XDEM_REQUIRE(r.get(np) && r.get(ntri) && r.get(nobj) && r.get(nplane) && r.get(nmot) &&
-                     r.get(step) && r.get(dt),
+                     r.get(step) && r.get(dt) && r.get(rel_every) && r.get(rel_group) &&
+                     r.get(np_act) && r.get(rel_next) && r.get(ngroups),
                  "%s: short header", path);
+    XDEM_REQUIRE(np_act <= np, "%s: %u active of %u particles", path, np_act, np);
+    XDEM_REQUIRE(ngroups <= np && rel_next <= ngroups, "%s: release group %u of %u", path,
+                 rel_next, ngroups);
+    std::vector<uint32_t> groups(ngroups);
+    XDEM_REQUIRE(r.arr(groups.data(), ngroups), "%s: truncated release groups", path);
     XDEM_REQUIRE(np == s->np && ntri == s->ntri && nobj == s->nobj &&
                      nmot == (uint32_t)s->motions.size(),
                  "%s: built for %u particles, %u triangles, %u objects, %u motions "
@@ -862,6 +923,11 @@ int32_t xdem_sim_load_checkpoint(xdem_sim* s, const char* path) {
                            hipMemcpyHostToDevice));
     s->motions = mot;
     s->step = step;
+    s->release_every = rel_every;
+    s->release_group = rel_group;
+    s->np_act = np_act;
+    s->release_next = rel_next;
+    s->release_groups = std::move(groups);
     return XDEM_OK;
 }
 

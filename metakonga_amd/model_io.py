@@ -302,8 +302,10 @@ class Model:
     has_mbd: bool = False
     base_dir: str = "."
 
-    def particle_positions(self, seed=0):
-        out = []
+    def _generate(self, seed):
+        """Positions of every generator, and the layer sizes that the plane and
+        circle generators append to particleManager::np_group (:362, :477)."""
+        out, groups = [], []
         for k, g in enumerate(self.particles):
             p = g.params
             if g.shape == "cube":
@@ -312,12 +314,40 @@ class Model:
                     raise ValueError("the cube generator is monodisperse")
             elif g.shape == "plane":
                 dx, ny, dz = p["DIMENSION"]
-                pos, _ = plane_particles(dx, int(ny), dz, p["STARTPOINT"], *p["RADIUS"], seed + k)
+                pos, gs = plane_particles(dx, int(ny), dz, p["STARTPOINT"], *p["RADIUS"], seed + k)
+                groups += gs
             else:
-                pos, _ = circle_particles(p["DIAMETER"], int(p["NUM_HEIGHT"]), p["STARTPOINT"],
-                                          *p["RADIUS"], seed + k)
+                pos, gs = circle_particles(p["DIAMETER"], int(p["NUM_HEIGHT"]), p["STARTPOINT"],
+                                           *p["RADIUS"], seed + k)
+                groups += gs
             out.append(pos)
-        return np.concatenate(out) if out else np.zeros((0, 4))
+        return (np.concatenate(out) if out else np.zeros((0, 4))), groups
+
+    def particle_positions(self, seed=0):
+        return self._generate(seed)[0]
+
+    def release(self, dt, seed=0):
+        """Real-time creation as the generators set it up (particleManager.cpp:392-397
+        and :588-594; `MAKING realtime np per_np time_per_group one_by_one file`) and
+        dem_simulation::initialize starts it (dem_simulation.cpp:210-222): the flags
+        and rates of the last plane or circle generator apply, and groups release
+        the layer sizes of every plane and circle generator in creation order.
+        Returns the Scene.release tuple, or None."""
+        last = [g for g in self.particles if g.shape in ("plane", "circle")]
+        if not last or not int(last[-1].params["MAKING"][0]):
+            return None
+        g = last[-1]
+        mk = g.params["MAKING"]
+        if int(mk[4]):
+            if g.shape == "plane":
+                # CreatePlaneParticle sets per_np = np / ndim.y with ndim.y = 0
+                # (particleManager.cpp:332, :395): the reference divides by zero
+                raise ValueError("one-by-one release of the plane generator is undefined "
+                                 "in the reference (its rate divides by zero)")
+            every = scenes.release_period(dt, per_second=int(mk[2]))
+            return (every, 1) if every else None
+        every = scenes.release_period(dt, time_per_group=float(mk[3]))
+        return (every, self._generate(seed)[1]) if every else None
 
     def to_scene(self, dt, nsteps, cell_size=0.0, grid_size=(128, 128, 128),
                  world_origin=(-1.0, -1.0, -1.0), seed=0, rolling=True):
@@ -327,7 +357,7 @@ class Model:
         particle-particle pair of CONTACT_ELEMENTS_DATA (ratio = DHS ks/kn);
         walls use the first particle-wall pair (the simulation takes one wall
         pair per scene)."""
-        pos = self.particle_positions(seed)
+        pos, _ = self._generate(seed)
         if not self.particles:
             raise ValueError("the model creates no particles")
         y, rho, nu, g = self.particles[0].material
@@ -362,7 +392,7 @@ class Model:
         sc = scenes.Scene(name=self.name or "xdm", pos=np.ascontiguousarray(pos), dt=dt,
                           nsteps=nsteps, gravity=tuple(self.gravity), world_origin=world_origin,
                           cell_size=cell_size, grid_size=grid_size, material=mat, contact=contact,
-                          planes=planes, rolling=rolling)
+                          planes=planes, rolling=rolling, release=self.release(dt, seed))
         if tri:
             sc.tri = np.concatenate(tri)
             sc.tri_object = np.concatenate(tri_obj)

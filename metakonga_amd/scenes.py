@@ -224,6 +224,10 @@ class Scene:
     tcm: int = 1
     seed: int = 0
     tangential_history: bool = False   # fused path: DHS with a tangential spring per contact
+    # real-time release (FP64 simulation): (steps between groups, particles per group or
+    # a list of group sizes); particles join in index order, the first group at step 1
+    # (dem_simulation.cpp:449-450); release_period gives the steps between groups
+    release: tuple | None = None
 
     @property
     def np(self) -> int:
@@ -288,6 +292,22 @@ def _pow2_at_least(n: int) -> int:
     while p < n:
         p <<= 1
     return p
+
+
+def release_period(dt: float, time_per_group: float | None = None,
+                   per_second: int | None = None) -> int:
+    """Steps between real-time releases, dem_simulation::initialize
+    (dem_simulation.cpp:210-219): one group every time_per_group seconds gives
+    (unsigned)((time_per_group + 1e-9) / dt); one particle at a time at
+    per_second particles a second gives (unsigned)((1.0 / per_second) / dt).
+    0 means no release (every particle is simulated from the start)."""
+    if (time_per_group is None) == (per_second is None):
+        raise ValueError("give time_per_group or per_second")
+    if per_second is not None:
+        if int(per_second) <= 0:
+            raise ValueError("per_second must be a positive count")
+        return int((1.0 / int(per_second)) / float(dt))
+    return int((float(time_per_group) + 1e-9) / float(dt))
 
 
 def as_f32_exact(x: np.ndarray) -> np.ndarray:

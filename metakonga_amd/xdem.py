@@ -69,10 +69,23 @@ class Simulation:
             if ob.motion is not None:
                 abi.check(self.lib, self.lib.xdem_sim_set_motion(h, abi.OBJECT_POLYGON, o,
                                                                   C.byref(ob.motion)))
+        if scene.release is not None and int(scene.release[0]) > 0:
+            every, group = scene.release
+            if np.ndim(group) == 0:
+                abi.check(self.lib, self.lib.xdem_sim_set_release(h, int(every), int(group)))
+            else:
+                g = np.ascontiguousarray(group, dtype=np.uint32)
+                abi.check(self.lib, self.lib.xdem_sim_set_release_groups(
+                    h, int(every), abi.ptr(g), len(g)))
 
     @property
     def np(self):
         return int(self.lib.xdem_sim_np(self.h))
+
+    @property
+    def active(self):
+        """Particles simulated so far (all of them unless a release is set)."""
+        return int(self.lib.xdem_sim_active(self.h))
 
     def step(self, n=1):
         abi.check(self.lib, self.lib.xdem_sim_step(self.h, n))

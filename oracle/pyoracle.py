@@ -34,7 +34,9 @@ class OrcWorld(C.Structure):
                                                               "cend")] + [
         ("drums", C.c_void_p), ("nmotion", C.c_uint32), ("motions", C.c_void_p),
         ("planes0", C.c_void_p), ("obj0", C.c_void_p), ("vlist", C.c_void_p),
-        ("step", C.c_uint64)]
+        ("step", C.c_uint64), ("np_total", C.c_uint32), ("release_every", C.c_uint32),
+        ("release_group", C.c_uint32), ("nrelease_groups", C.c_uint32),
+        ("release_next", C.c_uint32), ("release_groups", C.c_void_p)]
 
 
 class OrcMotion(C.Structure):
@@ -331,6 +333,18 @@ class World:
         w.tcm = scene.tcm
         w.semantics = semantics
         w.np = n
+        w.np_total = n
+        if scene.release is not None and int(scene.release[0]) > 0:
+            # (steps between groups, group size or sizes)
+            w.np = 0
+            every, group = scene.release
+            w.release_every = int(every)
+            if np.ndim(group) == 0:
+                w.release_group = int(group)
+            else:
+                self.release_groups = np.ascontiguousarray(group, dtype=np.uint32)
+                w.nrelease_groups = len(self.release_groups)
+                w.release_groups = self.release_groups.ctypes.data
         for k in ("pos", "vel", "omega", "acc", "alpha", "force", "moment", "mass", "iner"):
             setattr(w, k, getattr(self, k).ctypes.data)
         w.cp_pp = scene.contact_pp()

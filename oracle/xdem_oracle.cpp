@@ -1181,12 +1181,24 @@ void step_host(orc_world* w, int nthreads) {
                     w->iner, np);
 }
 
+// The release of the coming step (w->step = k - 1 before step k): the next
+// group joins the simulated prefix when (k - 1) % release_every == 0.
+void release_step(orc_world* w) {
+    if (w->release_every && w->step % w->release_every == 0 && w->np < w->np_total) {
+        uint32_t add = w->release_group;
+        if (w->nrelease_groups)
+            add = w->release_next < w->nrelease_groups ? w->release_groups[w->release_next++] : 0;
+        w->np = std::min(w->np_total, w->np + add);
+    }
+}
+
 }  // namespace
 
 extern "C" {
 
 void orc_step(orc_world* w, uint32_t nsteps) {
     for (uint32_t s = 0; s < nsteps; ++s) {
+        release_step(w);
         if (w->semantics == ORC_HOST)
             step_host(w, 1);
         else
@@ -1196,6 +1208,7 @@ void orc_step(orc_world* w, uint32_t nsteps) {
 
 void orc_step_omp(orc_world* w, uint32_t nsteps, int32_t nthreads) {
     for (uint32_t s = 0; s < nsteps; ++s) {
+        release_step(w);
         if (w->semantics == ORC_HOST)
             step_host(w, nthreads);
         else

@@ -89,6 +89,18 @@ typedef struct orc_world {
     const xdem_device_polygon_mass_info* obj0;   /* object poses as set */
     const double* vlist;                         /* 9*ntri local vertex coordinates */
     uint64_t step;                               /* steps taken */
+    /* real-time release (dem_simulation::oneStepAnalysis, dem_simulation.cpp:449-450):
+     * with release_every != 0 only the first np of np_total particles are simulated, and
+     * at step k (from 1) with (k - 1) % release_every == 0 the next group joins:
+     * release_group particles, or with nrelease_groups != 0 the entry release_next
+     * of release_groups (none once the list is exhausted,
+     * particleManager::NextCreatingPerGroup, particleManager.cpp:132-139) */
+    uint32_t np_total;
+    uint32_t release_every;
+    uint32_t release_group;
+    uint32_t nrelease_groups;
+    uint32_t release_next;
+    const uint32_t* release_groups;
 } orc_world;
 
 /* grid_base::getCellNumber / calcGridPos + calcGridHash */
