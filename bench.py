@@ -327,8 +327,9 @@ def main():
         k_ms = fused["phase_ms"]["contact_verlet"]
         achieved = KSTEP_BYTES * scene.np / (k_ms * 1e-3)
         ch = (fused.get("autotune") or {}).get("chosen", 2256)
-        sig = (f"k_step<true, false, {(ch // 1000) % 10}, {'true' if ch >= 10000 else 'false'}, "
-               "false, false>")
+        sig = (f"k_step<true, false, {(ch // 1000) % 10}, "
+               f"{'true' if (ch // 10000) % 10 else 'false'}, false, false, "
+               f"{'true' if ch >= 100000 else 'false'}>")
         traffic = profiled_traffic(sig) if scene.np == 1_000_000 else None
         roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic,

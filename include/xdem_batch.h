@@ -130,16 +130,22 @@ int32_t xdem_batch_set_block(xdem_batch* b, uint32_t threads);
  * contact records loaded one contact ahead (0/1; ignored while a scene has
  * tangential history).  Results do not depend on it. */
 int32_t xdem_batch_set_launch(xdem_batch* b, uint32_t threads, uint32_t unroll, int32_t load_ahead);
-/* Autotuner: times the contact kernel for block sizes 64, 128 and 256, each
- * with 1, 2 or 4 neighbour candidates loaded together and with the contact
- * records loaded one contact ahead or not, over steps_per_trial real steps
- * (the scenes advance 18 * (steps_per_trial + 1) steps), and keeps the
- * fastest.  *chosen = block size + 1000 * candidates-per-load-batch + 10000 *
- * (1 if contact records are loaded ahead); ms_per_candidate (18 entries:
- * block-major, then candidates-per-batch, then load-ahead off/on; may be
- * NULL) receives the measured kernel time per launch (-1 for the load-ahead
- * shapes while a scene has history on: they are not used then).  Results do
- * not depend on the choice (bit-identical). */
+/* How a thread walks its neighbour rows (the cell loop of cu_calculate_p2p,
+ * mphysics_cuda_impl.cuh:403-467, in the same order either way): per_lane = 1
+ * (default) walks its own non-empty rows as one stream, 0 walks row by row in
+ * step with the other lanes of its wave.  Results do not depend on it. */
+int32_t xdem_batch_set_walk(xdem_batch* b, int32_t per_lane);
+/* Autotuner: times the contact kernel for both walks (xdem_batch_set_walk), block
+ * sizes 64, 128 and 256, each with 1, 2 or 4 neighbour candidates loaded
+ * together and with the contact records loaded one contact ahead or not, over
+ * steps_per_trial real steps (the scenes advance 36 * (steps_per_trial + 1)
+ * steps), and keeps the fastest.  *chosen = block size + 1000 *
+ * candidates-per-load-batch + 10000 * (1 if contact records are loaded ahead)
+ * + 100000 * (1 for the per-lane walk); ms_per_candidate (36 entries: walk-major
+ * (row by row, then per lane), then block size, then candidates-per-batch, then
+ * load-ahead off/on; may be NULL) receives the measured kernel time per launch
+ * (-1 for the load-ahead shapes while a scene has history on or drums/meshes:
+ * they are not used then).  Results do not depend on the choice (bit-identical). */
 int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* chosen,
                             double* ms_per_candidate);
 int32_t xdem_batch_synchronize(xdem_batch* b);

@@ -50,6 +50,7 @@ def declare(lib):
         "xdem_batch_set_graphs": ([vp, i32], i32),
         "xdem_batch_set_block": ([vp, u32], i32),
         "xdem_batch_set_launch": ([vp, u32, u32, i32], i32),
+        "xdem_batch_set_walk": ([vp, i32], i32),
         "xdem_batch_synchronize": ([vp], i32),
         "xdem_batch_autotune": ([vp, u32, C.POINTER(u32), vp], i32),
         "xdem_batch_get_state": ([vp, u32, vp, vp, vp, vp, vp], i32),
@@ -185,14 +186,17 @@ class Batch:
         abi.check(self.lib, self.lib.xdem_batch_set_launch(self.h, int(threads), int(unroll),
                                                            int(bool(load_ahead))))
 
+    def walk(self, per_lane=True):
+        abi.check(self.lib, self.lib.xdem_batch_set_walk(self.h, int(bool(per_lane))))
+
     def autotune(self, steps_per_trial=10):
         """Pick the contact-kernel launch shape by timing (advances the scenes)."""
         chosen = C.c_uint32()
-        ms = np.zeros(18)
+        ms = np.zeros(36)
         abi.check(self.lib, self.lib.xdem_batch_autotune(self.h, steps_per_trial, C.byref(chosen),
                                                          abi.ptr(ms)))
-        keys = [f"block{b}-unroll{u}{'-ahead' if p else ''}" for b in (64, 128, 256)
-                for u in (1, 2, 4) for p in (0, 1)]
+        keys = [f"{w}-block{b}-unroll{u}{'-ahead' if p else ''}" for w in ("rows", "lane")
+                for b in (64, 128, 256) for u in (1, 2, 4) for p in (0, 1)]
         return chosen.value, dict(zip(keys, ms.tolist()))
 
     def state(self, sid):

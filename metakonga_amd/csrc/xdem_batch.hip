@@ -159,6 +159,7 @@ struct xdem_batch {
     uint32_t block_threads = 256;
     int unroll = 2;                   // k_step variant: candidates loaded together (1, 2 or 4)
     bool pipe = false;                // k_step variant: contact records loaded one ahead
+    bool flat = true;                 // k_step variant: per-lane row stream (else row by row)
     bool use_graphs = true;
     std::map<std::pair<uint32_t, int>, hipGraphExec_t> graphs;
     PhaseTimer timer;
@@ -742,12 +743,16 @@ int32_t launch_step(xdem_batch* b, bool adv) {
     }
     XDEM_TRY(b->timer.begin(1, b->st));
     dim3 g(b->nblk_sorted), t(b->block_threads);
-#define XDEM_STEP_LAUNCH(U, P, H, G)                                                   \
-    do {                                                                               \
-        if (adv)                                                                       \
-            hipLaunchKernelGGL((k_step<true, false, U, P, H, G>), g, t, 0, b->st, a);  \
-        else                                                                           \
-            hipLaunchKernelGGL((k_step<false, false, U, P, H, G>), g, t, 0, b->st, a); \
+#define XDEM_STEP_LAUNCH(U, P, H, G)                                                          \
+    do {                                                                                      \
+        if (adv && b->flat)                                                                   \
+            hipLaunchKernelGGL((k_step<true, false, U, P, H, G, true>), g, t, 0, b->st, a);   \
+        else if (adv)                                                                         \
+            hipLaunchKernelGGL((k_step<true, false, U, P, H, G, false>), g, t, 0, b->st, a);  \
+        else if (b->flat)                                                                     \
+            hipLaunchKernelGGL((k_step<false, false, U, P, H, G, true>), g, t, 0, b->st, a);  \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_step<false, false, U, P, H, G, false>), g, t, 0, b->st, a); \
     } while (0)
     if (b->geom_active()) {
         // drums or meshes: the GEOM variants (contact records not loaded ahead)
@@ -972,6 +977,15 @@ int32_t xdem_batch_set_launch(xdem_batch* b, uint32_t threads, uint32_t unroll,
     return XDEM_OK;
 }
 
+int32_t xdem_batch_set_walk(xdem_batch* b, int32_t per_lane) {
+    XDEM_REQUIRE(b, "null batch");
+    if ((per_lane != 0) != b->flat) {
+        b->flat = per_lane != 0;
+        b->drop_graphs();
+    }
+    return XDEM_OK;
+}
+
 int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps) {
     XDEM_REQUIRE(b, "null batch");
     if (nsteps == 0) return XDEM_OK;
@@ -1043,16 +1057,19 @@ int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* c
     double best = 1e300;
     uint32_t best_c = b->block_threads;
     int best_u = b->unroll;
-    bool best_p = b->pipe;
+    bool best_p = b->pipe, best_f = b->flat;
+    for (int w = 0; w < 2; ++w)
     for (int i = 0; i < 3; ++i)
         for (int u = 0; u < 3; ++u)
             for (int pp = 0; pp < 2; ++pp) {
+                const int slot = 18 * w + 6 * i + 2 * u + pp;
                 if (pp && (b->hist_active || b->geom_active())) {  // not a launch shape here
-                    if (ms_per_candidate) ms_per_candidate[6 * i + 2 * u + pp] = -1.0;
+                    if (ms_per_candidate) ms_per_candidate[slot] = -1.0;
                     continue;
                 }
                 b->unroll = unr[u];
                 b->pipe = pp != 0;
+                b->flat = w == 1;
                 b->drop_graphs();
                 XDEM_TRY(xdem_batch_set_block(b, cand[i]));
                 XDEM_TRY(xdem_batch_step(b, 1));  // applies the new tables
@@ -1062,21 +1079,25 @@ int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* c
                 uint32_t cnt[6];
                 XDEM_TRY(xdem_batch_phase_times(b, ms, cnt));
                 double per = cnt[1] ? ms[1] / cnt[1] : 1e300;
-                if (ms_per_candidate) ms_per_candidate[6 * i + 2 * u + pp] = per;
+                if (ms_per_candidate) ms_per_candidate[slot] = per;
                 if (per < best) {
                     best = per;
                     best_c = cand[i];
                     best_u = unr[u];
                     best_p = pp != 0;
+                    best_f = w == 1;
                 }
                 XDEM_TRY(xdem_batch_enable_timing(b, 0));
             }
     b->unroll = best_u;
     b->pipe = best_p;
+    b->flat = best_f;
     b->drop_graphs();
     XDEM_TRY(xdem_batch_set_block(b, best_c));
     if (was_on) XDEM_TRY(xdem_batch_enable_timing(b, 1));
-    if (chosen) *chosen = best_c + 1000u * (uint32_t)best_u + (best_p ? 10000u : 0u);
+    if (chosen)
+        *chosen = best_c + 1000u * (uint32_t)best_u + (best_p ? 10000u : 0u) +
+                  (best_f ? 100000u : 0u);
     return XDEM_OK;
 }
 

@@ -23,6 +23,7 @@ namespace fused {
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;      // empty cell / sentinel key
 constexpr int CMAX = 16;                     // contact-list slots per particle (LDS)
 constexpr int HSLOTS = 16;                   // tangential-history slots per particle
+constexpr int RTAB = 7;                      // neighbour rows kept in LDS during the walk
 constexpr uint32_t WALL_KEY = 0x80000000u;   // history key of plane p: WALL_KEY | p
 constexpr int LDS_T = 256;                   // lane stride of the per-thread LDS lists
 constexpr int SORT_LDS_MAX = 4096;           // movers sorted in LDS up to this count
@@ -663,11 +664,16 @@ constexpr int step_waves(int unr, bool pipe, bool hist, bool geom) {
 
 // GEOM: the variant that also evaluates drum walls and triangle meshes (used
 // when some scene of the batch has them; the others carry no registers for it).
-template <bool ADV, bool DBG, int UNR, bool PIPE, bool HIST, bool GEOM = false>
+template <bool ADV, bool DBG, int UNR, bool PIPE, bool HIST, bool GEOM = false, bool FLAT = true>
 __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step(StepArgs A) {
     // per-thread LDS lists, lane-contiguous (entry c of thread t at c * LDS_T + t)
     __shared__ uint32_t clist[LDS_T * CMAX];
     __shared__ uint32_t hkeys[HIST ? LDS_T * HSLOTS : 1];
+    // the non-empty neighbour rows after the first two (per thread, lane stride
+    // LDS_T); the history variants borrow the key lists, filled after the walk
+    static_assert(2 * RTAB <= HSLOTS, "row table must fit in the history keys");
+    __shared__ uint2 rtab_own[HIST || !FLAT ? 1 : LDS_T * RTAB];
+    uint2* const rtab = (HIST ? reinterpret_cast<uint2*>(hkeys) : rtab_own) + threadIdx.x;
     const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
     const BlockEntry be = A.blocks[L];
     const BScene S = A.scenes[be.scene];
@@ -697,7 +703,6 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
         hc.out = A.hout + (size_t)j * HSLOTS;
         hc.overflow = A.hoverflow;
         hc.nh = hc.on ? min(A.hcin[s], (uint32_t)HSLOTS) : 0u;
-        for (uint32_t k = 0; k < hc.nh; ++k) hc.keys[k * LDS_T] = __float_as_uint(hc.in[k].w);
     }
     Own o;
     o.x = ip.x;
@@ -781,13 +786,20 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     };
     if (cx >= 1 && cx + 1 < (int)S.gx) {
         // Away from the x wrap the three cells of an x-row have consecutive keys,
-        // so their particles are one contiguous run of sorted positions.  The
-        // range of row r+1 is loaded before row r is walked, and a row is
-        // walked UNR candidates at a time: UNR index loads, then UNR FP32
-        // screening records in flight; the FP64 records of the candidates that
-        // pass the screen are loaded for the exact test.
-        // A row is skipped when even its middle cell is out of reach; its
-        // end cells are dropped when out of reach.
+        // so their particles are one contiguous run of sorted positions.  A row
+        // is skipped when even its middle cell is out of reach; its end cells
+        // are dropped when out of reach.  Both walks visit the rows in order and
+        // each row in ascending sorted position (the order of the cell loop) and
+        // load UNR index words, then UNR FP32 screening records, per chunk; the
+        // FP64 records of the candidates that pass the screen are loaded for the
+        // exact test.
+        //   FLAT: the non-empty rows (at most 9) are listed first, the first two
+        // in registers and the rest in LDS, and each lane walks its rows as one
+        // stream: a slot past the end of the current row takes the next row's
+        // candidates, so a chunk may cross one row boundary and lanes with short
+        // rows do not wait for the longest row of the wave.
+        //   Row walk: the wave walks row r (the range of row r+1 loaded ahead)
+        // until its longest lane is done, then row r+1.
         auto row_range = [&](int r) {
             const int dy = (r % 3) - 1, dz = (r / 3) - 1;
             const float b2 = face2(dy, loy) + face2(dz, loz);
@@ -802,28 +814,94 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
             rg.y = c2.x != EMPTY ? c2.y : (c1.x != EMPTY ? c1.y : c0.y);
             return rg;
         };
-        uint2 next = row_range(0);
+        if constexpr (FLAT) {
+            // jj..e1: the current row's remaining run; nb..ne: the next row's run
+            uint32_t jj = 0, e1 = 0, nb = 0, ne = 0, nt = 0;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                const uint2 rg = row_range(r);
+                const uint32_t rb = rg.x, re = min(rg.y, A.n);
+                if (rb == EMPTY || rb >= re) continue;
+                jj = nt == 0 ? rb : jj;
+                e1 = nt == 0 ? re : e1;
+                nb = nt == 1 ? rb : nb;
+                ne = nt == 1 ? re : ne;
+                if (nt >= 2) rtab[(nt - 2) * LDS_T] = make_uint2(rb, re);
+                ++nt;
+            }
+            const uint32_t ntab = nt > 2 ? nt - 2 : 0;
+            uint32_t ti = 0;  // next row table entry
 #pragma unroll 1
-        for (int r = 0; r < 9; ++r) {
-            const uint2 row = next;
-            if (r < 8) next = row_range(r + 1);
-            if (row.x == EMPTY) continue;
-            const uint32_t e1 = min(row.y, A.n);
-            for (uint32_t jj = row.x; jj < e1; jj += UNR) {
+            while (jj < e1) {
                 uint32_t t[UNR];
                 float4 f[UNR];
+                const uint32_t nbase = nb - e1;  // slot k >= e1 maps to nb + (k - e1)
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) {
                     const uint32_t k = jj + u;
-                    const bool ok = k < e1 && k != j;  // e1 <= n
-                    t[u] = ok ? at32(A.perm, k) : A.n;
+                    const uint32_t q = k < e1 ? k : nbase + k;
+                    const bool ok = (k < e1 || q < ne) && q != j;  // ne <= n
+                    t[u] = ok ? at32(A.perm, q) : A.n;
                 }
 #pragma unroll
                 for (int u = 0; u < UNR; ++u)
                     if (t[u] < A.n) f[u] = at32(A.in.pf, t[u]);
+                // advance: stay in the current row, continue inside the next one,
+                // or (that one used up too) start the next chunk at the row after it
+                jj += UNR;
+                if (jj >= e1) {
+                    const uint32_t c2 = jj - e1;
+                    if (c2 < ne - nb) {
+                        jj = nb + c2;
+                        e1 = ne;
+                        nb = ne = 0;
+                        if (ti < ntab) {
+                            const uint2 rg = rtab[(ti++) * LDS_T];
+                            nb = rg.x;
+                            ne = rg.y;
+                        }
+                    } else {
+                        jj = e1 = nb = ne = 0;
+                        if (ti < ntab) {
+                            const uint2 rg = rtab[(ti++) * LDS_T];
+                            jj = rg.x;
+                            e1 = rg.y;
+                        }
+                        if (ti < ntab) {
+                            const uint2 rg = rtab[(ti++) * LDS_T];
+                            nb = rg.x;
+                            ne = rg.y;
+                        }
+                    }
+                }
 #pragma unroll
                 for (int u = 0; u < UNR; ++u)
                     if (t[u] < A.n && may_touch(f[u])) consider(t[u], at32(A.in.pr, t[u]));
+            }
+        } else {
+            uint2 next = row_range(0);
+#pragma unroll 1
+            for (int r = 0; r < 9; ++r) {
+                const uint2 row = next;
+                if (r < 8) next = row_range(r + 1);
+                if (row.x == EMPTY) continue;
+                const uint32_t e1 = min(row.y, A.n);
+                for (uint32_t jj = row.x; jj < e1; jj += UNR) {
+                    uint32_t t[UNR];
+                    float4 f[UNR];
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+                        const uint32_t k = jj + u;
+                        const bool ok = k < e1 && k != j;  // e1 <= n
+                        t[u] = ok ? at32(A.perm, k) : A.n;
+                    }
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u)
+                        if (t[u] < A.n) f[u] = at32(A.in.pf, t[u]);
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u)
+                        if (t[u] < A.n && may_touch(f[u])) consider(t[u], at32(A.in.pr, t[u]));
+                }
             }
         }
     } else {
@@ -836,6 +914,8 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     o.v = F3(vm.x, vm.y, vm.z);
     o.w = F3(wa.x, wa.y, wa.z);
     o.m = vm.w;
+    if (HIST)
+        for (uint32_t k = 0; k < hc.nh; ++k) hc.keys[k * LDS_T] = __float_as_uint(hc.in[k].w);
     // phase 2: the listed contacts, in order; then the further chunks
     const uint32_t found = ntotal;
     pp_contacts<PIPE, HIST>(A, S, o, mylist, min(found, (uint32_t)CMAX), hc, Ft, Fn, M, sumF, sumM);
