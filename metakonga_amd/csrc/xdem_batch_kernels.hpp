@@ -817,10 +817,22 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
         if constexpr (FLAT) {
             // jj..e1: the current row's remaining run; nb..ne: the next row's run
             uint32_t jj = 0, e1 = 0, nb = 0, ne = 0, nt = 0;
+            // the 27 range loads are issued unconditionally (key_of wraps into
+            // the scene's grid and 1 <= cx <= gx - 2, so every key is in range)
+            // and the culled cells are masked afterwards
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
-                const uint2 rg = row_range(r);
-                const uint32_t rb = rg.x, re = min(rg.y, A.n);
+                const int dy = (r % 3) - 1, dz = (r / 3) - 1;
+                const uint32_t k1 = key_of(S, cx, cy + dy, cz + dz);
+                const uint2 c0 = at32(A.crange, k1 - 1), c1 = at32(A.crange, k1),
+                            c2 = at32(A.crange, k1 + 1);
+                const float b2 = face2(dy, loy) + face2(dz, loz);
+                const bool alive = b2 < reach2;
+                const bool h0 = alive && face2(-1, lox) + b2 < reach2 && c0.x != EMPTY;
+                const bool h1 = alive && c1.x != EMPTY;
+                const bool h2 = alive && face2(1, lox) + b2 < reach2 && c2.x != EMPTY;
+                const uint32_t rb = h0 ? c0.x : (h1 ? c1.x : (h2 ? c2.x : EMPTY));
+                const uint32_t re = min(h2 ? c2.y : (h1 ? c1.y : c0.y), A.n);
                 if (rb == EMPTY || rb >= re) continue;
                 jj = nt == 0 ? rb : jj;
                 e1 = nt == 0 ? re : e1;
@@ -836,16 +848,22 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
                 uint32_t t[UNR];
                 float4 f[UNR];
                 const uint32_t nbase = nb - e1;  // slot k >= e1 maps to nb + (k - e1)
+                // idle slots (past both rows, or the thread itself) load the
+                // thread's own entries, so the loads need no branch; ok[]
+                // keeps them out of the screen
+                bool ok[UNR];
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) {
                     const uint32_t k = jj + u;
                     const uint32_t q = k < e1 ? k : nbase + k;
-                    const bool ok = (k < e1 || q < ne) && q != j;  // ne <= n
-                    t[u] = ok ? at32(A.perm, q) : A.n;
+                    ok[u] = (k < e1 || q < ne) && q != j;  // ne <= n
+                    t[u] = at32(A.perm, ok[u] ? q : j);
                 }
 #pragma unroll
-                for (int u = 0; u < UNR; ++u)
-                    if (t[u] < A.n) f[u] = at32(A.in.pf, t[u]);
+                for (int u = 0; u < UNR; ++u) {
+                    ok[u] = ok[u] && t[u] < A.n;
+                    f[u] = at32(A.in.pf, min(t[u], A.n - 1));
+                }
                 // advance: stay in the current row, continue inside the next one,
                 // or (that one used up too) start the next chunk at the row after it
                 jj += UNR;
@@ -876,7 +894,7 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
                 }
 #pragma unroll
                 for (int u = 0; u < UNR; ++u)
-                    if (t[u] < A.n && may_touch(f[u])) consider(t[u], at32(A.in.pr, t[u]));
+                    if (ok[u] && may_touch(f[u])) consider(t[u], at32(A.in.pr, t[u]));
             }
         } else {
             uint2 next = row_range(0);
