@@ -130,8 +130,8 @@ int32_t xdem_batch_set_block(xdem_batch* b, uint32_t threads);
  * contact records loaded one contact ahead (0/1; ignored while a scene has
  * tangential history).  Results do not depend on it. */
 int32_t xdem_batch_set_launch(xdem_batch* b, uint32_t threads, uint32_t unroll, int32_t load_ahead);
-/* How a thread walks its neighbour rows (the cell loop of cu_calculate_p2p,
- * mphysics_cuda_impl.cuh:403-467, in the same order either way): per_lane = 1
+/* How a thread walks its neighbour rows (the 27-cell loop of calculate_p2p_kernel,
+ * mphysics_cuda_impl.cuh:435-482, in the same order either way): per_lane = 1
  * (default) walks its own non-empty rows as one stream, 0 walks row by row in
  * step with the other lanes of its wave.  Results do not depend on it. */
 int32_t xdem_batch_set_walk(xdem_batch* b, int32_t per_lane);
