@@ -166,14 +166,24 @@ __global__ __launch_bounds__(512) void k_hash(KParams k, uint32_t* hash, uint32_
     index[sid + id] = sid + id;
 }
 
-// K8 calculate_p2p_kernel<TCM> — mphysics_cuda_impl.cuh:403-485
-template <int TCM>
+// K8 calculate_p2p_kernel<TCM> — mphysics_cuda_impl.cuh:403-485.  SORTED: thread t
+// evaluates the particle at sorted position t (of nsorted entries, triangle
+// spheres included and skipped) instead of particle t, so a wave's particles
+// are neighbours in space and share their cells; each particle's arithmetic
+// and neighbour order are the same either way.
+template <int TCM, bool SORTED>
 __global__ __launch_bounds__(256) void k_p2p(KParams k, const double* pos, const double* vel,
                                              const double* omega, double* force, double* moment,
                                              const double* mass, const uint32_t* sorted_index,
                                              const uint32_t* cstart, const uint32_t* cend,
-                                             const xdem_device_contact_property* cpp, uint32_t np) {
-    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+                                             const xdem_device_contact_property* cpp, uint32_t np,
+                                             uint32_t nsorted) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t id = t;
+    if (SORTED) {
+        if (t >= nsorted) return;
+        id = sorted_index[t];
+    }
     if (id >= np) return;
     const xdem_device_contact_property cp = *cpp;
     d4 ip4 = ld4(pos, id);
@@ -743,14 +753,24 @@ int32_t launch_hash(const KParams& k, uint32_t* hash, uint32_t* index, const dou
 int32_t launch_p2p(int tcm, const KParams& k, const double* pos, const double* vel,
                    const double* omega, double* force, double* moment, const double* mass,
                    const uint32_t* sorted, const uint32_t* cstart, const uint32_t* cend,
-                   const xdem_device_contact_property* cp, uint32_t np, hipStream_t s) {
+                   const xdem_device_contact_property* cp, uint32_t np, uint32_t nsorted,
+                   hipStream_t s) {
     if (!np) return XDEM_OK;
-    if (tcm == 0)
-        hipLaunchKernelGGL(k_p2p<0>, dim3(div_up(np, 256)), dim3(256), 0, s, k, pos, vel, omega,
-                           force, moment, mass, sorted, cstart, cend, cp, np);
-    else
-        hipLaunchKernelGGL(k_p2p<1>, dim3(div_up(np, 256)), dim3(256), 0, s, k, pos, vel, omega,
-                           force, moment, mass, sorted, cstart, cend, cp, np);
+    const dim3 g(div_up(nsorted ? nsorted : np, 256));
+    if (nsorted) {
+        if (tcm == 0)
+            hipLaunchKernelGGL((k_p2p<0, true>), g, dim3(256), 0, s, k, pos, vel, omega, force,
+                               moment, mass, sorted, cstart, cend, cp, np, nsorted);
+        else
+            hipLaunchKernelGGL((k_p2p<1, true>), g, dim3(256), 0, s, k, pos, vel, omega, force,
+                               moment, mass, sorted, cstart, cend, cp, np, nsorted);
+    } else if (tcm == 0) {
+        hipLaunchKernelGGL((k_p2p<0, false>), g, dim3(256), 0, s, k, pos, vel, omega, force,
+                           moment, mass, sorted, cstart, cend, cp, np, 0u);
+    } else {
+        hipLaunchKernelGGL((k_p2p<1, false>), g, dim3(256), 0, s, k, pos, vel, omega, force,
+                           moment, mass, sorted, cstart, cend, cp, np, 0u);
+    }
     XDEM_LAUNCHED();
     return XDEM_OK;
 }
@@ -946,7 +966,7 @@ int32_t xdem_calculate_p2p_f64(int32_t tcm, const double* pos, const double* vel
                              cstart && cend && cp),
                  "null array");
     return launch_p2p(tcm, params_from(host_params()), pos, vel, omega, force, moment, mass,
-                      sorted_index, cstart, cend, cp, np, stream());
+                      sorted_index, cstart, cend, cp, np, 0u, stream());
 }
 
 int32_t xdem_plane_contact_force_f64(int32_t tcm, const xdem_device_plane_info* plane,

@@ -15,7 +15,8 @@ int32_t launch_hash(const KParams& k, uint32_t* hash, uint32_t* index, const dou
 int32_t launch_p2p(int tcm, const KParams& k, const double* pos, const double* vel,
                    const double* omega, double* force, double* moment, const double* mass,
                    const uint32_t* sorted, const uint32_t* cstart, const uint32_t* cend,
-                   const xdem_device_contact_property* cp, uint32_t np, hipStream_t s);
+                   const xdem_device_contact_property* cp, uint32_t np, uint32_t nsorted,
+                   hipStream_t s);
 int32_t launch_planes(int tcm, const KParams& k, const xdem_device_plane_info* planes,
                       const xdem_device_contact_property* cps, uint32_t nplane, const double* pos,
                       const double* vel, const double* omega, double* force, double* moment,

@@ -384,7 +384,7 @@ static int32_t sim_step_once(xdem_sim* s, hipStream_t st) {
     const uint32_t* ce0 = s->grid.d_cell_end;
     XDEM_TRY(T.begin(4, st));
     XDEM_TRY(launch_p2p(tcm, k, s->pos, s->vel, s->omega, s->force, s->moment, s->mass, sid, cs0,
-                        ce0, s->d_cp_pp, np, st));
+                        ce0, s->d_cp_pp, np, np + s->ntri, st));
     XDEM_TRY(T.end(st));
     XDEM_TRY(T.begin(5, st));
     for (const WallOp& op : s->ops) {
