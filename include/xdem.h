@@ -186,7 +186,15 @@ int32_t xdem_cube_contact_force_f64(int32_t tcm, const xdem_device_plane_info* p
 /* cu_particle_polygonObject_collision — mphysics_cuda_dec.cuh:142-147, def :203-221.
  * cp and dpmi are indexed by polygon-object id (dpi[t].id). The per-object
  * reaction sums dpmi[].force/moment are written by an ordered reduction
- * (no racing +=), so they are deterministic. */
+ * (no racing +=), so they are deterministic: dpmi[o].force = -sum (Fn + Ft)
+ * and dpmi[o].moment = -sum (cp - origin) x (Fn + Ft) over the contacts of
+ * this call, as the reference caller reads them back
+ * (source/contact_particles_polygonObjects.cpp:469-477).  nobject = 0 leaves
+ * dpmi untouched; nobject = XDEM_NOBJECT_AUTO takes the number of objects as
+ * 1 + the largest dpi[t].id over the ntriangle triangles (ntriangle = 0: the
+ * parameter block's nsphere, the triangle count the reference's caller
+ * sets), which costs one device-to-host read. */
+#define XDEM_NOBJECT_AUTO 0xFFFFFFFFu
 int32_t xdem_particle_polygon_object_collision_f64(
     int32_t tcm, const xdem_device_polygon_info* dpi, const double* dsph,
     xdem_device_polygon_mass_info* dpmi, uint32_t nobject, uint32_t ntriangle,

@@ -41,8 +41,13 @@ typedef struct xdem_batch_scene_desc {
     xdem_uint3 grid_size;          /* powers of two, product <= 2^24 */
     bool rolling;                  /* device_parameters.rollingCondition */
     xdem_material particle;
-    xdem_contact_pair_desc pp;     /* particle-particle pair (pp.wall = particle material) */
-    xdem_contact_pair_desc wall;   /* particle-plane pair (wall.wall = wall material) */
+    xdem_contact_pair_desc pp;     /* particle-particle pair (pp.wall = particle material);
+                                      pp.cohesion >= 0 adds cohesionForce
+                                      (mphysics_cuda_impl.cuh:259-279) to the normal
+                                      force; refused with tangential_history */
+    xdem_contact_pair_desc wall;   /* particle-plane pair (wall.wall = wall material);
+                                      wall contacts carry no cohesion term, as the
+                                      FP64 entry points (xdem.h) */
     uint32_t nplane;
     const xdem_device_plane_info* planes;  /* host array of nplane planes, applied in order */
     int32_t tangential_history;    /* 0: the reference DHS law (memoryless tangential force);

@@ -94,9 +94,11 @@ inline void cu_cube_contact_force(const int tcm, device_plane_info* plan, double
         xdem_cube_contact_force_f64(tcm, plan, pos, vel, omega, force, moment, mass, np, cp),
         "cu_cube_contact_force");
 }
-// The reference accumulates per-object reactions into dpmi with a data race;
-// this form leaves dpmi untouched (use xdem_particle_polygon_object_collision_f64
-// with nobject > 0 for deterministic reaction sums).
+// The reference accumulates per-object reactions into dpmi with a data race
+// and never zeroes them; here dpmi[].force/moment receive this call's ordered
+// reaction sums (the objects are those the triangles name; the triangle count
+// is the parameter block's nsphere), which is what the reference's caller
+// copies back and hands to its bodies.
 inline void cu_particle_polygonObject_collision(const int tcm, device_polygon_info* dpi,
                                                 double* dsph, device_polygon_mass_info* dpmi,
                                                 double* pos, double* vel, double* omega,
@@ -105,7 +107,8 @@ inline void cu_particle_polygonObject_collision(const int tcm, device_polygon_in
                                                 unsigned int* cend, device_contact_property* cp,
                                                 unsigned int np) {
     xdem_compat::check(xdem_particle_polygon_object_collision_f64(
-                           tcm, dpi, dsph, dpmi, 0, 0, pos, vel, omega, force, moment, mass, sidx,
+                           tcm, dpi, dsph, dpmi, XDEM_NOBJECT_AUTO, 0, pos, vel, omega, force,
+                           moment, mass, sidx,
                            cstart, cend, cp, np),
                        "cu_particle_polygonObject_collision");
 }

@@ -308,6 +308,8 @@ BScene make_bscene(const HostScene& h, uint32_t plane0) {
     s.mass_c = (float)(d.particle.density * (4.0 / 3.0) * M_PI);
     s.rolling = d.rolling ? 1 : 0;
     s.history = d.tangential_history ? 1 : 0;
+    // particle-particle cohesion (walls carry none, as on the FP64 path)
+    s.cohpi = (float)(d.pp.cohesion * M_PI);
     s.rmax = h.rmax;
     s.pf_range = 2.0 * d.cell_size * std::max(d.grid_size.x, std::max(d.grid_size.y, d.grid_size.z));
     s.pf_eps = (float)std::ldexp(s.pf_range, -20);
@@ -881,6 +883,10 @@ int32_t xdem_batch_add_scene_history(xdem_batch* b, const xdem_batch_scene_desc*
     XDEM_REQUIRE(d->nplane == 0 || d->planes, "null planes");
     XDEM_REQUIRE(!hist_count || (d->tangential_history && hist_key && hist_xi),
                  "history arrays need tangential_history on and all three arrays");
+    XDEM_REQUIRE(d->pp.cohesion >= 0 && std::isfinite(d->pp.cohesion),
+                 "cohesion must be finite and >= 0");
+    XDEM_REQUIRE(!(d->tangential_history && d->pp.cohesion != 0),
+                 "cohesion is not available together with tangential_history");
     if (hist_count)
         for (uint32_t i = 0; i < d->np; ++i) {
             XDEM_REQUIRE(hist_count[i] <= XDEM_BATCH_HISTORY_SLOTS, "history count %u > %d",

@@ -43,7 +43,7 @@ struct BScene {
     int32_t rolling;
     uint32_t ncell;
     int32_t history;          // tangential-spring history on (DHS with a spring per contact)
-    uint32_t pad;
+    float cohpi;              // particle-particle cohesion x pi (0: none)
     double rmax;              // largest radius of the scene (cell culling)
     double pf_range;          // |coordinate - world origin| bound of the FP32 copy
     float pf_eps;             // distance error bound of the FP32 copy (2^-20 pf_range)
@@ -296,10 +296,11 @@ __device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 // along u, multiplied by |dv|/|e| in the dot product, would make the friction
 // of a near head-on contact (|e| << |dv|) lose digits the reference keeps.
 __device__ inline void dhs_f32(float kn, float vn, float ks, float vs, float mu, float dt,
-                               float ir, float delta, f3 dv, f3 u, f3& Ft, f3& Fn, f3& M) {
+                               float ir, float delta, f3 dv, f3 u, f3& Ft, f3& Fn, f3& M,
+                               float fca = 0.0f) {
     float fsn = -kn * (delta * fsqrt(delta));
     float fdn = vn * dotf(dv, u);
-    Fn = (fsn + fdn) * u;
+    Fn = ((fsn + fca) + fdn) * u;
     f3 e = dv - dotf(dv, u) * u;
     float mag_e = fsqrt(dotf(e, e));
     if (mag_e != 0.0f) {
@@ -396,6 +397,11 @@ __device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own&
     float Ms = (o.m * jvm.w) * frcp(o.m + jvm.w);
     float kn = S.ckn * fsqrt(R);
     float vn = fsqrt(S.cvn * Ms * kn);
+    // cohesionForce (mphysics_cuda_impl.cuh:259-279) is coh pi rc^2 with
+    // rc = (3 R* |fsn| / (4 E*))^(1/3), the Hertz contact radius; with the DHS
+    // stiffness kn = 4/3 E* sqrt(R*) and |fsn| = kn delta^1.5 that radius is
+    // exactly sqrt(R* delta), so the term is coh pi R* delta (no cube root)
+    const float fca = S.cohpi * R * cdist;
     if (HIST && hc.on) {
         f3 xi = hc.lookup(jid), cFt, cFn, cM;
         hist_f32(kn, vn, kn * S.sr, vn * S.sr, S.mu, S.dtf, o.r, cdist, dv, u, xi, cFt, cFn,
@@ -405,7 +411,7 @@ __device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own&
         sumM += cM;
         return;
     }
-    dhs_f32(kn, vn, kn * S.sr, vn * S.sr, S.mu, S.dtf, o.r, cdist, dv, u, Ft, Fn, M);
+    dhs_f32(kn, vn, kn * S.sr, vn * S.sr, S.mu, S.dtf, o.r, cdist, dv, u, Ft, Fn, M, fca);
     sumF += Fn + Ft;
     sumM += M;
 }
@@ -670,10 +676,26 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     __shared__ uint32_t clist[LDS_T * CMAX];
     __shared__ uint32_t hkeys[HIST ? LDS_T * HSLOTS : 1];
     // the non-empty neighbour rows after the first two (per thread, lane stride
-    // LDS_T); the history variants borrow the key lists, filled after the walk
+    // LDS_T); the history variants borrow the thread's own key slots (row i in
+    // slots 2i and 2i+1), which the thread fills only after its walk, so no
+    // thread ever touches another thread's slots
     static_assert(2 * RTAB <= HSLOTS, "row table must fit in the history keys");
     __shared__ uint2 rtab_own[HIST || !FLAT ? 1 : LDS_T * RTAB];
-    uint2* const rtab = (HIST ? reinterpret_cast<uint2*>(hkeys) : rtab_own) + threadIdx.x;
+    auto rtab_put = [&](uint32_t i, uint32_t rb, uint32_t re) {
+        if constexpr (HIST) {
+            hkeys[(2 * i) * LDS_T + threadIdx.x] = rb;
+            hkeys[(2 * i + 1) * LDS_T + threadIdx.x] = re;
+        } else {
+            rtab_own[i * LDS_T + threadIdx.x] = make_uint2(rb, re);
+        }
+    };
+    auto rtab_get = [&](uint32_t i) -> uint2 {
+        if constexpr (HIST)
+            return make_uint2(hkeys[(2 * i) * LDS_T + threadIdx.x],
+                              hkeys[(2 * i + 1) * LDS_T + threadIdx.x]);
+        else
+            return rtab_own[i * LDS_T + threadIdx.x];
+    };
     const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
     const BlockEntry be = A.blocks[L];
     const BScene S = A.scenes[be.scene];
@@ -838,7 +860,7 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
                 e1 = nt == 0 ? re : e1;
                 nb = nt == 1 ? rb : nb;
                 ne = nt == 1 ? re : ne;
-                if (nt >= 2) rtab[(nt - 2) * LDS_T] = make_uint2(rb, re);
+                if (nt >= 2) rtab_put(nt - 2, rb, re);
                 ++nt;
             }
             const uint32_t ntab = nt > 2 ? nt - 2 : 0;
@@ -874,19 +896,19 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
                         e1 = ne;
                         nb = ne = 0;
                         if (ti < ntab) {
-                            const uint2 rg = rtab[(ti++) * LDS_T];
+                            const uint2 rg = rtab_get(ti++);
                             nb = rg.x;
                             ne = rg.y;
                         }
                     } else {
                         jj = e1 = nb = ne = 0;
                         if (ti < ntab) {
-                            const uint2 rg = rtab[(ti++) * LDS_T];
+                            const uint2 rg = rtab_get(ti++);
                             jj = rg.x;
                             e1 = rg.y;
                         }
                         if (ti < ntab) {
-                            const uint2 rg = rtab[(ti++) * LDS_T];
+                            const uint2 rg = rtab_get(ti++);
                             nb = rg.x;
                             ne = rg.y;
                         }

@@ -9,6 +9,7 @@
 // (the oracle is compiled the same way).  Transcendentals (pow, log, sqrt of
 // the force constants, atan/sin/cos of the cylinder rim) come from the ROCm
 // device library and may differ from glibc by an ulp.
+#include <algorithm>
 #include <vector>
 
 #include "xdem_common.hpp"
@@ -635,6 +636,17 @@ __global__ __launch_bounds__(512) void k_rows_partial(const d3* in, uint32_t n, 
     if (threadIdx.x == 0) partial[(size_t)blockIdx.y * ROW_PARTS + blockIdx.x] = s[0];
 }
 
+// Largest polygon-object id over ntri triangles (objects = that + 1).
+__global__ void k_max_object_id(const xdem_device_polygon_info* dpi, uint32_t ntri,
+                                uint32_t* out) {
+    uint32_t m = 0;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntri;
+         t += gridDim.x * blockDim.x)
+        m = max(m, (uint32_t)dpi[t].id);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
 __global__ void k_store_reactions(const d3* sums, xdem_device_polygon_mass_info* dpmi,
                                   uint32_t nobj) {
     uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1005,7 +1017,6 @@ int32_t xdem_particle_polygon_object_collision_f64(
     const uint32_t* sorted_index, const uint32_t* cstart, const uint32_t* cend,
     const xdem_device_contact_property* cp, uint32_t np) {
     (void)dsph;
-    (void)ntriangle;
     XDEM_TRY(need_params());
     XDEM_REQUIRE(tcm == 0 || tcm == 1, "tcm must be 0 (HMCM) or 1 (DHS)");
     XDEM_REQUIRE(np == 0 || (dpi && dpmi && pos && vel && omega && force && moment && mass &&
@@ -1013,6 +1024,27 @@ int32_t xdem_particle_polygon_object_collision_f64(
                  "null array");
     XDEM_REQUIRE(host_params().np == np, "np (%u) must equal the parameter block np (%u): "
                  "triangle entries are the sorted indices >= np", np, host_params().np);
+    if (nobject == XDEM_NOBJECT_AUTO) {
+        // the objects the triangles name: 1 + the largest dpi[t].id
+        const uint32_t ntri = ntriangle ? ntriangle : host_params().nsphere;
+        nobject = 0;
+        if (ntri && np) {
+            XDEM_REQUIRE(dpi, "null dpi");
+            hipStream_t s = stream();
+            XDEM_TRY(workspace(3).reserve(sizeof(uint32_t)));
+            uint32_t* d = static_cast<uint32_t*>(workspace(3).ptr);
+            XDEM_TRY(fill_u32(d, 0u, 1, s));
+            hipLaunchKernelGGL(k_max_object_id, dim3(std::min<uint32_t>(div_up(ntri, 256), 1024)),
+                               dim3(256), 0, s, dpi, ntri, d);
+            XDEM_LAUNCHED();
+            uint32_t mx = 0;
+            XDEM_HIP(hipMemcpyAsync(&mx, d, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            XDEM_HIP(hipStreamSynchronize(s));
+            XDEM_REQUIRE(mx < (1u << 16), "polygon-object id %u out of range (ids must be < 65536)",
+                         mx);
+            nobject = mx + 1;
+        }
+    }
     return launch_polygon(tcm, params_from(host_params()), dpi, dpmi, nobject, pos, vel, omega,
                           force, moment, mass, sorted_index, cstart, cend, cp, np, stream());
 }

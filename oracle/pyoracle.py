@@ -98,6 +98,9 @@ def _declare(L):
         "orc_hash_and_index": ([vp, C.c_int, vp, u32, u32, vp, vp], None),
         "orc_reorder_and_find_cell_start": ([vp, vp, u32, u32, vp, vp, vp], None),
         "orc_force_constant": ([i32, f64, f64, f64, f64, vp, vp], None),
+        "orc_cohesion_force": ([f64] * 8, f64),
+        "orc_host_cohesion": ([f64] * 4, f64),
+        "orc_set_force_magnitudes": ([vp], None),
         "orc_p2p": ([vp, i32, C.c_int] + [vp] * 10 + [u32, vp, u64], u64),
         "orc_pairs_bruteforce": ([vp, u32, vp, u64], u64),
         "orc_plane_detect": ([vp, C.c_int, vp, vp, vp, f64], f64),
@@ -181,6 +184,30 @@ def force_constant(tcm, ir, jr, im, jm, cp):
     out = abi.ForceConstant()
     load().orc_force_constant(tcm, ir, jr, im, jm, byref(cp), byref(out))
     return out
+
+
+def cohesion_force(ri, rj, Ei, Ej, pri, prj, coh, Fn):
+    return load().orc_cohesion_force(ri, rj, Ei, Ej, pri, prj, coh, Fn)
+
+
+def host_cohesion(coh, coh_r, coh_e, Fn):
+    return load().orc_host_cohesion(coh, coh_r, coh_e, Fn)
+
+
+class force_magnitudes:
+    """Context: every contact force the oracle evaluates inside the block adds
+    its magnitude |Fn + Ft| to .fmag[particle] (float64[n])."""
+
+    def __init__(self, n):
+        self.fmag = np.zeros(n)
+
+    def __enter__(self):
+        load().orc_set_force_magnitudes(P(self.fmag))
+        return self.fmag
+
+    def __exit__(self, *exc):
+        load().orc_set_force_magnitudes(None)
+        return False
 
 
 def p2p(prm, tcm, pos4, vel, omega, force, moment, mass, sorted_, cstart, cend, cp,

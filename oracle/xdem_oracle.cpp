@@ -114,6 +114,15 @@ xdem_device_force_constant force_constant(int tcm, double ir, double jr, double 
     return dfc;
 }
 
+// Optional per-particle sum of contact-force magnitudes |Fn + Ft| (set by
+// orc_set_force_magnitudes): the cancellation-safe scale of a per-particle
+// force comparison.  Each particle's entry is written only by the loop
+// iteration of that particle.
+double* g_fmag = nullptr;
+inline void add_mag(uint32_t id, const V3& f) {
+    if (g_fmag) g_fmag[id] += length(f);
+}
+
 // cohesionForce — mphysics_cuda_impl.cuh:259-279 (particle-particle only; walls
 // and triangles get no cohesion term, see header).
 double cohesion_force(double ri, double rj, double Ei, double Ej, double pri, double prj,
@@ -294,6 +303,17 @@ void orc_reorder_and_find_cell_start(uint32_t* hash, uint32_t* index, uint32_t n
     sort_cells(hash, index, n, ncell, cstart, cend, sorted);
 }
 
+double orc_cohesion_force(double ri, double rj, double Ei, double Ej, double pri, double prj,
+                          double coh, double Fn) {
+    return cohesion_force(ri, rj, Ei, Ej, pri, prj, coh, Fn);
+}
+
+double orc_host_cohesion(double coh, double coh_r, double coh_e, double Fn) {
+    return host_cohesion(coh, coh_r, coh_e, Fn);
+}
+
+void orc_set_force_magnitudes(double* fmag) { g_fmag = fmag; }
+
 void orc_force_constant(int32_t tcm, double ir, double jr, double im, double jm,
                         const xdem_device_contact_property* cp, xdem_device_force_constant* out) {
     *out = force_constant(tcm, ir, jr, im, jm, *cp);
@@ -359,6 +379,7 @@ void p2p_one(const xdem_device_parameters* p, int tcm, int sem, uint32_t id, con
                         double fca = host_cohesion(cp.coh, Req, coh_e, fsn);
                         V3 mf = mk(0, 0, 0), mm = mk(0, 0, 0);
                         host_dhs(c, p->dt, fca, cdist, cpt, rv, unit, mf, mm);
+                        add_mag(id, mf);
                         sumF += mf;
                         sumM += mm;
                     } else {
@@ -368,6 +389,7 @@ void p2p_one(const xdem_device_parameters* p, int tcm, int sem, uint32_t id, con
                         double fsn = -c.kn * std::pow(cdist, 1.5);
                         double fca = cohesion_force(ir, jr, cp.Ei, cp.Ej, cp.pri, cp.prj, cp.coh, fsn);
                         gpu_law(tcm, c, p->dt, ir, fca, rcon, cdist, iomega, rv, unit, Ft, Fn, M);
+                        add_mag(id, Fn + Ft);
                         sumF += Fn + Ft;
                         sumM += M;
                     }
@@ -520,6 +542,7 @@ void orc_plane(const xdem_device_parameters* p, int32_t tcm, const xdem_device_p
             V3 dv = p->rollingCondition ? -(ivel + cross(iomega, r * un)) : -ivel;
             xdem_device_force_constant c = force_constant(tcm, r, 0.0, m, 0.0, *cp);
             gpu_law(tcm, c, p->dt, 0.0, 0.0, rcon, cdist, iomega, dv, un, Ft, Fn, M);
+            add_mag(id, Fn + Ft);
         }
         V3 s = Fn + Ft;
         double* f = force + 3 * id;
@@ -639,6 +662,7 @@ void orc_cylinder(const xdem_device_parameters* p, int32_t tcm, const xdem_devic
             V3 dv = of(cy->vel) + cross(of(cy->omega), cy2cp) - (ivel + cross(iomega, ip[3] * un));
             xdem_device_force_constant c = force_constant(tcm, ip[3], 0, im, 0, *cp);
             gpu_law(tcm, c, p->dt, 0.0, 0.0, rcon, cdist, iomega, dv, un, Ft, Fn, M);
+            add_mag(id, Fn + Ft);
         }
         V3 sum_f = Fn + Ft;
         double* f = force + 3 * id;
@@ -653,8 +677,8 @@ void orc_cylinder(const xdem_device_parameters* p, int32_t tcm, const xdem_devic
 namespace {
 // One drum contact: the cylinder kernel's evaluation (ir = 0, wall point
 // velocity minus the particle's contact-point velocity) on an inner surface.
-void drum_one(const xdem_device_parameters* p, int32_t tcm, const xdem_drum_info* dr, double r,
-              double im, V3 iomega, V3 ivel, V3 unit, V3 cpt, double cdist,
+void drum_one(const xdem_device_parameters* p, int32_t tcm, const xdem_drum_info* dr, uint32_t id,
+              double r, double im, V3 iomega, V3 ivel, V3 unit, V3 cpt, double cdist,
               const xdem_device_contact_property& cp, V3& F, V3& M, V3& R, V3& RM) {
     V3 o = of(dr->origin);
     V3 wv = dr->omega * cross(of(dr->axis), cpt - o);
@@ -664,6 +688,7 @@ void drum_one(const xdem_device_parameters* p, int32_t tcm, const xdem_drum_info
     xdem_device_force_constant c = force_constant(tcm, r, 0, im, 0, cp);
     gpu_law(tcm, c, p->dt, 0.0, 0.0, rcon, cdist, iomega, dv, unit, Ft, Fn, Mc);
     V3 f = Fn + Ft;
+    add_mag(id, f);
     F += f;
     M += Mc;
     R += -f;
@@ -694,14 +719,14 @@ void orc_drum(const xdem_device_parameters* p, int32_t tcm, const xdem_drum_info
             if (cs > 0 && rho > 0) {
                 V3 u = q / rho;
                 V3 cpt = of(dr->origin) + a * ax + dr->radius * u;
-                drum_one(p, tcm, dr, r, im, iomega, ivel, u, cpt, cs, *cp, F, M, R, RM);
+                drum_one(p, tcm, dr, id, r, im, iomega, ivel, u, cpt, cs, *cp, F, M, R, RM);
             }
             double cc = r - (dr->half_length - std::fabs(a));
             if (dr->caps && cc > 0 && rho < dr->radius) {
                 double sg = a >= 0 ? 1.0 : -1.0;
                 V3 u = sg * ax;
                 V3 cpt = of(dr->origin) + (sg * dr->half_length) * ax + q;
-                drum_one(p, tcm, dr, r, im, iomega, ivel, u, cpt, cc, *cp, F, M, R, RM);
+                drum_one(p, tcm, dr, id, r, im, iomega, ivel, u, cpt, cc, *cp, F, M, R, RM);
             }
         }
         double* f = force + 3 * (size_t)id;
@@ -806,6 +831,7 @@ void poly_one(const xdem_device_parameters* p, int tcm, uint32_t id,
                                     : of(pmi.vel) - ivel;
                         xdem_device_force_constant c = force_constant(tcm, ir, 0, im, 0, cmp);
                         gpu_law(tcm, c, p->dt, 0.0, 0.0, rcon, cdist, iomega, dv, unit, Ft, Fn, M);
+                        add_mag(id, Fn + Ft);
                         sum_force += Fn + Ft;
                         sum_moment += M;
                         if (rf) {
@@ -1309,6 +1335,7 @@ void hist_one(const orc_world* w, const orc_hist* in, orc_hist* out, uint32_t id
                     V3 xi = hc.lookup(k), Ft, Fn, M;
                     hist_law(c, p->dt, ir, cdist, rv, unit, xi, Ft, Fn, M);
                     hc.store(k, xi, overflow);
+                    add_mag(id, Fn + Ft);
                     sumF += Fn + Ft;
                     sumM += M;
                 }
@@ -1333,6 +1360,7 @@ void hist_one(const orc_world* w, const orc_hist* in, orc_hist* out, uint32_t id
         V3 xi = hc.lookup(key), Ft, Fn, M;
         hist_law(c, p->dt, 0.0, cdist, dv, un, xi, Ft, Fn, M);
         hc.store(key, xi, overflow);
+        add_mag(id, Fn + Ft);
         F += Fn + Ft;
     }
     out->count[id] = hc.nout;

@@ -113,6 +113,15 @@ void orc_hash_and_index(const xdem_device_parameters* p, int semantics, const do
 /* K5-K7: stable sort by hash, cell ranges over ncell, sorted_index */
 void orc_reorder_and_find_cell_start(uint32_t* hash, uint32_t* index, uint32_t n, uint32_t ncell,
                                      uint32_t* cstart, uint32_t* cend, uint32_t* sorted);
+/* cohesionForce (mphysics_cuda_impl.cuh:259-279) and contact::cohesionForce
+ * (source/collision/contact.cpp:6-18) */
+double orc_cohesion_force(double ri, double rj, double Ei, double Ej, double pri, double prj,
+                          double coh, double Fn);
+double orc_host_cohesion(double coh, double coh_r, double coh_e, double Fn);
+/* From now on every contact force Fn + Ft the oracle evaluates (particle,
+ * plane, cylinder, drum, triangle, history twin) adds |Fn + Ft| to fmag[id]
+ * of its particle; NULL stops. */
+void orc_set_force_magnitudes(double* fmag);
 /* getConstant (mphysics_cuda_impl.cuh:214-256) */
 void orc_force_constant(int32_t tcm, double ir, double jr, double im, double jm,
                         const xdem_device_contact_property* cp, xdem_device_force_constant* out);

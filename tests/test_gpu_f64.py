@@ -178,6 +178,49 @@ def test_p2p_forces_and_pairs(lib, orc, dense):
     assert np.array_equal(got, pairs)
 
 
+@pytest.mark.parametrize("tcm", [1, 0])
+def test_p2p_cohesion(lib, orc, dense, tcm):
+    """cohesionForce (mphysics_cuda_impl.cuh:259-279) inside DHSModel (:386)
+    and HMCModel (:357): a cohesive dense packing (coh = 1e5 Pa, about a
+    third of the Hertz repulsion at these overlaps), forces within 1e-12 of
+    the oracle's and the pair list bit-exact."""
+    p, pos, vel, om, m, I = dense
+    n = pos.shape[0]
+    cp = scenes.contact_property(MAT, MAT, scenes.ContactSpec(cohesion=1e5))
+    h, idx, cs, ce, srt = _grid(lib, orc, p, pos)
+    f_ref, t_ref = np.zeros((n, 3)), np.zeros((n, 3))
+    f_nc, t_nc = np.zeros((n, 3)), np.zeros((n, 3))
+    pairs = orc.p2p(p, tcm, pos, vel, om, f_ref, t_ref, m, srt, cs, ce, cp, want_pairs=True)
+    orc.p2p(p, tcm, pos, vel, om, f_nc, t_nc, m, srt, cs, ce, CP)
+    D = {k: DBuf(lib, v) for k, v in dict(pos=pos, vel=vel, om=om, m=m, srt=srt, cs=cs,
+                                            ce=ce).items()}
+    df = DBuf(lib, np.zeros((n, 3)))
+    dt = DBuf(lib, np.zeros((n, 3)))
+    dcp, _ = struct_dev(lib, abi.ContactProperty, [cp])
+    abi.check(lib, lib.xdem_calculate_p2p_f64(tcm, D["pos"].ptr, D["vel"].ptr, D["om"].ptr, df.ptr,
+                                              dt.ptr, D["m"].ptr, D["srt"].ptr, D["cs"].ptr,
+                                              D["ce"].ptr, dcp.ptr, n))
+    scale = contact_scale(pos, vel, om, m, pairs)
+    f = df.get(np.float64, (n, 3))
+    ferr = (np.abs(f - f_ref).max(1) / scale).max()
+    terr = (np.abs(dt.get(np.float64, (n, 3)) - t_ref).max(1) / (scale * pos[:, 3])).max()
+    print(f"cohesion tcm={tcm}: max force err {ferr:.2e}, torque err {terr:.2e}")
+    assert ferr < TOL, ferr
+    assert terr < TOL, terr
+    # the cohesion term is material: it changes the forces well above the bar
+    assert (np.abs(f_ref - f_nc).max(1) / scale).max() > 1e-2
+    maxp = 32
+    dcnt = DBuf(lib, nbytes=4 * n)
+    dpart = DBuf(lib, nbytes=4 * n * maxp)
+    abi.check(lib, lib.xdem_contact_pairs_f64(D["pos"].ptr, D["srt"].ptr, D["cs"].ptr, D["ce"].ptr,
+                                              n, maxp, dcnt.ptr, dpart.ptr))
+    cnt = dcnt.get(np.uint32, n)
+    part = dpart.get(np.uint32, (n, maxp))
+    got = np.concatenate([np.stack([np.full(c, i, np.uint32), part[i, :c]], 1)
+                          for i, c in enumerate(cnt) if c])
+    assert np.array_equal(got, pairs)
+
+
 def test_p2p_hmcm(lib, orc, dense):
     """TCM = 0 (HMCM variant, mphysics_cuda_impl.cuh:347-377) with rolling friction."""
     p, pos, vel, om, m, I = dense

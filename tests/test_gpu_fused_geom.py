@@ -1,10 +1,10 @@
 """Drum walls and triangle meshes on the fused FP32c path against the FP64
 oracle (the same scene the FP64 GPU path is tested on: a spinning drum with a
 fin turning with it, particles against the shell, the caps and both faces of
-the fin).  Bars: accelerations of a step within the FP32c bound of the other
-fused tests (2e-5 of the largest); the fin's reaction force and moment within
-2e-5 of the summed magnitudes of its contact forces (times the largest lever
-for the moment); runs alone vs in a mix, graph vs eager and through a
+the fin).  Bars: per-particle force m a of a step within FP32C_TOL (1e-5) of
+the oracle, relative to the particle's summed contact-force magnitudes plus
+its weight; the fin's reaction force and moment within 1e-5 of the summed
+magnitudes of its contact forces (times the largest lever for the moment); runs alone vs in a mix, graph vs eager and through a
 checkpoint bit-identical."""
 import numpy as np
 import pytest
@@ -12,18 +12,19 @@ import pytest
 from metakonga_amd import abi, scenes
 from metakonga_amd.batch import Batch
 from test_gpu_drum import _drum_scene
+from xdem_testutil import FP32C_TOL, acc_error, world_step_magnitudes
 
 pytestmark = pytest.mark.gpu
-FP32C_TOL = 2e-5
 
 
 def _batch_for(scs):
     return Batch(sum(s.np for s in scs), len(scs), sum(int(np.prod(s.grid_size)) for s in scs))
 
 
-def _acc_err(st, w):
-    scale = np.abs(w.acc).max()
-    return np.abs(st["acc"] - w.acc).max() / scale
+def _acc_err(st, w, fm):
+    err = acc_error(st["acc"], w, fm)
+    print(f"FP32c vs FP64 oracle: force err {err:.2e}")
+    return err
 
 
 def test_first_step_vs_oracle(lib, orc):
@@ -35,9 +36,9 @@ def test_first_step_vs_oracle(lib, orc):
     st = b.state(sid)
     w = orc.World(sc)
     w.vel[:] = vel
-    w.step(1)
+    fm = world_step_magnitudes(orc, w)
     assert np.abs(st["pos"] - w.pos).max() < 1e-9
-    assert _acc_err(st, w) < FP32C_TOL
+    assert _acc_err(st, w, fm) < FP32C_TOL
     b.close()
 
 
@@ -57,10 +58,10 @@ def test_later_step_vs_oracle(lib, orc, start_steps):
     for m in w.motions:
         m.n_act, m.active = start_steps, 1
     b.step(1)
-    w.step(1)
+    fm = world_step_magnitudes(orc, w)
     st2 = b.state(sid)
     assert np.abs(st2["pos"] - w.pos).max() < 1e-9
-    assert _acc_err(st2, w) < FP32C_TOL
+    assert _acc_err(st2, w, fm) < FP32C_TOL
     b.close()
 
 
@@ -86,10 +87,10 @@ def test_motion_window_vs_oracle(lib, orc, start_steps):
         # the motion is off then (as after the transition the fused run went through)
         m.n_act, m.active = sum(act), 1
     b.step(1)
-    w.step(1)
+    fm = world_step_magnitudes(orc, w)
     st2 = b.state(sid)
     assert np.abs(st2["pos"] - w.pos).max() < 1e-9
-    assert _acc_err(st2, w) < FP32C_TOL
+    assert _acc_err(st2, w, fm) < FP32C_TOL
     b.close()
 
 

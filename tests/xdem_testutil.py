@@ -131,3 +131,40 @@ def as4(a3, w):
     out[:, :3] = a3
     out[:, 3] = w
     return out
+
+
+# ------------------------------------------------- FP32c error against the oracle
+# North-star bar of the fused FP32c path: relative force (and torque) error
+# against the FP64 oracle at most 1e-5.  "Relative" is per particle, against
+# the sum of the magnitudes of its contact forces plus its weight,
+# sum_k |Fn_k + Ft_k| + m |g| (the oracle reports the sum exactly), so that a
+# particle whose contact forces cancel is not compared against its tiny net
+# force; torques against the same scale times the radius.
+FP32C_TOL = 1e-5
+
+
+def world_step_magnitudes(orc, w, nsteps=1):
+    """Step an oracle World; return the per-particle contact-force magnitude
+    sums of its last step."""
+    if nsteps > 1:
+        w.step(nsteps - 1)
+    with orc.force_magnitudes(w.pos.shape[0]) as fm:
+        w.step(1)
+    return fm.copy()
+
+
+def rel_error(got, ref, scale):
+    """max over particles of |got - ref|_inf / scale (0 for no particles)."""
+    got = np.asarray(got).reshape(len(scale), -1)
+    ref = np.asarray(ref).reshape(len(scale), -1)
+    if len(scale) == 0:
+        return 0.0
+    return float((np.abs(got - ref).max(1) / scale).max())
+
+
+def acc_error(acc, w, fmag, g=9.80665):
+    """Per-particle acceleration error m |a - a_ref| relative to the step's
+    contact-force magnitudes plus the weight (World w after its step)."""
+    scale = fmag + w.mass * g
+    return rel_error(acc * w.mass[:, None], w.acc * w.mass[:, None], scale)
+
