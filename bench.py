@@ -220,6 +220,84 @@ def run_serving(duration, lib, rate=20.0, block=100, seed=0xD3E4):
     return rep
 
 
+def fp32c_error_vs_oracle(lib, n=1_000_000, steps=300, speed=1.0):
+    """Relative force and torque error of the fused FP32c contact evaluation
+    against the FP64 oracle on one state of the hopper workload: the seeded
+    fill given random velocities of `speed` m/s and advanced `steps` steps on
+    the fused path (so that spheres collide), then every particle's contact
+    force and torque evaluated by both on that state.  Per particle the error
+    is |F_gpu - F_oracle|_inf / (sum_k |F_k| + m g), the sum over its contacts
+    as the oracle evaluates them; the maximum over particles is reported."""
+    from metakonga_amd import batch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.load()
+    sc = scenes.hopper(n=n)
+    vel = scenes.as_f32_exact(scenes.uniform(0xABC, 3 * n, 5).reshape(n, 3) * 2.0 * speed - speed)
+    ncell = int(np.prod(sc.grid_size))
+    b = batch.Batch(n, 1, ncell, lib)
+    sid = b.add(sc, vel)
+    done = 0
+    while done < steps:
+        b.step(min(100, steps - done))
+        done += 100
+    st = b.state(sid)
+    b.prepare()
+    out = b.probe(sid, ncell, max_per=32)
+    b.close()
+    p = sc.params()
+    pos = np.ascontiguousarray(st["pos"])
+    vv, om = np.ascontiguousarray(st["vel"]), np.ascontiguousarray(st["omega"])
+    h, idx = pyoracle.hash_and_index(p, pos)
+    h, idx, cs, ce, srt = pyoracle.reorder(h, idx, p.ncell)
+    m, _ = sc.mass_inertia()
+    f, t = np.zeros((n, 3)), np.zeros((n, 3))
+    with pyoracle.force_magnitudes(n) as fm:
+        npairs = pyoracle.p2p(p, 1, pos, vv, om, f, t, m, srt, cs, ce, sc.contact_pp())
+        for pl in sc.planes:
+            pyoracle.plane(p, 1, pl, pos, vv, om, f, t, m, sc.contact_pp())
+    wg = m * 9.80665
+    df = np.abs(out["force"] - f).max(1)
+    dt = np.abs(out["moment"] - t).max(1)
+    st, sn = fm.terms + wg, np.asarray(fm) + wg
+    en = df / sn
+    exact = bool(np.array_equal(out["sorted_ids"], srt) and np.array_equal(out["cstart"], cs) and
+                 np.array_equal(out["cend"], ce) and int(out["counts"].sum()) == int(npairs))
+    return {"force_rel_err_max": float((df / st).max()),
+            "torque_rel_err_max": float((dt / (st * pos[:, 3])).max()),
+            "scale": "per particle: sum over its contacts of |fsn| + |fca| + (|vn| + |vs|) V + "
+                     "|Ft| (the contact-law term magnitudes) + m g",
+            "force_rel_err_max_net_scale": float(en.max()),
+            "force_rel_err_p9999_net_scale": float(np.quantile(en, 0.9999)),
+            "net_scale": "per particle: sum over its contacts of |Fn + Ft| + m g",
+            "share_within_1e-5_net_scale": float((en <= 1e-5).mean()),
+            "contact_pairs": int(npairs), "grid_and_pair_count_exact": exact,
+            "state": f"hopper {n} spheres after {steps} steps from the fill with random "
+                     f"velocities of {speed} m/s (FP32c path), forces of every particle"}
+
+
+def cpu_subset(scs, budget_s, threads=1):
+    """Oracle CPU path over a fixed subset of scenes: each scene stepped for
+    its share of the time budget; particle-steps/s over the whole subset."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.load()
+    total_ps, total_t, steps = 0, 0.0, []
+    for sc in scs:
+        w = pyoracle.World(sc, semantics=pyoracle.ORC_HOST)
+        t0 = time.perf_counter()
+        w.step(1, threads=threads)
+        t1 = time.perf_counter() - t0
+        k = max(1, min(200, int(budget_s / len(scs) / max(t1, 1e-4)) - 1))
+        t0 = time.perf_counter()
+        w.step(k, threads=threads)
+        total_t += time.perf_counter() - t0
+        total_ps += sc.np * k
+        steps.append(k)
+    return dict(value=total_ps / total_t, seconds=total_t, particle_steps=total_ps,
+                steps_min=min(steps), steps_max=max(steps))
+
+
 def cpu_baseline(scene, budget_s=15.0, threads=1):
     """Oracle CPU path (serial, the reference's CPU semantics) on a bounded
     sample of the same workload: all particles, as many steps as fit ~budget."""
@@ -252,14 +330,22 @@ def cpu_settling_full():
 
 
 def cpu_model():
+    """lscpu's full model name, sockets and threads, and nproc."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout)
+    except Exception:
+        pass
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
         for line in out.splitlines():
-            if line.startswith("Model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Thread(s) per core", "Core(s) per socket",
+                             "CPU(s)"):
+                info[k.strip()] = v.strip()
     except Exception:
-        pass
-    return platform.processor()
+        info["Model name"] = platform.processor()
+    return info
 
 
 def main():
@@ -325,7 +411,9 @@ def main():
         head = fused
         dtype = "f32 contact + f64 positions"
         k_ms = fused["phase_ms"]["contact_verlet"]
-        achieved = KSTEP_BYTES * scene.np / (k_ms * 1e-3)
+        # algorithmic bytes of one launch: the per-particle-step figure of the
+        # FP32c data model (192 B) x the particles one launch advances
+        achieved = STEP_F32C_BYTES * scene.np / (k_ms * 1e-3)
         ch = (fused.get("autotune") or {}).get("chosen", 2256)
         sig = (f"k_step<true, false, {(ch // 1000) % 10}, "
                f"{'true' if (ch // 10000) % 10 else 'false'}, false, false, "
@@ -336,7 +424,8 @@ def main():
                 "traffic_source": "profiles/hbm_traffic_hopper1M_fp32c.md (rocprofv3 FETCH_SIZE + "
                                   "WRITE_SIZE, copy-kernel calibrated)" if traffic else None,
                 "kernel": f"{sig} (fused contact + Verlet, FP32c), block {ch % 1000}",
-                "bytes_per_particle": KSTEP_BYTES, "kernel_ms": k_ms,
+                "bytes_per_particle": STEP_F32C_BYTES, "kernel_record_bytes": KSTEP_BYTES,
+                "kernel_ms": k_ms,
                 "step_frac": STEP_F32C_BYTES * fused["value"] / HBM_PEAK}
     else:
         head = f64
@@ -372,24 +461,45 @@ def main():
     cpu = None
     cpu_omp = None
     cpu_small = None
+    cpu_c3 = cpu_c4 = None
+    err = None
     if not args.no_cpu:
+        host = cpu_model()
+        log(f"host: {host}")
+        if fused is not None and args.np == 1_000_000:
+            err = fp32c_error_vs_oracle(lib)
+            log(f"fp32c vs fp64 oracle: {err}")
         log("cpu baseline (oracle, CPU-path semantics, 1 thread) ...")
         c = cpu_baseline(scene, budget_s=args.cpu_budget)
         cpu = {"value": c["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
                "sample": f"{scene.name} all {scene.np} particles, {c['steps']} steps "
                          f"({c['seconds']:.1f}s), serial CPU-path semantics",
-               "cpu": cpu_model()}
+               "cpu": host}
         log(f"cpu: {c['value']:.4g} particle-steps/s over {c['steps']} steps")
-        nthr = min(16, os.cpu_count() or 1)
+        nthr = host.get("nproc") or os.cpu_count() or 1
         co = cpu_baseline(scene, budget_s=args.cpu_budget / 2, threads=nthr)
         cpu_omp = {"value": co["value"], "unit": "particle-steps/s", "cores": nthr,
-                   "kind": "port", "sample": f"same, OpenMP over particles, {co['steps']} steps"}
+                   "kind": "port", "sample": f"same, OpenMP over particles on all {nthr} host "
+                                             f"threads, {co['steps']} steps"}
         log(f"cpu omp x{nthr}: {co['value']:.4g} particle-steps/s")
         cs = cpu_settling_full()
         cpu_small = {"value": cs["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
                      "sample": f"settling box {cs['np']} spheres in full, {cs['steps']} steps "
                                f"({cs['seconds']:.1f}s), serial CPU-path semantics"}
         log(f"cpu settling 8k in full: {cs['value']:.4g} particle-steps/s")
+        from metakonga_amd import serve
+        c3 = cpu_subset(scenes.batch_scenes()[:16], args.cpu_budget / 2, threads=nthr)
+        cpu_c3 = {"value": c3["value"], "unit": "particle-steps/s", "cores": nthr, "kind": "port",
+                  "sample": f"the first 16 of the 256 box-settling scenes, "
+                            f"{c3['steps_min']}-{c3['steps_max']} steps each "
+                            f"({c3['seconds']:.1f}s), OpenMP over particles"}
+        tr = serve.poisson_trace(seed=0xD3E4, rate=20.0, duration=60.0, block=100)[:16]
+        c4 = cpu_subset([r.scene() for r in tr], args.cpu_budget / 2, threads=nthr)
+        cpu_c4 = {"value": c4["value"], "unit": "particle-steps/s", "cores": nthr, "kind": "port",
+                  "sample": f"the first 16 requests of the serving trace, "
+                            f"{c4['steps_min']}-{c4['steps_max']} steps each "
+                            f"({c4['seconds']:.1f}s), OpenMP over particles"}
+        log(f"cpu 16-scene subsets: batched {c3['value']:.4g}, serving {c4['value']:.4g}")
 
     out = {
         "metric": METRIC,
@@ -427,6 +537,11 @@ def main():
         out["cpu_baseline_multicore"] = cpu_omp
     if cpu_small is not None:
         out["cpu_baseline_settling_8k"] = cpu_small
+    if cpu_c3 is not None:
+        out["cpu_baseline_batched_16"] = cpu_c3
+        out["cpu_baseline_serving_16"] = cpu_c4
+    if err is not None:
+        out["fp32c_force_err_vs_oracle"] = err
     if serving is not None:
         out["serving"] = serving
     print(json.dumps(out), flush=True)

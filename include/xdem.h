@@ -123,6 +123,8 @@ int32_t xdem_memset(void* dptr, int value, size_t bytes);
 int32_t xdem_memcpy_htod(void* dst, const void* src, size_t bytes);
 int32_t xdem_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int32_t xdem_memcpy_dtod(void* dst, const void* src, size_t bytes);
+/* Free and total device memory of the current device (hipMemGetInfo). */
+int32_t xdem_device_memory(size_t* free_bytes, size_t* total_bytes);
 /* A plain copy kernel of `bytes` (multiple of 16, 16-byte aligned pointers):
  * the known-size reference for calibrating profiler byte counters.  No
  * counterpart in the reference. */

@@ -101,6 +101,13 @@ int32_t xdem_batch_add_polygon_object(xdem_batch* b, uint32_t scene_id,
  * with 32-bit byte offsets); larger runs use several batches. */
 int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
                           xdem_batch** out);
+/* Device bytes a batch of these capacities holds: what xdem_batch_create
+ * allocates, its scene and launch tables at their largest, and with
+ * tangential_history the history buffers the first such scene adds (the
+ * parity probe's buffers are not counted).  Host arithmetic only (no GPU
+ * needed); admission control budgets HBM with it. */
+uint64_t xdem_batch_device_bytes(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
+                                 int32_t tangential_history);
 int32_t xdem_batch_destroy(xdem_batch* b);
 /* Add a scene (host arrays, reference layout: pos4 = x,y,z,r; vel3/omega3 may be
  * NULL for zero).  Takes effect at the next xdem_batch_step.  Fails with

@@ -218,6 +218,7 @@ def _declare(lib):
         "xdem_memcpy_htod": ([vp, vp, C.c_size_t], i32),
         "xdem_memcpy_dtoh": ([vp, vp, C.c_size_t], i32),
         "xdem_memcpy_dtod": ([vp, vp, C.c_size_t], i32),
+        "xdem_device_memory": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], i32),
         "xdem_copy_kernel": ([vp, vp, C.c_size_t], i32),
         "xdem_set_symbolic_parameter": ([C.POINTER(DeviceParameters)], i32),
         "xdem_get_symbolic_parameter": ([C.POINTER(DeviceParameters)], i32),

@@ -32,6 +32,7 @@ def declare(lib):
     vp, u32, i32, u64 = C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64
     sig = {
         "xdem_batch_create": ([u32, u32, u64, C.POINTER(vp)], i32),
+        "xdem_batch_device_bytes": ([u32, u32, u64, i32], u64),
         "xdem_batch_destroy": ([vp], i32),
         "xdem_batch_add_scene": ([vp, C.POINTER(BatchSceneDesc), vp, vp, vp, C.POINTER(u32)], i32),
         "xdem_batch_add_scene_state": ([vp, C.POINTER(BatchSceneDesc), vp, vp, vp, vp, vp,

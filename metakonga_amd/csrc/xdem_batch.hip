@@ -809,7 +809,33 @@ int32_t enqueue_block(xdem_batch* b, uint32_t nsteps, bool first_full_sort) {
 
 }  // namespace
 
+namespace {
+// Device bytes of a batch: the arrays xdem_batch_create allocates (keep in
+// step with it), the per-scene tables and block tables at their largest
+// (64-thread blocks), and the history buffers ensure_history adds.
+uint64_t batch_device_bytes(uint64_t n, uint64_t nscene, uint64_t ncell, bool history) {
+    const uint64_t nblk = div_up((uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFull), ROWS_T);
+    uint64_t b = 0;
+    b += 2 * n * (sizeof(double4) + 3 * sizeof(float4) + sizeof(uint2) + sizeof(float4));  // buf[2]
+    b += 4 * n * sizeof(uint32_t);                        // keynew, skey[2], perm
+    b += ncell * sizeof(uint2);                           // crange
+    b += (nblk * (ROWS_T / 64) + 1) * sizeof(uint64_t);   // moverbits
+    b += (nblk + 1) * sizeof(uint32_t) + 4 * sizeof(uint32_t);  // blockcnt, counters
+    b += n * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t) + 2 * sizeof(uint32_t));  // movers
+    b += 1024 * sizeof(double);                           // d_red
+    b += nscene * sizeof(BScene) + 2 * div_up((uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFull), 64) *
+                                       sizeof(BlockEntry);
+    if (history) b += 2 * n * (HSLOTS * sizeof(float4) + sizeof(uint32_t)) + sizeof(uint32_t);
+    return b;
+}
+}  // namespace
+
 extern "C" {
+
+uint64_t xdem_batch_device_bytes(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
+                                 int32_t tangential_history) {
+    return batch_device_bytes(max_particles, max_scenes, max_cells, tangential_history != 0);
+}
 
 int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
                           xdem_batch** out) {

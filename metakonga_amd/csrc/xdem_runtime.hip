@@ -195,6 +195,12 @@ int32_t xdem_memcpy_dtod(void* dst, const void* src, size_t bytes) {
     return XDEM_OK;
 }
 
+int32_t xdem_device_memory(size_t* free_bytes, size_t* total_bytes) {
+    XDEM_REQUIRE(free_bytes && total_bytes, "null argument");
+    XDEM_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return XDEM_OK;
+}
+
 // setSymbolicParameter — mphysics_cuda_def.cu:13-16 (cudaMemcpyToSymbol(cte)).
 int32_t xdem_set_symbolic_parameter(const xdem_device_parameters* h) {
     XDEM_REQUIRE(h, "null parameters");

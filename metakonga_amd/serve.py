@@ -20,12 +20,17 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import scenes
+from . import abi, scenes
 
-# Device bytes per particle of the fused path: two state buffers of 84 B,
-# keys/perm/sort scratch ~40 B.
-BYTES_PER_PARTICLE = 2 * 84 + 40
-BYTES_PER_CELL = 8
+
+def device_bytes(particles, nscenes, cells, history=False, lib=None):
+    """Device bytes of a fused batch holding `particles`, `nscenes` scenes and
+    `cells` cells: the library's own allocation arithmetic
+    (xdem_batch_device_bytes; host only, no GPU needed)."""
+    from . import batch as _batch
+    L = lib or abi.load()
+    _batch.declare(L)
+    return int(L.xdem_batch_device_bytes(int(particles), int(nscenes), int(cells), int(history)))
 
 
 @dataclass
@@ -95,14 +100,17 @@ class Server:
     finished: list = field(default_factory=list)
     used_particles: int = 0
     used_cells: int = 0
+    used_history: bool = False
     particle_steps: int = 0
 
     def footprint(self, sc):
-        return sc.np * BYTES_PER_PARTICLE + int(np.prod(sc.grid_size)) * BYTES_PER_CELL
+        """Device bytes of a batch holding this scene alone."""
+        return device_bytes(sc.np, 1, int(np.prod(sc.grid_size)), sc.tangential_history)
 
     def fits(self, sc):
         cells = int(np.prod(sc.grid_size))
-        mem = (self.used_particles + sc.np) * BYTES_PER_PARTICLE + (self.used_cells + cells) * BYTES_PER_CELL
+        mem = device_bytes(self.used_particles + sc.np, len(self.running) + 1,
+                           self.used_cells + cells, self.used_history or sc.tangential_history)
         return (self.used_particles + sc.np <= self.max_particles and
                 self.used_cells + cells <= self.max_cells and
                 len(self.running) < self.max_scenes and mem <= self.hbm_budget)
@@ -129,6 +137,7 @@ class Server:
             rec.scene_cells = int(np.prod(sc.grid_size))
             self.used_particles += sc.np
             self.used_cells += rec.scene_cells
+            self.used_history = self.used_history or sc.tangential_history
             self.running[rec.sid] = rec
 
     def retire(self, now):

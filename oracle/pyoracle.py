@@ -12,7 +12,8 @@ import numpy as np
 from metakonga_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# XDEM_ORACLE_LIB selects another build of the same sources (the sanitizer build)
+LIB = os.environ.get("XDEM_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 ORC_GPU, ORC_HOST = 0, 1
 ORC_WALL_PLANE, ORC_WALL_CYLINDER, ORC_WALL_DRUM = 0, 1, 2
@@ -100,7 +101,7 @@ def _declare(L):
         "orc_force_constant": ([i32, f64, f64, f64, f64, vp, vp], None),
         "orc_cohesion_force": ([f64] * 8, f64),
         "orc_host_cohesion": ([f64] * 4, f64),
-        "orc_set_force_magnitudes": ([vp], None),
+        "orc_set_force_magnitudes": ([vp, vp], None),
         "orc_p2p": ([vp, i32, C.c_int] + [vp] * 10 + [u32, vp, u64], u64),
         "orc_pairs_bruteforce": ([vp, u32, vp, u64], u64),
         "orc_plane_detect": ([vp, C.c_int, vp, vp, vp, f64], f64),
@@ -194,19 +195,29 @@ def host_cohesion(coh, coh_r, coh_e, Fn):
     return load().orc_host_cohesion(coh, coh_r, coh_e, Fn)
 
 
+class Magnitudes(np.ndarray):
+    """Per-particle sums of the net contact-force magnitudes |Fn + Ft| (the
+    array itself) with the sums of the contact-law term magnitudes as .terms."""
+
+    def __array_finalize__(self, obj):
+        self.terms = getattr(obj, "terms", None)
+
+
 class force_magnitudes:
     """Context: every contact force the oracle evaluates inside the block adds
-    its magnitude |Fn + Ft| to .fmag[particle] (float64[n])."""
+    |Fn + Ft| to fmag[particle] and |fsn| + |fca| + (|vn| + |vs|) V + |Ft| to
+    fmag.terms[particle] (float64[n] each)."""
 
     def __init__(self, n):
-        self.fmag = np.zeros(n)
+        self.fmag = np.zeros(n).view(Magnitudes)
+        self.fmag.terms = np.zeros(n)
 
     def __enter__(self):
-        load().orc_set_force_magnitudes(P(self.fmag))
+        load().orc_set_force_magnitudes(P(np.asarray(self.fmag)), P(self.fmag.terms))
         return self.fmag
 
     def __exit__(self, *exc):
-        load().orc_set_force_magnitudes(None)
+        load().orc_set_force_magnitudes(None, None)
         return False
 
 

@@ -118,9 +118,23 @@ xdem_device_force_constant force_constant(int tcm, double ir, double jr, double 
 // orc_set_force_magnitudes): the cancellation-safe scale of a per-particle
 // force comparison.  Each particle's entry is written only by the loop
 // iteration of that particle.
+//
+// A second sum (g_tmag) adds the magnitudes of the terms the contact law
+// combines: |fsn| + |fca| + (|vn| + |vs|) V + |Ft|, V the summed magnitudes of
+// the velocities entering the relative velocity (|v_i| + |w_i| r_i, plus the
+// partner's |v_j| + |w_j| r_j or the wall point's speed).  Rounding in a
+// lower-precision evaluation of the law is bounded relative to that sum; the
+// net force |Fn + Ft| can be much smaller when the terms cancel.
 double* g_fmag = nullptr;
-inline void add_mag(uint32_t id, const V3& f) {
+double* g_tmag = nullptr;
+inline void add_mag(uint32_t id, const V3& f, double terms = 0.0) {
     if (g_fmag) g_fmag[id] += length(f);
+    if (g_tmag) g_tmag[id] += terms;
+}
+inline double term_scale(const xdem_device_force_constant& c, double cdist, double fca, double V,
+                         const V3& Ft) {
+    return std::fabs(c.kn * std::pow(cdist, 1.5)) + std::fabs(fca) +
+           (std::fabs(c.vn) + std::fabs(c.vs)) * V + length(Ft);
 }
 
 // cohesionForce — mphysics_cuda_impl.cuh:259-279 (particle-particle only; walls
@@ -312,7 +326,10 @@ double orc_host_cohesion(double coh, double coh_r, double coh_e, double Fn) {
     return host_cohesion(coh, coh_r, coh_e, Fn);
 }
 
-void orc_set_force_magnitudes(double* fmag) { g_fmag = fmag; }
+void orc_set_force_magnitudes(double* fmag, double* tmag) {
+    g_fmag = fmag;
+    g_tmag = tmag;
+}
 
 void orc_force_constant(int32_t tcm, double ir, double jr, double im, double jm,
                         const xdem_device_contact_property* cp, xdem_device_force_constant* out) {
@@ -379,7 +396,11 @@ void p2p_one(const xdem_device_parameters* p, int tcm, int sem, uint32_t id, con
                         double fca = host_cohesion(cp.coh, Req, coh_e, fsn);
                         V3 mf = mk(0, 0, 0), mm = mk(0, 0, 0);
                         host_dhs(c, p->dt, fca, cdist, cpt, rv, unit, mf, mm);
-                        add_mag(id, mf);
+                        add_mag(id, mf,
+                                term_scale(c, cdist, fca,
+                                           length(ivel) + length(iomega) * ir + length(jvel) +
+                                               length(jomega) * jr,
+                                           mf));
                         sumF += mf;
                         sumM += mm;
                     } else {
@@ -389,7 +410,11 @@ void p2p_one(const xdem_device_parameters* p, int tcm, int sem, uint32_t id, con
                         double fsn = -c.kn * std::pow(cdist, 1.5);
                         double fca = cohesion_force(ir, jr, cp.Ei, cp.Ej, cp.pri, cp.prj, cp.coh, fsn);
                         gpu_law(tcm, c, p->dt, ir, fca, rcon, cdist, iomega, rv, unit, Ft, Fn, M);
-                        add_mag(id, Fn + Ft);
+                        add_mag(id, Fn + Ft,
+                                term_scale(c, cdist, fca,
+                                           length(ivel) + length(iomega) * ir + length(jvel) +
+                                               length(jomega) * jr,
+                                           Ft));
                         sumF += Fn + Ft;
                         sumM += M;
                     }
@@ -542,7 +567,7 @@ void orc_plane(const xdem_device_parameters* p, int32_t tcm, const xdem_device_p
             V3 dv = p->rollingCondition ? -(ivel + cross(iomega, r * un)) : -ivel;
             xdem_device_force_constant c = force_constant(tcm, r, 0.0, m, 0.0, *cp);
             gpu_law(tcm, c, p->dt, 0.0, 0.0, rcon, cdist, iomega, dv, un, Ft, Fn, M);
-            add_mag(id, Fn + Ft);
+            add_mag(id, Fn + Ft, term_scale(c, cdist, 0.0, length(ivel) + length(iomega) * r, Ft));
         }
         V3 s = Fn + Ft;
         double* f = force + 3 * id;
@@ -662,7 +687,11 @@ void orc_cylinder(const xdem_device_parameters* p, int32_t tcm, const xdem_devic
             V3 dv = of(cy->vel) + cross(of(cy->omega), cy2cp) - (ivel + cross(iomega, ip[3] * un));
             xdem_device_force_constant c = force_constant(tcm, ip[3], 0, im, 0, *cp);
             gpu_law(tcm, c, p->dt, 0.0, 0.0, rcon, cdist, iomega, dv, un, Ft, Fn, M);
-            add_mag(id, Fn + Ft);
+            add_mag(id, Fn + Ft,
+                    term_scale(c, cdist, 0.0,
+                               length(ivel) + length(iomega) * ip[3] +
+                                   length(of(cy->vel) + cross(of(cy->omega), cy2cp)),
+                               Ft));
         }
         V3 sum_f = Fn + Ft;
         double* f = force + 3 * id;
@@ -688,7 +717,7 @@ void drum_one(const xdem_device_parameters* p, int32_t tcm, const xdem_drum_info
     xdem_device_force_constant c = force_constant(tcm, r, 0, im, 0, cp);
     gpu_law(tcm, c, p->dt, 0.0, 0.0, rcon, cdist, iomega, dv, unit, Ft, Fn, Mc);
     V3 f = Fn + Ft;
-    add_mag(id, f);
+    add_mag(id, f, term_scale(c, cdist, 0.0, length(ivel) + length(iomega) * r + length(wv), Ft));
     F += f;
     M += Mc;
     R += -f;
@@ -831,7 +860,11 @@ void poly_one(const xdem_device_parameters* p, int tcm, uint32_t id,
                                     : of(pmi.vel) - ivel;
                         xdem_device_force_constant c = force_constant(tcm, ir, 0, im, 0, cmp);
                         gpu_law(tcm, c, p->dt, 0.0, 0.0, rcon, cdist, iomega, dv, unit, Ft, Fn, M);
-                        add_mag(id, Fn + Ft);
+                        add_mag(id, Fn + Ft,
+                                term_scale(c, cdist, 0.0,
+                                           length(ivel) + length(iomega) * ir +
+                                               length(of(pmi.vel) + cross(of(pmi.omega), po2cp)),
+                                           Ft));
                         sum_force += Fn + Ft;
                         sum_moment += M;
                         if (rf) {
@@ -1335,7 +1368,11 @@ void hist_one(const orc_world* w, const orc_hist* in, orc_hist* out, uint32_t id
                     V3 xi = hc.lookup(k), Ft, Fn, M;
                     hist_law(c, p->dt, ir, cdist, rv, unit, xi, Ft, Fn, M);
                     hc.store(k, xi, overflow);
-                    add_mag(id, Fn + Ft);
+                    add_mag(id, Fn + Ft,
+                            term_scale(c, cdist, 0.0,
+                                       length(ivel) + length(iomega) * ir + length(jvel) +
+                                           length(jomega) * jr,
+                                       Ft));
                     sumF += Fn + Ft;
                     sumM += M;
                 }
@@ -1360,7 +1397,7 @@ void hist_one(const orc_world* w, const orc_hist* in, orc_hist* out, uint32_t id
         V3 xi = hc.lookup(key), Ft, Fn, M;
         hist_law(c, p->dt, 0.0, cdist, dv, un, xi, Ft, Fn, M);
         hc.store(key, xi, overflow);
-        add_mag(id, Fn + Ft);
+        add_mag(id, Fn + Ft, term_scale(c, cdist, 0.0, length(ivel) + length(iomega) * ir, Ft));
         F += Fn + Ft;
     }
     out->count[id] = hc.nout;

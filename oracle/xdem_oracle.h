@@ -120,8 +120,10 @@ double orc_cohesion_force(double ri, double rj, double Ei, double Ej, double pri
 double orc_host_cohesion(double coh, double coh_r, double coh_e, double Fn);
 /* From now on every contact force Fn + Ft the oracle evaluates (particle,
  * plane, cylinder, drum, triangle, history twin) adds |Fn + Ft| to fmag[id]
- * of its particle; NULL stops. */
-void orc_set_force_magnitudes(double* fmag);
+ * of its particle and the magnitudes of the terms of the contact law,
+ * |fsn| + |fca| + (|vn| + |vs|) V + |Ft| (V: summed magnitudes of the
+ * velocities entering the relative velocity), to tmag[id]; NULL stops. */
+void orc_set_force_magnitudes(double* fmag, double* tmag);
 /* getConstant (mphysics_cuda_impl.cuh:214-256) */
 void orc_force_constant(int32_t tcm, double ir, double jr, double im, double jm,
                         const xdem_device_contact_property* cp, xdem_device_force_constant* out);

@@ -94,3 +94,24 @@ def test_hbm_budget_defers_and_rejects():
     assert by[2].result == {"rejected": True}
     assert by[1].admitted >= by[0].finished   # deferred until the first one retired
     assert be.max_live == 1
+
+
+def test_footprint_is_the_library_allocation_arithmetic():
+    """Admission budgets with xdem_batch_device_bytes: linear in particles and
+    cells, the history buffers added when a scene has history, and a budget of
+    exactly two scenes' batch admits two of three equal scenes at a time."""
+    b1 = serve.device_bytes(100_000, 1, 1 << 20)
+    b2 = serve.device_bytes(200_000, 1, 1 << 20)
+    b3 = serve.device_bytes(100_000, 1, 1 << 21)
+    per_p = (b2 - b1) / 100_000
+    per_c = (b3 - b1) / (1 << 20)
+    assert 200 < per_p < 400 and per_c == 8
+    assert serve.device_bytes(100_000, 1, 1 << 20, history=True) - b1 >= 100_000 * 2 * 16 * 16
+    clock = serve.FakeClock()
+    be = FakeBackend(clock)
+    trace = [serve.Request(k, 0.0, "drop", 20_000, 200, 1) for k in range(3)]
+    sc = trace[0].scene()
+    cells = int(np.prod(sc.grid_size))
+    s = _server(clock, be, budget=serve.device_bytes(2 * sc.np, 2, 2 * cells))
+    s.run(trace)
+    assert be.max_live == 2
