@@ -168,6 +168,22 @@ int32_t xdem_calculate_hash_and_index_polygon_sphere_f64(uint32_t* hash, uint32_
 int32_t xdem_reorder_data_and_find_cell_start(uint32_t* hash, uint32_t* index, uint32_t* cstart,
                                               uint32_t* cend, uint32_t* sorted_index, uint32_t n,
                                               uint32_t ncell);
+/* No counterpart in the reference: xdem_reorder_data_and_find_cell_start with
+ * the same arguments and results, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It calls no allocating routine: its sort scratch is a
+ * fixed device array of the library (about 8.7 MB, held by each device the
+ * library runs on until the process ends), so n is limited to
+ * XDEM_NOALLOC_MAX_KEYS (larger n: an error, nothing written).  All calls on
+ * one device share that scratch: calls that overlap on one device (several
+ * host threads, or streams other than the library's) race.  Every scratch
+ * word is written before it is read in a call, so nothing carries over from
+ * the previous call. */
+#define XDEM_NOALLOC_MAX_KEYS (1u << 20)
+int32_t xdem_reorder_data_and_find_cell_start_noalloc(uint32_t* hash, uint32_t* index,
+                                                      uint32_t* cstart, uint32_t* cend,
+                                                      uint32_t* sorted_index, uint32_t n,
+                                                      uint32_t ncell);
 /* cu_calculate_p2p — mphysics_cuda_dec.cuh:121-126, def mphysics_cuda_def.cu:110-136 */
 int32_t xdem_calculate_p2p_f64(int32_t tcm, const double* pos, const double* vel,
                                const double* omega, double* force, double* moment,

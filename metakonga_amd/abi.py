@@ -228,6 +228,7 @@ def _declare(lib):
         "xdem_calculate_hash_and_index_f64": ([vp, vp, vp, u32], i32),
         "xdem_calculate_hash_and_index_polygon_sphere_f64": ([vp, vp, u32, u32, vp], i32),
         "xdem_reorder_data_and_find_cell_start": ([vp] * 5 + [u32, u32], i32),
+        "xdem_reorder_data_and_find_cell_start_noalloc": ([vp] * 5 + [u32, u32], i32),
         "xdem_calculate_p2p_f64": ([i32] + [vp] * 10 + [u32], i32),
         "xdem_plane_contact_force_f64": ([i32] + [vp] * 7 + [u32, vp], i32),
         "xdem_cube_contact_force_f64": ([i32] + [vp] * 7 + [u32, vp], i32),

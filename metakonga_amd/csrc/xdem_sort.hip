@@ -18,6 +18,9 @@
 //    steps) costs no LDS atomics and no serialised conflicts.
 //  * the scatter stages the tile in LDS in digit order and writes each digit run
 //    with consecutive lanes, so global writes are coalesced.
+#include <mutex>
+
+#include "../../include/xdem.h"
 #include "xdem_common.hpp"
 
 namespace xdem {
@@ -256,13 +259,20 @@ int32_t fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s) {
     return XDEM_OK;
 }
 
-int32_t sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, hipStream_t s) {
+// 32-bit words of scratch the sort of n keys uses: ping-pong keys and values,
+// the per-tile digit histogram and the digit totals.  Every word is written
+// before it is read within one sort, so the scratch needs no clearing.
+size_t sort_scratch_words(uint32_t n) {
+    return 2 * (size_t)n + (size_t)RADIX * div_up(n, TILE) + RADIX + 64;
+}
+
+// The sort with a caller-provided scratch of sort_scratch_words(n) words.
+int32_t sort_pairs_in(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits,
+                      hipStream_t s, uint32_t* ws) {
     if (n <= 1 || key_bits == 0) return XDEM_OK;
     const uint32_t ntiles = div_up(n, TILE);
     const size_t hist_n = (size_t)RADIX * ntiles;
-    const size_t need = sizeof(uint32_t) * (2 * (size_t)n + hist_n + RADIX) + 256;
-    XDEM_TRY(workspace(0).reserve(need));
-    uint32_t* tk = static_cast<uint32_t*>(workspace(0).ptr);
+    uint32_t* tk = ws;
     uint32_t* tv = tk + n;
     uint32_t* hist = tv + n;
     uint32_t* dtot = hist + hist_n;
@@ -292,6 +302,12 @@ int32_t sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits
     return XDEM_OK;
 }
 
+int32_t sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, hipStream_t s) {
+    if (n <= 1 || key_bits == 0) return XDEM_OK;
+    XDEM_TRY(workspace(0).reserve(sizeof(uint32_t) * sort_scratch_words(n)));
+    return sort_pairs_in(keys, vals, n, key_bits, s, static_cast<uint32_t*>(workspace(0).ptr));
+}
+
 int32_t cell_ranges(const uint32_t* keys, const uint32_t* vals, uint32_t* cstart, uint32_t* cend,
                     uint32_t* sorted_index, uint32_t n, hipStream_t s) {
     if (n == 0) return XDEM_OK;
@@ -301,7 +317,62 @@ int32_t cell_ranges(const uint32_t* keys, const uint32_t* vals, uint32_t* cstart
     return XDEM_OK;
 }
 
+// ------------------------------------------- no-allocation form of the reorder
+// The scratch of xdem_reorder_data_and_find_cell_start_noalloc: a file-scope
+// device array (each device holds its own copy for the life of the process),
+// sized for XDEM_NOALLOC_MAX_KEYS keys.  Host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device.
+constexpr size_t NOALLOC_WORDS = 2 * (size_t)XDEM_NOALLOC_MAX_KEYS +
+                                 (size_t)RADIX * ((XDEM_NOALLOC_MAX_KEYS + TILE - 1) / TILE) +
+                                 RADIX + 64;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch[MAX_DEVICES] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device (looked up once per device).
+int32_t noalloc_scratch(uint32_t** out) {
+    int dev = 0;
+    XDEM_HIP(hipGetDevice(&dev));
+    XDEM_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "device %d beyond %d", dev, MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        XDEM_REQUIRE(e == hipSuccess && p, "hipGetSymbolAddress(scratch) failed on device %d: %s",
+                     dev, hipGetErrorString(e));
+        g_scratch[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = g_scratch[dev];
+    return XDEM_OK;
+}
+
 }  // namespace xdem
+
+extern "C" int32_t xdem_reorder_data_and_find_cell_start_noalloc(uint32_t* hash, uint32_t* index,
+                                                                 uint32_t* cstart, uint32_t* cend,
+                                                                 uint32_t* sorted_index,
+                                                                 uint32_t n, uint32_t ncell) {
+    using namespace xdem;
+    XDEM_REQUIRE(ncell > 0, "ncell == 0");
+    XDEM_REQUIRE(n <= XDEM_NOALLOC_MAX_KEYS,
+                 "n = %u exceeds the %u keys of the no-allocation scratch (use "
+                 "xdem_reorder_data_and_find_cell_start)",
+                 n, (uint32_t)XDEM_NOALLOC_MAX_KEYS);
+    XDEM_REQUIRE(n == 0 || (hash && index && sorted_index), "null array");
+    XDEM_REQUIRE(cstart && cend, "null cell arrays");
+    uint32_t* ws = nullptr;
+    XDEM_TRY(noalloc_scratch(&ws));
+    hipStream_t s = stream();
+    XDEM_TRY(sort_pairs_in(hash, index, n, bits_for(ncell), s, ws));
+    XDEM_TRY(fill_u32(cstart, 0xFFFFFFFFu, ncell, s));
+    XDEM_TRY(fill_u32(cend, 0u, ncell, s));
+    XDEM_TRY(cell_ranges(hash, index, cstart, cend, sorted_index, n, s));
+    return XDEM_OK;
+}
 
 extern "C" int32_t xdem_reorder_data_and_find_cell_start(uint32_t* hash, uint32_t* index,
                                                          uint32_t* cstart, uint32_t* cend,
