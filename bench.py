@@ -415,9 +415,9 @@ def main():
         # FP32c data model (192 B) x the particles one launch advances
         achieved = STEP_F32C_BYTES * scene.np / (k_ms * 1e-3)
         ch = (fused.get("autotune") or {}).get("chosen", 2256)
-        sig = (f"k_step<true, false, {(ch // 1000) % 10}, "
-               f"{'true' if (ch // 10000) % 10 else 'false'}, false, false, "
-               f"{'true' if ch >= 100000 else 'false'}>")
+        tf = lambda v: "true" if v else "false"  # noqa: E731
+        sig = (f"k_step<true, false, {(ch // 1000) % 10}, {tf((ch // 10000) % 10)}, false, false, "
+               f"{tf((ch // 100000) % 10)}, {tf(ch // 1000000)}>")
         traffic = profiled_traffic(sig) if scene.np == 1_000_000 else None
         roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic,

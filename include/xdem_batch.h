@@ -102,12 +102,36 @@ int32_t xdem_batch_add_polygon_object(xdem_batch* b, uint32_t scene_id,
 int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
                           xdem_batch** out);
 /* Device bytes a batch of these capacities holds: what xdem_batch_create
- * allocates, its scene and launch tables at their largest, and with
- * tangential_history the history buffers the first such scene adds (the
- * parity probe's buffers are not counted).  Host arithmetic only (no GPU
+ * allocates, its scene and launch tables at their largest, with
+ * XDEM_BATCH_FOOTPRINT_HISTORY the history buffers the first such scene adds,
+ * with XDEM_BATCH_FOOTPRINT_LISTS the candidate lists of xdem_batch_set_lists
+ * (the parity probe's buffers are not counted).  Host arithmetic only (no GPU
  * needed); admission control budgets HBM with it. */
+#define XDEM_BATCH_FOOTPRINT_HISTORY 1
+#define XDEM_BATCH_FOOTPRINT_LISTS 2
 uint64_t xdem_batch_device_bytes(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
-                                 int32_t tangential_history);
+                                 int32_t options);
+/* Candidate lists (no counterpart in the reference): the contact kernel keeps,
+ * per particle, the neighbours within ri + rj + skin (skin = skin_fraction x
+ * the scene's cell size; 0 keeps the current fraction, 0.05 at creation) in
+ * the reference traversal order and evaluates the contact predicate on them
+ * instead of walking the 27 cells.  A particle walks its cells again (and
+ * rebuilds its list) at the first step of every xdem_batch_step call, when
+ * its scene's accumulated displacement bound reaches half the skin, when more
+ * than 64 particles of the batch changed cell in the last step, when it moved
+ * in the sorted order, or when a particle changed cell within one cell of its
+ * own; contacts and their order, hence the results, are bit-identical to the
+ * walk.  Plane-only batches (no history, drums or meshes); the lists take
+ * (32 + 1) x 4 bytes per particle of capacity and a bit per cell. */
+int32_t xdem_batch_set_lists(xdem_batch* b, int32_t on, float skin_fraction);
+/* Scenes whose lists were all rebuilt for the last step (the rest reused
+ * theirs, particle by particle as above). */
+int32_t xdem_batch_list_rebuilds(xdem_batch* b, uint32_t* scenes_rebuilding);
+/* The re-sort between steps: with at most 64 particles changed cell (on, the
+ * default) only the windows of the sorted order those particles shift are
+ * rewritten, in one kernel; off: the full merge every step.  The results are
+ * identical. */
+int32_t xdem_batch_set_sort_patch(xdem_batch* b, int32_t on);
 int32_t xdem_batch_destroy(xdem_batch* b);
 /* Add a scene (host arrays, reference layout: pos4 = x,y,z,r; vel3/omega3 may be
  * NULL for zero).  Takes effect at the next xdem_batch_step.  Fails with

@@ -13,7 +13,7 @@ import numpy as np
 
 from . import abi
 
-PHASES = ["pos_hash", "contact_verlet", "sort_movers", "sort_order", "sort_merge", "cell_ranges"]
+PHASES = ["pos_hash", "contact_verlet", "sort_patch", "sort_movers", "sort_merge", "cell_ranges"]
 
 
 class BatchSceneDesc(C.Structure):
@@ -52,6 +52,9 @@ def declare(lib):
         "xdem_batch_set_block": ([vp, u32], i32),
         "xdem_batch_set_launch": ([vp, u32, u32, i32], i32),
         "xdem_batch_set_walk": ([vp, i32], i32),
+        "xdem_batch_set_lists": ([vp, i32, C.c_float], i32),
+        "xdem_batch_list_rebuilds": ([vp, C.POINTER(u32)], i32),
+        "xdem_batch_set_sort_patch": ([vp, i32], i32),
         "xdem_batch_synchronize": ([vp], i32),
         "xdem_batch_autotune": ([vp, u32, C.POINTER(u32), vp], i32),
         "xdem_batch_get_state": ([vp, u32, vp, vp, vp, vp, vp], i32),
@@ -190,13 +193,29 @@ class Batch:
     def walk(self, per_lane=True):
         abi.check(self.lib, self.lib.xdem_batch_set_walk(self.h, int(bool(per_lane))))
 
+    def lists(self, on=True, skin_fraction=0.0):
+        """Candidate lists in the contact kernel (plane-only batches); skin as a
+        fraction of the cell size (0: keep the current one, 0.05 initially)."""
+        abi.check(self.lib, self.lib.xdem_batch_set_lists(self.h, int(bool(on)),
+                                                          float(skin_fraction)))
+
+    def list_rebuilds(self):
+        """Scenes whose candidate lists were all rebuilt for the last step."""
+        m = C.c_uint32()
+        abi.check(self.lib, self.lib.xdem_batch_list_rebuilds(self.h, C.byref(m)))
+        return m.value
+
+    def sort_patch(self, on=True):
+        abi.check(self.lib, self.lib.xdem_batch_set_sort_patch(self.h, int(bool(on))))
+
     def autotune(self, steps_per_trial=10):
         """Pick the contact-kernel launch shape by timing (advances the scenes)."""
         chosen = C.c_uint32()
-        ms = np.zeros(36)
+        ms = np.zeros(54)
         abi.check(self.lib, self.lib.xdem_batch_autotune(self.h, steps_per_trial, C.byref(chosen),
                                                          abi.ptr(ms)))
-        keys = [f"{w}-block{b}-unroll{u}{'-ahead' if p else ''}" for w in ("rows", "lane")
+        keys = [f"{w}-block{b}-unroll{u}{'-ahead' if p else ''}"
+                for w in ("rows", "lane", "lane-lists")
                 for b in (64, 128, 256) for u in (1, 2, 4) for p in (0, 1)]
         return chosen.value, dict(zip(keys, ms.tolist()))
 

@@ -32,6 +32,11 @@ static_assert(WALL_KEY == XDEM_BATCH_WALL_KEY, "wall key");
 
 namespace {
 
+// Wave slots of the displacement table: the sorted block table has at most
+// n / T + scenes blocks of T threads (each scene's last block may be partial),
+// so at most n / 64 + 4 scenes waves.
+inline size_t wdisp_words(uint64_t n, uint64_t nscene) { return n / 64 + 4 * nscene + 4; }
+
 // A triangle-mesh object of a scene: its triangles in the object frame, sorted
 // by local cell, and the local cell table (offsets filled at layout time).
 struct HostObject {
@@ -105,20 +110,40 @@ struct xdem_batch {
     hipStream_t st = nullptr;
     BState buf[2];
     int cur = 0;                      // buffer holding the state
-    uint32_t* keynew = nullptr;
+    // keys: skey[skcur] = sorted keys of the last sort (storage order of the
+    // state); the next keys are written into skey[1 - skcur] by k_step / k_pos
+    // and sorted there in place
     uint32_t* skey[2] = {nullptr, nullptr};
-    int skcur = 0;                    // skey buffer of the last sort (storage order)
-    uint32_t* perm = nullptr;
+    int skcur = 0;
+    uint32_t* perm = nullptr;         // sorted position -> storage index
+    uint32_t* invperm = nullptr;      // storage index -> sorted position
     uint2* crange = nullptr;
     // sort scratch
     uint64_t* moverbits = nullptr;
     uint32_t* blockcnt = nullptr;
-    uint32_t* counters = nullptr;     // [0] mcount, [1] msorted
+    uint32_t* counters = nullptr;     // [0],[1] movers by key parity, [2] movers of the last
+                                      // sort, [3] patched (the window patch placed them)
+    uint32_t* win = nullptr;          // windows of the last two sorts (k_patch)
+    uint2* mold = nullptr;            // per mover: old cell size, scene
     uint64_t* mkey = nullptr;
     uint32_t* ms = nullptr;
     uint64_t* mkey2 = nullptr;
     uint32_t* ms2 = nullptr;
     uint32_t* mplo = nullptr;         // 2 n: insertion points, then radix scratch
+    // candidate lists (plane-only scenes)
+    bool lists_on = false;
+    bool patch_on = true;             // the window patch for few movers (else the full merge)
+    float skin_frac = 0.05f;          // skin = skin_frac x cell size
+    uint32_t* vl = nullptr;           // VLK entries per particle, lane-interleaved
+    uint32_t* vc = nullptr;           // entries per particle
+    uint64_t* dirty = nullptr;        // bit per cell key
+    size_t dirty_words = 0;
+    uint32_t* dprev = nullptr;        // marked neighbourhoods of the last two sorts
+    float* disp = nullptr;            // per scene (cap_scenes each)
+    uint32_t* wdisp = nullptr;        // per wave of k_step: its largest displacement
+    uint2* d_scene_blk = nullptr;     // per scene: first and end block of the sorted table
+    uint32_t* force = nullptr;
+    uint32_t* rebuild = nullptr;
     // tables
     BScene* d_scenes = nullptr;
     BPlane* d_planes = nullptr;
@@ -181,7 +206,10 @@ struct xdem_batch {
         free_state(buf[0]);
         free_state(buf[1]);
         if (d_blocks_store == d_blocks_sorted) d_blocks_store = nullptr;
-        for (void* p : {(void*)keynew, (void*)skey[0], (void*)skey[1], (void*)perm, (void*)crange,
+        for (void* p : {(void*)invperm, (void*)win, (void*)mold, (void*)vl, (void*)vc, (void*)dirty,
+                        (void*)dprev, (void*)disp, (void*)wdisp, (void*)d_scene_blk, (void*)force, (void*)rebuild})
+            if (p) (void)hipFree(p);
+        for (void* p : {(void*)skey[0], (void*)skey[1], (void*)perm, (void*)crange,
                         (void*)moverbits, (void*)blockcnt, (void*)counters, (void*)mkey, (void*)ms,
                         (void*)mkey2, (void*)ms2, (void*)mplo, (void*)d_scenes, (void*)d_planes,
                         (void*)d_blocks_sorted, (void*)d_blocks_store, (void*)d_red, (void*)dbg_f,
@@ -215,13 +243,18 @@ struct xdem_batch {
         return b;
     }
 
+    // where k_step / k_pos write the keys of the positions they compute
+    uint32_t* knext() const { return skey[1 - skcur]; }
+
     // Where k_step / k_pos record the movers of the keys they compute, against
-    // the keys of the last sort (on = 0 when a full sort follows instead).
+    // the keys of the last sort (on = 0 when a full sort follows instead); the
+    // mover counter alternates with the key buffer.
     MoverOut mover_out(bool on) const {
         MoverOut m{};
         m.key_old = skey[skcur];
         m.crange = crange;
-        m.mcount = counters;
+        m.mcount = counters + (1 - skcur);
+        m.mold = mold;
         m.mkey = mkey;
         m.ms = ms;
         m.mplo = mplo;
@@ -233,29 +266,63 @@ struct xdem_batch {
         return m;
     }
 
-    SortArgs sort_args(const uint32_t* key_old, const uint32_t* key_new, const uint2* zi,
-                       uint32_t* skey_out) {
+    SortArgs sort_args(const uint2* zi) {
         SortArgs a{};
         a.n = n_total;
         a.ncell_cap = (uint32_t)cap_cells;
-        a.key_old = key_old;
-        a.key_new = key_new;
+        a.key_old = skey[skcur];
         a.zi = zi;
+        a.patched = counters + 3;
         a.crange = crange;
         a.moverbits = moverbits;
         a.blockcnt = blockcnt;
-        a.mcount = counters;
-        a.msorted = counters + 1;
+        a.mcount = counters + (1 - skcur);
+        a.msorted = counters + 2;
         a.mkey = mkey;
         a.ms = ms;
         a.mkey2 = mkey2;
         a.ms2 = ms2;
         a.mplo = mplo;
         a.perm = perm;
-        a.skey = skey_out;
+        a.invperm = invperm;
+        a.skey = skey[1 - skcur];
         a.nblk = nblk_rows();
         a.key_bits = key_bits();
         a.id_bits = id_bits();
+        return a;
+    }
+
+    PatchArgs patch_args(const uint2* zi) {
+        PatchArgs a{};
+        a.n = n_total;
+        a.ncell_cap = (uint32_t)cap_cells;
+        a.par = 1u - (uint32_t)skcur;
+        a.patch_on = patch_on ? 1u : 0u;
+        a.key_old = skey[skcur];
+        a.skey = skey[1 - skcur];
+        a.zi = zi;
+        a.crange = crange;
+        a.perm = perm;
+        a.invperm = invperm;
+        a.counters = counters;
+        a.mkey = mkey;
+        a.ms = ms;
+        a.mplo = mplo;
+        a.mold = mold;
+        a.moverbits = moverbits;
+        a.blockcnt = blockcnt;
+        a.win = win;
+        a.lists = vl ? 1u : 0u;
+        a.dirty = dirty;
+        a.dprev = dprev;
+        a.scenes = d_scenes;
+        a.nscenes = (uint32_t)scenes.size();
+        a.disp = disp;
+        a.wdisp = wdisp;
+        a.scene_blk = d_scene_blk;
+        a.wpb = block_threads / 64;
+        a.force = force;
+        a.rebuild = rebuild;
         return a;
     }
 };
@@ -282,7 +349,7 @@ int32_t alloc_cells(xdem_batch* b, uint32_t ncell, uint32_t* base) {
     return XDEM_OK;
 }
 
-BScene make_bscene(const HostScene& h, uint32_t plane0) {
+BScene make_bscene(const HostScene& h, uint32_t plane0, float skin_frac = 0.05f) {
     const xdem_batch_scene_desc& d = h.desc;
     BScene s{};
     s.wox = d.world_origin.x;
@@ -313,6 +380,7 @@ BScene make_bscene(const HostScene& h, uint32_t plane0) {
     s.rmax = h.rmax;
     s.pf_range = 2.0 * d.cell_size * std::max(d.grid_size.x, std::max(d.grid_size.y, d.grid_size.z));
     s.pf_eps = (float)std::ldexp(s.pf_range, -20);
+    s.skin = (float)(skin_frac * d.cell_size);
     return s;
 }
 
@@ -411,14 +479,15 @@ int32_t apply_layout(xdem_batch* b) {
         for (HostScene* s : old) {
             XDEM_TRY(copy_segment(A, B, s->p0, at, s->n, st));
             XDEM_TRY(copy_hist(b, b->cur, 1 - b->cur, s->p0, at, s->n));
-            XDEM_HIP(hipMemcpyAsync(b->keynew + at, b->skey[b->skcur] + s->p0, 4ull * s->n,
+            XDEM_HIP(hipMemcpyAsync(b->knext() + at, b->skey[b->skcur] + s->p0, 4ull * s->n,
                                     hipMemcpyDeviceToDevice, st));
             s->p0 = at;
             at += s->n;
         }
         XDEM_TRY(copy_segment(B, A, 0, 0, n_old, st));
         XDEM_TRY(copy_hist(b, 1 - b->cur, b->cur, 0, 0, n_old));
-        XDEM_HIP(hipMemcpyAsync(b->skey[b->skcur], b->keynew, 4ull * n_old, hipMemcpyDeviceToDevice, st));
+        XDEM_HIP(hipMemcpyAsync(b->skey[b->skcur], b->knext(), 4ull * n_old, hipMemcpyDeviceToDevice,
+                                st));
         // the sorted positions of the remaining scenes moved: rebuild their cell ranges
         for (HostScene* s : old) {
             hipLaunchKernelGGL(k_fill_u2, dim3(std::min<uint32_t>(4096, div_up(s->ncell, 256))),
@@ -429,7 +498,7 @@ int32_t apply_layout(xdem_batch* b) {
         if (n_old) {
             hipLaunchKernelGGL(k_ranges, dim3(div_up(n_old, ROWS_T)), dim3(ROWS_T), 0, st,
                                b->skey[b->skcur], b->crange, n_old, (uint32_t)b->cap_cells,
-                               (uint64_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr);
+                               (uint64_t*)nullptr, (uint32_t*)nullptr, (const uint32_t*)nullptr);
             XDEM_LAUNCHED();
         }
     }
@@ -538,7 +607,7 @@ int32_t apply_layout(xdem_batch* b) {
     std::vector<uint2> hcl;
     std::vector<BPose> hpose;
     for (HostScene* s : all) {
-        hs.push_back(make_bscene(*s, (uint32_t)hp.size()));
+        hs.push_back(make_bscene(*s, (uint32_t)hp.size(), b->skin_frac));
         float ckn, cvn, sr, mu;
         pair_consts(s->desc.particle, s->desc.wall.wall, s->desc.wall, ckn, cvn, sr, mu);
         for (auto& pl : s->planes) hp.push_back(BPlane{pl, ckn, cvn, sr, mu});
@@ -586,16 +655,19 @@ int32_t apply_layout(xdem_batch* b) {
     // reaction tables: per body (every mesh object, then every drum) the block
     // range of its scene (blocks are contiguous per scene in the sorted layout)
     // and its slot in the scene (objects 0.., drums after them)
+    std::vector<uint2> range(hs.size(), make_uint2(0, 0));
+    std::vector<bool> seen(hs.size(), false);
+    for (uint32_t L = 0; L < bs.size(); ++L) {
+        const uint32_t k = bs[L].scene;
+        if (!seen[k]) range[k].x = L;
+        seen[k] = true;
+        range[k].y = L + 1;
+    }
+    XDEM_REQUIRE((uint64_t)bs.size() * (b->block_threads / 64) <= wdisp_words(b->cap_np, b->cap_scenes),
+                 "internal: %zu blocks exceed the wave table", bs.size());
+    XDEM_TRY(upload_vec(&b->d_scene_blk, range, st));
     b->react_valid = false;
     if (b->nobj_total || b->ndrum_total) {
-        std::vector<uint2> range(hs.size(), make_uint2(0, 0));
-        std::vector<bool> seen(hs.size(), false);
-        for (uint32_t L = 0; L < bs.size(); ++L) {
-            const uint32_t k = bs[L].scene;
-            if (!seen[k]) range[k].x = L;
-            seen[k] = true;
-            range[k].y = L + 1;
-        }
         std::vector<uint4> robj;
         uint32_t slots = 0;
         for (uint32_t k = 0; k < hs.size(); ++k) {
@@ -624,6 +696,10 @@ int32_t apply_layout(xdem_batch* b) {
             b->react_cap = nbody * 6;
         }
     }
+    // the storage moved: the sort and list state starts over
+    if (b->dirty) XDEM_HIP(hipMemsetAsync(b->dirty, 0, 8 * b->dirty_words, st));
+    hipLaunchKernelGGL(k_sort_reset, dim3(1), dim3(256), 0, st, b->patch_args(b->buf[b->cur].zi));
+    XDEM_LAUNCHED();
     XDEM_HIP(hipStreamSynchronize(st));
     b->layout_dirty = false;
     return XDEM_OK;
@@ -636,11 +712,11 @@ int32_t launch_pos(xdem_batch* b, bool adv, const BlockEntry* blocks, uint32_t n
     BState& A = b->buf[b->cur];
     const MoverOut mv = b->mover_out(movers);
     if (adv)
-        hipLaunchKernelGGL(k_pos<true>, dim3(nblk), dim3(b->block_threads), 0, b->st, A, b->keynew,
-                           b->d_scenes, blocks, mv);
+        hipLaunchKernelGGL(k_pos<true>, dim3(nblk), dim3(b->block_threads), 0, b->st, A, b->knext(),
+                           b->d_scenes, blocks, mv, b->force);
     else
-        hipLaunchKernelGGL(k_pos<false>, dim3(nblk), dim3(b->block_threads), 0, b->st, A, b->keynew,
-                           b->d_scenes, blocks, mv);
+        hipLaunchKernelGGL(k_pos<false>, dim3(nblk), dim3(b->block_threads), 0, b->st, A,
+                           b->knext(), b->d_scenes, blocks, mv, b->force);
     XDEM_LAUNCHED();
     return XDEM_OK;
 }
@@ -653,20 +729,27 @@ int32_t run_sort(xdem_batch* b) {
     if (!n) return XDEM_OK;
     hipStream_t st = b->st;
     PhaseTimer& T = b->timer;
-    const uint32_t* kold = b->skey[b->skcur];
-    uint32_t* knew_sorted = b->skey[1 - b->skcur];
-    SortArgs a = b->sort_args(kold, b->keynew, b->buf[b->cur].zi, knew_sorted);
+    uint32_t* knew_sorted = b->knext();
+    const uint2* zi = b->buf[b->cur].zi;
+    SortArgs a = b->sort_args(zi);
+    // at most FEW movers: k_patch re-places them and the three kernels after it
+    // exit; otherwise k_patch only does the bookkeeping
+    XDEM_TRY(T.begin(2, st));
+    hipLaunchKernelGGL(k_patch, dim3(PATCH_GRID), dim3(256), 0, st, b->patch_args(zi));
+    XDEM_LAUNCHED();
+    XDEM_TRY(T.end(st));
     XDEM_TRY(T.begin(3, st));
     hipLaunchKernelGGL(k_sort_movers, dim3(1), dim3(1024), 0, st, a);
     XDEM_LAUNCHED();
     XDEM_TRY(T.end(st));
     XDEM_TRY(T.begin(4, st));
-    hipLaunchKernelGGL(k_merge, dim3(a.nblk + 64), dim3(ROWS_T), 0, st, a);
+    hipLaunchKernelGGL(k_merge, dim3(SWEEP_GRID + 64), dim3(ROWS_T), 0, st, a);
     XDEM_LAUNCHED();
     XDEM_TRY(T.end(st));
     XDEM_TRY(T.begin(5, st));
-    hipLaunchKernelGGL(k_ranges, dim3(a.nblk), dim3(ROWS_T), 0, st, knew_sorted, b->crange, n,
-                       (uint32_t)b->cap_cells, b->moverbits, b->blockcnt, b->counters);
+    hipLaunchKernelGGL(k_ranges, dim3(std::min<uint32_t>(SWEEP_GRID, a.nblk)), dim3(ROWS_T), 0, st,
+                       knew_sorted, b->crange, n, (uint32_t)b->cap_cells, b->moverbits,
+                       b->blockcnt, (const uint32_t*)(b->counters + 3));
     XDEM_LAUNCHED();
     XDEM_TRY(T.end(st));
     b->skcur = 1 - b->skcur;
@@ -678,14 +761,15 @@ int32_t run_sort(xdem_batch* b) {
 int32_t run_full_sort(xdem_batch* b) {
     const uint32_t n = b->n_total;
     hipStream_t st = b->st;
-    uint32_t* out = b->skey[1 - b->skcur];
-    XDEM_HIP(hipMemcpyAsync(out, b->keynew, 4ull * n, hipMemcpyDeviceToDevice, st));
+    uint32_t* out = b->knext();  // the keys k_pos wrote, sorted in place
     hipLaunchKernelGGL(k_iota, dim3(div_up(n, 256)), dim3(256), 0, st, b->perm, n);
     XDEM_LAUNCHED();
     XDEM_HIP(hipStreamSynchronize(st));  // sort_pairs may grow the library workspace
     XDEM_TRY(sort_pairs(out, b->perm, n, b->key_bits(), st));
     hipLaunchKernelGGL(k_ranges, dim3(div_up(n, ROWS_T)), dim3(ROWS_T), 0, st, out, b->crange, n,
-                       (uint32_t)b->cap_cells, b->moverbits, b->blockcnt, b->counters);
+                       (uint32_t)b->cap_cells, b->moverbits, b->blockcnt, (const uint32_t*)nullptr);
+    XDEM_LAUNCHED();
+    hipLaunchKernelGGL(k_invert, dim3(div_up(n, 256)), dim3(256), 0, st, b->perm, b->invperm, n);
     XDEM_LAUNCHED();
     b->skcur = 1 - b->skcur;
     return XDEM_OK;
@@ -697,8 +781,15 @@ StepArgs step_args(xdem_batch* b, bool dbg) {
     a.out = b->buf[1 - b->cur];
     a.perm = b->perm;
     a.crange = b->crange;
-    a.keynew = b->keynew;
+    a.keynew = b->knext();
     a.mv = b->mover_out(true);
+    a.vl = b->vl;
+    a.vc = b->vc;
+    a.vl_stride = b->cap_np;
+    a.invperm = b->invperm;
+    a.rebuild = b->rebuild;
+    a.dirty = b->dirty;
+    a.wdisp = b->wdisp;
     a.scenes = b->d_scenes;
     a.planes = b->d_planes;
     a.drums = b->d_drums;
@@ -772,6 +863,20 @@ int32_t launch_step(xdem_batch* b, bool adv) {
             XDEM_STEP_LAUNCH(4, false, true, false);
         else
             XDEM_STEP_LAUNCH(2, false, true, false);
+    } else if (adv && b->flat && b->lists_on && b->vl) {
+        // candidate lists (per-lane walk where a list is rebuilt)
+#define XDEM_VLIST_LAUNCH(U, P) \
+    hipLaunchKernelGGL((k_step<true, false, U, P, false, false, true, true>), g, t, 0, b->st, a)
+        const int v = b->unroll * 2 + (b->pipe ? 1 : 0);
+        switch (v) {
+            case 2: XDEM_VLIST_LAUNCH(1, false); break;
+            case 3: XDEM_VLIST_LAUNCH(1, true); break;
+            case 5: XDEM_VLIST_LAUNCH(2, true); break;
+            case 8: XDEM_VLIST_LAUNCH(4, false); break;
+            case 9: XDEM_VLIST_LAUNCH(4, true); break;
+            default: XDEM_VLIST_LAUNCH(2, false); break;
+        }
+#undef XDEM_VLIST_LAUNCH
     } else {
         const int v = b->unroll * 2 + (b->pipe ? 1 : 0);
         switch (v) {
@@ -813,19 +918,25 @@ namespace {
 // Device bytes of a batch: the arrays xdem_batch_create allocates (keep in
 // step with it), the per-scene tables and block tables at their largest
 // (64-thread blocks), and the history buffers ensure_history adds.
-uint64_t batch_device_bytes(uint64_t n, uint64_t nscene, uint64_t ncell, bool history) {
+uint64_t batch_device_bytes(uint64_t n, uint64_t nscene, uint64_t ncell, bool history,
+                            bool lists) {
     const uint64_t nblk = div_up((uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFull), ROWS_T);
     uint64_t b = 0;
     b += 2 * n * (sizeof(double4) + 3 * sizeof(float4) + sizeof(uint2) + sizeof(float4));  // buf[2]
-    b += 4 * n * sizeof(uint32_t);                        // keynew, skey[2], perm
+    b += 4 * n * sizeof(uint32_t);                        // skey[2], perm, invperm
     b += ncell * sizeof(uint2);                           // crange
     b += (nblk * (ROWS_T / 64) + 1) * sizeof(uint64_t);   // moverbits
-    b += (nblk + 1) * sizeof(uint32_t) + 4 * sizeof(uint32_t);  // blockcnt, counters
-    b += n * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t) + 2 * sizeof(uint32_t));  // movers
+    b += (nblk + 1) * sizeof(uint32_t) + 8 * sizeof(uint32_t);  // blockcnt, counters
+    b += n * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t) + 2 * sizeof(uint32_t) +
+              sizeof(uint2));                             // movers
+    b += (2 * WIN_WORDS + 2 * DPREV_WORDS) * sizeof(uint32_t);  // windows, marked cells
+    b += 5 * nscene * sizeof(uint32_t);                   // disp, force, rebuild, scene blocks
+    b += wdisp_words(n, nscene) * sizeof(uint32_t);       // wave displacements
     b += 1024 * sizeof(double);                           // d_red
     b += nscene * sizeof(BScene) + 2 * div_up((uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFull), 64) *
                                        sizeof(BlockEntry);
     if (history) b += 2 * n * (HSLOTS * sizeof(float4) + sizeof(uint32_t)) + sizeof(uint32_t);
+    if (lists) b += n * (VLK + 1) * sizeof(uint32_t) + (ncell / 64 + 1) * sizeof(uint64_t);
     return b;
 }
 }  // namespace
@@ -833,8 +944,51 @@ uint64_t batch_device_bytes(uint64_t n, uint64_t nscene, uint64_t ncell, bool hi
 extern "C" {
 
 uint64_t xdem_batch_device_bytes(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
-                                 int32_t tangential_history) {
-    return batch_device_bytes(max_particles, max_scenes, max_cells, tangential_history != 0);
+                                 int32_t options) {
+    return batch_device_bytes(max_particles, max_scenes, max_cells,
+                              (options & XDEM_BATCH_FOOTPRINT_HISTORY) != 0,
+                              (options & XDEM_BATCH_FOOTPRINT_LISTS) != 0);
+}
+
+int32_t xdem_batch_set_sort_patch(xdem_batch* b, int32_t on) {
+    XDEM_REQUIRE(b, "null batch");
+    if ((on != 0) != b->patch_on) {
+        b->patch_on = on != 0;
+        b->drop_graphs();
+    }
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_list_rebuilds(xdem_batch* b, uint32_t* scenes_rebuilding) {
+    XDEM_REQUIRE(b && scenes_rebuilding, "null argument");
+    XDEM_HIP(hipStreamSynchronize(b->st));
+    // the count of the last k_patch (its parity is the current key buffer's)
+    XDEM_HIP(hipMemcpy(scenes_rebuilding, b->counters + 4 + b->skcur, 4, hipMemcpyDeviceToHost));
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_set_lists(xdem_batch* b, int32_t on, float skin_fraction) {
+    XDEM_REQUIRE(b, "null batch");
+    XDEM_REQUIRE(skin_fraction >= 0.0f && skin_fraction < 1.0f,
+                 "skin fraction %g outside [0, 1)", (double)skin_fraction);
+    if (on && !b->vl) {
+        XDEM_TRY(dalloc(&b->vl, (size_t)VLK * b->cap_np));
+        XDEM_TRY(dalloc(&b->vc, (size_t)b->cap_np));
+        b->dirty_words = b->cap_cells / 64 + 1;
+        XDEM_TRY(dalloc(&b->dirty, b->dirty_words));
+        XDEM_HIP(hipMemsetAsync(b->dirty, 0, 8 * b->dirty_words, b->st));
+        XDEM_HIP(hipStreamSynchronize(b->st));
+        b->layout_dirty = true;  // the sort state restarts with the lists
+    }
+    if (skin_fraction > 0.0f && skin_fraction != b->skin_frac) {
+        b->skin_frac = skin_fraction;
+        b->layout_dirty = true;  // scene tables carry the skin
+    }
+    if ((on != 0) != b->lists_on) {
+        b->lists_on = on != 0;
+        b->drop_graphs();
+    }
+    return XDEM_OK;
 }
 
 int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t max_cells,
@@ -860,16 +1014,26 @@ int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t 
     size_t n = max_particles;
     uint32_t nblk = div_up(max_particles, ROWS_T);
     if ((rc = alloc_state(b->buf[0], n)) || (rc = alloc_state(b->buf[1], n)) ||
-        (rc = dalloc(&b->keynew, n)) || (rc = dalloc(&b->skey[0], n)) ||
-        (rc = dalloc(&b->skey[1], n)) || (rc = dalloc(&b->perm, n)) ||
+        (rc = dalloc(&b->skey[0], n)) || (rc = dalloc(&b->skey[1], n)) ||
+        (rc = dalloc(&b->perm, n)) || (rc = dalloc(&b->invperm, n)) ||
         (rc = dalloc(&b->crange, (size_t)max_cells)) ||
         (rc = dalloc(&b->moverbits, (size_t)nblk * (ROWS_T / 64) + 1)) ||
-        (rc = dalloc(&b->blockcnt, (size_t)nblk + 1)) || (rc = dalloc(&b->counters, 4)) ||
+        (rc = dalloc(&b->blockcnt, (size_t)nblk + 1)) || (rc = dalloc(&b->counters, 8)) ||
+        (rc = dalloc(&b->win, 2 * WIN_WORDS)) || (rc = dalloc(&b->mold, n)) ||
         (rc = dalloc(&b->mkey, n)) || (rc = dalloc(&b->ms, n)) || (rc = dalloc(&b->mkey2, n)) ||
         (rc = dalloc(&b->ms2, n)) || (rc = dalloc(&b->mplo, 2 * n)) ||
-        (rc = dalloc(&b->d_red, 8 * 1024)))
+        (rc = dalloc(&b->dprev, 2 * DPREV_WORDS)) || (rc = dalloc(&b->disp, max_scenes)) ||
+        (rc = dalloc(&b->wdisp, wdisp_words(max_particles, max_scenes))) ||
+        (rc = dalloc(&b->force, max_scenes)) ||
+        (rc = dalloc(&b->rebuild, max_scenes)) || (rc = dalloc(&b->d_red, 8 * 1024)))
         return fail(rc);
-    if (hipMemset(b->counters, 0, 16) != hipSuccess ||
+    if (hipMemset(b->counters, 0, 32) != hipSuccess ||
+        hipMemset(b->win, 0, 8 * WIN_WORDS) != hipSuccess ||
+        hipMemset(b->dprev, 0, 8 * DPREV_WORDS) != hipSuccess ||
+        hipMemset(b->disp, 0, 4ull * max_scenes) != hipSuccess ||
+        hipMemset(b->wdisp, 0, 4 * wdisp_words(max_particles, max_scenes)) != hipSuccess ||
+        hipMemset(b->force, 0, 4ull * max_scenes) != hipSuccess ||
+        hipMemset(b->rebuild, 0, 4ull * max_scenes) != hipSuccess ||
         hipMemset(b->moverbits, 0, 8 * ((size_t)nblk * (ROWS_T / 64) + 1)) != hipSuccess ||
         hipMemset(b->blockcnt, 0, 4 * ((size_t)nblk + 1)) != hipSuccess)
         return fail(XDEM_ERR_HIP);
@@ -1074,11 +1238,15 @@ int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps) {
 }
 
 // Autotuner: times the contact kernel (HIP events) for each candidate launch
-// shape — block size 64/128/256 x candidates loaded together 1/2/4 x contact
-// records loaded one ahead or not — over `steps_per_trial` real steps (the
-// scenes advance) and keeps the fastest.  Returns the choice (block size +
-// 1000 * unroll + 10000 * pipelined) and the per-candidate kernel time per
-// launch (18 entries: block-major, then unroll, then pipelined).
+// shape — neighbour walk (row by row, per lane, per lane with candidate lists)
+// x block size 64/128/256 x candidates loaded together 1/2/4 x contact records
+// loaded one ahead or not — over `steps_per_trial` real steps (the scenes
+// advance) and keeps the fastest.  Returns the choice (block size + 1000 *
+// unroll + 10000 * pipelined + 100000 * per-lane + 1000000 * lists) and the
+// per-candidate kernel time per launch (54 entries: walk-major, then block,
+// unroll, pipelined; -1 where a shape does not apply).  The step time of the
+// candidate-list shapes includes their list rebuilds (the first timed step of
+// each trial rebuilds every list).
 int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* chosen,
                             double* ms_per_candidate) {
     XDEM_REQUIRE(b, "null batch");
@@ -1089,19 +1257,23 @@ int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* c
     double best = 1e300;
     uint32_t best_c = b->block_threads;
     int best_u = b->unroll;
-    bool best_p = b->pipe, best_f = b->flat;
-    for (int w = 0; w < 2; ++w)
+    bool best_p = b->pipe, best_f = b->flat, best_l = b->lists_on;
+    XDEM_TRY(apply_layout(b));
+    const bool lists_possible = !b->hist_active && !b->geom_active();
+    for (int w = 0; w < 3; ++w)
     for (int i = 0; i < 3; ++i)
         for (int u = 0; u < 3; ++u)
             for (int pp = 0; pp < 2; ++pp) {
                 const int slot = 18 * w + 6 * i + 2 * u + pp;
-                if (pp && (b->hist_active || b->geom_active())) {  // not a launch shape here
+                if ((pp && (b->hist_active || b->geom_active())) || (w == 2 && !lists_possible)) {
+                    // not a launch shape here
                     if (ms_per_candidate) ms_per_candidate[slot] = -1.0;
                     continue;
                 }
                 b->unroll = unr[u];
                 b->pipe = pp != 0;
-                b->flat = w == 1;
+                b->flat = w >= 1;
+                XDEM_TRY(xdem_batch_set_lists(b, w == 2, 0.0f));
                 b->drop_graphs();
                 XDEM_TRY(xdem_batch_set_block(b, cand[i]));
                 XDEM_TRY(xdem_batch_step(b, 1));  // applies the new tables
@@ -1117,19 +1289,21 @@ int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* c
                     best_c = cand[i];
                     best_u = unr[u];
                     best_p = pp != 0;
-                    best_f = w == 1;
+                    best_f = w >= 1;
+                    best_l = w == 2;
                 }
                 XDEM_TRY(xdem_batch_enable_timing(b, 0));
             }
     b->unroll = best_u;
     b->pipe = best_p;
     b->flat = best_f;
+    XDEM_TRY(xdem_batch_set_lists(b, best_l, 0.0f));
     b->drop_graphs();
     XDEM_TRY(xdem_batch_set_block(b, best_c));
     if (was_on) XDEM_TRY(xdem_batch_enable_timing(b, 1));
     if (chosen)
         *chosen = best_c + 1000u * (uint32_t)best_u + (best_p ? 10000u : 0u) +
-                  (best_f ? 100000u : 0u);
+                  (best_f ? 100000u : 0u) + (best_l ? 1000000u : 0u);
     return XDEM_OK;
 }
 
@@ -1301,7 +1475,7 @@ int32_t xdem_batch_phase_times(xdem_batch* b, double* ms6, uint32_t* count6) {
 int32_t xdem_batch_last_movers(xdem_batch* b, uint32_t* movers) {
     XDEM_REQUIRE(b && movers, "null argument");
     XDEM_HIP(hipStreamSynchronize(b->st));
-    XDEM_HIP(hipMemcpy(movers, b->counters + 1, 4, hipMemcpyDeviceToHost));
+    XDEM_HIP(hipMemcpy(movers, b->counters + 2, 4, hipMemcpyDeviceToHost));
     return XDEM_OK;
 }
 
@@ -1454,6 +1628,12 @@ int32_t xdem_batch_prepare(xdem_batch* b) {
     }
     b->cur = 1 - b->cur;
     hipLaunchKernelGGL(k_iota, dim3(div_up(b->n_total, 256)), dim3(256), 0, b->st, b->perm, b->n_total);
+    XDEM_LAUNCHED();
+    hipLaunchKernelGGL(k_iota, dim3(div_up(b->n_total, 256)), dim3(256), 0, b->st, b->invperm,
+                       b->n_total);
+    XDEM_LAUNCHED();
+    // storage moved: the window sets and the lists start over
+    hipLaunchKernelGGL(k_sort_reset, dim3(1), dim3(256), 0, b->st, b->patch_args(b->buf[b->cur].zi));
     XDEM_LAUNCHED();
     b->storage_sorted_all_new = false;
     if (b->d_blocks_store != b->d_blocks_sorted) {

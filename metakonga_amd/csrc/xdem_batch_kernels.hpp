@@ -29,6 +29,11 @@ constexpr int LDS_T = 256;                   // lane stride of the per-thread LD
 constexpr int SORT_LDS_MAX = 4096;           // movers sorted in LDS up to this count
 constexpr int MERGE_LDS = 2048;              // sorted movers staged in LDS by k_merge up to this count
 constexpr int ROWS_T = 256;                  // threads of the row kernels
+constexpr int FEW = 64;                      // movers re-placed by the window patch (k_patch)
+constexpr int PATCH_GRID = 512;              // blocks of k_patch (grid-stride over the windows)
+constexpr int SWEEP_GRID = 1024;             // blocks of the grid-stride sort kernels
+constexpr int VLK = 32;                      // candidate-list entries per particle
+constexpr uint32_t VL_BAD = 0xFFFFFFFFu;     // no usable candidate list (walk every step)
 
 // Scene constants (FP64 grid/time, FP32 material).
 struct BScene {
@@ -47,7 +52,7 @@ struct BScene {
     double rmax;              // largest radius of the scene (cell culling)
     double pf_range;          // |coordinate - world origin| bound of the FP32 copy
     float pf_eps;             // distance error bound of the FP32 copy (2^-20 pf_range)
-    uint32_t pad2;
+    float skin;               // candidate-list skin (distance beyond contact kept in a list)
     uint32_t drum0, ndrum;    // into the drum table (applied after the planes)
     uint32_t obj0, nobj;      // into the polygon-object and pose tables
 };
@@ -149,6 +154,7 @@ struct MoverOut {
     uint64_t* mkey;            // (key << 32 | id)
     uint32_t* ms;              // storage index
     uint32_t* mplo;            // insertion point
+    uint2* mold;               // (particles in the old cell, scene index)
     uint64_t* moverbits;
     uint32_t* blockcnt;
     uint32_t n, ncell_cap;
@@ -178,6 +184,18 @@ struct StepArgs {
     float4* hout;
     uint32_t* hcout;
     uint32_t* hoverflow;       // contacts found with every slot taken (not kept)
+    // candidate lists (VLIST variants): per particle up to VLK entries in the
+    // reference traversal order, lane-interleaved (entry k of storage index s
+    // at vl[k * vl_stride + s]); entries name storage indices of `in` when
+    // read and sorted positions (storage of `out`) when written
+    uint32_t* vl;
+    uint32_t* vc;              // entries per storage index; VL_BAD: none usable
+    uint32_t vl_stride;
+    const uint32_t* invperm;   // storage index of `in` -> sorted position
+    const uint32_t* rebuild;   // per scene: rebuild every list this step
+    const uint64_t* dirty;     // bit per cell key: a particle changed cell within one cell
+    uint32_t* wdisp;           // per wave (block L, wave w: L * waves per block + w): the
+                               // largest displacement of this step (float bits, bounded above)
     // parity probe outputs (per sorted position) — null in production
     float* dbg_f;              // 6 floats: force xyz, torque xyz
     uint32_t* dbg_cnt;
@@ -205,6 +223,20 @@ __device__ inline uint32_t key_of(const BScene& S, int cx, int cy, int cz) {
     uint32_t y = (uint32_t)cy & (S.gy - 1);
     uint32_t z = (uint32_t)cz & (S.gz - 1);
     return S.cell_base + (__umul24(__umul24(z, S.gy), S.gx) + __umul24(y, S.gx) + x);
+}
+
+// An upper bound of the largest u over the active lanes of the wave: the
+// largest value of the high 16 bits any lane has (found bit by bit with
+// ballots, so inactive lanes take no part), low bits set.  For non-negative
+// floats' bit patterns the result read as a float is >= every lane's value
+// (and a NaN, whose pattern exceeds every finite one, stays above them).
+__device__ inline uint32_t wave_max_hi16(uint32_t u) {
+    uint32_t cand = 0;
+    for (int b = 31; b >= 16; --b) {
+        const uint32_t t = cand | (1u << b);
+        if (__ballot(u >= t)) cand = t;
+    }
+    return cand | 0xFFFFu;
 }
 
 __device__ inline uint64_t lanemask_lt64() {
@@ -244,8 +276,10 @@ __device__ inline uint32_t insertion_point(const uint2* crange, const uint32_t* 
 
 // Called by every active lane of a wave with its storage index, new key and
 // particle id: one count reservation per wave that has movers.
-__device__ inline void record_mover(const MoverOut& O, uint32_t s, uint32_t nw, uint32_t id) {
-    const bool mover = O.key_old[s] != nw;
+__device__ inline void record_mover(const MoverOut& O, uint32_t s, uint32_t nw, uint32_t id,
+                                    uint32_t scene) {
+    const uint32_t ko = O.key_old[s];
+    const bool mover = ko != nw;
     const uint64_t word = __ballot(mover);
     if (!word) return;
     const uint32_t leader = (uint32_t)__ffsll((unsigned long long)word) - 1;
@@ -258,6 +292,12 @@ __device__ inline void record_mover(const MoverOut& O, uint32_t s, uint32_t nw, 
             O.mkey[p] = ((uint64_t)nw << 32) | id;
             O.ms[p] = s;
             O.mplo[p] = insertion_point(O.crange, O.key_old, O.n, O.ncell_cap, nw);
+            uint32_t size = 0;
+            if (ko < O.ncell_cap) {
+                const uint2 c = O.crange[ko];
+                size = c.x == EMPTY ? 0u : c.y - c.x;
+            }
+            O.mold[p] = make_uint2(size, scene);
         }
         atomicOr((unsigned long long*)&O.moverbits[s >> 6], 1ull << (s & 63));
         atomicAdd(&O.blockcnt[s / ROWS_T], 1u);
@@ -670,8 +710,11 @@ constexpr int step_waves(int unr, bool pipe, bool hist, bool geom) {
 
 // GEOM: the variant that also evaluates drum walls and triangle meshes (used
 // when some scene of the batch has them; the others carry no registers for it).
-template <bool ADV, bool DBG, int UNR, bool PIPE, bool HIST, bool GEOM = false, bool FLAT = true>
+template <bool ADV, bool DBG, int UNR, bool PIPE, bool HIST, bool GEOM = false, bool FLAT = true,
+          bool VLIST = false>
 __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step(StepArgs A) {
+    static_assert(!VLIST || (ADV && !DBG && !HIST && !GEOM && FLAT),
+                  "candidate lists: advancing plane-only per-lane-walk variants");
     // per-thread LDS lists, lane-contiguous (entry c of thread t at c * LDS_T + t)
     __shared__ uint32_t clist[LDS_T * CMAX];
     __shared__ uint32_t hkeys[HIST ? LDS_T * HSLOTS : 1];
@@ -700,7 +743,13 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     const BlockEntry be = A.blocks[L];
     const BScene S = A.scenes[be.scene];
     const uint32_t j = be.j0 + threadIdx.x;
-    if (j >= be.j1 || j >= A.n) return;
+    const uint32_t wslot = L * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (j >= be.j1 || j >= A.n) {
+        // a wave without particles reports no displacement (its slot is read)
+        if (VLIST && threadIdx.x % 64 == 0 && be.j0 + threadIdx.x >= min(be.j1, A.n))
+            A.wdisp[wslot] = 0;
+        return;
+    }
     uint32_t* mylist = clist + threadIdx.x;
     const uint32_t s = A.perm[j];
     if (s >= A.n) return;  // defensive: perm is a permutation of [0, n)
@@ -733,6 +782,22 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     o.r = (float)ip.w;
     int cx, cy, cz;
     cell_of(S, o.x, o.y, o.z, cx, cy, cz);
+    // Candidate lists (VLIST): a lane reuses its list (the candidates within
+    // ri + rj + skin when it was built, in the reference traversal order)
+    // unless the scene rebuilds every list this step (displacement bound past
+    // half the skin, or a block start), the particle moved in the sorted
+    // order (its list slot moves with it), a particle changed cell within one
+    // cell of its own (the traversal order around it may have changed, or a
+    // new candidate arrived), or it has no usable list.  Otherwise its
+    // contacts and their order are those of the walk.
+    bool rebuild = true;
+    uint32_t vcnt = 0;
+    if constexpr (VLIST) {
+        vcnt = A.vc[s];
+        const uint32_t kown = A.mv.key_old[j];
+        const bool dirty = (A.dirty[kown >> 6] >> (kown & 63)) & 1ull;
+        rebuild = A.rebuild[be.scene] != 0 || s != j || dirty || vcnt > (uint32_t)VLK;
+    }
     f3 Ft = F3(0, 0, 0), Fn = F3(0, 0, 0), M = F3(0, 0, 0);
     f3 sumF = F3(0, 0, 0), sumM = F3(0, 0, 0);
     uint32_t ntotal = 0;
@@ -745,7 +810,9 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     // the cell assignment) holds no contact of i and is not visited; skipping
     // it changes neither the contacts nor their order.
     const float csf = (float)S.cs;
-    const float reach2 = (float)((ri + S.rmax) * (ri + S.rmax)) * (1.0f + 1e-4f);
+    // a list build also keeps the candidates within the skin
+    const double reach = ri + S.rmax + (VLIST ? (double)S.skin : 0.0);
+    const float reach2 = (float)(reach * reach) * (1.0f + 1e-4f);
     const float lox = (float)(o.x - (S.wox + (double)cx * S.cs));
     const float loy = (float)(o.y - (S.woy + (double)cy * S.cs));
     const float loz = (float)(o.z - (S.woz + (double)cz * S.cs));
@@ -758,10 +825,19 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     // false); out-of-range copies (x NaN, so d2 is NaN) are never rejected.
     const float4 ownf = pf_of(S, o.x, o.y, o.z, o.r);
     const float sr0 = ownf.w + S.pf_eps;
-    auto may_touch = [&](const float4& f) {
+    auto screen_d2 = [&](const float4& f) {
         const float dx = f.x - ownf.x, dy = f.y - ownf.y, dz = f.z - ownf.z;
-        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    };
+    auto may_touch = [&](const float4& f) {
+        const float d2 = screen_d2(f);
         const float sr = sr0 + f.w;
+        return !(d2 >= sr * sr * (1.0f + 0x1p-19f));
+    };
+    // within ri + rj + skin (list builds; same widened FP32 bound)
+    auto in_skin = [&](const float4& f) {
+        const float d2 = screen_d2(f);
+        const float sr = sr0 + f.w + S.skin;
         return !(d2 >= sr * sr * (1.0f + 0x1p-19f));
     };
     // phase 1: candidates in the reference order (z, y, x cells; ascending j).
@@ -839,9 +915,11 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
         if constexpr (FLAT) {
             // jj..e1: the current row's remaining run; nb..ne: the next row's run
             uint32_t jj = 0, e1 = 0, nb = 0, ne = 0, nt = 0;
+            uint32_t lcnt = 0;  // list entries written (VLIST rebuild)
             // the 27 range loads are issued unconditionally (key_of wraps into
             // the scene's grid and 1 <= cx <= gx - 2, so every key is in range)
             // and the culled cells are masked afterwards
+            if (!VLIST || rebuild)
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
                 const int dy = (r % 3) - 1, dz = (r / 3) - 1;
@@ -868,6 +946,7 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
 #pragma unroll 1
             while (jj < e1) {
                 uint32_t t[UNR];
+                uint32_t qq[UNR];
                 float4 f[UNR];
                 const uint32_t nbase = nb - e1;  // slot k >= e1 maps to nb + (k - e1)
                 // idle slots (past both rows, or the thread itself) load the
@@ -879,6 +958,7 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
                     const uint32_t k = jj + u;
                     const uint32_t q = k < e1 ? k : nbase + k;
                     ok[u] = (k < e1 || q < ne) && q != j;  // ne <= n
+                    qq[u] = q;
                     t[u] = at32(A.perm, ok[u] ? q : j);
                 }
 #pragma unroll
@@ -914,9 +994,49 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
                         }
                     }
                 }
+                if constexpr (VLIST) {
+                    // the list keeps sorted positions: the storage indices of `out`
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u)
+                        if (ok[u] && in_skin(f[u])) {
+                            if (lcnt < (uint32_t)VLK) A.vl[(size_t)lcnt * A.vl_stride + j] = qq[u];
+                            ++lcnt;
+                        }
+                }
 #pragma unroll
                 for (int u = 0; u < UNR; ++u)
                     if (ok[u] && may_touch(f[u])) consider(t[u], at32(A.in.pr, t[u]));
+            }
+            if constexpr (VLIST) {
+                if (rebuild) {
+                    A.vc[j] = lcnt > (uint32_t)VLK ? VL_BAD : lcnt;
+                } else {
+                    // the list: entries are storage indices of `in`; each is
+                    // rewritten with its sorted position (its storage index in
+                    // `out`) where that differs, in place (slot s == j)
+                    constexpr int LU = 2;  // entries whose loads are issued together
+#pragma unroll 1
+                    for (uint32_t k = 0; k < vcnt; k += LU) {
+                        uint32_t e[LU], nx2[LU];
+                        float4 f[LU];
+#pragma unroll
+                        for (int u = 0; u < LU; ++u)
+                            e[u] = k + u < vcnt ? A.vl[(size_t)(k + u) * A.vl_stride + s] : s;
+#pragma unroll
+                        for (int u = 0; u < LU; ++u) {
+                            e[u] = min(e[u], A.n - 1);  // defensive: entries are < n
+                            f[u] = at32(A.in.pf, e[u]);
+                            nx2[u] = at32(A.invperm, e[u]);
+                        }
+#pragma unroll
+                        for (int u = 0; u < LU; ++u)
+                            if (k + u < vcnt) {
+                                if (nx2[u] != e[u])
+                                    A.vl[(size_t)(k + u) * A.vl_stride + s] = nx2[u];
+                                if (may_touch(f[u])) consider(e[u], at32(A.in.pr, e[u]));
+                            }
+                    }
+                }
             }
         } else {
             uint2 next = row_range(0);
@@ -946,6 +1066,7 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
         }
     } else {
         scan_cells();
+        if constexpr (VLIST) A.vc[j] = VL_BAD;  // next to the x wrap: no list, always walked
     }
     if (!EARLY) {
         vm = A.in.vm[s];
@@ -1029,7 +1150,15 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
         cell_of(S, nx, ny, nz, kx, ky, kz);
         const uint32_t kn = key_of(S, kx, ky, kz);
         A.keynew[j] = kn;
-        if (A.mv.on) record_mover(A.mv, j, kn, zi.y);
+        if (A.mv.on) record_mover(A.mv, j, kn, zi.y, be.scene);
+        if constexpr (VLIST) {
+            // the scene's largest displacement of the step (bounded above)
+            const float ddx = (float)(nx - o.x), ddy = (float)(ny - o.y), ddz = (float)(nz - o.z);
+            const float d = fsqrt(ddx * ddx + ddy * ddy + ddz * ddz) * 1.0001f;
+            const uint32_t m = wave_max_hi16(__float_as_uint(d));
+            if (__lane_id() == (uint32_t)__ffsll((unsigned long long)__ballot(1)) - 1)
+                A.wdisp[wslot] = m;
+        }
     }
     A.out.pr[j] = make_double4(nx, ny, nz, ip.w);
     A.out.pf[j] = pf_of(S, nx, ny, nz, o.r);
@@ -1118,10 +1247,13 @@ __global__ __launch_bounds__(256) void k_react_sum(const double* part, uint32_t 
 // in storage order (block start / parity prepare).
 template <bool ADV>
 __global__ __launch_bounds__(256) void k_pos(BState st, uint32_t* keynew, const BScene* scenes,
-                                             const BlockEntry* blocks, MoverOut mv) {
+                                             const BlockEntry* blocks, MoverOut mv,
+                                             uint32_t* force) {
     const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
     const BlockEntry be = blocks[L];
     const BScene S = scenes[be.scene];
+    // a step block starts: every candidate list of the scene is rebuilt
+    if (force && threadIdx.x == 0) force[be.scene] = 1;
     const uint32_t j = be.j0 + threadIdx.x;
     if (j >= be.j1) return;
     double4 p = st.pr[j];
@@ -1138,7 +1270,7 @@ __global__ __launch_bounds__(256) void k_pos(BState st, uint32_t* keynew, const 
     cell_of(S, x, y, z, cx, cy, cz);
     const uint32_t kn = key_of(S, cx, cy, cz);
     keynew[j] = kn;
-    if (mv.on) record_mover(mv, j, kn, st.zi[j].y);
+    if (mv.on) record_mover(mv, j, kn, st.zi[j].y, be.scene);
 }
 
 // ------------------------------------------------------------------ sort
@@ -1146,8 +1278,8 @@ struct SortArgs {
     uint32_t n;                 // particles (storage positions)
     uint32_t ncell_cap;         // size of crange
     const uint32_t* key_old;    // sorted keys of the previous sort, storage order
-    const uint32_t* key_new;    // keys of the current positions, storage order
     const uint2* zi;            // (alpha.z bits, particle id), storage order
+    const uint32_t* patched;    // 1: k_patch already placed the movers (the kernels below exit)
     uint2* crange;
     uint64_t* moverbits;        // one bit per storage position (set by record_mover)
     uint32_t* blockcnt;         // movers per ROWS_T block; scanned in place (+1 total)
@@ -1162,7 +1294,10 @@ struct SortArgs {
     uint64_t* mkey2;            // sorted movers (and radix ping-pong)
     uint32_t* ms2;              // sorted movers' append index
     uint32_t* perm;             // out: sorted position -> storage index
-    uint32_t* skey;             // out: sorted keys
+    uint32_t* invperm;          // out: storage index -> sorted position
+    uint32_t* skey;             // the keys of the current positions, storage order (written by
+                                // k_step / k_pos), sorted in place: stayers' keys are read from
+                                // key_old, movers' from their records, never from here
     uint32_t nblk;              // ROWS_T blocks over n
     uint32_t key_bits, id_bits;
 };
@@ -1236,6 +1371,7 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
     __shared__ uint32_t hist[256];
     uint32_t* wcnt = lv;  // 16 waves x 256 digits, only in the radix path (lv unused there)
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (*A.patched) return;
     const uint32_t M = min(*A.mcount, A.n);
     const bool few = M <= (uint32_t)MERGE_LDS;
     // exclusive scan of blockcnt[0..nblk) in place, total at [nblk]
@@ -1435,41 +1571,44 @@ __device__ inline uint32_t count_below(const uint32_t* v, uint32_t M, uint32_t x
     return lo;
 }
 
+// Grid: SWEEP_GRID blocks stepping over the ROWS_T tiles of stayers, then 64
+// blocks for the movers.
 __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
     __shared__ uint64_t smk[MERGE_LDS];
     __shared__ uint32_t sst[MERGE_LDS];
+    if (*A.patched) return;
     const uint32_t M = *A.msorted;
     const bool few = M <= (uint32_t)MERGE_LDS;
     const uint32_t* mst = A.mplo + A.n;  // sorted mover storage indices (few movers)
-    if (blockIdx.x < A.nblk) {
-        // the particle's own words first: their loads overlap the staging
-        const uint32_t s = blockIdx.x * ROWS_T + threadIdx.x;
-        const bool in = s < A.n;
-        const uint32_t kn = in ? A.key_new[s] : 0u;
-        const uint32_t ko = in ? A.key_old[s] : 0u;
-        const uint32_t id = in ? A.zi[s].y : 0u;
+    if (blockIdx.x < (uint32_t)SWEEP_GRID) {
         if (few)
             for (uint32_t i = threadIdx.x; i < M; i += ROWS_T) {
                 smk[i] = A.mkey2[i];
                 sst[i] = mst[i];
             }
         __syncthreads();
-        if (!in) return;
-        if (M && kn != ko) return;  // a mover
-        uint32_t out = s;
-        if (M) {
-            uint64_t X = ((uint64_t)kn << 32) | id;
-            uint32_t lo = movers_below(few ? smk : A.mkey2, M, X);
-            out = s - (few ? count_below(sst, M, s) : mover_prefix(A, s)) + lo;
-        }
-        if (out < A.n) {
-            A.perm[out] = s;
-            A.skey[out] = kn;
+        for (uint32_t tile = blockIdx.x; tile < A.nblk; tile += SWEEP_GRID) {
+            const uint32_t s = tile * ROWS_T + threadIdx.x;
+            if (s >= A.n) continue;
+            // a mover (its bit was set by record_mover) is placed by the mover blocks
+            if (M && ((A.moverbits[s >> 6] >> (s & 63)) & 1ull)) continue;
+            const uint32_t kn = A.key_old[s];  // a stayer keeps its key
+            uint32_t out = s;
+            if (M) {
+                const uint64_t X = ((uint64_t)kn << 32) | A.zi[s].y;
+                const uint32_t lo = movers_below(few ? smk : A.mkey2, M, X);
+                out = s - (few ? count_below(sst, M, s) : mover_prefix(A, s)) + lo;
+            }
+            if (out < A.n) {
+                A.perm[out] = s;
+                A.invperm[s] = out;
+                A.skey[out] = kn;
+            }
         }
         return;
     }
-    const uint32_t nmb = gridDim.x - A.nblk;
-    for (uint32_t r = (blockIdx.x - A.nblk) * ROWS_T + threadIdx.x; r < M; r += nmb * ROWS_T) {
+    const uint32_t nmb = gridDim.x - SWEEP_GRID;
+    for (uint32_t r = (blockIdx.x - SWEEP_GRID) * ROWS_T + threadIdx.x; r < M; r += nmb * ROWS_T) {
         const uint64_t X = A.mkey2[r];
         const uint32_t p = A.ms2[r];
         if (p >= A.n) continue;  // defensive: payloads are append indices < M
@@ -1481,6 +1620,7 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
         uint32_t out = r + (lo - (few ? count_below(mst, M, lo) : mover_prefix(A, lo)));
         if (out < A.n && s < A.n) {
             A.perm[out] = s;
+            A.invperm[s] = out;
             A.skey[out] = kn;
             const uint32_t old = A.key_old[s];
             if (old < A.ncell_cap) A.crange[old] = make_uint2(EMPTY, 0);
@@ -1490,24 +1630,386 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
 
 // Cell ranges of the sorted keys (reorderDataAndFindCellStart_kernel, :166-212).
 // Also clears the mover marks of the sort just finished (when given).
+// Grid-stride over the positions; exits when k_patch already updated the
+// ranges (patched non-null and set).
 __global__ __launch_bounds__(ROWS_T) void k_ranges(const uint32_t* skey, uint2* crange, uint32_t n,
                                                   uint32_t ncell_cap, uint64_t* moverbits,
-                                                  uint32_t* blockcnt, uint32_t* mcount) {
-    const uint32_t j = blockIdx.x * ROWS_T + threadIdx.x;
-    if (j >= n) return;
-    if (moverbits && (j & 63) == 0) moverbits[j >> 6] = 0;
-    if (blockcnt && j % ROWS_T == 0) blockcnt[j / ROWS_T] = 0;
-    if (j == 0) {
-        if (blockcnt) blockcnt[(n + ROWS_T - 1) / ROWS_T] = 0;
-        if (mcount) *mcount = 0;
+                                                  uint32_t* blockcnt, const uint32_t* patched) {
+    if (patched && *patched) return;
+    for (uint32_t j = blockIdx.x * ROWS_T + threadIdx.x; j < n; j += gridDim.x * ROWS_T) {
+        if (moverbits && (j & 63) == 0) moverbits[j >> 6] = 0;
+        if (blockcnt && j % ROWS_T == 0) blockcnt[j / ROWS_T] = 0;
+        if (j == 0 && blockcnt) blockcnt[(n + ROWS_T - 1) / ROWS_T] = 0;
+        const uint32_t h = skey[j];
+        if (h >= ncell_cap) continue;  // defensive: keys are < the cell capacity
+        if (j == 0 || skey[j - 1] != h) {
+            crange[h].x = j;
+            if (j > 0 && skey[j - 1] < ncell_cap) crange[skey[j - 1]].y = j;
+        }
+        if (j == n - 1) crange[h].y = n;
     }
-    uint32_t h = skey[j];
-    if (h >= ncell_cap) return;  // defensive: keys are < the cell capacity
-    if (j == 0 || skey[j - 1] != h) {
-        crange[h].x = j;
-        if (j > 0 && skey[j - 1] < ncell_cap) crange[skey[j - 1]].y = j;
+}
+
+// ------------------------------------------------------- window patch
+// The re-sort when at most FEW particles changed cell (the usual case between
+// two steps).  With storage in the previous sorted order, a mover at storage
+// src whose new (key, id) has insertion point lo in that order (the number of
+// previous entries below it) shifts exactly the entries between src and lo,
+// so the new order differs from the old one only inside the windows
+// [src, lo - 1] (moving up) or [lo, src] (moving down), merged where they
+// overlap or touch; every position outside them keeps its element, key and
+// cell boundaries.  Each block recomputes the movers' order and windows (a
+// wave-level sort of <= 64 records) and the grid strides over the window
+// positions: position j's new element is found by inverting the merge (the
+// mover placed at j, or the stayer of rank j - (movers placed before j) in the
+// old order), and the cell boundaries at window positions are rewritten from
+// the new keys.  Positions of the previous sort's windows outside the new
+// ones get the identity permutation back.  More movers: nothing is done here
+// and k_sort_movers / k_merge / k_ranges sort (patched = 0).
+constexpr int WIN_WORDS = 2 + 2 * FEW;   // per parity: count, then (first, last) pairs
+constexpr int DPREV_WORDS = 1 + 2 * FEW; // per parity: count, then (cell key, scene) pairs
+
+struct PatchArgs {
+    uint32_t n, ncell_cap, par;
+    uint32_t patch_on;         // 0: every sort takes the full merge
+    const uint32_t* key_old;   // sorted keys of the previous sort (storage order now)
+    uint32_t* skey;            // this sort's keys, storage order (k_step / k_pos), patched in place
+    const uint2* zi;           // particle ids, storage order
+    uint2* crange;
+    uint32_t* perm;
+    uint32_t* invperm;
+    uint32_t* counters;        // [0], [1] movers by key-buffer parity, [2] movers of the
+                               // sort, [3] patched, [4], [5] scenes rebuilding every list
+                               // by parity
+    const uint64_t* mkey;      // mover records, append order (record_mover)
+    const uint32_t* ms;
+    const uint32_t* mplo;
+    const uint2* mold;
+    uint64_t* moverbits;
+    uint32_t* blockcnt;
+    uint32_t* win;             // windows by parity (this sort: par; the previous: 1 - par)
+    // candidate lists (lists != 0)
+    uint32_t lists;
+    uint64_t* dirty;           // bit per cell key
+    uint32_t* dprev;           // marked neighbourhoods by parity
+    const BScene* scenes;
+    uint32_t nscenes;
+    float* disp;               // per scene: displacement bound since its lists were built
+    const uint32_t* wdisp;     // per wave of the last k_step: its largest displacement
+    const uint2* scene_blk;    // per scene: first and end block of the launch table
+    uint32_t wpb;              // waves per block of k_step
+    uint32_t* force;           // per scene: a block start asks for a rebuild
+    uint32_t* rebuild;         // per scene: out, rebuild every list in the next step
+};
+
+// The 27 cells around cell key k of scene S (the walk's wrap-around) get
+// their dirty bits set (set = true) or cleared.
+__device__ inline void mark_neighbourhood(const BScene& S, uint32_t k, uint32_t which,
+                                          uint64_t* dirty, bool set) {
+    const uint32_t local = k - S.cell_base;
+    const int x = (int)(local % S.gx), y = (int)((local / S.gx) % S.gy),
+              z = (int)(local / (S.gx * S.gy));
+    const int dz = (int)(which / 9) - 1, dy = (int)((which / 3) % 3) - 1, dx = (int)(which % 3) - 1;
+    const uint32_t kk = key_of(S, x + dx, y + dy, z + dz);
+    const uint64_t bit = 1ull << (kk & 63);
+    if (set)
+        atomicOr((unsigned long long*)&dirty[kk >> 6], (unsigned long long)bit);
+    else
+        atomicAnd((unsigned long long*)&dirty[kk >> 6], (unsigned long long)~bit);
+}
+
+__global__ __launch_bounds__(256) void k_patch(PatchArgs A) {
+    __shared__ uint64_t sX[FEW];      // movers by new (key, id)
+    __shared__ uint32_t sApp[FEW];    // their append index
+    __shared__ uint32_t sSrc[FEW];    // their storage index
+    __shared__ uint32_t sOut[FEW];    // their new position (increasing)
+    __shared__ uint32_t sAsc[FEW];    // the movers' storage indices, ascending
+    __shared__ uint32_t sWin[2 * FEW], sPrev[2 * FEW];
+    __shared__ uint32_t sAux[2 * FEW];  // blocks 0 and 1: per mover, scene / old key
+    __shared__ uint32_t sM, sNW, sNP;
+    __shared__ uint32_t sred[4];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t n = A.n;
+    uint32_t* const win_now = A.win + A.par * WIN_WORDS;
+    const uint32_t* const win_prev = A.win + (1 - A.par) * WIN_WORDS;
+    // ---- wave 0 of every block: the movers' order and the windows.  The
+    // mover count, the previous windows and the first 64 mover records are
+    // loaded together; the movers' storage indices follow their sort by
+    // shuffle.
+    if (w == 0) {
+        const uint32_t m0 = A.counters[A.par];
+        const uint32_t np0 = win_prev[0];
+        const uint32_t pa = win_prev[2 + 2 * lane], pb = win_prev[3 + 2 * lane];
+        const bool rec = lane < n;
+        const uint64_t mk = rec ? A.mkey[lane] : ~0ull;
+        const uint32_t msl = rec ? A.ms[lane] : 0u;
+        const uint32_t M = min(m0, n);
+        const bool few = M <= (uint32_t)FEW && A.patch_on;
+        if (lane == 0) sM = M;
+        if (few) {
+            uint64_t key = lane < M ? mk : ~0ull;
+            uint32_t val = lane;
+            bitonic_wave(key, val);
+            const uint32_t src = (uint32_t)__shfl((int)msl, (int)(val & 63u), 64);
+            sX[lane] = key;
+            sApp[lane] = val;
+            sSrc[lane] = src;
+            uint64_t sk = lane < M ? (uint64_t)msl : ~0ull;
+            uint32_t unused = 0;
+            bitonic_wave(sk, unused);
+            sAsc[lane] = (uint32_t)sk;
+            const uint32_t np = min(np0, (uint32_t)FEW);
+            if (lane < np) {
+                sPrev[2 * lane] = pa;
+                sPrev[2 * lane + 1] = pb;
+            }
+            if (lane == 0) sNP = np;
+            __builtin_amdgcn_wave_barrier();
+            // insertion point in the previous order (past the entries of the new
+            // cell with a smaller id), new position and window of mover `lane`
+            uint64_t win = ~0ull;
+            if (lane < M) {
+                const uint32_t kn = (uint32_t)(key >> 32), id = (uint32_t)key;
+                uint32_t lo = A.mplo[val];
+                while (lo < n && A.key_old[lo] == kn && A.zi[lo].y < id) ++lo;
+                uint32_t below = 0;
+                for (uint32_t r = 0; r < M; ++r) below += sAsc[r] < lo ? 1u : 0u;
+                sOut[lane] = lane + lo - below;
+                const uint32_t a = src < lo ? src : lo, b = src < lo ? lo - 1 : src;
+                win = ((uint64_t)a << 32) | b;
+            }
+            // windows sorted by first position, merged where they overlap or
+            // touch: a running maximum of the last positions decides where a
+            // merged window starts and ends
+            uint32_t unused2 = 0;
+            bitonic_wave(win, unused2);
+            const uint32_t a = (uint32_t)(win >> 32), b = (uint32_t)win;
+            uint32_t run = b;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t u = (uint32_t)__shfl_up((int)run, o, 64);
+                if (lane >= (uint32_t)o) run = max(run, u);
+            }
+            const uint32_t prev_run = (uint32_t)__shfl_up((int)run, 1, 64);
+            const bool valid = lane < M;
+            const bool start = valid && (lane == 0 || a > prev_run + 1);
+            const uint64_t starts = __ballot(start);
+            const uint32_t k = (uint32_t)__popcll(starts & (lanemask_lt64() | (1ull << lane))) - 1;
+            const bool next_start = lane + 1 < 64 && ((starts >> (lane + 1)) & 1ull);
+            if (start) sWin[2 * k] = a;
+            if (valid && (lane + 1 == M || next_start)) sWin[2 * k + 1] = run;
+            if (lane == 0) sNW = (uint32_t)__popcll(starts);
+        } else if (lane == 0) {
+            sNW = 0;
+            sNP = 0;
+        }
     }
-    if (j == n - 1) crange[h].y = n;
+    __syncthreads();
+    const uint32_t M = sM;
+    const bool few = M <= (uint32_t)FEW && A.patch_on;
+    if (blockIdx.x == 0 && t == 0) {
+        A.counters[1 - A.par] = 0;  // the next step's movers go there
+        A.counters[2] = M;
+        A.counters[3] = few ? 1u : 0u;
+        A.counters[4 + (1 - A.par)] = 0;
+        if (!few) {  // the full merge rewrites every position
+            win_now[0] = 1;
+            win_now[2] = 0;
+            win_now[3] = n - 1;
+        }
+    }
+    // ---- per scene (blocks in turn): the displacement bound since its lists
+    // were built, the step's largest wave value added; every list is rebuilt
+    // when it reaches half the skin (both particles of a pair may have moved),
+    // at a block start, or after many movers
+    if (A.lists)
+        for (uint32_t sc = blockIdx.x; sc < A.nscenes; sc += gridDim.x) {
+            const uint2 br = A.scene_blk[sc];
+            uint32_t m = 0;
+            const uint32_t q1 = br.y * A.wpb;
+            for (uint32_t q0 = br.x * A.wpb + t; q0 < q1; q0 += 8 * 256) {
+                uint32_t v[8];  // eight loads in flight
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const uint32_t q = q0 + u * 256;
+                    v[u] = q < q1 ? A.wdisp[q] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) m = max(m, v[u]);
+            }
+            for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+            if (lane == 0) sred[w] = m;
+            __syncthreads();
+            if (t == 0) {
+                m = max(max(sred[0], sred[1]), max(sred[2], sred[3]));
+                const float D = A.disp[sc] + __uint_as_float(m);
+                const bool rb = A.force[sc] != 0 || !few || !(2.0f * D < A.scenes[sc].skin);
+                A.disp[sc] = rb ? 0.0f : D;
+                A.rebuild[sc] = rb ? 1u : 0u;
+                A.force[sc] = 0;
+                if (rb) atomicAdd(&A.counters[4 + A.par], 1u);
+            }
+            __syncthreads();
+        }
+    if (blockIdx.x == 0 && A.lists) {
+        // ---- dirty cells (one block): clear the previous sort's marks, then
+        // mark the 27 cells around each mover's new cell (the movers' records
+        // are at hand on the patch path only; otherwise every list is rebuilt)
+        uint32_t* const dp = A.dprev + (1 - A.par) * DPREV_WORDS;
+        uint32_t* const dn = A.dprev + A.par * DPREV_WORDS;
+        const uint32_t npv = min(dp[0], (uint32_t)FEW);
+        for (uint32_t q = t; q < 27 * npv; q += 256) {
+            const uint32_t e = q / 27;
+            mark_neighbourhood(A.scenes[dp[2 + 2 * e]], dp[1 + 2 * e], q % 27, A.dirty, false);
+        }
+        const uint32_t nm = few ? M : 0u;
+        if (t < nm) sAux[t] = A.mold[sApp[t]].y;
+        __syncthreads();
+        for (uint32_t q = t; q < 27 * nm; q += 256) {
+            const uint32_t e = q / 27;
+            mark_neighbourhood(A.scenes[sAux[e]], (uint32_t)(sX[e] >> 32), q % 27, A.dirty, true);
+        }
+        if (t < nm) {
+            dn[1 + 2 * t] = (uint32_t)(sX[t] >> 32);
+            dn[2 + 2 * t] = sAux[t];
+        }
+        if (t == 0) dn[0] = nm;
+    }
+    if (!few) return;
+    if (blockIdx.x == 1) {
+        // ---- a cell every particle left (and none entered) becomes empty;
+        // the movers' marks of record_mover are cleared for the next step; the
+        // windows of this sort are recorded
+        if (t < M) sAux[t] = A.key_old[sSrc[t]];
+        __syncthreads();
+        if (t < M) {
+            const uint32_t src = sSrc[t], ko = sAux[t];
+            if (ko < A.ncell_cap) {
+                uint32_t leave = 0, enter = 0;
+                for (uint32_t r = 0; r < M; ++r) {
+                    leave += sAux[r] == ko ? 1u : 0u;
+                    enter += (uint32_t)(sX[r] >> 32) == ko ? 1u : 0u;
+                }
+                if (A.mold[sApp[t]].x + enter == leave) A.crange[ko] = make_uint2(EMPTY, 0);
+            }
+            A.moverbits[src >> 6] = 0;
+            A.blockcnt[src / ROWS_T] = 0;
+        }
+        if (t < sNW) {
+            win_now[2 + 2 * t] = sWin[2 * t];
+            win_now[3 + 2 * t] = sWin[2 * t + 1];
+        }
+        if (t == 0) win_now[0] = sNW;
+        return;
+    }
+    if (blockIdx.x < 2) return;
+    // ---- the sweep (blocks 2..): new windows (one position past each, for
+    // its last boundary), then the previous windows' positions outside the
+    // new ones
+    const uint32_t nw = sNW, npv = sNP, M2 = M;
+    auto elem = [&](uint32_t j, uint32_t& key) -> uint32_t {
+        uint32_t c1 = 0;
+        for (uint32_t r = 0; r < M2; ++r) {
+            const uint32_t o = sOut[r];
+            if (o == j) {
+                key = (uint32_t)(sX[r] >> 32);
+                return sSrc[r];
+            }
+            if (o > j) break;
+            ++c1;
+        }
+        uint32_t s = j - c1;
+        for (uint32_t r = 0; r < M2; ++r) {
+            if (sAsc[r] <= s)
+                ++s;
+            else
+                break;
+        }
+        s = min(s, n - 1);  // defensive: the windows hold their own stayers
+        key = A.key_old[s];
+        return s;
+    };
+    auto in_new = [&](uint32_t j) {
+        for (uint32_t c = 0; c < nw; ++c)
+            if (j >= sWin[2 * c] && j <= sWin[2 * c + 1]) return true;
+        return false;
+    };
+    uint32_t total = 0;
+    for (uint32_t c = 0; c < nw; ++c) total += sWin[2 * c + 1] - sWin[2 * c] + 2;
+    const uint32_t total_new = total;
+    for (uint32_t c = 0; c < npv; ++c) total += sPrev[2 * c + 1] - sPrev[2 * c] + 1;
+    for (uint32_t g = (blockIdx.x - 2) * 256 + t; g < total; g += (gridDim.x - 2) * 256) {
+        if (g < total_new) {
+            uint32_t c = 0, off = g;
+            while (off >= sWin[2 * c + 1] - sWin[2 * c] + 2) {
+                off -= sWin[2 * c + 1] - sWin[2 * c] + 2;
+                ++c;
+            }
+            const uint32_t a = sWin[2 * c], b = sWin[2 * c + 1], j = a + off;
+            if (j > n) continue;
+            uint32_t kc = EMPTY, kp = EMPTY;
+            if (j <= b) {
+                const uint32_t e = elem(j, kc);
+                A.perm[j] = e;
+                A.invperm[e] = j;
+                A.skey[j] = kc;
+            } else if (j < n) {
+                kc = A.skey[j];  // outside every window: unchanged
+            }
+            if (j > 0) {
+                if (j - 1 >= a)
+                    (void)elem(j - 1, kp);
+                else
+                    kp = A.skey[j - 1];
+            }
+            if (j == 0 || j == n || kp != kc) {
+                if (j < n && kc < A.ncell_cap) A.crange[kc].x = j;
+                if (j > 0 && kp < A.ncell_cap) A.crange[kp].y = j;
+            }
+        } else {
+            uint32_t c = 0, off = g - total_new;
+            while (off >= sPrev[2 * c + 1] - sPrev[2 * c] + 1) {
+                off -= sPrev[2 * c + 1] - sPrev[2 * c] + 1;
+                ++c;
+            }
+            const uint32_t j = sPrev[2 * c] + off;
+            if (j < n && !in_new(j)) {
+                A.perm[j] = j;
+                A.invperm[j] = j;
+            }
+        }
+    }
+}
+
+// Reset of the sort and list state after a layout change or a regather:
+// both mover counters zero, both window sets the whole range (the next patch
+// restores the identity everywhere), no marked neighbourhoods recorded (the
+// dirty bitmap itself is cleared by the host), every scene's lists rebuilt.
+// One block.
+__global__ __launch_bounds__(256) void k_sort_reset(PatchArgs A) {
+    const uint32_t t = threadIdx.x;
+    if (t < 2) {
+        A.counters[t] = 0;
+        A.counters[4 + t] = 0;
+        A.win[t * WIN_WORDS] = A.n ? 1u : 0u;
+        A.win[t * WIN_WORDS + 2] = 0;
+        A.win[t * WIN_WORDS + 3] = A.n ? A.n - 1 : 0u;
+        if (A.lists) A.dprev[t * DPREV_WORDS] = 0;
+    }
+    if (A.lists)
+        for (uint32_t sc = t; sc < A.nscenes; sc += 256) {
+            A.force[sc] = 1;
+            A.disp[sc] = 0.0f;
+            A.rebuild[sc] = 1;
+        }
+}
+
+// invperm[perm[j]] = j (after a full sort).
+__global__ void k_invert(const uint32_t* perm, uint32_t* invperm, uint32_t n) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) {
+        const uint32_t s = perm[j];
+        if (s < n) invperm[s] = j;
+    }
 }
 
 __global__ void k_iota(uint32_t* p, uint32_t n) {

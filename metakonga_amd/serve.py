@@ -23,14 +23,15 @@ import numpy as np
 from . import abi, scenes
 
 
-def device_bytes(particles, nscenes, cells, history=False, lib=None):
+def device_bytes(particles, nscenes, cells, history=False, lists=True, lib=None):
     """Device bytes of a fused batch holding `particles`, `nscenes` scenes and
-    `cells` cells: the library's own allocation arithmetic
-    (xdem_batch_device_bytes; host only, no GPU needed)."""
+    `cells` cells (with candidate lists unless lists=False): the library's own
+    allocation arithmetic (xdem_batch_device_bytes; host only, no GPU needed)."""
     from . import batch as _batch
     L = lib or abi.load()
     _batch.declare(L)
-    return int(L.xdem_batch_device_bytes(int(particles), int(nscenes), int(cells), int(history)))
+    return int(L.xdem_batch_device_bytes(int(particles), int(nscenes), int(cells),
+                                         (1 if history else 0) | (2 if lists else 0)))
 
 
 @dataclass

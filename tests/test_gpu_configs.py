@@ -153,8 +153,17 @@ def test_batch_memory_matches_footprint(lib):
     b.sync()
     abi.check(lib, lib.xdem_device_memory(C.byref(free1), C.byref(tot)))
     used = free0.value - free1.value
-    want = serve.device_bytes(max_p, max_s, max_c)
+    want = serve.device_bytes(max_p, max_s, max_c, lists=False)
     print(f"batch device memory {used / 1e6:.1f} MB, footprint {want / 1e6:.1f} MB")
+    assert abs(used - want) <= 0.03 * want + (64 << 20), (used, want)
+    # with candidate lists
+    b.lists(True)
+    b.step(1)
+    b.sync()
+    abi.check(lib, lib.xdem_device_memory(C.byref(free1), C.byref(tot)))
+    used = free0.value - free1.value
+    want = serve.device_bytes(max_p, max_s, max_c, lists=True)
+    print(f"with lists {used / 1e6:.1f} MB, footprint {want / 1e6:.1f} MB")
     b.close()
     assert abs(used - want) <= 0.03 * want + (64 << 20), (used, want)
 

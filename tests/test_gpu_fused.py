@@ -285,8 +285,9 @@ def test_block_size_does_not_change_results(lib):
     b.add(sc, vel, om)
     chosen, ms = b.autotune(steps_per_trial=5)
     assert chosen % 1000 in (64, 128, 256) and (chosen // 1000) % 10 in (1, 2, 4)
-    assert (chosen // 10000) % 10 in (0, 1) and chosen // 100000 in (0, 1)
-    assert len(ms) == 36 and all(v > 0 for v in ms.values())
+    assert (chosen // 10000) % 10 in (0, 1) and (chosen // 100000) % 10 in (0, 1)
+    assert chosen // 1000000 in (0, 1)
+    assert len(ms) == 54 and all(v > 0 for v in ms.values())
     b.close()
 
 

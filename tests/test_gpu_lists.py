@@ -1,0 +1,120 @@
+"""Candidate lists and the window-patch re-sort of the fused path against the
+composition they replace (the 27-cell walk every step and the full merge
+every step): bit-identical states after every run.
+
+Lists keep, per particle, its neighbours within ri + rj + skin in the
+reference traversal order; contacts found from a list are the walk's
+contacts in the walk's order, so every result must match bit for bit —
+across skins, launch shapes, graph replay, scene mixes, few movers (the
+window patch) and many (the full merge and a rebuild of every list).
+"""
+import numpy as np
+import pytest
+
+from metakonga_amd import scenes
+from metakonga_amd.batch import Batch
+from xdem_testutil import dense_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes(kind):
+    if kind == "dense":  # contact-rich, slow: few particles change cell
+        sc, vel, om = dense_scene(n_side=10, seed=21, vel_scale=0.05)
+        return [sc], [vel], [om]
+    if kind == "fast":  # most particles change cell every few steps
+        sc, vel, om = dense_scene(n_side=12, seed=22, pitch=2.2, jitter=0.1, vel_scale=0.8)
+        sc.gravity = (0.0, 0.0, 0.0)
+        return [sc], [vel], [om]
+    if kind == "boxes":  # three settling boxes in one batch
+        scs = [scenes.box_scene(n, r, 500 + k, 5e-6, 100) for k, (n, r) in
+               enumerate([(3000, 3e-3), (1200, 4.5e-3), (4000, 2e-3)])]
+        return scs, [None] * 3, [None] * 3
+    if kind == "hopper":  # polydisperse, colliding at 0.3 m/s
+        sc = scenes.hopper(n=60_000)
+        n = sc.np
+        vel = scenes.as_f32_exact(scenes.uniform(0x51, 3 * n, 5).reshape(n, 3) * 0.6 - 0.3)
+        return [sc], [vel], [None]
+    if kind == "crowd":  # 24 contacts on one sphere: more than the 16-entry contact list
+        R, r = float(scenes.as_f32_exact(5e-3)), float(scenes.as_f32_exact(1.2e-3))
+        k = np.arange(24)
+        z = 1 - 2 * (k + 0.5) / 24
+        rr = np.sqrt(1 - z * z)
+        g = np.pi * (3 - np.sqrt(5))
+        dirs = np.c_[rr * np.cos(g * k), rr * np.sin(g * k), z]
+        c = np.array([0.05, 0.05, 0.05])
+        pos = np.vstack([np.r_[c, R], np.c_[c + dirs * (R + r - 2e-5), np.full(24, r)]])
+        sc = scenes.Scene(name="crowd", pos=pos, dt=1e-5, nsteps=1, world_origin=(0.0, 0.0, 0.0),
+                          cell_size=2 * R, grid_size=(16, 16, 16), gravity=(0.0, 0.0, 0.0))
+        vel = scenes.as_f32_exact(scenes.uniform(9, 3 * sc.np).reshape(-1, 3) * 0.02 - 0.01)
+        return [sc], [vel], [None]
+    raise ValueError(kind)
+
+
+def _run(kind, steps=90, block=30, patch=True, lists=False, skin=0.0, threads=256, unroll=2,
+         ahead=False, graphs=True):
+    scs, vels, oms = _scenes(kind)
+    b = Batch(sum(s.np for s in scs), len(scs), sum(int(np.prod(s.grid_size)) for s in scs))
+    b.graphs(graphs)
+    b.launch(threads, unroll, ahead)
+    b.sort_patch(patch)
+    if lists:
+        b.lists(True, skin)
+    ids = [b.add(s, v, o) for s, v, o in zip(scs, vels, oms)]
+    done, rebuilds, movers = 0, [], []
+    while done < steps:
+        b.step(min(block, steps - done))
+        done += block
+        rebuilds.append(b.list_rebuilds())
+        movers.append(b.last_movers())
+    out = [b.state(i) for i in ids]
+    b.close()
+    return out, rebuilds, movers
+
+
+VARIANTS = [
+    dict(patch=True),
+    dict(patch=True, lists=True, skin=0.05),
+    dict(patch=True, lists=True, skin=0.3),
+    dict(patch=True, lists=True, skin=0.05, threads=64, unroll=4),
+    dict(patch=True, lists=True, skin=0.05, unroll=1, ahead=True),
+    dict(patch=True, lists=True, skin=0.05, graphs=False),
+]
+
+
+@pytest.mark.parametrize("kind", ["dense", "fast", "boxes", "hopper", "crowd"])
+def test_lists_and_patch_bitexact(lib, kind):
+    ref, _, ref_movers = _run(kind, patch=False)
+    print(f"{kind}: movers of the last sort of each block {ref_movers}")
+    for v in VARIANTS:
+        out, rebuilds, _ = _run(kind, **v)
+        for k, (a, b) in enumerate(zip(ref, out)):
+            for key in a:
+                assert np.array_equal(a[key], b[key]), (kind, v, k, key)
+        if v.get("lists") and kind in ("dense", "crowd"):
+            # quiet scenes: the lists were reused (no scene rebuilt them all)
+            assert rebuilds[-1] == 0, (kind, v, rebuilds)
+
+
+def test_lists_alone_vs_mix(lib):
+    boxes, _, _ = _scenes("boxes")
+    dense, dv, do = _scenes("dense")
+    scs = [boxes[0], dense[0], boxes[2]]
+    vels, oms = [None, dv[0], None], [None, do[0], None]
+
+    def run(sel):
+        b = Batch(sum(scs[k].np for k in sel), len(sel),
+                  sum(int(np.prod(scs[k].grid_size)) for k in sel))
+        b.lists(True, 0.1)
+        ids = [b.add(scs[k], vels[k], oms[k]) for k in sel]
+        for _ in range(3):
+            b.step(25)
+        out = [b.state(i) for i in ids]
+        b.close()
+        return out
+
+    mixed = run([0, 1, 2])
+    for k in range(3):
+        alone = run([k])[0]
+        for key in alone:
+            assert np.array_equal(alone[key], mixed[k][key]), (k, key)

@@ -100,12 +100,15 @@ def test_footprint_is_the_library_allocation_arithmetic():
     """Admission budgets with xdem_batch_device_bytes: linear in particles and
     cells, the history buffers added when a scene has history, and a budget of
     exactly two scenes' batch admits two of three equal scenes at a time."""
-    b1 = serve.device_bytes(100_000, 1, 1 << 20)
-    b2 = serve.device_bytes(200_000, 1, 1 << 20)
-    b3 = serve.device_bytes(100_000, 1, 1 << 21)
+    b1 = serve.device_bytes(100_000, 1, 1 << 20, lists=False)
+    b2 = serve.device_bytes(200_000, 1, 1 << 20, lists=False)
+    b3 = serve.device_bytes(100_000, 1, 1 << 21, lists=False)
     per_p = (b2 - b1) / 100_000
     per_c = (b3 - b1) / (1 << 20)
     assert 200 < per_p < 400 and per_c == 8
+    # candidate lists: 32 entries and a count per particle, a bit per cell
+    bl = serve.device_bytes(100_000, 1, 1 << 20)
+    assert bl - b1 == 100_000 * 33 * 4 + ((1 << 20) // 64 + 1) * 8
     assert serve.device_bytes(100_000, 1, 1 << 20, history=True) - b1 >= 100_000 * 2 * 16 * 16
     clock = serve.FakeClock()
     be = FakeBackend(clock)
