@@ -124,7 +124,7 @@ struct xdem_batch {
     uint32_t* counters = nullptr;     // [0],[1] movers by key parity, [2] movers of the last
                                       // sort, [3] patched (the window patch placed them)
     uint32_t* win = nullptr;          // windows of the last two sorts (k_patch)
-    uint2* mold = nullptr;            // per mover: old cell size, scene
+    uint4* mold = nullptr;            // first FEW movers: old cell size, scene, old key
     uint64_t* mkey = nullptr;
     uint32_t* ms = nullptr;
     uint64_t* mkey2 = nullptr;
@@ -249,8 +249,10 @@ struct xdem_batch {
     // Where k_step / k_pos record the movers of the keys they compute, against
     // the keys of the last sort (on = 0 when a full sort follows instead); the
     // mover counter alternates with the key buffer.
-    MoverOut mover_out(bool on) const {
+    MoverOut mover_out(bool on, const uint2* zi, const uint32_t* zperm) const {
         MoverOut m{};
+        m.zi = zi;
+        m.perm = zperm;
         m.key_old = skey[skcur];
         m.crange = crange;
         m.mcount = counters + (1 - skcur);
@@ -710,7 +712,7 @@ int32_t launch_pos(xdem_batch* b, bool adv, const BlockEntry* blocks, uint32_t n
                    bool movers) {
     if (!nblk) return XDEM_OK;
     BState& A = b->buf[b->cur];
-    const MoverOut mv = b->mover_out(movers);
+    const MoverOut mv = b->mover_out(movers, A.zi, nullptr);
     if (adv)
         hipLaunchKernelGGL(k_pos<true>, dim3(nblk), dim3(b->block_threads), 0, b->st, A, b->knext(),
                            b->d_scenes, blocks, mv, b->force);
@@ -782,7 +784,7 @@ StepArgs step_args(xdem_batch* b, bool dbg) {
     a.perm = b->perm;
     a.crange = b->crange;
     a.keynew = b->knext();
-    a.mv = b->mover_out(true);
+    a.mv = b->mover_out(true, a.in.zi, b->perm);
     a.vl = b->vl;
     a.vc = b->vc;
     a.vl_stride = b->cap_np;
@@ -927,8 +929,8 @@ uint64_t batch_device_bytes(uint64_t n, uint64_t nscene, uint64_t ncell, bool hi
     b += ncell * sizeof(uint2);                           // crange
     b += (nblk * (ROWS_T / 64) + 1) * sizeof(uint64_t);   // moverbits
     b += (nblk + 1) * sizeof(uint32_t) + 8 * sizeof(uint32_t);  // blockcnt, counters
-    b += n * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t) + 2 * sizeof(uint32_t) +
-              sizeof(uint2));                             // movers
+    b += n * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t) + 2 * sizeof(uint32_t)) +
+         FEW * sizeof(uint4);                             // movers
     b += (2 * WIN_WORDS + 2 * DPREV_WORDS) * sizeof(uint32_t);  // windows, marked cells
     b += 5 * nscene * sizeof(uint32_t);                   // disp, force, rebuild, scene blocks
     b += wdisp_words(n, nscene) * sizeof(uint32_t);       // wave displacements
@@ -1019,7 +1021,7 @@ int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t 
         (rc = dalloc(&b->crange, (size_t)max_cells)) ||
         (rc = dalloc(&b->moverbits, (size_t)nblk * (ROWS_T / 64) + 1)) ||
         (rc = dalloc(&b->blockcnt, (size_t)nblk + 1)) || (rc = dalloc(&b->counters, 8)) ||
-        (rc = dalloc(&b->win, 2 * WIN_WORDS)) || (rc = dalloc(&b->mold, n)) ||
+        (rc = dalloc(&b->win, 2 * WIN_WORDS)) || (rc = dalloc(&b->mold, (size_t)FEW)) ||
         (rc = dalloc(&b->mkey, n)) || (rc = dalloc(&b->ms, n)) || (rc = dalloc(&b->mkey2, n)) ||
         (rc = dalloc(&b->ms2, n)) || (rc = dalloc(&b->mplo, 2 * n)) ||
         (rc = dalloc(&b->dprev, 2 * DPREV_WORDS)) || (rc = dalloc(&b->disp, max_scenes)) ||

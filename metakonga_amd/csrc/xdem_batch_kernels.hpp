@@ -153,8 +153,10 @@ struct MoverOut {
     uint32_t* mcount;
     uint64_t* mkey;            // (key << 32 | id)
     uint32_t* ms;              // storage index
-    uint32_t* mplo;            // insertion point
-    uint2* mold;               // (particles in the old cell, scene index)
+    uint32_t* mplo;            // insertion point: previous entries below the new (key, id)
+    uint4* mold;               // first FEW movers: (particles in the old cell, scene, old key)
+    const uint2* zi;           // particle ids, storage order
+    const uint32_t* perm;      // sorted position of the last sort -> storage index (null: identity)
     uint64_t* moverbits;
     uint32_t* blockcnt;
     uint32_t n, ncell_cap;
@@ -291,13 +293,18 @@ __device__ inline void record_mover(const MoverOut& O, uint32_t s, uint32_t nw, 
         if (p < O.n) {
             O.mkey[p] = ((uint64_t)nw << 32) | id;
             O.ms[p] = s;
-            O.mplo[p] = insertion_point(O.crange, O.key_old, O.n, O.ncell_cap, nw);
-            uint32_t size = 0;
-            if (ko < O.ncell_cap) {
-                const uint2 c = O.crange[ko];
-                size = c.x == EMPTY ? 0u : c.y - c.x;
+            // past the entries of the new cell with a smaller id
+            uint32_t lo = insertion_point(O.crange, O.key_old, O.n, O.ncell_cap, nw);
+            while (lo < O.n && O.key_old[lo] == nw && O.zi[O.perm ? O.perm[lo] : lo].y < id) ++lo;
+            O.mplo[p] = lo;
+            if (p < (uint32_t)FEW) {
+                uint32_t size = 0;
+                if (ko < O.ncell_cap) {
+                    const uint2 c = O.crange[ko];
+                    size = c.x == EMPTY ? 0u : c.y - c.x;
+                }
+                O.mold[p] = make_uint4(size, scene, ko, 0u);
             }
-            O.mold[p] = make_uint2(size, scene);
         }
         atomicOr((unsigned long long*)&O.moverbits[s >> 6], 1ull << (s & 63));
         atomicAdd(&O.blockcnt[s / ROWS_T], 1u);
@@ -1613,10 +1620,9 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
         const uint32_t p = A.ms2[r];
         if (p >= A.n) continue;  // defensive: payloads are append indices < M
         const uint32_t s = A.ms[p];
-        const uint32_t kn = (uint32_t)(X >> 32), id = (uint32_t)X;
-        // entries of the previous order with (key, id) < X
-        uint32_t lo = A.mplo[p];
-        while (lo < A.n && A.key_old[lo] == kn && A.zi[lo].y < id) ++lo;
+        const uint32_t kn = (uint32_t)(X >> 32);
+        // entries of the previous order with (key, id) < X (recorded with the mover)
+        const uint32_t lo = A.mplo[p];
         uint32_t out = r + (lo - (few ? count_below(mst, M, lo) : mover_prefix(A, lo)));
         if (out < A.n && s < A.n) {
             A.perm[out] = s;
@@ -1684,7 +1690,7 @@ struct PatchArgs {
     const uint64_t* mkey;      // mover records, append order (record_mover)
     const uint32_t* ms;
     const uint32_t* mplo;
-    const uint2* mold;
+    const uint4* mold;
     uint64_t* moverbits;
     uint32_t* blockcnt;
     uint32_t* win;             // windows by parity (this sort: par; the previous: 1 - par)
@@ -1718,14 +1724,27 @@ __device__ inline void mark_neighbourhood(const BScene& S, uint32_t k, uint32_t 
         atomicAnd((unsigned long long*)&dirty[kk >> 6], (unsigned long long)~bit);
 }
 
+// Rank of this lane's key among the keys of lanes 0..M-1 (ties by lane).
+__device__ inline uint32_t wave_rank(uint64_t key, uint32_t M) {
+    const uint32_t lane = __lane_id();
+    const int khi = (int)(uint32_t)(key >> 32), klo = (int)(uint32_t)key;
+    uint32_t rank = 0;
+    for (uint32_t r = 0; r < M; ++r) {
+        const uint64_t o = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(khi, (int)r) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane(klo, (int)r);
+        rank += (o < key || (o == key && r < lane)) ? 1u : 0u;
+    }
+    return rank;
+}
+
 __global__ __launch_bounds__(256) void k_patch(PatchArgs A) {
     __shared__ uint64_t sX[FEW];      // movers by new (key, id)
-    __shared__ uint32_t sApp[FEW];    // their append index
     __shared__ uint32_t sSrc[FEW];    // their storage index
     __shared__ uint32_t sOut[FEW];    // their new position (increasing)
     __shared__ uint32_t sAsc[FEW];    // the movers' storage indices, ascending
     __shared__ uint32_t sWin[2 * FEW], sPrev[2 * FEW];
-    __shared__ uint32_t sAux[2 * FEW];  // blocks 0 and 1: per mover, scene / old key
+    __shared__ uint4 sOld[FEW];       // their (old cell size, scene, old key)
+    __shared__ uint32_t sAux[FEW];    // window ends while the windows are sorted
     __shared__ uint32_t sM, sNW, sNP;
     __shared__ uint32_t sred[4];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -1734,8 +1753,8 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A) {
     const uint32_t* const win_prev = A.win + (1 - A.par) * WIN_WORDS;
     // ---- wave 0 of every block: the movers' order and the windows.  The
     // mover count, the previous windows and the first 64 mover records are
-    // loaded together; the movers' storage indices follow their sort by
-    // shuffle.
+    // loaded together (one memory round trip).  The sorts are rank sorts over
+    // the M records (all-pairs by lane reads, M iterations) scattered to LDS.
     if (w == 0) {
         const uint32_t m0 = A.counters[A.par];
         const uint32_t np0 = win_prev[0];
@@ -1743,58 +1762,70 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A) {
         const bool rec = lane < n;
         const uint64_t mk = rec ? A.mkey[lane] : ~0ull;
         const uint32_t msl = rec ? A.ms[lane] : 0u;
-        const uint32_t M = min(m0, n);
+        const uint32_t mpl = rec ? A.mplo[lane] : 0u;
+        const uint4 mo = rec ? A.mold[lane] : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t M = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(m0, n));
         const bool few = M <= (uint32_t)FEW && A.patch_on;
         if (lane == 0) sM = M;
         if (few) {
-            uint64_t key = lane < M ? mk : ~0ull;
-            uint32_t val = lane;
-            bitonic_wave(key, val);
-            const uint32_t src = (uint32_t)__shfl((int)msl, (int)(val & 63u), 64);
-            sX[lane] = key;
-            sApp[lane] = val;
-            sSrc[lane] = src;
-            uint64_t sk = lane < M ? (uint64_t)msl : ~0ull;
-            uint32_t unused = 0;
-            bitonic_wave(sk, unused);
-            sAsc[lane] = (uint32_t)sk;
+            // movers by (key, id), and their storage indices ascending
+            const uint32_t rk = wave_rank(mk, M), rs = wave_rank((uint64_t)msl, M);
+            if (lane < M) {
+                sX[rk] = mk;
+                sSrc[rk] = msl;
+                sOut[rk] = mpl;  // the insertion point for now
+                sOld[rk] = mo;
+                sAsc[rs] = msl;
+            }
             const uint32_t np = min(np0, (uint32_t)FEW);
             if (lane < np) {
                 sPrev[2 * lane] = pa;
                 sPrev[2 * lane + 1] = pb;
             }
             if (lane == 0) sNP = np;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            // insertion point in the previous order (past the entries of the new
-            // cell with a smaller id), new position and window of mover `lane`
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // new position and window of the mover of rank `lane`: previous
+            // entries below it, less the movers among them, plus the movers
+            // before it
             uint64_t win = ~0ull;
+            uint32_t out = 0;
             if (lane < M) {
-                const uint32_t kn = (uint32_t)(key >> 32), id = (uint32_t)key;
-                uint32_t lo = A.mplo[val];
-                while (lo < n && A.key_old[lo] == kn && A.zi[lo].y < id) ++lo;
+                const uint32_t lo = sOut[lane], src = sSrc[lane];
                 uint32_t below = 0;
-                for (uint32_t r = 0; r < M; ++r) below += sAsc[r] < lo ? 1u : 0u;
-                sOut[lane] = lane + lo - below;
+                for (uint32_t r = 0; r < M; ++r)
+                    below += (uint32_t)__builtin_amdgcn_readlane((int)msl, (int)r) < lo ? 1u : 0u;
+                out = lane + lo - below;
                 const uint32_t a = src < lo ? src : lo, b = src < lo ? lo - 1 : src;
                 win = ((uint64_t)a << 32) | b;
             }
-            // windows sorted by first position, merged where they overlap or
-            // touch: a running maximum of the last positions decides where a
-            // merged window starts and ends
-            uint32_t unused2 = 0;
-            bitonic_wave(win, unused2);
-            const uint32_t a = (uint32_t)(win >> 32), b = (uint32_t)win;
+            // windows by first position, merged where they overlap or touch: a
+            // running maximum of the last positions decides where a merged
+            // window starts and ends
+            const uint32_t rw = wave_rank(win, M);
+            __builtin_amdgcn_wave_barrier();
+            if (lane < M) {
+                sOut[lane] = out;
+                sWin[rw] = (uint32_t)(win >> 32);
+                sAux[rw] = (uint32_t)win;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const bool valid = lane < M;
+            const uint32_t a = valid ? sWin[lane] : 0xFFFFFFFFu, b = valid ? sAux[lane] : 0xFFFFFFFFu;
             uint32_t run = b;
             for (int o = 1; o < 64; o <<= 1) {
                 const uint32_t u = (uint32_t)__shfl_up((int)run, o, 64);
                 if (lane >= (uint32_t)o) run = max(run, u);
             }
             const uint32_t prev_run = (uint32_t)__shfl_up((int)run, 1, 64);
-            const bool valid = lane < M;
             const bool start = valid && (lane == 0 || a > prev_run + 1);
             const uint64_t starts = __ballot(start);
             const uint32_t k = (uint32_t)__popcll(starts & (lanemask_lt64() | (1ull << lane))) - 1;
             const bool next_start = lane + 1 < 64 && ((starts >> (lane + 1)) & 1ull);
+            __builtin_amdgcn_wave_barrier();
             if (start) sWin[2 * k] = a;
             if (valid && (lane + 1 == M || next_start)) sWin[2 * k + 1] = run;
             if (lane == 0) sNW = (uint32_t)__popcll(starts);
@@ -1862,15 +1893,14 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A) {
             mark_neighbourhood(A.scenes[dp[2 + 2 * e]], dp[1 + 2 * e], q % 27, A.dirty, false);
         }
         const uint32_t nm = few ? M : 0u;
-        if (t < nm) sAux[t] = A.mold[sApp[t]].y;
         __syncthreads();
         for (uint32_t q = t; q < 27 * nm; q += 256) {
             const uint32_t e = q / 27;
-            mark_neighbourhood(A.scenes[sAux[e]], (uint32_t)(sX[e] >> 32), q % 27, A.dirty, true);
+            mark_neighbourhood(A.scenes[sOld[e].y], (uint32_t)(sX[e] >> 32), q % 27, A.dirty, true);
         }
         if (t < nm) {
             dn[1 + 2 * t] = (uint32_t)(sX[t] >> 32);
-            dn[2 + 2 * t] = sAux[t];
+            dn[2 + 2 * t] = sOld[t].y;
         }
         if (t == 0) dn[0] = nm;
     }
@@ -1879,17 +1909,15 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A) {
         // ---- a cell every particle left (and none entered) becomes empty;
         // the movers' marks of record_mover are cleared for the next step; the
         // windows of this sort are recorded
-        if (t < M) sAux[t] = A.key_old[sSrc[t]];
-        __syncthreads();
         if (t < M) {
-            const uint32_t src = sSrc[t], ko = sAux[t];
+            const uint32_t src = sSrc[t], ko = sOld[t].z;
             if (ko < A.ncell_cap) {
                 uint32_t leave = 0, enter = 0;
                 for (uint32_t r = 0; r < M; ++r) {
-                    leave += sAux[r] == ko ? 1u : 0u;
+                    leave += sOld[r].z == ko ? 1u : 0u;
                     enter += (uint32_t)(sX[r] >> 32) == ko ? 1u : 0u;
                 }
-                if (A.mold[sApp[t]].x + enter == leave) A.crange[ko] = make_uint2(EMPTY, 0);
+                if (sOld[t].x + enter == leave) A.crange[ko] = make_uint2(EMPTY, 0);
             }
             A.moverbits[src >> 6] = 0;
             A.blockcnt[src / ROWS_T] = 0;
