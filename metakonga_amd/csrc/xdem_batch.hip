@@ -32,10 +32,10 @@ static_assert(WALL_KEY == XDEM_BATCH_WALL_KEY, "wall key");
 
 namespace {
 
-// Wave slots of the displacement table: the sorted block table has at most
-// n / T + scenes blocks of T threads (each scene's last block may be partial),
-// so at most n / 64 + 4 scenes waves.
-inline size_t wdisp_words(uint64_t n, uint64_t nscene) { return n / 64 + 4 * nscene + 4; }
+// Slots of the displacement table, one per block of the sorted block table:
+// at most n / T + scenes blocks of T >= 64 threads (each scene's last block may
+// be partial).
+inline size_t wdisp_words(uint64_t n, uint64_t nscene) { return n / 64 + nscene + 4; }
 
 // A triangle-mesh object of a scene: its triangles in the object frame, sorted
 // by local cell, and the local cell table (offsets filled at layout time).
@@ -140,7 +140,7 @@ struct xdem_batch {
     size_t dirty_words = 0;
     uint32_t* dprev = nullptr;        // marked neighbourhoods of the last two sorts
     float* disp = nullptr;            // per scene (cap_scenes each)
-    uint32_t* wdisp = nullptr;        // per wave of k_step: its largest displacement
+    uint32_t* wdisp = nullptr;        // per block of k_step: its largest displacement
     uint2* d_scene_blk = nullptr;     // per scene: first and end block of the sorted table
     uint32_t* force = nullptr;
     uint32_t* rebuild = nullptr;
@@ -322,7 +322,6 @@ struct xdem_batch {
         a.disp = disp;
         a.wdisp = wdisp;
         a.scene_blk = d_scene_blk;
-        a.wpb = block_threads / 64;
         a.force = force;
         a.rebuild = rebuild;
         return a;
@@ -665,7 +664,7 @@ int32_t apply_layout(xdem_batch* b) {
         seen[k] = true;
         range[k].y = L + 1;
     }
-    XDEM_REQUIRE((uint64_t)bs.size() * (b->block_threads / 64) <= wdisp_words(b->cap_np, b->cap_scenes),
+    XDEM_REQUIRE((uint64_t)bs.size() <= wdisp_words(b->cap_np, b->cap_scenes),
                  "internal: %zu blocks exceed the wave table", bs.size());
     XDEM_TRY(upload_vec(&b->d_scene_blk, range, st));
     b->react_valid = false;
@@ -933,7 +932,7 @@ uint64_t batch_device_bytes(uint64_t n, uint64_t nscene, uint64_t ncell, bool hi
          FEW * sizeof(uint4);                             // movers
     b += (2 * WIN_WORDS + 2 * DPREV_WORDS) * sizeof(uint32_t);  // windows, marked cells
     b += 5 * nscene * sizeof(uint32_t);                   // disp, force, rebuild, scene blocks
-    b += wdisp_words(n, nscene) * sizeof(uint32_t);       // wave displacements
+    b += wdisp_words(n, nscene) * sizeof(uint32_t);       // block displacements
     b += 1024 * sizeof(double);                           // d_red
     b += nscene * sizeof(BScene) + 2 * div_up((uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFull), 64) *
                                        sizeof(BlockEntry);

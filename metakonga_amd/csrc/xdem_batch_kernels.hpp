@@ -196,8 +196,8 @@ struct StepArgs {
     const uint32_t* invperm;   // storage index of `in` -> sorted position
     const uint32_t* rebuild;   // per scene: rebuild every list this step
     const uint64_t* dirty;     // bit per cell key: a particle changed cell within one cell
-    uint32_t* wdisp;           // per wave (block L, wave w: L * waves per block + w): the
-                               // largest displacement of this step (float bits, bounded above)
+    uint32_t* wdisp;           // per block L of the launch: the largest displacement of this
+                               // step (float bits, bounded above)
     // parity probe outputs (per sorted position) — null in production
     float* dbg_f;              // 6 floats: force xyz, torque xyz
     uint32_t* dbg_cnt;
@@ -731,6 +731,15 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     // thread ever touches another thread's slots
     static_assert(2 * RTAB <= HSLOTS, "row table must fit in the history keys");
     __shared__ uint2 rtab_own[HIST || !FLAT ? 1 : LDS_T * RTAB];
+    // candidate lists: the block's largest displacement and the lanes done
+    __shared__ uint32_t sdisp[2];
+    if constexpr (VLIST) {
+        if (threadIdx.x == 0) {
+            sdisp[0] = 0;
+            sdisp[1] = 0;
+        }
+        __syncthreads();
+    }
     auto rtab_put = [&](uint32_t i, uint32_t rb, uint32_t re) {
         if constexpr (HIST) {
             hkeys[(2 * i) * LDS_T + threadIdx.x] = rb;
@@ -750,16 +759,27 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     const BlockEntry be = A.blocks[L];
     const BScene S = A.scenes[be.scene];
     const uint32_t j = be.j0 + threadIdx.x;
-    const uint32_t wslot = L * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // Every lane arrives once with its wave's largest displacement (0 for a
+    // lane without a particle); the lane that completes the block's count
+    // writes the block's maximum (LDS atomics, one per wave and path).
+    auto disp_arrive = [&](uint32_t m) {
+        const uint64_t act = __ballot(1);
+        if (__lane_id() == (uint32_t)__ffsll((unsigned long long)act) - 1) {
+            if (m) atomicMax(&sdisp[0], m);
+            const uint32_t k = (uint32_t)__popcll(act);
+            if (atomicAdd(&sdisp[1], k) + k == blockDim.x) A.wdisp[L] = atomicMax(&sdisp[0], 0u);
+        }
+    };
     if (j >= be.j1 || j >= A.n) {
-        // a wave without particles reports no displacement (its slot is read)
-        if (VLIST && threadIdx.x % 64 == 0 && be.j0 + threadIdx.x >= min(be.j1, A.n))
-            A.wdisp[wslot] = 0;
+        if constexpr (VLIST) disp_arrive(0u);
         return;
     }
     uint32_t* mylist = clist + threadIdx.x;
     const uint32_t s = A.perm[j];
-    if (s >= A.n) return;  // defensive: perm is a permutation of [0, n)
+    if (s >= A.n) {  // defensive: perm is a permutation of [0, n)
+        if constexpr (VLIST) disp_arrive(0u);
+        return;
+    }
     const double4 ip = A.in.pr[s];
     // The 5-wave variants load the rest of the own record after the neighbour
     // walk (the walk does not hold it in registers); the 4-wave ones have the
@@ -1162,9 +1182,7 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
             // the scene's largest displacement of the step (bounded above)
             const float ddx = (float)(nx - o.x), ddy = (float)(ny - o.y), ddz = (float)(nz - o.z);
             const float d = fsqrt(ddx * ddx + ddy * ddy + ddz * ddz) * 1.0001f;
-            const uint32_t m = wave_max_hi16(__float_as_uint(d));
-            if (__lane_id() == (uint32_t)__ffsll((unsigned long long)__ballot(1)) - 1)
-                A.wdisp[wslot] = m;
+            disp_arrive(wave_max_hi16(__float_as_uint(d)));
         }
     }
     A.out.pr[j] = make_double4(nx, ny, nz, ip.w);
@@ -1701,9 +1719,8 @@ struct PatchArgs {
     const BScene* scenes;
     uint32_t nscenes;
     float* disp;               // per scene: displacement bound since its lists were built
-    const uint32_t* wdisp;     // per wave of the last k_step: its largest displacement
+    const uint32_t* wdisp;     // per block of the last k_step: its largest displacement
     const uint2* scene_blk;    // per scene: first and end block of the launch table
-    uint32_t wpb;              // waves per block of k_step
     uint32_t* force;           // per scene: a block start asks for a rebuild
     uint32_t* rebuild;         // per scene: out, rebuild every list in the next step
 };
@@ -1856,16 +1873,15 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A) {
         for (uint32_t sc = blockIdx.x; sc < A.nscenes; sc += gridDim.x) {
             const uint2 br = A.scene_blk[sc];
             uint32_t m = 0;
-            const uint32_t q1 = br.y * A.wpb;
-            for (uint32_t q0 = br.x * A.wpb + t; q0 < q1; q0 += 8 * 256) {
-                uint32_t v[8];  // eight loads in flight
+            for (uint32_t q0 = br.x + t; q0 < br.y; q0 += 16 * 256) {
+                uint32_t v[16];  // sixteen loads in flight
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < 16; ++u) {
                     const uint32_t q = q0 + u * 256;
-                    v[u] = q < q1 ? A.wdisp[q] : 0u;
+                    v[u] = q < br.y ? A.wdisp[q] : 0u;
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) m = max(m, v[u]);
+                for (int u = 0; u < 16; ++u) m = max(m, v[u]);
             }
             for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
             if (lane == 0) sred[w] = m;
@@ -1881,7 +1897,7 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A) {
             }
             __syncthreads();
         }
-    if (blockIdx.x == 0 && A.lists) {
+    if (blockIdx.x == 2 && A.lists) {
         // ---- dirty cells (one block): clear the previous sort's marks, then
         // mark the 27 cells around each mover's new cell (the movers' records
         // are at hand on the patch path only; otherwise every list is rebuilt)
@@ -1929,77 +1945,110 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A) {
         if (t == 0) win_now[0] = sNW;
         return;
     }
-    if (blockIdx.x < 2) return;
-    // ---- the sweep (blocks 2..): new windows (one position past each, for
+    if (blockIdx.x < 3) return;
+    // ---- the sweep (blocks 3..): new windows (one position past each, for
     // its last boundary), then the previous windows' positions outside the
     // new ones
-    const uint32_t nw = sNW, npv = sNP, M2 = M;
-    auto elem = [&](uint32_t j, uint32_t& key) -> uint32_t {
-        uint32_t c1 = 0;
+    // The movers and windows are held in registers (lane r: mover r, window
+    // r) and read by uniform lane index; the keys around a position are loaded
+    // before its element is worked out (a stayer in a window comes from at
+    // most one position away per overlapping mover, usually one).
+    const uint32_t nw = (uint32_t)__builtin_amdgcn_readfirstlane((int)sNW);
+    const uint32_t npv = (uint32_t)__builtin_amdgcn_readfirstlane((int)sNP);
+    const uint32_t M2 = (uint32_t)__builtin_amdgcn_readfirstlane((int)M);
+    const int vOut = (int)(lane < M2 ? sOut[lane] : 0xFFFFFFFFu);
+    const int vKey = (int)(lane < M2 ? (uint32_t)(sX[lane] >> 32) : 0u);
+    const int vSrc = (int)(lane < M2 ? sSrc[lane] : 0u);
+    const int vAsc = (int)(lane < M2 ? sAsc[lane] : 0xFFFFFFFFu);
+    const int vWa = (int)(lane < nw ? sWin[2 * lane] : 0u), vWb = (int)(lane < nw ? sWin[2 * lane + 1] : 0u);
+    const int vPa = (int)(lane < npv ? sPrev[2 * lane] : 0u), vPb = (int)(lane < npv ? sPrev[2 * lane + 1] : 0u);
+    auto rl = [](int v, uint32_t r) { return (uint32_t)__builtin_amdgcn_readlane(v, (int)r); };
+    // element at new position j: the mover placed there, or the stayer of
+    // rank j - (movers placed before j) in the previous order; its key from
+    // the preloaded keys q[0..3] = key_old[j0 - 2 .. j0 + 1] when near j0
+    auto elem = [&](uint32_t j, uint32_t j0, const uint4 q, uint32_t& key) -> uint32_t {
+        uint32_t c1 = 0, hk = 0, hs = 0;
+        bool hit = false;
         for (uint32_t r = 0; r < M2; ++r) {
-            const uint32_t o = sOut[r];
+            const uint32_t o = rl(vOut, r), rk = rl(vKey, r), rs = rl(vSrc, r);
+            c1 += o < j ? 1u : 0u;
             if (o == j) {
-                key = (uint32_t)(sX[r] >> 32);
-                return sSrc[r];
+                hit = true;
+                hk = rk;
+                hs = rs;
             }
-            if (o > j) break;
-            ++c1;
+        }
+        if (hit) {
+            key = hk;
+            return hs;
         }
         uint32_t s = j - c1;
-        for (uint32_t r = 0; r < M2; ++r) {
-            if (sAsc[r] <= s)
-                ++s;
-            else
-                break;
-        }
+        for (uint32_t r = 0; r < M2; ++r) s += rl(vAsc, r) <= s ? 1u : 0u;
         s = min(s, n - 1);  // defensive: the windows hold their own stayers
-        key = A.key_old[s];
+        if (s + 2 >= j0 && s <= j0 + 1) {
+            const uint32_t d = s + 2 - j0;
+            key = d == 0 ? q.x : d == 1 ? q.y : d == 2 ? q.z : q.w;
+        } else {
+            key = A.key_old[s];
+        }
         return s;
     };
     auto in_new = [&](uint32_t j) {
-        for (uint32_t c = 0; c < nw; ++c)
-            if (j >= sWin[2 * c] && j <= sWin[2 * c + 1]) return true;
-        return false;
+        bool in = false;
+        for (uint32_t c = 0; c < nw; ++c) in = in || (j >= rl(vWa, c) && j <= rl(vWb, c));
+        return in;
     };
     uint32_t total = 0;
-    for (uint32_t c = 0; c < nw; ++c) total += sWin[2 * c + 1] - sWin[2 * c] + 2;
+    for (uint32_t c = 0; c < nw; ++c) total += rl(vWb, c) - rl(vWa, c) + 2;
     const uint32_t total_new = total;
-    for (uint32_t c = 0; c < npv; ++c) total += sPrev[2 * c + 1] - sPrev[2 * c] + 1;
-    for (uint32_t g = (blockIdx.x - 2) * 256 + t; g < total; g += (gridDim.x - 2) * 256) {
+    for (uint32_t c = 0; c < npv; ++c) total += rl(vPb, c) - rl(vPa, c) + 1;
+    for (uint32_t g = (blockIdx.x - 3) * 256 + t; g < total; g += (gridDim.x - 3) * 256) {
         if (g < total_new) {
-            uint32_t c = 0, off = g;
-            while (off >= sWin[2 * c + 1] - sWin[2 * c] + 2) {
-                off -= sWin[2 * c + 1] - sWin[2 * c] + 2;
-                ++c;
+            // the window holding slot g (windows are disjoint and ordered)
+            uint32_t a = 0, b = 0, j = n + 1, base = 0;
+            for (uint32_t c = 0; c < nw; ++c) {
+                const uint32_t ca = rl(vWa, c), cb = rl(vWb, c), len = cb - ca + 2;
+                if (g >= base && g < base + len) {
+                    a = ca;
+                    b = cb;
+                    j = ca + (g - base);
+                }
+                base += len;
             }
-            const uint32_t a = sWin[2 * c], b = sWin[2 * c + 1], j = a + off;
             if (j > n) continue;
+            // the previous order's keys at j - 2 .. j + 1 (outside every window
+            // a position keeps its key)
+            auto kat = [&](uint32_t x, int d) -> uint32_t {
+                const int64_t y = (int64_t)x + d;
+                return y >= 0 && y < (int64_t)n ? A.key_old[(uint32_t)y] : EMPTY;
+            };
+            const uint4 q = make_uint4(kat(j, -2), kat(j, -1), kat(j, 0), kat(j, 1));
             uint32_t kc = EMPTY, kp = EMPTY;
             if (j <= b) {
-                const uint32_t e = elem(j, kc);
+                const uint32_t e = elem(j, j, q, kc);
                 A.perm[j] = e;
                 A.invperm[e] = j;
                 A.skey[j] = kc;
             } else if (j < n) {
-                kc = A.skey[j];  // outside every window: unchanged
+                kc = q.z;  // outside every window: unchanged
             }
             if (j > 0) {
                 if (j - 1 >= a)
-                    (void)elem(j - 1, kp);
+                    (void)elem(j - 1, j, q, kp);
                 else
-                    kp = A.skey[j - 1];
+                    kp = q.y;
             }
             if (j == 0 || j == n || kp != kc) {
                 if (j < n && kc < A.ncell_cap) A.crange[kc].x = j;
                 if (j > 0 && kp < A.ncell_cap) A.crange[kp].y = j;
             }
         } else {
-            uint32_t c = 0, off = g - total_new;
-            while (off >= sPrev[2 * c + 1] - sPrev[2 * c] + 1) {
-                off -= sPrev[2 * c + 1] - sPrev[2 * c] + 1;
-                ++c;
+            uint32_t j = n, base = total_new;
+            for (uint32_t c = 0; c < npv; ++c) {
+                const uint32_t ca = rl(vPa, c), len = rl(vPb, c) - ca + 1;
+                if (g >= base && g < base + len) j = ca + (g - base);
+                base += len;
             }
-            const uint32_t j = sPrev[2 * c] + off;
             if (j < n && !in_new(j)) {
                 A.perm[j] = j;
                 A.invperm[j] = j;
