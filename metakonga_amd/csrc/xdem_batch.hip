@@ -278,7 +278,6 @@ struct xdem_batch {
         a.crange = crange;
         a.moverbits = moverbits;
         a.blockcnt = blockcnt;
-        a.mcount = counters + (1 - skcur);
         a.msorted = counters + 2;
         a.mkey = mkey;
         a.ms = ms;
@@ -733,14 +732,11 @@ int32_t run_sort(xdem_batch* b) {
     uint32_t* knew_sorted = b->knext();
     const uint2* zi = b->buf[b->cur].zi;
     SortArgs a = b->sort_args(zi);
-    // at most FEW movers: k_patch re-places them and the three kernels after it
-    // exit; otherwise k_patch only does the bookkeeping
+    // at most FEW movers: k_patch re-places them and the two kernels after it
+    // exit; otherwise k_patch does the bookkeeping and sorts the movers (one
+    // block) for the full merge (phase 3, the separate mover sort, is unused)
     XDEM_TRY(T.begin(2, st));
-    hipLaunchKernelGGL(k_patch, dim3(PATCH_GRID), dim3(256), 0, st, b->patch_args(zi));
-    XDEM_LAUNCHED();
-    XDEM_TRY(T.end(st));
-    XDEM_TRY(T.begin(3, st));
-    hipLaunchKernelGGL(k_sort_movers, dim3(1), dim3(1024), 0, st, a);
+    hipLaunchKernelGGL(k_patch, dim3(PATCH_GRID), dim3(256), 0, st, b->patch_args(zi), a);
     XDEM_LAUNCHED();
     XDEM_TRY(T.end(st));
     XDEM_TRY(T.begin(4, st));

@@ -6,13 +6,16 @@
 //            the scene's planes) + Verlet velocity update + next Verlet position
 //            update + next cell key, written in the new sorted order (the
 //            physical reorder rides on the writes);
-//   k_sort_movers / k_merge / k_ranges
+//   k_patch / k_merge / k_ranges
 //            re-sort by (cell key, particle id): only the particles whose cell
 //            changed ("movers", recorded by k_step / k_pos together with their
-//            insertion point) are sorted; the rest keep their order, and a
-//            merge gives every particle its new position.  The result is the
-//            unique (key, id) order, i.e. what the reference's stable sort of
-//            the id-ordered keys produces (source/mphysics_cuda_def.cu:92-94).
+//            insertion point) are sorted; the rest keep their order.  Up to 64
+//            movers, k_patch rewrites only the windows of positions they shift
+//            (k_merge and k_ranges exit); otherwise one block of k_patch sorts
+//            the movers and k_merge / k_ranges merge them with the rest and
+//            rebuild the cell ranges.  The result is the unique (key, id)
+//            order, i.e. what the reference's stable sort of the id-ordered
+//            keys produces (source/mphysics_cuda_def.cu:92-94).
 #pragma once
 #include "xdem_common.hpp"
 #include "xdem_geom.hpp"
@@ -1041,19 +1044,27 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
                     // the list: entries are storage indices of `in`; each is
                     // rewritten with its sorted position (its storage index in
                     // `out`) where that differs, in place (slot s == j)
-                    constexpr int LU = 2;  // entries whose loads are issued together
+                    // LU entries per batch; the next batch's entries are loaded
+                    // while this batch's records are in flight
+                    constexpr int LU = UNR < 2 ? 2 : UNR;
+                    uint32_t en[LU];
+#pragma unroll
+                    for (int u = 0; u < LU; ++u)
+                        en[u] = (uint32_t)u < vcnt ? A.vl[(size_t)u * A.vl_stride + s] : s;
 #pragma unroll 1
                     for (uint32_t k = 0; k < vcnt; k += LU) {
                         uint32_t e[LU], nx2[LU];
                         float4 f[LU];
 #pragma unroll
-                        for (int u = 0; u < LU; ++u)
-                            e[u] = k + u < vcnt ? A.vl[(size_t)(k + u) * A.vl_stride + s] : s;
-#pragma unroll
                         for (int u = 0; u < LU; ++u) {
-                            e[u] = min(e[u], A.n - 1);  // defensive: entries are < n
+                            e[u] = min(en[u], A.n - 1);  // defensive: entries are < n
                             f[u] = at32(A.in.pf, e[u]);
                             nx2[u] = at32(A.invperm, e[u]);
+                        }
+#pragma unroll
+                        for (int u = 0; u < LU; ++u) {
+                            const uint32_t kn = k + LU + u;
+                            en[u] = kn < vcnt ? A.vl[(size_t)kn * A.vl_stride + s] : s;
                         }
 #pragma unroll
                         for (int u = 0; u < LU; ++u)
@@ -1309,8 +1320,7 @@ struct SortArgs {
     uint64_t* moverbits;        // one bit per storage position (set by record_mover)
     uint32_t* blockcnt;         // movers per ROWS_T block; scanned in place (+1 total)
                                 // when there are more than MERGE_LDS movers
-    uint32_t* mcount;           // device counter of movers (reset by k_sort_movers)
-    uint32_t* msorted;          // number of sorted movers (written by k_sort_movers)
+    uint32_t* msorted;          // number of movers of this sort (written by k_patch)
     uint64_t* mkey;             // mover (key << 32 | id), append order
     uint32_t* ms;               // mover storage index, append order
     uint32_t* mplo;             // mover insertion point in the previous order, append order;
@@ -1382,49 +1392,47 @@ __device__ inline void bitonic_wave(uint64_t& key, uint32_t& val) {
         }
 }
 
-// Single-workgroup: the movers sorted by (key, id) — one wave with register
-// shuffles up to 64 movers, bitonic in LDS up to
-// SORT_LDS_MAX, else an LSD radix sort in global memory by the one workgroup
-// (large admissions).  Up to MERGE_LDS movers, their storage indices are
-// sorted too (k_merge counts the movers below a position in that list);
-// beyond, the per-block mover counts are scanned for k_merge's bit counts.
-__global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
+// One workgroup of NT threads (called by one block of k_patch when the window
+// patch does not apply): the M movers sorted by (key, id) — one wave with
+// register shuffles up to 64 movers, bitonic in LDS up to SORT_LDS_MAX, else
+// an LSD radix sort in global memory by the one workgroup (large admissions).
+// Up to MERGE_LDS movers, their storage indices are sorted too (k_merge
+// counts the movers below a position in that list); beyond, the per-block
+// mover counts are scanned for k_merge's bit counts.
+template <int NT>
+__device__ void sort_movers_block(const SortArgs& A, const uint32_t M) {
+    constexpr int NW = NT / 64;
+    static_assert(NW * 256 <= SORT_LDS_MAX, "radix wave counts live in lv");
     __shared__ uint64_t lk[SORT_LDS_MAX];
     __shared__ uint32_t lv[SORT_LDS_MAX];
-    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t wsum[NW];
     __shared__ uint32_t carry_s;
     __shared__ uint32_t hist[256];
-    uint32_t* wcnt = lv;  // 16 waves x 256 digits, only in the radix path (lv unused there)
+    uint32_t* wcnt = lv;  // NW waves x 256 digits, only in the radix path (lv unused there)
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    if (*A.patched) return;
-    const uint32_t M = min(*A.mcount, A.n);
     const bool few = M <= (uint32_t)MERGE_LDS;
     // exclusive scan of blockcnt[0..nblk) in place, total at [nblk]
     if (t == 0) carry_s = 0;
     __syncthreads();
-    for (uint32_t base = 0; !few && base < A.nblk; base += 1024) {
+    for (uint32_t base = 0; !few && base < A.nblk; base += NT) {
         uint32_t e = base + t;
         uint32_t v = e < A.nblk ? A.blockcnt[e] : 0;
         uint32_t inc = wave_scan_incl(v);
         if (lane == 63) wsum[w] = inc;
         __syncthreads();
         if (w == 0) {
-            uint32_t x = lane < 16 ? wsum[lane] : 0;
+            uint32_t x = lane < (uint32_t)NW ? wsum[lane] : 0;
             uint32_t xi = wave_scan_incl(x);
-            if (lane < 16) wsum[lane] = xi - x;
+            if (lane < (uint32_t)NW) wsum[lane] = xi - x;
         }
         __syncthreads();
         uint32_t carry = carry_s;
         if (e < A.nblk) A.blockcnt[e] = carry + wsum[w] + inc - v;
         __syncthreads();
-        if (t == 1023) carry_s = carry + wsum[w] + inc;
+        if (t == NT - 1) carry_s = carry + wsum[w] + inc;
         __syncthreads();
     }
-    if (t == 0) {
-        if (!few) A.blockcnt[A.nblk] = carry_s;
-        *A.msorted = M;
-        *A.mcount = 0;
-    }
+    if (t == 0 && !few) A.blockcnt[A.nblk] = carry_s;
     if (M == 0) return;
     if (M <= 64) {
         // wave 0 sorts the keys, wave 1 the storage indices (independent)
@@ -1448,32 +1456,32 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
     if (M <= (uint32_t)SORT_LDS_MAX) {
         uint32_t P = 2;
         while (P < M) P <<= 1;
-        for (uint32_t i = t; i < P; i += 1024) {
+        for (uint32_t i = t; i < P; i += NT) {
             lk[i] = i < M ? A.mkey[i] : ~0ull;
             lv[i] = i;  // payload: the append index
         }
         __syncthreads();
         bitonic_lds(lk, lv, P);
-        for (uint32_t i = t; i < M; i += 1024) {
+        for (uint32_t i = t; i < M; i += NT) {
             A.mkey2[i] = lk[i];
             A.ms2[i] = lv[i];
         }
         if (few) {
             // sorted storage indices (P <= MERGE_LDS fits the free part of lv)
             __syncthreads();
-            for (uint32_t i = t; i < P; i += 1024) lv[i] = i < M ? A.ms[i] : 0xFFFFFFFFu;
+            for (uint32_t i = t; i < P; i += NT) lv[i] = i < M ? A.ms[i] : 0xFFFFFFFFu;
             __syncthreads();
             bitonic_lds(lv, (uint32_t*)nullptr, P);
-            for (uint32_t i = t; i < M; i += 1024) A.mplo[A.n + i] = lv[i];
+            for (uint32_t i = t; i < M; i += NT) A.mplo[A.n + i] = lv[i];
         }
         return;
     }
     // LSD radix over the id bits then the key bits; 8-bit digits; stable by
-    // processing 1024-item chunks in order.  Payload: the append index,
+    // processing NT-item chunks in order.  Payload: the append index,
     // ping-ponged between the second half of the mplo buffer and ms2 in step
     // with the keys (mkey, mkey2).
     uint32_t* scratch = A.mplo + A.n;
-    for (uint32_t i = t; i < M; i += 1024) scratch[i] = i;
+    for (uint32_t i = t; i < M; i += NT) scratch[i] = i;
     __syncthreads();
     uint64_t* src_k = A.mkey;
     uint32_t* src_v = scratch;
@@ -1487,7 +1495,7 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
         int sh = shifts[p];
         if (t < 256) hist[t] = 0;
         __syncthreads();
-        for (uint32_t i = t; i < M; i += 1024) atomicAdd(&hist[(uint32_t)(src_k[i] >> sh) & 255u], 1u);
+        for (uint32_t i = t; i < M; i += NT) atomicAdd(&hist[(uint32_t)(src_k[i] >> sh) & 255u], 1u);
         __syncthreads();
         if (t == 0) {
             uint32_t run = 0;
@@ -1499,8 +1507,8 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
         }
         __syncthreads();
         const uint64_t lt = lanemask_lt64();
-        for (uint32_t base = 0; base < M; base += 1024) {
-            for (uint32_t i = t; i < 16 * 256; i += 1024) wcnt[i] = 0;
+        for (uint32_t base = 0; base < M; base += NT) {
+            for (uint32_t i = t; i < NW * 256; i += NT) wcnt[i] = 0;
             __syncthreads();
             uint32_t e = base + t;
             bool valid = e < M;
@@ -1519,7 +1527,7 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
             __syncthreads();
             if (t < 256) {  // exclusive over waves per digit; chunk total of digit t
                 uint32_t run = 0;
-                for (int k = 0; k < 16; ++k) {
+                for (int k = 0; k < NW; ++k) {
                     uint32_t c = wcnt[k * 256 + t];
                     wcnt[k * 256 + t] = run;
                     run += c;
@@ -1545,7 +1553,7 @@ __global__ __launch_bounds__(1024) void k_sort_movers(SortArgs A) {
         __syncthreads();
     }
     if (src_k != A.mkey2) {
-        for (uint32_t i = t; i < M; i += 1024) {
+        for (uint32_t i = t; i < M; i += NT) {
             A.mkey2[i] = src_k[i];
             A.ms2[i] = src_v[i];
         }
@@ -1688,8 +1696,8 @@ __global__ __launch_bounds__(ROWS_T) void k_ranges(const uint32_t* skey, uint2* 
 // mover placed at j, or the stayer of rank j - (movers placed before j) in the
 // old order), and the cell boundaries at window positions are rewritten from
 // the new keys.  Positions of the previous sort's windows outside the new
-// ones get the identity permutation back.  More movers: nothing is done here
-// and k_sort_movers / k_merge / k_ranges sort (patched = 0).
+// ones get the identity permutation back.  More movers: block 0 sorts them
+// (sort_movers_block) and k_merge / k_ranges do the rest (patched = 0).
 constexpr int WIN_WORDS = 2 + 2 * FEW;   // per parity: count, then (first, last) pairs
 constexpr int DPREV_WORDS = 1 + 2 * FEW; // per parity: count, then (cell key, scene) pairs
 
@@ -1754,7 +1762,7 @@ __device__ inline uint32_t wave_rank(uint64_t key, uint32_t M) {
     return rank;
 }
 
-__global__ __launch_bounds__(256) void k_patch(PatchArgs A) {
+__global__ __launch_bounds__(256) void k_patch(PatchArgs A, SortArgs SA) {
     __shared__ uint64_t sX[FEW];      // movers by new (key, id)
     __shared__ uint32_t sSrc[FEW];    // their storage index
     __shared__ uint32_t sOut[FEW];    // their new position (increasing)
@@ -1920,7 +1928,11 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A) {
         }
         if (t == 0) dn[0] = nm;
     }
-    if (!few) return;
+    if (!few) {
+        // the full merge follows: block 0 sorts the movers for it
+        if (blockIdx.x == 0) sort_movers_block<256>(SA, M);
+        return;
+    }
     if (blockIdx.x == 1) {
         // ---- a cell every particle left (and none entered) becomes empty;
         // the movers' marks of record_mover are cleared for the next step; the
