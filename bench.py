@@ -106,7 +106,7 @@ def run_batched(nscene, steps, lib, block=100, seed=0xD3E3):
     for s in scs:
         b.add(s)
     b.step(block)          # upload + first (full) sort + first block, untimed
-    tuned = b.autotune(steps_per_trial=5)
+    tuned = b.autotune(steps_per_trial=20)
     b.step(block)
     b.sync()
     t0 = time.perf_counter()
@@ -119,7 +119,7 @@ def run_batched(nscene, steps, lib, block=100, seed=0xD3E3):
     b.close()
     return dict(scenes=nscene, particles=ntot, steps=done, cells=cells, wall_s=wall,
                 value=ntot * done / wall, generation_s=gen,
-                autotune={"chosen": tuned[0], "kernel_ms": tuned[1]})
+                autotune={"chosen": tuned[0], "step_ms": tuned[1]})
 
 
 def run_settling_8k(lib, steps=2000):
@@ -162,7 +162,7 @@ def run_c2_drum(lib, steps=200, f64_steps=20):
     b = batch.Batch(sc.np, 1, int(np.prod(sc.grid_size)), lib)
     sid = b.add(sc)
     b.step(20)
-    tuned = b.autotune(steps_per_trial=5)
+    tuned = b.autotune(steps_per_trial=20)
     b.step(100)
     b.sync()
     t0 = time.perf_counter()
@@ -220,20 +220,27 @@ def run_serving(duration, lib, rate=20.0, block=100, seed=0xD3E4):
     return rep
 
 
-def fp32c_error_vs_oracle(lib, n=1_000_000, steps=300, speed=1.0):
+def fp32c_error_vs_oracle(lib, n=1_000_000, steps=300, speed=1.0, sc=None, label=None):
     """Relative force and torque error of the fused FP32c contact evaluation
-    against the FP64 oracle on one state of the hopper workload: the seeded
-    fill given random velocities of `speed` m/s and advanced `steps` steps on
-    the fused path (so that spheres collide), then every particle's contact
-    force and torque evaluated by both on that state.  Per particle the error
-    is |F_gpu - F_oracle|_inf / (sum_k |F_k| + m g), the sum over its contacts
-    as the oracle evaluates them; the maximum over particles is reported."""
+    against the FP64 oracle on one state of the hopper workload (or of `sc`):
+    the seeded fill given random velocities of `speed` m/s (none for `sc`) and
+    advanced `steps` steps on the fused path (so that spheres collide), then
+    every particle's contact force and torque evaluated by both on that state.
+    Per particle the error is |F_gpu - F_oracle|_inf / (sum_k |F_k| + m g), the
+    sum over its contacts as the oracle evaluates them; the maximum over
+    particles is reported."""
     from metakonga_amd import batch
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     pyoracle.load()
-    sc = scenes.hopper(n=n)
-    vel = scenes.as_f32_exact(scenes.uniform(0xABC, 3 * n, 5).reshape(n, 3) * 2.0 * speed - speed)
+    if sc is None:
+        sc = scenes.hopper(n=n)
+        vel = scenes.as_f32_exact(scenes.uniform(0xABC, 3 * n, 5).reshape(n, 3) * 2.0 * speed - speed)
+        label = (f"hopper {n} spheres after {steps} steps from the fill with random "
+                 f"velocities of {speed} m/s (FP32c path), forces of every particle")
+    else:
+        n = sc.np
+        vel = None
     ncell = int(np.prod(sc.grid_size))
     b = batch.Batch(n, 1, ncell, lib)
     sid = b.add(sc, vel)
@@ -271,9 +278,39 @@ def fp32c_error_vs_oracle(lib, n=1_000_000, steps=300, speed=1.0):
             "force_rel_err_p9999_net_scale": float(np.quantile(en, 0.9999)),
             "net_scale": "per particle: sum over its contacts of |Fn + Ft| + m g",
             "share_within_1e-5_net_scale": float((en <= 1e-5).mean()),
-            "contact_pairs": int(npairs), "grid_and_pair_count_exact": exact,
-            "state": f"hopper {n} spheres after {steps} steps from the fill with random "
-                     f"velocities of {speed} m/s (FP32c path), forces of every particle"}
+            "contact_pairs": int(npairs), "contacts_per_particle": int(npairs) / n,
+            "grid_and_pair_count_exact": exact, "state": label}
+
+
+def run_dense_bed(lib, steps=200, f64_steps=20, err=True):
+    """Contact-rich leg: 1M spheres (r U[3.92, 4] mm) in a compressed
+    face-centred cubic packing in a closed box (about 9-10 contacts per sphere,
+    solid fraction about 0.7), on the fused FP32c path (autotuned, graph
+    blocks; kernel time from HIP events for the roofline) and on the plain
+    FP64 path, with the FP32c force error against the FP64 oracle on the state
+    after 100 steps."""
+    from metakonga_amd import batch
+    sc = scenes.dense_bed()
+    fused = batch.bench_fused(sc, steps, 20, lib)
+    k_ms = fused["phase_ms"]["contact_verlet"]
+    f64 = run_fp64(sc, f64_steps, 3, lib)
+    out = {"workload": f"dense bed {sc.np} spheres, FCC lattice at 0.985 x 2r, radii "
+                       f"U[3.92, 4] mm, closed 6-plane box, dt 5e-6 s, cell 8 mm, "
+                       f"grid {sc.grid_size[0]}^3",
+           "fp32c_fused": fused["value"], "ms_per_step": fused["ms_per_step"],
+           "phase_ms": fused["phase_ms"], "autotune_chosen": (fused.get("autotune") or {}).get("chosen"),
+           "fp64_plain": f64["value"], "fp64_steps": f64_steps,
+           "speedup_vs_fp64": fused["value"] / f64["value"],
+           "roofline": {"bound": "hbm", "kernel_ms": k_ms,
+                        "achieved": STEP_F32C_BYTES * sc.np / (k_ms * 1e-3) / 1e9,
+                        "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": STEP_F32C_BYTES * sc.np / (k_ms * 1e-3) / HBM_PEAK,
+                        "step_frac": STEP_F32C_BYTES * fused["value"] / HBM_PEAK}}
+    if err:
+        out["fp32c_force_err_vs_oracle"] = fp32c_error_vs_oracle(
+            lib, steps=100, sc=sc, label=f"dense bed {sc.np} spheres after 100 steps (FP32c "
+                                         f"path), forces of every particle")
+    return out
 
 
 def cpu_subset(scs, budget_s, threads=1):
@@ -360,6 +397,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=1000)
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="0 to skip serving")
     ap.add_argument("--skip-fp64", action="store_true", help="skip the plain FP64 leg")
+    ap.add_argument("--dense-steps", type=int, default=200,
+                    help="steps of the contact-rich 1M dense-bed leg (0 to skip)")
     ap.add_argument("--c2-steps", type=int, default=200,
                     help="steps of the 4e6-sphere drum leg (0 to skip)")
     ap.add_argument("--no-graphs", action="store_true", help="eager launches (profiler runs)")
@@ -453,6 +492,11 @@ def main():
         small = run_settling_8k(lib)
         log(f"settling 8k: {small}")
 
+    dense = None
+    if fused is not None and not args.skip_fp64 and args.dense_steps > 0:
+        dense = run_dense_bed(lib, args.dense_steps, err=not args.no_cpu)
+        log(f"dense bed: {dense}")
+
     c2 = None
     if fused is not None and not args.skip_fp64 and args.c2_steps > 0:
         c2 = run_c2_drum(lib, args.c2_steps)
@@ -531,6 +575,8 @@ def main():
         out["batched"] = batched
     if small is not None:
         out["settling_8k"] = small
+    if dense is not None:
+        out["dense_bed"] = dense
     if c2 is not None:
         out["c2_drum"] = c2
     if cpu_omp is not None:

@@ -171,17 +171,21 @@ int32_t xdem_batch_set_launch(xdem_batch* b, uint32_t threads, uint32_t unroll, 
  * (default) walks its own non-empty rows as one stream, 0 walks row by row in
  * step with the other lanes of its wave.  Results do not depend on it. */
 int32_t xdem_batch_set_walk(xdem_batch* b, int32_t per_lane);
-/* Autotuner: times the contact kernel for both walks (xdem_batch_set_walk), block
- * sizes 64, 128 and 256, each with 1, 2 or 4 neighbour candidates loaded
- * together and with the contact records loaded one contact ahead or not, over
- * steps_per_trial real steps (the scenes advance 36 * (steps_per_trial + 1)
- * steps), and keeps the fastest.  *chosen = block size + 1000 *
- * candidates-per-load-batch + 10000 * (1 if contact records are loaded ahead)
- * + 100000 * (1 for the per-lane walk); ms_per_candidate (36 entries: walk-major
- * (row by row, then per lane), then block size, then candidates-per-batch, then
- * load-ahead off/on; may be NULL) receives the measured kernel time per launch
- * (-1 for the load-ahead shapes while a scene has history on or drums/meshes:
- * they are not used then).  Results do not depend on the choice (bit-identical). */
+/* Autotuner: for each launch shape — neighbour search (the walk row by row, the
+ * per-lane walk, or the per-lane walk with candidate lists, xdem_batch_set_walk /
+ * xdem_batch_set_lists), block sizes 64, 128 and 256, 1, 2 or 4 neighbour
+ * candidates loaded together, contact records loaded one contact ahead or not —
+ * times a whole graph-replayed block of steps_per_trial steps (rounded up to
+ * even; 0 means 20) after a warm block of the same length (HIP events on the
+ * batch stream: contacts, Verlet and re-sort), and keeps the fastest.  The
+ * scenes advance 54 * (2 * steps_per_trial + 1) steps at most.  *chosen = block
+ * size + 1000 * candidates-per-load-batch + 10000 * (1 if contact records are
+ * loaded ahead) + 100000 * (1 for the per-lane walk) + 1000000 * (1 with
+ * candidate lists); ms_per_candidate (54 entries: search-major, then block size,
+ * then candidates-per-batch, then load-ahead off/on; may be NULL) receives the
+ * measured time per step (-1 for shapes not used here: load-ahead while a scene
+ * has history on or drums/meshes, and candidate lists then too).  Results do
+ * not depend on the choice (bit-identical). */
 int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* chosen,
                             double* ms_per_candidate);
 int32_t xdem_batch_synchronize(xdem_batch* b);

@@ -208,8 +208,9 @@ class Batch:
     def sort_patch(self, on=True):
         abi.check(self.lib, self.lib.xdem_batch_set_sort_patch(self.h, int(bool(on))))
 
-    def autotune(self, steps_per_trial=10):
-        """Pick the contact-kernel launch shape by timing (advances the scenes)."""
+    def autotune(self, steps_per_trial=20):
+        """Pick the launch shape by timing whole graph-replayed step blocks
+        (advances the scenes).  Returns (chosen code, {shape: ms per step})."""
         chosen = C.c_uint32()
         ms = np.zeros(54)
         abi.check(self.lib, self.lib.xdem_batch_autotune(self.h, steps_per_trial, C.byref(chosen),
@@ -321,7 +322,7 @@ def bench_fused(scene, steps, warmup, lib, block=100, graphs=True, autotune=True
     sid = b.add(scene)
     blk = min(block, max(1, steps))
     b.step(max(1, warmup))
-    tuned = b.autotune(steps_per_trial=5) if autotune else None
+    tuned = b.autotune(steps_per_trial=20) if autotune else None
     b.step(blk)  # settle the chosen launch shape (and capture its graph)
     b.sync()
     t0 = time.perf_counter()
@@ -360,7 +361,7 @@ def bench_fused(scene, steps, warmup, lib, block=100, graphs=True, autotune=True
     b.close()
     return dict(wall_s=wall, ms_per_step=wall * 1e3 / steps, value=scene.np * steps / wall,
                 phase_ms=per, movers_last=movers, timed_steps_events=nt,
-                autotune=None if tuned is None else {"chosen": tuned[0], "kernel_ms": tuned[1]},
+                autotune=None if tuned is None else {"chosen": tuned[0], "step_ms": tuned[1]},
                 pcie_inclusive={"value": scene.np * steps / pcie_wall, "wall_s": pcie_wall,
                                 "steps": steps})
 

@@ -736,7 +736,7 @@ int32_t run_sort(xdem_batch* b) {
     // exit; otherwise k_patch does the bookkeeping and sorts the movers (one
     // block) for the full merge (phase 3, the separate mover sort, is unused)
     XDEM_TRY(T.begin(2, st));
-    hipLaunchKernelGGL(k_patch, dim3(PATCH_GRID), dim3(256), 0, st, b->patch_args(zi), a);
+    hipLaunchKernelGGL(k_patch, dim3(PATCH_GRID), dim3(PATCH_T), 0, st, b->patch_args(zi), a);
     XDEM_LAUNCHED();
     XDEM_TRY(T.end(st));
     XDEM_TRY(T.begin(4, st));
@@ -1249,8 +1249,24 @@ int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* c
     XDEM_REQUIRE(b, "null batch");
     const uint32_t cand[3] = {64, 128, 256};
     const int unr[3] = {1, 2, 4};
-    if (steps_per_trial == 0) steps_per_trial = 10;
+    if (steps_per_trial == 0) steps_per_trial = 20;
+    steps_per_trial += steps_per_trial & 1u;  // even: the timed block replays the warm one's graph
     bool was_on = b->timer.on;
+    XDEM_TRY(xdem_batch_enable_timing(b, 0));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    XDEM_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        set_error("hipEventCreate failed");
+        return XDEM_ERR_HIP;
+    }
+    struct EvGuard {
+        hipEvent_t a, b;
+        ~EvGuard() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{e0, e1};
     double best = 1e300;
     uint32_t best_c = b->block_threads;
     int best_u = b->unroll;
@@ -1274,12 +1290,14 @@ int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* c
                 b->drop_graphs();
                 XDEM_TRY(xdem_batch_set_block(b, cand[i]));
                 XDEM_TRY(xdem_batch_step(b, 1));  // applies the new tables
-                XDEM_TRY(xdem_batch_enable_timing(b, 1));
+                XDEM_TRY(xdem_batch_step(b, steps_per_trial));  // warm (graph captured)
+                XDEM_HIP(hipEventRecord(e0, b->st));
                 XDEM_TRY(xdem_batch_step(b, steps_per_trial));
-                double ms[6];
-                uint32_t cnt[6];
-                XDEM_TRY(xdem_batch_phase_times(b, ms, cnt));
-                double per = cnt[1] ? ms[1] / cnt[1] : 1e300;
+                XDEM_HIP(hipEventRecord(e1, b->st));
+                XDEM_HIP(hipEventSynchronize(e1));
+                float el = 0.0f;
+                XDEM_HIP(hipEventElapsedTime(&el, e0, e1));
+                const double per = (double)el / steps_per_trial;
                 if (ms_per_candidate) ms_per_candidate[slot] = per;
                 if (per < best) {
                     best = per;
@@ -1289,7 +1307,6 @@ int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* c
                     best_f = w >= 1;
                     best_l = w == 2;
                 }
-                XDEM_TRY(xdem_batch_enable_timing(b, 0));
             }
     b->unroll = best_u;
     b->pipe = best_p;

@@ -29,11 +29,13 @@ constexpr int HSLOTS = 16;                   // tangential-history slots per par
 constexpr int RTAB = 7;                      // neighbour rows kept in LDS during the walk
 constexpr uint32_t WALL_KEY = 0x80000000u;   // history key of plane p: WALL_KEY | p
 constexpr int LDS_T = 256;                   // lane stride of the per-thread LDS lists
-constexpr int SORT_LDS_MAX = 4096;           // movers sorted in LDS up to this count
 constexpr int MERGE_LDS = 2048;              // sorted movers staged in LDS by k_merge up to this count
 constexpr int ROWS_T = 256;                  // threads of the row kernels
 constexpr int FEW = 64;                      // movers re-placed by the window patch (k_patch)
-constexpr int PATCH_GRID = 512;              // blocks of k_patch (grid-stride over the windows)
+constexpr int PATCH_T = 1024;                // threads of a k_patch block
+constexpr int PATCH_GRID = 256;              // blocks of k_patch (grid-stride over the windows)
+constexpr int RANK_MAX = 32768;              // movers sorted by counting ranks (k_patch, every block)
+constexpr int RANK_CHUNK = 2048;             // mover keys staged in LDS per counting pass
 constexpr int SWEEP_GRID = 1024;             // blocks of the grid-stride sort kernels
 constexpr int VLK = 32;                      // candidate-list entries per particle
 constexpr uint32_t VL_BAD = 0xFFFFFFFFu;     // no usable candidate list (walk every step)
@@ -1346,136 +1348,49 @@ __device__ inline uint32_t wave_scan_incl(uint32_t v) {
     return v;
 }
 
-// In-LDS bitonic sort of P (a power of two) entries by the whole-block loop.
-template <class K>
-__device__ inline void bitonic_lds(K* key, uint32_t* val, uint32_t P) {
-    const uint32_t t = threadIdx.x;
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-        for (uint32_t jb = k >> 1; jb > 0; jb >>= 1) {
-            for (uint32_t i = t; i < P; i += blockDim.x) {
-                uint32_t ix = i ^ jb;
-                if (ix > i) {
-                    bool up = (i & k) == 0;
-                    K a = key[i], b = key[ix];
-                    if ((a > b) == up) {
-                        key[i] = b;
-                        key[ix] = a;
-                        if (val) {
-                            uint32_t tv = val[i];
-                            val[i] = val[ix];
-                            val[ix] = tv;
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// Bitonic sort of one value per lane across a 64-lane wave (register
-// shuffles, no barrier); val rides with key.  Keys must be distinct except
-// padding (the largest value), whose payloads are never read.
-__device__ inline void bitonic_wave(uint64_t& key, uint32_t& val) {
-    const uint32_t lane = __lane_id();
-    for (uint32_t k = 2; k <= 64; k <<= 1)
-        for (uint32_t jb = k >> 1; jb > 0; jb >>= 1) {
-            const uint32_t lo = __shfl_xor((uint32_t)key, (int)jb, 64);
-            const uint32_t hi = __shfl_xor((uint32_t)(key >> 32), (int)jb, 64);
-            const uint32_t ov = __shfl_xor(val, (int)jb, 64);
-            const uint64_t other = ((uint64_t)hi << 32) | lo;
-            const bool take_min = ((lane & jb) == 0) == ((lane & k) == 0);
-            if (take_min ? other < key : other > key) {
-                key = other;
-                val = ov;
-            }
-        }
-}
-
-// One workgroup of NT threads (called by one block of k_patch when the window
-// patch does not apply): the M movers sorted by (key, id) — one wave with
-// register shuffles up to 64 movers, bitonic in LDS up to SORT_LDS_MAX, else
-// an LSD radix sort in global memory by the one workgroup (large admissions).
-// Up to MERGE_LDS movers, their storage indices are sorted too (k_merge
-// counts the movers below a position in that list); beyond, the per-block
-// mover counts are scanned for k_merge's bit counts.
+// Exclusive scan of the per-block mover counts in place (total at [nblk]),
+// for k_merge's bit counts when more than MERGE_LDS movers; one block of NT.
 template <int NT>
-__device__ void sort_movers_block(const SortArgs& A, const uint32_t M) {
+__device__ void scan_blockcnt(const SortArgs& A) {
     constexpr int NW = NT / 64;
-    static_assert(NW * 256 <= SORT_LDS_MAX, "radix wave counts live in lv");
-    __shared__ uint64_t lk[SORT_LDS_MAX];
-    __shared__ uint32_t lv[SORT_LDS_MAX];
     __shared__ uint32_t wsum[NW];
     __shared__ uint32_t carry_s;
-    __shared__ uint32_t hist[256];
-    uint32_t* wcnt = lv;  // NW waves x 256 digits, only in the radix path (lv unused there)
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const bool few = M <= (uint32_t)MERGE_LDS;
-    // exclusive scan of blockcnt[0..nblk) in place, total at [nblk]
     if (t == 0) carry_s = 0;
     __syncthreads();
-    for (uint32_t base = 0; !few && base < A.nblk; base += NT) {
-        uint32_t e = base + t;
-        uint32_t v = e < A.nblk ? A.blockcnt[e] : 0;
-        uint32_t inc = wave_scan_incl(v);
+    for (uint32_t base = 0; base < A.nblk; base += NT) {
+        const uint32_t e = base + t;
+        const uint32_t v = e < A.nblk ? A.blockcnt[e] : 0;
+        const uint32_t inc = wave_scan_incl(v);
         if (lane == 63) wsum[w] = inc;
         __syncthreads();
         if (w == 0) {
-            uint32_t x = lane < (uint32_t)NW ? wsum[lane] : 0;
-            uint32_t xi = wave_scan_incl(x);
+            const uint32_t x = lane < (uint32_t)NW ? wsum[lane] : 0;
+            const uint32_t xi = wave_scan_incl(x);
             if (lane < (uint32_t)NW) wsum[lane] = xi - x;
         }
         __syncthreads();
-        uint32_t carry = carry_s;
+        const uint32_t carry = carry_s;
         if (e < A.nblk) A.blockcnt[e] = carry + wsum[w] + inc - v;
         __syncthreads();
         if (t == NT - 1) carry_s = carry + wsum[w] + inc;
         __syncthreads();
     }
-    if (t == 0 && !few) A.blockcnt[A.nblk] = carry_s;
+    if (t == 0) A.blockcnt[A.nblk] = carry_s;
+}
+
+// One workgroup of NT threads (block 0 of k_patch, beyond RANK_MAX movers:
+// large admissions): the M movers sorted by (key, id) with an LSD radix sort
+// in global memory, and the per-block mover counts scanned for k_merge.
+template <int NT>
+__device__ void sort_movers_block(const SortArgs& A, const uint32_t M) {
+    constexpr int NW = NT / 64;
+    __shared__ uint32_t lk[256];        // per-digit chunk totals
+    __shared__ uint32_t wcnt[NW * 256]; // per wave and digit
+    __shared__ uint32_t hist[256];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    scan_blockcnt<NT>(A);
     if (M == 0) return;
-    if (M <= 64) {
-        // wave 0 sorts the keys, wave 1 the storage indices (independent)
-        if (w == 0) {
-            uint64_t key = lane < M ? A.mkey[lane] : ~0ull;
-            uint32_t val = lane;  // payload: the append index
-            bitonic_wave(key, val);
-            if (lane < M) {
-                A.mkey2[lane] = key;
-                A.ms2[lane] = val;
-            }
-        } else if (w == 1) {
-            // sorted storage indices (distinct; padding sorts last)
-            uint64_t sk = lane < M ? (uint64_t)A.ms[lane] : ~0ull;
-            uint32_t unused = 0;
-            bitonic_wave(sk, unused);
-            if (lane < M) A.mplo[A.n + lane] = (uint32_t)sk;
-        }
-        return;
-    }
-    if (M <= (uint32_t)SORT_LDS_MAX) {
-        uint32_t P = 2;
-        while (P < M) P <<= 1;
-        for (uint32_t i = t; i < P; i += NT) {
-            lk[i] = i < M ? A.mkey[i] : ~0ull;
-            lv[i] = i;  // payload: the append index
-        }
-        __syncthreads();
-        bitonic_lds(lk, lv, P);
-        for (uint32_t i = t; i < M; i += NT) {
-            A.mkey2[i] = lk[i];
-            A.ms2[i] = lv[i];
-        }
-        if (few) {
-            // sorted storage indices (P <= MERGE_LDS fits the free part of lv)
-            __syncthreads();
-            for (uint32_t i = t; i < P; i += NT) lv[i] = i < M ? A.ms[i] : 0xFFFFFFFFu;
-            __syncthreads();
-            bitonic_lds(lv, (uint32_t*)nullptr, P);
-            for (uint32_t i = t; i < M; i += NT) A.mplo[A.n + i] = lv[i];
-        }
-        return;
-    }
     // LSD radix over the id bits then the key bits; 8-bit digits; stable by
     // processing NT-item chunks in order.  Payload: the append index,
     // ping-ponged between the second half of the mplo buffer and ms2 in step
@@ -1532,7 +1447,7 @@ __device__ void sort_movers_block(const SortArgs& A, const uint32_t M) {
                     wcnt[k * 256 + t] = run;
                     run += c;
                 }
-                lk[t] = run;  // lk is free in this path: 256-entry scratch
+                lk[t] = run;
             }
             __syncthreads();
             if (valid) {
@@ -1557,6 +1472,40 @@ __device__ void sort_movers_block(const SortArgs& A, const uint32_t M) {
             A.mkey2[i] = src_k[i];
             A.ms2[i] = src_v[i];
         }
+    }
+}
+
+// The M movers sorted by counting ranks (keys are distinct): groups of 64
+// movers, one group per block at a time (grid-stride); lane l of the group's
+// 64 holds mover l, and the block's T / 64 waves split the M keys between them
+// (staged through LDS RANK_CHUNK at a time), so each thread counts M / (T / 64)
+// keys; the partial counts are summed in LDS.  Mover r lands at its rank:
+// mkey2 / ms2 (append index).  Every thread of the block calls it.
+__device__ void rank_movers(const SortArgs& A, const uint32_t M) {
+    __shared__ uint64_t cX[RANK_CHUNK];
+    __shared__ uint32_t part[PATCH_T];
+    const uint32_t t = threadIdx.x, T = blockDim.x, lane = t & 63, w = t >> 6, W = T >> 6;
+    for (uint32_t g0 = blockIdx.x * 64; g0 < M; g0 += gridDim.x * 64) {  // block-uniform
+        const uint32_t r = g0 + lane;
+        const uint64_t myX = r < M ? A.mkey[r] : ~0ull;
+        uint32_t rk = 0;
+        for (uint32_t c0 = 0; c0 < M; c0 += RANK_CHUNK) {
+            const uint32_t cn = min((uint32_t)RANK_CHUNK, M - c0);
+            for (uint32_t i = t; i < cn; i += T) cX[i] = A.mkey[c0 + i];
+            __syncthreads();
+#pragma unroll 4
+            for (uint32_t i = w; i < cn; i += W) rk += cX[i] < myX ? 1u : 0u;
+            __syncthreads();
+        }
+        part[t] = rk;
+        __syncthreads();
+        if (w == 0 && r < M) {
+            uint32_t sum = 0;
+            for (uint32_t q = 0; q < W; ++q) sum += part[q * 64 + lane];
+            A.mkey2[sum] = myX;
+            A.ms2[sum] = r;
+        }
+        __syncthreads();
     }
 }
 
@@ -1591,34 +1540,16 @@ __device__ inline uint32_t movers_below(const uint64_t* mk, uint32_t M, uint64_t
     return lo;
 }
 
-// Entries of the sorted list v[0..M) below x.
-__device__ inline uint32_t count_below(const uint32_t* v, uint32_t M, uint32_t x) {
-    uint32_t lo = 0, hi = M;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (v[mid] < x)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 // Grid: SWEEP_GRID blocks stepping over the ROWS_T tiles of stayers, then 64
 // blocks for the movers.
 __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
     __shared__ uint64_t smk[MERGE_LDS];
-    __shared__ uint32_t sst[MERGE_LDS];
     if (*A.patched) return;
     const uint32_t M = *A.msorted;
     const bool few = M <= (uint32_t)MERGE_LDS;
-    const uint32_t* mst = A.mplo + A.n;  // sorted mover storage indices (few movers)
     if (blockIdx.x < (uint32_t)SWEEP_GRID) {
         if (few)
-            for (uint32_t i = threadIdx.x; i < M; i += ROWS_T) {
-                smk[i] = A.mkey2[i];
-                sst[i] = mst[i];
-            }
+            for (uint32_t i = threadIdx.x; i < M; i += ROWS_T) smk[i] = A.mkey2[i];
         __syncthreads();
         for (uint32_t tile = blockIdx.x; tile < A.nblk; tile += SWEEP_GRID) {
             const uint32_t s = tile * ROWS_T + threadIdx.x;
@@ -1630,7 +1561,7 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
             if (M) {
                 const uint64_t X = ((uint64_t)kn << 32) | A.zi[s].y;
                 const uint32_t lo = movers_below(few ? smk : A.mkey2, M, X);
-                out = s - (few ? count_below(sst, M, s) : mover_prefix(A, s)) + lo;
+                out = s - mover_prefix(A, s) + lo;
             }
             if (out < A.n) {
                 A.perm[out] = s;
@@ -1649,7 +1580,7 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
         const uint32_t kn = (uint32_t)(X >> 32);
         // entries of the previous order with (key, id) < X (recorded with the mover)
         const uint32_t lo = A.mplo[p];
-        uint32_t out = r + (lo - (few ? count_below(mst, M, lo) : mover_prefix(A, lo)));
+        uint32_t out = r + (lo - mover_prefix(A, lo));
         if (out < A.n && s < A.n) {
             A.perm[out] = s;
             A.invperm[s] = out;
@@ -1762,7 +1693,7 @@ __device__ inline uint32_t wave_rank(uint64_t key, uint32_t M) {
     return rank;
 }
 
-__global__ __launch_bounds__(256) void k_patch(PatchArgs A, SortArgs SA) {
+__global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
     __shared__ uint64_t sX[FEW];      // movers by new (key, id)
     __shared__ uint32_t sSrc[FEW];    // their storage index
     __shared__ uint32_t sOut[FEW];    // their new position (increasing)
@@ -1771,7 +1702,7 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A, SortArgs SA) {
     __shared__ uint4 sOld[FEW];       // their (old cell size, scene, old key)
     __shared__ uint32_t sAux[FEW];    // window ends while the windows are sorted
     __shared__ uint32_t sM, sNW, sNP;
-    __shared__ uint32_t sred[4];
+    __shared__ uint32_t sred[PATCH_T / 64];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     const uint32_t n = A.n;
     uint32_t* const win_now = A.win + A.par * WIN_WORDS;
@@ -1881,21 +1812,21 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A, SortArgs SA) {
         for (uint32_t sc = blockIdx.x; sc < A.nscenes; sc += gridDim.x) {
             const uint2 br = A.scene_blk[sc];
             uint32_t m = 0;
-            for (uint32_t q0 = br.x + t; q0 < br.y; q0 += 16 * 256) {
-                uint32_t v[16];  // sixteen loads in flight
+            for (uint32_t q0 = br.x + t; q0 < br.y; q0 += 4 * PATCH_T) {
+                uint32_t v[4];  // four loads in flight
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    const uint32_t q = q0 + u * 256;
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t q = q0 + u * PATCH_T;
                     v[u] = q < br.y ? A.wdisp[q] : 0u;
                 }
 #pragma unroll
-                for (int u = 0; u < 16; ++u) m = max(m, v[u]);
+                for (int u = 0; u < 4; ++u) m = max(m, v[u]);
             }
             for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
             if (lane == 0) sred[w] = m;
             __syncthreads();
             if (t == 0) {
-                m = max(max(sred[0], sred[1]), max(sred[2], sred[3]));
+                for (uint32_t q = 0; q < PATCH_T / 64; ++q) m = max(m, sred[q]);
                 const float D = A.disp[sc] + __uint_as_float(m);
                 const bool rb = A.force[sc] != 0 || !few || !(2.0f * D < A.scenes[sc].skin);
                 A.disp[sc] = rb ? 0.0f : D;
@@ -1912,13 +1843,13 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A, SortArgs SA) {
         uint32_t* const dp = A.dprev + (1 - A.par) * DPREV_WORDS;
         uint32_t* const dn = A.dprev + A.par * DPREV_WORDS;
         const uint32_t npv = min(dp[0], (uint32_t)FEW);
-        for (uint32_t q = t; q < 27 * npv; q += 256) {
+        for (uint32_t q = t; q < 27 * npv; q += PATCH_T) {
             const uint32_t e = q / 27;
             mark_neighbourhood(A.scenes[dp[2 + 2 * e]], dp[1 + 2 * e], q % 27, A.dirty, false);
         }
         const uint32_t nm = few ? M : 0u;
         __syncthreads();
-        for (uint32_t q = t; q < 27 * nm; q += 256) {
+        for (uint32_t q = t; q < 27 * nm; q += PATCH_T) {
             const uint32_t e = q / 27;
             mark_neighbourhood(A.scenes[sOld[e].y], (uint32_t)(sX[e] >> 32), q % 27, A.dirty, true);
         }
@@ -1929,8 +1860,16 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A, SortArgs SA) {
         if (t == 0) dn[0] = nm;
     }
     if (!few) {
-        // the full merge follows: block 0 sorts the movers for it
-        if (blockIdx.x == 0) sort_movers_block<256>(SA, M);
+        // the full merge follows: the movers are sorted for it — by counting
+        // ranks spread over the blocks (one mover per thread), or beyond
+        // RANK_MAX by one block (large admissions)
+        if (M <= (uint32_t)RANK_MAX) {
+            // the last block scans the per-block mover counts for k_merge
+            if (blockIdx.x == gridDim.x - 1) scan_blockcnt<PATCH_T>(SA);
+            rank_movers(SA, M);
+        } else if (blockIdx.x == 0) {
+            sort_movers_block<PATCH_T>(SA, M);
+        }
         return;
     }
     if (blockIdx.x == 1) {
@@ -2014,7 +1953,7 @@ __global__ __launch_bounds__(256) void k_patch(PatchArgs A, SortArgs SA) {
     for (uint32_t c = 0; c < nw; ++c) total += rl(vWb, c) - rl(vWa, c) + 2;
     const uint32_t total_new = total;
     for (uint32_t c = 0; c < npv; ++c) total += rl(vPb, c) - rl(vPa, c) + 1;
-    for (uint32_t g = (blockIdx.x - 3) * 256 + t; g < total; g += (gridDim.x - 3) * 256) {
+    for (uint32_t g = (blockIdx.x - 3) * PATCH_T + t; g < total; g += (gridDim.x - 3) * PATCH_T) {
         if (g < total_new) {
             // the window holding slot g (windows are disjoint and ordered)
             uint32_t a = 0, b = 0, j = n + 1, base = 0;

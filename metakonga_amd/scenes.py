@@ -391,6 +391,42 @@ def hopper(n=1_000_000, r_min=2e-3, r_max=4e-3, seed=0xD3E1, dt=5e-6, nsteps=500
                  world_origin=wo, cell_size=cs, grid_size=(g, g, g), planes=planes, seed=seed)
 
 
+def dense_bed(n=1_000_000, r=4e-3, seed=0xD3E5, dt=5e-6, nsteps=1000, squeeze=0.985, poly=0.02,
+              name="dense-bed") -> Scene:
+    """Contact-rich compressed packing (a benchmark state, not a reference
+    config): a face-centred cubic lattice with nearest-neighbour distance
+    d = squeeze * 2r (4 sites per cubic cell of side d * sqrt(2)), filled in
+    site order (x, then z, then y layers) in a closed 6-plane box whose walls
+    sit squeeze * r outside the outermost sites; radii U[(1 - poly) r, r]
+    rounded to FP32 values.  Neighbours whose radii sum exceeds d overlap by at
+    most (1 - squeeze) * 2r: with the defaults about 7/8 of the 12 lattice
+    neighbours touch, solid fraction about 0.7.  Cell size 2r."""
+    r = float(as_f32_exact(r))
+    d = squeeze * 2.0 * r
+    a = d * math.sqrt(2.0)
+    m = int(math.ceil((n / 4.0) ** (1.0 / 3.0)))
+    basis = np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]])
+    c = np.arange(m)
+    # cells in (y, z, x) order, x fastest; 4 sites per cell
+    cy, cz, cx = np.meshgrid(c, c, c, indexing="ij")
+    cells = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], 1).astype(np.float64)
+    sites = (cells[:, None, :] + basis[None, :, :]).reshape(-1, 3)[:n] * a
+    rad = as_f32_exact(r * (1.0 - poly * uniform(seed, n, 1)))
+    lo = -squeeze * r
+    hi = (m - 0.5) * a + squeeze * r
+    top = float(sites[:, 1].max()) + squeeze * r
+    off = 2.0 * r  # box corner at (off, off, off) - lo
+    pos = np.concatenate([sites + (off - lo), rad[:, None]], 1)
+    mn = (off, off, off)
+    mx = (off + hi - lo, off + top - lo, off + hi - lo)
+    cs = 2.0 * r
+    g = _pow2_at_least(int(math.ceil((max(mx) + 2.0 * r) / cs)) + 2)
+    planes = cube_planes(mn, mx)
+    return Scene(name=name, pos=np.ascontiguousarray(pos), dt=dt, nsteps=nsteps,
+                 world_origin=(0.0, 0.0, 0.0), cell_size=cs, grid_size=(g, g, g), planes=planes,
+                 seed=seed)
+
+
 def drum(n=4_000_000, radius=3e-3, R=0.5, L=1.6, rpm=30.0, ntri=20000, seed=0xD3E2, dt=5e-6,
          nsteps=2000, name="C2-drum", grid=(256, 256, 256)) -> Scene:
     """C2: n spheres of radius r in a drum (inner shell and end caps, axis z,

@@ -18,6 +18,8 @@
   matches the library's footprint figure.
 * A NaN in one scene leaves the scenes sharing its batch bit-identical to
   their runs alone.
+* The contact-rich compressed packing (200,000 spheres) with candidate lists:
+  grid and pairs bit-exact, forces within the FP32c bar.
 """
 import ctypes as C
 
@@ -101,6 +103,50 @@ def test_hopper_1m_grid_pairs_forces(lib, orc):
     print(f"hopper 1M: FP32c vs FP64 oracle force {fe}, torque {te}")
     assert fe["term"] < FP32C_TOL and te["term"] < FP32C_TOL
     assert fe["net_ok_share"] > 0.999
+
+
+def test_dense_bed_grid_pairs_forces(lib, orc):
+    """The contact-rich compressed packing (200,000 spheres, FCC at 0.985 x 2r,
+    about 9 contacts each) after 100 steps with candidate lists: grid and
+    contact pairs bit-exact against the oracle on the same state, forces and
+    torques within the FP32c bar."""
+    sc = scenes.dense_bed(n=200_000)
+    n = sc.np
+    ncell = int(np.prod(sc.grid_size))
+    b = Batch(n, 1, ncell)
+    b.lists(True, 0.05)
+    sid = b.add(sc)
+    b.step(50)
+    b.step(50)
+    st = b.state(sid)
+    assert np.all(np.isfinite(st["pos"])) and np.all(np.isfinite(st["vel"]))
+    b.prepare()
+    out = b.probe(sid, ncell, max_per=32)
+    b.close()
+    p = sc.params()
+    pos = np.ascontiguousarray(st["pos"])
+    h, idx = orc.hash_and_index(p, pos)
+    h, idx, cs, ce, srt = orc.reorder(h, idx, p.ncell)
+    assert np.array_equal(out["sorted_ids"], srt)
+    assert np.array_equal(out["cstart"], cs) and np.array_equal(out["cend"], ce)
+    m, _ = sc.mass_inertia()
+    f, t = np.zeros((n, 3)), np.zeros((n, 3))
+    with orc.force_magnitudes(n) as fm:
+        pairs = orc.p2p(p, 1, pos, np.ascontiguousarray(st["vel"]), np.ascontiguousarray(st["omega"]),
+                        f, t, m, srt, cs, ce, sc.contact_pp(), want_pairs=True, max_pairs=16 * n)
+        for pl in sc.planes:
+            orc.plane(p, 1, pl, pos, np.ascontiguousarray(st["vel"]),
+                      np.ascontiguousarray(st["omega"]), f, t, m, sc.contact_pp())
+    cnt = out["counts"]
+    print(f"dense bed: {len(pairs) / n:.2f} contacts per sphere", flush=True)
+    assert len(pairs) > 8 * n
+    got = np.repeat(np.arange(n, dtype=np.uint32), cnt)
+    part = out["partners"][np.arange(32)[None, :] < cnt[:, None]]
+    assert np.array_equal(np.stack([got, part], 1), pairs)
+    fe = force_errors(out["force"], f, fm, m)
+    te = force_errors(out["moment"], t, fm, m, pos[:, 3])
+    print(f"dense bed: FP32c vs FP64 oracle force {fe}, torque {te}")
+    assert fe["term"] < FP32C_TOL and te["term"] < FP32C_TOL
 
 
 def test_batch_256_scenes_alone_vs_mix(lib):

@@ -168,14 +168,14 @@ def test_graph_replay_equals_eager(lib):
         assert np.array_equal(a[key], e[key]), key
 
 
-@pytest.mark.parametrize("speed", [0.05, 600.0])
-def test_incremental_sort_bitexact(lib, orc, speed):
-    """After steps (the mover merge path; at 600 m/s most of the 10648
-    particles change cell every step, more than the 4096 sorted in LDS, which
-    exercises the one-workgroup radix fallback), the grid of the current
-    positions equals the oracle's stable sort."""
-    sc, vel, om = dense_scene(n_side=11 if speed < 1 else 22, seed=9, pitch=2.3, jitter=0.1,
-                              vel_scale=speed)
+@pytest.mark.parametrize("speed,side", [(0.05, 11), (600.0, 22), (600.0, 44)])
+def test_incremental_sort_bitexact(lib, orc, speed, side):
+    """After steps, the grid of the current positions equals the oracle's
+    stable sort: few movers (the window patch), most of 10,648 particles
+    changing cell every step (the movers ranked by counting across the blocks,
+    more than the 2048 k_merge keeps in LDS), and most of 85,184 (more than
+    the 32,768 ranked: the one-workgroup radix fallback)."""
+    sc, vel, om = dense_scene(n_side=side, seed=9, pitch=2.3, jitter=0.1, vel_scale=speed)
     sc.planes = []
     sc.gravity = (0.0, 0.0, 0.0)
     b = Batch(sc.np, 1, int(np.prod(sc.grid_size)), lib)
@@ -184,7 +184,7 @@ def test_incremental_sort_bitexact(lib, orc, speed):
     b.step(5)
     moved = b.last_movers()
     if speed > 1:
-        assert moved > 4096  # the large-M path was taken
+        assert moved > (32768 if side > 30 else 2048), moved  # the large-M path was taken
     st = b.state(sid)
     b.prepare()
     out = b.probe(sid, int(np.prod(sc.grid_size)))

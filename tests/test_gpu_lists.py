@@ -35,6 +35,14 @@ def _scenes(kind):
         n = sc.np
         vel = scenes.as_f32_exact(scenes.uniform(0x51, 3 * n, 5).reshape(n, 3) * 0.6 - 0.3)
         return [sc], [vel], [None]
+    if kind == "hopper_mid":  # a few hundred movers per step: the full merge, movers ranked
+        sc = scenes.hopper(n=200_000)
+        n = sc.np
+        vel = scenes.as_f32_exact(scenes.uniform(0x52, 3 * n, 5).reshape(n, 3) * 1.0 - 0.5)
+        return [sc], [vel], [None]
+    if kind == "bed":  # compressed FCC packing: about 9 contacts per sphere
+        sc = scenes.dense_bed(n=40_000)
+        return [sc], [None], [None]
     if kind == "crowd":  # 24 contacts on one sphere: more than the 16-entry contact list
         R, r = float(scenes.as_f32_exact(5e-3)), float(scenes.as_f32_exact(1.2e-3))
         k = np.arange(24)
@@ -82,7 +90,7 @@ VARIANTS = [
 ]
 
 
-@pytest.mark.parametrize("kind", ["dense", "fast", "boxes", "hopper", "crowd"])
+@pytest.mark.parametrize("kind", ["dense", "fast", "boxes", "hopper", "hopper_mid", "bed", "crowd"])
 def test_lists_and_patch_bitexact(lib, kind):
     ref, _, ref_movers = _run(kind, patch=False)
     print(f"{kind}: movers of the last sort of each block {ref_movers}")
@@ -91,7 +99,11 @@ def test_lists_and_patch_bitexact(lib, kind):
         for k, (a, b) in enumerate(zip(ref, out)):
             for key in a:
                 assert np.array_equal(a[key], b[key]), (kind, v, k, key)
-        if v.get("lists") and kind in ("dense", "crowd"):
+        if kind == "hopper_mid":
+            # more than 64 movers per sort (the full merge; the movers sorted by
+            # counting ranks across the blocks)
+            assert 64 < max(ref_movers) <= 32768, ref_movers
+        if v.get("lists") and kind in ("dense", "bed", "crowd"):
             # quiet scenes: the lists were reused (no scene rebuilt them all)
             assert rebuilds[-1] == 0, (kind, v, rebuilds)
 
