@@ -136,8 +136,8 @@ struct xdem_batch {
     float skin_frac = 0.05f;          // skin = skin_frac x cell size
     uint32_t* vl = nullptr;           // VLK entries per particle, lane-interleaved
     uint32_t* vc = nullptr;           // entries per particle
-    uint64_t* dirty = nullptr;        // bit per cell key
-    size_t dirty_words = 0;
+    uint8_t* dirty = nullptr;         // byte per cell key
+    size_t dirty_bytes = 0;
     uint32_t* dprev = nullptr;        // marked neighbourhoods of the last two sorts
     float* disp = nullptr;            // per scene (cap_scenes each)
     uint32_t* wdisp = nullptr;        // per block of k_step: its largest displacement
@@ -313,7 +313,7 @@ struct xdem_batch {
         a.moverbits = moverbits;
         a.blockcnt = blockcnt;
         a.win = win;
-        a.lists = vl ? 1u : 0u;
+        a.lists = lists_on && vl ? 1u : 0u;
         a.dirty = dirty;
         a.dprev = dprev;
         a.scenes = d_scenes;
@@ -697,7 +697,7 @@ int32_t apply_layout(xdem_batch* b) {
         }
     }
     // the storage moved: the sort and list state starts over
-    if (b->dirty) XDEM_HIP(hipMemsetAsync(b->dirty, 0, 8 * b->dirty_words, st));
+    if (b->dirty) XDEM_HIP(hipMemsetAsync(b->dirty, 0, b->dirty_bytes, st));
     hipLaunchKernelGGL(k_sort_reset, dim3(1), dim3(256), 0, st, b->patch_args(b->buf[b->cur].zi));
     XDEM_LAUNCHED();
     XDEM_HIP(hipStreamSynchronize(st));
@@ -933,7 +933,7 @@ uint64_t batch_device_bytes(uint64_t n, uint64_t nscene, uint64_t ncell, bool hi
     b += nscene * sizeof(BScene) + 2 * div_up((uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFull), 64) *
                                        sizeof(BlockEntry);
     if (history) b += 2 * n * (HSLOTS * sizeof(float4) + sizeof(uint32_t)) + sizeof(uint32_t);
-    if (lists) b += n * (VLK + 1) * sizeof(uint32_t) + (ncell / 64 + 1) * sizeof(uint64_t);
+    if (lists) b += n * (VLK + 1) * sizeof(uint32_t) + ncell;  // lists, counts, dirty bytes
     return b;
 }
 }  // namespace
@@ -971,9 +971,9 @@ int32_t xdem_batch_set_lists(xdem_batch* b, int32_t on, float skin_fraction) {
     if (on && !b->vl) {
         XDEM_TRY(dalloc(&b->vl, (size_t)VLK * b->cap_np));
         XDEM_TRY(dalloc(&b->vc, (size_t)b->cap_np));
-        b->dirty_words = b->cap_cells / 64 + 1;
-        XDEM_TRY(dalloc(&b->dirty, b->dirty_words));
-        XDEM_HIP(hipMemsetAsync(b->dirty, 0, 8 * b->dirty_words, b->st));
+        b->dirty_bytes = (size_t)b->cap_cells;
+        XDEM_TRY(dalloc(&b->dirty, b->dirty_bytes));
+        XDEM_HIP(hipMemsetAsync(b->dirty, 0, b->dirty_bytes, b->st));
         XDEM_HIP(hipStreamSynchronize(b->st));
         b->layout_dirty = true;  // the sort state restarts with the lists
     }
@@ -984,6 +984,8 @@ int32_t xdem_batch_set_lists(xdem_batch* b, int32_t on, float skin_fraction) {
     if ((on != 0) != b->lists_on) {
         b->lists_on = on != 0;
         b->drop_graphs();
+        // no marks are kept while the lists are off: start from a clean state
+        if (on) b->layout_dirty = true;
     }
     return XDEM_OK;
 }

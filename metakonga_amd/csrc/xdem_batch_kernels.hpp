@@ -31,7 +31,8 @@ constexpr uint32_t WALL_KEY = 0x80000000u;   // history key of plane p: WALL_KEY
 constexpr int LDS_T = 256;                   // lane stride of the per-thread LDS lists
 constexpr int MERGE_LDS = 2048;              // sorted movers staged in LDS by k_merge up to this count
 constexpr int ROWS_T = 256;                  // threads of the row kernels
-constexpr int FEW = 64;                      // movers re-placed by the window patch (k_patch)
+constexpr int FEW = 256;                     // movers re-placed by the window patch (k_patch)
+constexpr int FEW_WAVE = 64;                 // up to this many, one wave orders them
 constexpr int PATCH_T = 1024;                // threads of a k_patch block
 constexpr int PATCH_GRID = 256;              // blocks of k_patch (grid-stride over the windows)
 constexpr int RANK_MAX = 32768;              // movers sorted by counting ranks (k_patch, every block)
@@ -200,7 +201,7 @@ struct StepArgs {
     uint32_t vl_stride;
     const uint32_t* invperm;   // storage index of `in` -> sorted position
     const uint32_t* rebuild;   // per scene: rebuild every list this step
-    const uint64_t* dirty;     // bit per cell key: a particle changed cell within one cell
+    const uint8_t* dirty;      // byte per cell key: a particle changed cell within one cell
     uint32_t* wdisp;           // per block L of the launch: the largest displacement of this
                                // step (float bits, bounded above)
     // parity probe outputs (per sorted position) — null in production
@@ -827,7 +828,7 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     if constexpr (VLIST) {
         vcnt = A.vc[s];
         const uint32_t kown = A.mv.key_old[j];
-        const bool dirty = (A.dirty[kown >> 6] >> (kown & 63)) & 1ull;
+        const bool dirty = A.dirty[kown] != 0;
         rebuild = A.rebuild[be.scene] != 0 || s != j || dirty || vcnt > (uint32_t)VLK;
     }
     f3 Ft = F3(0, 0, 0), Fn = F3(0, 0, 0), M = F3(0, 0, 0);
@@ -1653,7 +1654,7 @@ struct PatchArgs {
     uint32_t* win;             // windows by parity (this sort: par; the previous: 1 - par)
     // candidate lists (lists != 0)
     uint32_t lists;
-    uint64_t* dirty;           // bit per cell key
+    uint8_t* dirty;            // byte per cell key
     uint32_t* dprev;           // marked neighbourhoods by parity
     const BScene* scenes;
     uint32_t nscenes;
@@ -1665,19 +1666,16 @@ struct PatchArgs {
 };
 
 // The 27 cells around cell key k of scene S (the walk's wrap-around) get
-// their dirty bits set (set = true) or cleared.
+// their dirty bytes set (set = true) or cleared (plain byte stores: no
+// read-modify-write, so marks of different movers never interfere).
 __device__ inline void mark_neighbourhood(const BScene& S, uint32_t k, uint32_t which,
-                                          uint64_t* dirty, bool set) {
+                                          uint8_t* dirty, bool set) {
     const uint32_t local = k - S.cell_base;
     const int x = (int)(local % S.gx), y = (int)((local / S.gx) % S.gy),
               z = (int)(local / (S.gx * S.gy));
     const int dz = (int)(which / 9) - 1, dy = (int)((which / 3) % 3) - 1, dx = (int)(which % 3) - 1;
     const uint32_t kk = key_of(S, x + dx, y + dy, z + dz);
-    const uint64_t bit = 1ull << (kk & 63);
-    if (set)
-        atomicOr((unsigned long long*)&dirty[kk >> 6], (unsigned long long)bit);
-    else
-        atomicAnd((unsigned long long*)&dirty[kk >> 6], (unsigned long long)~bit);
+    dirty[kk] = set ? 1u : 0u;
 }
 
 // Rank of this lane's key among the keys of lanes 0..M-1 (ties by lane).
@@ -1693,106 +1691,327 @@ __device__ inline uint32_t wave_rank(uint64_t key, uint32_t M) {
     return rank;
 }
 
+// Block-wide scans over PATCH_T threads (one value per thread): inclusive
+// maximum and exclusive sum; every thread of the block calls them.
+__device__ inline uint32_t block_incl_max(uint32_t v, uint32_t* wtmp) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)v, o, 64);
+        if (lane >= (uint32_t)o) v = max(v, u);
+    }
+    if (lane == 63) wtmp[w] = v;
+    __syncthreads();
+    for (uint32_t q = 0; q < w; ++q) v = max(v, wtmp[q]);
+    __syncthreads();
+    return v;
+}
+__device__ inline uint32_t block_excl_sum(uint32_t v, uint32_t* wtmp, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)x, o, 64);
+        if (lane >= (uint32_t)o) x += u;
+    }
+    if (lane == 63) wtmp[w] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t q = 0; q < PATCH_T / 64; ++q) {
+        before += q < w ? wtmp[q] : 0u;
+        all += wtmp[q];
+    }
+    __syncthreads();
+    total = all;
+    return before + x - v;
+}
+
+__device__ inline uint2 block_excl_sum2(uint2 v, uint32_t* wtmp, uint2& total) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint2 x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t ux = (uint32_t)__shfl_up((int)x.x, o, 64);
+        const uint32_t uy = (uint32_t)__shfl_up((int)x.y, o, 64);
+        if (lane >= (uint32_t)o) {
+            x.x += ux;
+            x.y += uy;
+        }
+    }
+    if (lane == 63) {
+        wtmp[2 * w] = x.x;
+        wtmp[2 * w + 1] = x.y;
+    }
+    __syncthreads();
+    uint2 before = make_uint2(0u, 0u), all = make_uint2(0u, 0u);
+    for (uint32_t q = 0; q < PATCH_T / 64; ++q) {
+        const uint32_t a = wtmp[2 * q], b = wtmp[2 * q + 1];
+        if (q < w) {
+            before.x += a;
+            before.y += b;
+        }
+        all.x += a;
+        all.y += b;
+    }
+    __syncthreads();
+    total = all;
+    return make_uint2(before.x + x.x - v.x, before.y + x.y - v.y);
+}
+
+// Entries of the ascending v[0..m) (stride st) below x / at most x.
+__device__ inline uint32_t lds_count_lt(const uint32_t* v, uint32_t m, uint32_t st, uint32_t x) {
+    uint32_t lo = 0, hi = m;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (v[mid * st] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+__device__ inline uint32_t lds_count_le(const uint32_t* v, uint32_t m, uint32_t st, uint32_t x) {
+    uint32_t lo = 0, hi = m;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (v[mid * st] <= x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+__device__ inline uint32_t lds_count_lt64(const uint64_t* v, uint32_t m, uint64_t x) {
+    uint32_t lo = 0, hi = m;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (v[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Each block works out the movers' order and the merged windows in LDS:
+// up to FEW_WAVE movers one wave does it with lane ranks and shuffles; up to
+// FEW the whole block (thread t for mover t: ranks by counting over the M
+// records, windows ranked by (first, last, mover), merged by a block-wide
+// running maximum of their last positions).  When the new windows hold more
+// than PATCH_SLOTS_MAX positions (movers across far-apart cells) the full
+// merge is taken instead.  Roles then: block 0 the counters, block 1 the
+// emptied cells, the movers' marks and the window record, block 2 the dirty
+// cells, every block a share of the scenes' displacement bounds, blocks 3..
+// the sweep: new window positions (one past each too, for its last boundary)
+// and the previous windows' positions outside the new ones; a position's
+// element comes from binary searches: the mover placed there, or the stayer
+// whose rank among the stayers is j - (movers placed before j).  On the full
+// merge the blocks sort the movers for k_merge instead.
+constexpr uint32_t PATCH_SLOTS_MAX = (PATCH_GRID - 3u) * PATCH_T;  // one per sweep thread
 __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
-    __shared__ uint64_t sX[FEW];      // movers by new (key, id)
-    __shared__ uint32_t sSrc[FEW];    // their storage index
-    __shared__ uint32_t sOut[FEW];    // their new position (increasing)
-    __shared__ uint32_t sAsc[FEW];    // the movers' storage indices, ascending
-    __shared__ uint32_t sWin[2 * FEW], sPrev[2 * FEW];
-    __shared__ uint4 sOld[FEW];       // their (old cell size, scene, old key)
-    __shared__ uint32_t sAux[FEW];    // window ends while the windows are sorted
-    __shared__ uint32_t sM, sNW, sNP;
-    __shared__ uint32_t sred[PATCH_T / 64];
+    __shared__ uint64_t rX[FEW];                  // block path: records in append order (key, id)
+    __shared__ uint32_t rS[FEW], rL[FEW];         // storage index, insertion point; later the
+                                                  // windows in order (first, last)
+    __shared__ uint64_t sX[FEW];                  // by rank: (key, id)
+    __shared__ uint32_t sSrc[FEW], sOut[FEW];     // by rank: storage index, new position
+    __shared__ uint32_t sAsc[FEW];                // storage indices ascending
+    __shared__ uint32_t sOsz[FEW], sOsc[FEW], sOkey[FEW];  // by rank: old cell size, scene, old key
+    __shared__ uint32_t wA[FEW], wB[FEW];         // scratch
+    __shared__ uint32_t sWin[2 * FEW];            // merged windows: (first, last) pairs, ordered
+    __shared__ uint32_t sPrev[2 * FEW];           // the previous sort's windows
+    __shared__ uint32_t sWpre[FEW + 1], sPpre[FEW + 1];  // slots before each window
+    __shared__ uint32_t wtmp[2 * (PATCH_T / 64)];
+    __shared__ uint32_t sNW, sTot[2];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     const uint32_t n = A.n;
     uint32_t* const win_now = A.win + A.par * WIN_WORDS;
     const uint32_t* const win_prev = A.win + (1 - A.par) * WIN_WORDS;
-    // ---- wave 0 of every block: the movers' order and the windows.  The
-    // mover count, the previous windows and the first 64 mover records are
-    // loaded together (one memory round trip).  The sorts are rank sorts over
-    // the M records (all-pairs by lane reads, M iterations) scattered to LDS.
-    if (w == 0) {
-        const uint32_t m0 = A.counters[A.par];
-        const uint32_t np0 = win_prev[0];
-        const uint32_t pa = win_prev[2 + 2 * lane], pb = win_prev[3 + 2 * lane];
-        const bool rec = lane < n;
+    const uint32_t M = min(A.counters[A.par], n);
+    const uint32_t npv = min(win_prev[0], (uint32_t)FEW);
+    // 0: full merge, 1: wave path, 2: block path
+    uint32_t mode = !A.patch_on || M > (uint32_t)FEW ? 0u : M <= (uint32_t)FEW_WAVE ? 1u : 2u;
+    if (t < npv) {
+        sPrev[2 * t] = win_prev[2 + 2 * t];
+        sPrev[2 * t + 1] = win_prev[3 + 2 * t];
+    }
+    uint4 mo = make_uint4(0u, 0u, 0u, 0u);
+    if (mode == 2 && t < M) {
+        rX[t] = A.mkey[t];
+        rS[t] = A.ms[t];
+        rL[t] = A.mplo[t];
+        mo = A.mold[t];
+    }
+    if (mode == 1 && w == 0) {
+        // ---- one wave: records in registers (one memory round trip), rank
+        // sorts over the M records by lane reads, scattered to LDS
+        const bool rec = lane < M;
         const uint64_t mk = rec ? A.mkey[lane] : ~0ull;
         const uint32_t msl = rec ? A.ms[lane] : 0u;
         const uint32_t mpl = rec ? A.mplo[lane] : 0u;
-        const uint4 mo = rec ? A.mold[lane] : make_uint4(0u, 0u, 0u, 0u);
-        const uint32_t M = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(m0, n));
-        const bool few = M <= (uint32_t)FEW && A.patch_on;
-        if (lane == 0) sM = M;
-        if (few) {
-            // movers by (key, id), and their storage indices ascending
-            const uint32_t rk = wave_rank(mk, M), rs = wave_rank((uint64_t)msl, M);
-            if (lane < M) {
-                sX[rk] = mk;
-                sSrc[rk] = msl;
-                sOut[rk] = mpl;  // the insertion point for now
-                sOld[rk] = mo;
-                sAsc[rs] = msl;
-            }
-            const uint32_t np = min(np0, (uint32_t)FEW);
-            if (lane < np) {
-                sPrev[2 * lane] = pa;
-                sPrev[2 * lane + 1] = pb;
-            }
-            if (lane == 0) sNP = np;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // new position and window of the mover of rank `lane`: previous
-            // entries below it, less the movers among them, plus the movers
-            // before it
-            uint64_t win = ~0ull;
-            uint32_t out = 0;
-            if (lane < M) {
-                const uint32_t lo = sOut[lane], src = sSrc[lane];
-                uint32_t below = 0;
-                for (uint32_t r = 0; r < M; ++r)
-                    below += (uint32_t)__builtin_amdgcn_readlane((int)msl, (int)r) < lo ? 1u : 0u;
-                out = lane + lo - below;
-                const uint32_t a = src < lo ? src : lo, b = src < lo ? lo - 1 : src;
-                win = ((uint64_t)a << 32) | b;
-            }
-            // windows by first position, merged where they overlap or touch: a
-            // running maximum of the last positions decides where a merged
-            // window starts and ends
-            const uint32_t rw = wave_rank(win, M);
-            __builtin_amdgcn_wave_barrier();
-            if (lane < M) {
-                sOut[lane] = out;
-                sWin[rw] = (uint32_t)(win >> 32);
-                sAux[rw] = (uint32_t)win;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const bool valid = lane < M;
-            const uint32_t a = valid ? sWin[lane] : 0xFFFFFFFFu, b = valid ? sAux[lane] : 0xFFFFFFFFu;
-            uint32_t run = b;
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t u = (uint32_t)__shfl_up((int)run, o, 64);
-                if (lane >= (uint32_t)o) run = max(run, u);
-            }
-            const uint32_t prev_run = (uint32_t)__shfl_up((int)run, 1, 64);
-            const bool start = valid && (lane == 0 || a > prev_run + 1);
-            const uint64_t starts = __ballot(start);
-            const uint32_t k = (uint32_t)__popcll(starts & (lanemask_lt64() | (1ull << lane))) - 1;
-            const bool next_start = lane + 1 < 64 && ((starts >> (lane + 1)) & 1ull);
-            __builtin_amdgcn_wave_barrier();
-            if (start) sWin[2 * k] = a;
-            if (valid && (lane + 1 == M || next_start)) sWin[2 * k + 1] = run;
-            if (lane == 0) sNW = (uint32_t)__popcll(starts);
-        } else if (lane == 0) {
-            sNW = 0;
-            sNP = 0;
+        const uint4 mw = rec ? A.mold[lane] : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t rk = wave_rank(mk, M), rs = wave_rank((uint64_t)msl, M);
+        if (rec) {
+            sX[rk] = mk;
+            sSrc[rk] = msl;
+            sOut[rk] = mpl;  // the insertion point for now
+            sOsz[rk] = mw.x;
+            sOsc[rk] = mw.y;
+            sOkey[rk] = mw.z;
+            sAsc[rs] = msl;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // new position and window of the mover of rank `lane`: previous entries
+        // below it, less the movers among them, plus the movers before it
+        uint64_t win = ~0ull;
+        uint32_t out = 0;
+        if (rec) {
+            const uint32_t lo = sOut[lane], src = sSrc[lane];
+            uint32_t below = 0;
+            for (uint32_t r = 0; r < M; ++r)
+                below += (uint32_t)__builtin_amdgcn_readlane((int)msl, (int)r) < lo ? 1u : 0u;
+            out = lane + lo - below;
+            const uint32_t a = src < lo ? src : lo, b = src < lo ? lo - 1 : src;
+            win = ((uint64_t)a << 32) | b;
+        }
+        // windows by first position, merged where they overlap or touch: a
+        // running maximum of the last positions decides where a merged window
+        // starts and ends
+        const uint32_t rw = wave_rank(win, M);
+        __builtin_amdgcn_wave_barrier();
+        if (rec) {
+            sOut[lane] = out;
+            wA[rw] = (uint32_t)(win >> 32);
+            wB[rw] = (uint32_t)win;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t a = rec ? wA[lane] : 0xFFFFFFFFu, b = rec ? wB[lane] : 0xFFFFFFFFu;
+        uint32_t run = b;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = (uint32_t)__shfl_up((int)run, o, 64);
+            if (lane >= (uint32_t)o) run = max(run, u);
+        }
+        const uint32_t prev_run = (uint32_t)__shfl_up((int)run, 1, 64);
+        const bool start = rec && (lane == 0 || a > prev_run + 1);
+        const uint64_t starts = __ballot(start);
+        const uint32_t k = (uint32_t)__popcll(starts & (lanemask_lt64() | (1ull << lane))) - 1;
+        const bool next_start = lane + 1 < 64 && ((starts >> (lane + 1)) & 1ull);
+        if (start) sWin[2 * k] = a;
+        if (rec && (lane + 1 == M || next_start)) sWin[2 * k + 1] = run;
+        const uint32_t nwv = (uint32_t)__popcll(starts);
+        if (lane == 0) sNW = nwv;
+        if (npv <= 64) {
+            // slot prefixes of the new and previous windows (this wave; the
+            // previous windows were loaded by the whole block above, so wait)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            uint32_t wl = lane < nwv ? sWin[2 * lane + 1] - sWin[2 * lane] + 2 : 0u;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t u = (uint32_t)__shfl_up((int)wl, o, 64);
+                if (lane >= (uint32_t)o) wl += u;
+            }
+            sWpre[lane + 1] = wl;
+            if (lane == 0) sWpre[0] = 0;
+            if (lane == 63) sTot[0] = wl;
+        }
+    } else if (mode != 1 && t == 0) {
+        sNW = 0;
     }
     __syncthreads();
-    const uint32_t M = sM;
-    const bool few = M <= (uint32_t)FEW && A.patch_on;
+    if (mode == 2) {
+        // ---- the whole block: ranks by (key, id), storage indices ascending,
+        // movers below the insertion point; new position and window
+        uint32_t wa = 0xFFFFFFFFu, wb = 0;
+        if (t < M) {
+            const uint64_t myX = rX[t];
+            const uint32_t myS = rS[t], myL = rL[t];
+            uint32_t rk = 0, rs = 0, below = 0;
+#pragma unroll 4
+            for (uint32_t r = 0; r < M; ++r) {
+                const uint64_t x = rX[r];
+                const uint32_t q = rS[r];
+                rk += x < myX ? 1u : 0u;
+                rs += q < myS ? 1u : 0u;
+                below += q < myL ? 1u : 0u;
+            }
+            sX[rk] = myX;
+            sSrc[rk] = myS;
+            sOut[rk] = rk + myL - below;
+            sOsz[rk] = mo.x;
+            sOsc[rk] = mo.y;
+            sOkey[rk] = mo.z;
+            sAsc[rs] = myS;
+            wa = myS < myL ? myS : myL;
+            wb = myS < myL ? myL - 1 : myS;
+            wA[t] = wa;
+            wB[t] = wb;
+        }
+        __syncthreads();
+        // windows ranked by (first, last, mover), into rS / rL
+        if (t < M) {
+            uint32_t rw = 0;
+#pragma unroll 4
+            for (uint32_t r = 0; r < M; ++r) {
+                const uint32_t a = wA[r], b = wB[r];
+                rw += (a < wa || (a == wa && (b < wb || (b == wb && r < t)))) ? 1u : 0u;
+            }
+            rS[rw] = wa;
+            rL[rw] = wb;
+        }
+        __syncthreads();
+        // merged where they overlap or touch
+        const bool valid = t < M;
+        const uint32_t a = valid ? rS[t] : 0xFFFFFFFFu, b = valid ? rL[t] : 0u;
+        const uint32_t run = block_incl_max(b, wtmp);
+        if (valid) wA[t] = run;  // (t < M <= FEW)
+        __syncthreads();
+        const bool start = valid && (t == 0 || a > wA[t - 1] + 1);
+        if (valid) wB[t] = start ? 1u : 0u;
+        uint32_t nstart = 0;
+        const uint32_t k = block_excl_sum(start ? 1u : 0u, wtmp, nstart);
+        const bool next_start = t + 1 < M && wB[t + 1] != 0;
+        if (start) sWin[2 * k] = a;
+        if (valid && (t + 1 == M || next_start)) sWin[2 * (k + (start ? 1u : 0u) - 1) + 1] = run;
+        if (t == 0) sNW = nstart;
+        __syncthreads();
+    }
+    const uint32_t nw = sNW;
+    // ---- slot prefixes of the new windows (one past each too) and of the
+    // previous ones: the sweep finds a slot's window by search (the one wave
+    // did the new windows' already when the previous ones were few)
+    uint32_t total_new = 0, total_prev = 0;
+    if (mode == 1 && npv <= 64) {
+        if (w == 1) {
+            uint32_t pl = lane < npv ? sPrev[2 * lane + 1] - sPrev[2 * lane] + 1 : 0u;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t u = (uint32_t)__shfl_up((int)pl, o, 64);
+                if (lane >= (uint32_t)o) pl += u;
+            }
+            sPpre[lane + 1] = pl;
+            if (lane == 0) sPpre[0] = 0;
+            if (lane == 63) sTot[1] = pl;
+        }
+        __syncthreads();
+        total_new = sTot[0];
+        total_prev = sTot[1];
+    } else {
+        uint2 tot;
+        const uint2 e = block_excl_sum2(
+            make_uint2(t < nw ? sWin[2 * t + 1] - sWin[2 * t] + 2 : 0u,
+                       t < npv ? sPrev[2 * t + 1] - sPrev[2 * t] + 1 : 0u), wtmp, tot);
+        if (t < nw) sWpre[t] = e.x;
+        if (t < npv) sPpre[t] = e.y;
+        if (t == 0) {
+            sWpre[nw] = tot.x;
+            sPpre[npv] = tot.y;
+        }
+        total_new = tot.x;
+        total_prev = tot.y;
+        __syncthreads();
+    }
+    if (mode != 0 && total_new > PATCH_SLOTS_MAX) mode = 0;  // the same decision in every block
+    const bool few = mode != 0;
     if (blockIdx.x == 0 && t == 0) {
         A.counters[1 - A.par] = 0;  // the next step's movers go there
         A.counters[2] = M;
@@ -1805,9 +2024,9 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         }
     }
     // ---- per scene (blocks in turn): the displacement bound since its lists
-    // were built, the step's largest wave value added; every list is rebuilt
+    // were built, the step's largest block value added; every list is rebuilt
     // when it reaches half the skin (both particles of a pair may have moved),
-    // at a block start, or after many movers
+    // at a block start, or on the full merge
     if (A.lists)
         for (uint32_t sc = blockIdx.x; sc < A.nscenes; sc += gridDim.x) {
             const uint2 br = A.scene_blk[sc];
@@ -1823,10 +2042,10 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
                 for (int u = 0; u < 4; ++u) m = max(m, v[u]);
             }
             for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
-            if (lane == 0) sred[w] = m;
+            if (lane == 0) wtmp[w] = m;
             __syncthreads();
             if (t == 0) {
-                for (uint32_t q = 0; q < PATCH_T / 64; ++q) m = max(m, sred[q]);
+                for (uint32_t q = 0; q < PATCH_T / 64; ++q) m = max(m, wtmp[q]);
                 const float D = A.disp[sc] + __uint_as_float(m);
                 const bool rb = A.force[sc] != 0 || !few || !(2.0f * D < A.scenes[sc].skin);
                 A.disp[sc] = rb ? 0.0f : D;
@@ -1842,8 +2061,8 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         // are at hand on the patch path only; otherwise every list is rebuilt)
         uint32_t* const dp = A.dprev + (1 - A.par) * DPREV_WORDS;
         uint32_t* const dn = A.dprev + A.par * DPREV_WORDS;
-        const uint32_t npv = min(dp[0], (uint32_t)FEW);
-        for (uint32_t q = t; q < 27 * npv; q += PATCH_T) {
+        const uint32_t npd = min(dp[0], (uint32_t)FEW);
+        for (uint32_t q = t; q < 27 * npd; q += PATCH_T) {
             const uint32_t e = q / 27;
             mark_neighbourhood(A.scenes[dp[2 + 2 * e]], dp[1 + 2 * e], q % 27, A.dirty, false);
         }
@@ -1851,18 +2070,18 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         __syncthreads();
         for (uint32_t q = t; q < 27 * nm; q += PATCH_T) {
             const uint32_t e = q / 27;
-            mark_neighbourhood(A.scenes[sOld[e].y], (uint32_t)(sX[e] >> 32), q % 27, A.dirty, true);
+            mark_neighbourhood(A.scenes[sOsc[e]], (uint32_t)(sX[e] >> 32), q % 27, A.dirty, true);
         }
         if (t < nm) {
             dn[1 + 2 * t] = (uint32_t)(sX[t] >> 32);
-            dn[2 + 2 * t] = sOld[t].y;
+            dn[2 + 2 * t] = sOsc[t];
         }
         if (t == 0) dn[0] = nm;
     }
     if (!few) {
         // the full merge follows: the movers are sorted for it — by counting
-        // ranks spread over the blocks (one mover per thread), or beyond
-        // RANK_MAX by one block (large admissions)
+        // ranks spread over the blocks (64 movers per block at a time), or
+        // beyond RANK_MAX by one block (large admissions)
         if (M <= (uint32_t)RANK_MAX) {
             // the last block scans the per-block mover counts for k_merge
             if (blockIdx.x == gridDim.x - 1) scan_blockcnt<PATCH_T>(SA);
@@ -1877,64 +2096,57 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         // the movers' marks of record_mover are cleared for the next step; the
         // windows of this sort are recorded
         if (t < M) {
-            const uint32_t src = sSrc[t], ko = sOld[t].z;
+            const uint32_t src = sSrc[t], ko = sOkey[t];
             if (ko < A.ncell_cap) {
-                uint32_t leave = 0, enter = 0;
-                for (uint32_t r = 0; r < M; ++r) {
-                    leave += sOld[r].z == ko ? 1u : 0u;
-                    enter += (uint32_t)(sX[r] >> 32) == ko ? 1u : 0u;
-                }
-                if (sOld[t].x + enter == leave) A.crange[ko] = make_uint2(EMPTY, 0);
+                uint32_t leave = 0;
+                for (uint32_t r = 0; r < M; ++r) leave += sOkey[r] == ko ? 1u : 0u;
+                const uint32_t enter = lds_count_lt64(sX, M, (uint64_t)(ko + 1) << 32) -
+                                       lds_count_lt64(sX, M, (uint64_t)ko << 32);
+                if (sOsz[t] + enter == leave) A.crange[ko] = make_uint2(EMPTY, 0);
             }
             A.moverbits[src >> 6] = 0;
             A.blockcnt[src / ROWS_T] = 0;
         }
-        if (t < sNW) {
+        if (t < nw) {
             win_now[2 + 2 * t] = sWin[2 * t];
             win_now[3 + 2 * t] = sWin[2 * t + 1];
         }
-        if (t == 0) win_now[0] = sNW;
+        if (t == 0) win_now[0] = nw;
         return;
     }
     if (blockIdx.x < 3) return;
-    // ---- the sweep (blocks 3..): new windows (one position past each, for
-    // its last boundary), then the previous windows' positions outside the
-    // new ones
-    // The movers and windows are held in registers (lane r: mover r, window
-    // r) and read by uniform lane index; the keys around a position are loaded
-    // before its element is worked out (a stayer in a window comes from at
-    // most one position away per overlapping mover, usually one).
-    const uint32_t nw = (uint32_t)__builtin_amdgcn_readfirstlane((int)sNW);
-    const uint32_t npv = (uint32_t)__builtin_amdgcn_readfirstlane((int)sNP);
-    const uint32_t M2 = (uint32_t)__builtin_amdgcn_readfirstlane((int)M);
-    const int vOut = (int)(lane < M2 ? sOut[lane] : 0xFFFFFFFFu);
-    const int vKey = (int)(lane < M2 ? (uint32_t)(sX[lane] >> 32) : 0u);
-    const int vSrc = (int)(lane < M2 ? sSrc[lane] : 0u);
-    const int vAsc = (int)(lane < M2 ? sAsc[lane] : 0xFFFFFFFFu);
-    const int vWa = (int)(lane < nw ? sWin[2 * lane] : 0u), vWb = (int)(lane < nw ? sWin[2 * lane + 1] : 0u);
-    const int vPa = (int)(lane < npv ? sPrev[2 * lane] : 0u), vPb = (int)(lane < npv ? sPrev[2 * lane + 1] : 0u);
-    auto rl = [](int v, uint32_t r) { return (uint32_t)__builtin_amdgcn_readlane(v, (int)r); };
-    // element at new position j: the mover placed there, or the stayer of
-    // rank j - (movers placed before j) in the previous order; its key from
-    // the preloaded keys q[0..3] = key_old[j0 - 2 .. j0 + 1] when near j0
+    // ---- the sweep (blocks 3..).  Up to FEW_WAVE movers their storage
+    // indices are held in registers (lane r: mover r) and read by uniform
+    // lane index; more, by binary search.  The keys around a position are
+    // loaded before its element is worked out (a stayer in a window comes
+    // from at most one position away per overlapping mover, usually one).
+    const uint32_t vAsc = lane < M ? sAsc[lane] : 0xFFFFFFFFu;
     auto elem = [&](uint32_t j, uint32_t j0, const uint4 q, uint32_t& key) -> uint32_t {
-        uint32_t c1 = 0, hk = 0, hs = 0;
-        bool hit = false;
-        for (uint32_t r = 0; r < M2; ++r) {
-            const uint32_t o = rl(vOut, r), rk = rl(vKey, r), rs = rl(vSrc, r);
-            c1 += o < j ? 1u : 0u;
-            if (o == j) {
-                hit = true;
-                hk = rk;
-                hs = rs;
+        const uint32_t c1 = lds_count_lt(sOut, M, 1, j);  // movers placed before j
+        if (c1 < M && sOut[c1] == j) {
+            key = (uint32_t)(sX[c1] >> 32);
+            return sSrc[c1];
+        }
+        // the stayer of rank q = j - c1: q plus the movers before it, i.e.
+        // those whose storage index exceeds the stayers before it by at most q
+        // (sAsc[i] - i, the stayers before mover i, does not decrease)
+        const uint32_t q0 = j - c1;
+        uint32_t s;
+        if (mode == 1) {
+            s = q0;
+            for (uint32_t r = 0; r < M; ++r)
+                s += (uint32_t)__builtin_amdgcn_readlane((int)vAsc, (int)r) <= s ? 1u : 0u;
+        } else {
+            uint32_t lo = 0, hi = M;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sAsc[mid] - mid <= q0)
+                    lo = mid + 1;
+                else
+                    hi = mid;
             }
+            s = q0 + lo;
         }
-        if (hit) {
-            key = hk;
-            return hs;
-        }
-        uint32_t s = j - c1;
-        for (uint32_t r = 0; r < M2; ++r) s += rl(vAsc, r) <= s ? 1u : 0u;
         s = min(s, n - 1);  // defensive: the windows hold their own stayers
         if (s + 2 >= j0 && s <= j0 + 1) {
             const uint32_t d = s + 2 - j0;
@@ -1944,36 +2156,19 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         }
         return s;
     };
-    auto in_new = [&](uint32_t j) {
-        bool in = false;
-        for (uint32_t c = 0; c < nw; ++c) in = in || (j >= rl(vWa, c) && j <= rl(vWb, c));
-        return in;
-    };
-    uint32_t total = 0;
-    for (uint32_t c = 0; c < nw; ++c) total += rl(vWb, c) - rl(vWa, c) + 2;
-    const uint32_t total_new = total;
-    for (uint32_t c = 0; c < npv; ++c) total += rl(vPb, c) - rl(vPa, c) + 1;
+    const uint32_t total = total_new + total_prev;
     for (uint32_t g = (blockIdx.x - 3) * PATCH_T + t; g < total; g += (gridDim.x - 3) * PATCH_T) {
         if (g < total_new) {
-            // the window holding slot g (windows are disjoint and ordered)
-            uint32_t a = 0, b = 0, j = n + 1, base = 0;
-            for (uint32_t c = 0; c < nw; ++c) {
-                const uint32_t ca = rl(vWa, c), cb = rl(vWb, c), len = cb - ca + 2;
-                if (g >= base && g < base + len) {
-                    a = ca;
-                    b = cb;
-                    j = ca + (g - base);
-                }
-                base += len;
-            }
+            const uint32_t c = lds_count_le(sWpre, nw, 1, g) - 1;  // the window holding slot g
+            const uint32_t a = sWin[2 * c], b = sWin[2 * c + 1], j = a + (g - sWpre[c]);
             if (j > n) continue;
             // the previous order's keys at j - 2 .. j + 1 (outside every window
             // a position keeps its key)
-            auto kat = [&](uint32_t x, int d) -> uint32_t {
-                const int64_t y = (int64_t)x + d;
+            auto kat = [&](int d) -> uint32_t {
+                const int64_t y = (int64_t)j + d;
                 return y >= 0 && y < (int64_t)n ? A.key_old[(uint32_t)y] : EMPTY;
             };
-            const uint4 q = make_uint4(kat(j, -2), kat(j, -1), kat(j, 0), kat(j, 1));
+            const uint4 q = make_uint4(kat(-2), kat(-1), kat(0), kat(1));
             uint32_t kc = EMPTY, kp = EMPTY;
             if (j <= b) {
                 const uint32_t e = elem(j, j, q, kc);
@@ -1994,13 +2189,13 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
                 if (j > 0 && kp < A.ncell_cap) A.crange[kp].y = j;
             }
         } else {
-            uint32_t j = n, base = total_new;
-            for (uint32_t c = 0; c < npv; ++c) {
-                const uint32_t ca = rl(vPa, c), len = rl(vPb, c) - ca + 1;
-                if (g >= base && g < base + len) j = ca + (g - base);
-                base += len;
-            }
-            if (j < n && !in_new(j)) {
+            const uint32_t gp = g - total_new;
+            const uint32_t c = lds_count_le(sPpre, npv, 1, gp) - 1;
+            const uint32_t j = sPrev[2 * c] + (gp - sPpre[c]);
+            // inside a new window? (merged windows are disjoint and ordered)
+            const uint32_t cn = lds_count_le(sWin, nw, 2, j);
+            const bool in_new = cn > 0 && j <= sWin[2 * (cn - 1) + 1];
+            if (j < n && !in_new) {
                 A.perm[j] = j;
                 A.invperm[j] = j;
             }
@@ -2011,7 +2206,7 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
 // Reset of the sort and list state after a layout change or a regather:
 // both mover counters zero, both window sets the whole range (the next patch
 // restores the identity everywhere), no marked neighbourhoods recorded (the
-// dirty bitmap itself is cleared by the host), every scene's lists rebuilt.
+// dirty bytes themselves are cleared by the host), every scene's lists rebuilt.
 // One block.
 __global__ __launch_bounds__(256) void k_sort_reset(PatchArgs A) {
     const uint32_t t = threadIdx.x;
