@@ -31,7 +31,7 @@ constexpr uint32_t WALL_KEY = 0x80000000u;   // history key of plane p: WALL_KEY
 constexpr int LDS_T = 256;                   // lane stride of the per-thread LDS lists
 constexpr int MERGE_LDS = 2048;              // sorted movers staged in LDS by k_merge up to this count
 constexpr int ROWS_T = 256;                  // threads of the row kernels
-constexpr int FEW = 256;                     // movers re-placed by the window patch (k_patch)
+constexpr int FEW = 1024;                    // movers re-placed by the window patch (k_patch)
 constexpr int FEW_WAVE = 64;                 // up to this many, one wave orders them
 constexpr int PATCH_T = 1024;                // threads of a k_patch block
 constexpr int PATCH_GRID = 256;              // blocks of k_patch (grid-stride over the windows)
@@ -1805,14 +1805,46 @@ __device__ inline uint32_t lds_count_lt64(const uint64_t* v, uint32_t m, uint64_
 // whose rank among the stayers is j - (movers placed before j).  On the full
 // merge the blocks sort the movers for k_merge instead.
 constexpr uint32_t PATCH_SLOTS_MAX = (PATCH_GRID - 3u) * PATCH_T;  // one per sweep thread
+// Bitonic sort of P (a power of two, at most PATCH_T) entries of key (with
+// payload val) and, in the same network, of k2 (its own order); one element
+// per thread; every thread of the block calls it.
+__device__ inline void block_bitonic(uint64_t* key, uint32_t* val, uint32_t* k2, uint32_t P) {
+    const uint32_t i = threadIdx.x;
+    for (uint32_t size = 2; size <= P; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            const uint32_t j = i ^ stride;
+            if (i < P && j > i) {
+                const bool up = (i & size) == 0;
+                const uint64_t a = key[i], b = key[j];
+                if ((a > b) == up) {
+                    key[i] = b;
+                    key[j] = a;
+                    if (val) {
+                        const uint32_t tv = val[i];
+                        val[i] = val[j];
+                        val[j] = tv;
+                    }
+                }
+                if (k2) {
+                    const uint32_t c = k2[i], d = k2[j];
+                    if ((c > d) == up) {
+                        k2[i] = d;
+                        k2[j] = c;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
 __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
-    __shared__ uint64_t rX[FEW];                  // block path: records in append order (key, id)
-    __shared__ uint32_t rS[FEW], rL[FEW];         // storage index, insertion point; later the
-                                                  // windows in order (first, last)
     __shared__ uint64_t sX[FEW];                  // by rank: (key, id)
+    __shared__ uint32_t pay[FEW];                 // by rank: append index of the record
     __shared__ uint32_t sSrc[FEW], sOut[FEW];     // by rank: storage index, new position
     __shared__ uint32_t sAsc[FEW];                // storage indices ascending
-    __shared__ uint32_t sOsz[FEW], sOsc[FEW], sOkey[FEW];  // by rank: old cell size, scene, old key
+    __shared__ uint32_t rS[FEW], rL[FEW];         // by append index: storage index, insertion point
+    __shared__ uint32_t rOsz[FEW], rOsc[FEW], rOkey[FEW];  // by append index: old cell size, scene, key
+    __shared__ uint64_t wk[FEW];                  // windows (first << 32 | last), then in order
     __shared__ uint32_t wA[FEW], wB[FEW];         // scratch
     __shared__ uint32_t sWin[2 * FEW];            // merged windows: (first, last) pairs, ordered
     __shared__ uint32_t sPrev[2 * FEW];           // the previous sort's windows
@@ -1831,12 +1863,60 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         sPrev[2 * t] = win_prev[2 + 2 * t];
         sPrev[2 * t + 1] = win_prev[3 + 2 * t];
     }
-    uint4 mo = make_uint4(0u, 0u, 0u, 0u);
-    if (mode == 2 && t < M) {
-        rX[t] = A.mkey[t];
-        rS[t] = A.ms[t];
-        rL[t] = A.mplo[t];
-        mo = A.mold[t];
+    if (mode == 2) {
+        // ---- records (one memory round trip); keys and storage indices
+        // padded to a power of two for the sorting network
+        uint32_t P = 64;
+        while (P < M) P <<= 1;
+        if (t < P) {
+            const bool rec = t < M;
+            sX[t] = rec ? A.mkey[t] : ~0ull;
+            pay[t] = t;
+            const uint32_t src = rec ? A.ms[t] : 0xFFFFFFFFu;
+            sAsc[t] = src;
+            if (rec) {
+                rS[t] = src;
+                rL[t] = A.mplo[t];
+                const uint4 mo = A.mold[t];
+                rOsz[t] = mo.x;
+                rOsc[t] = mo.y;
+                rOkey[t] = mo.z;
+            }
+        }
+        __syncthreads();
+        block_bitonic(sX, pay, sAsc, P);  // by (key, id); storage indices ascending
+        // new position and window of the mover of rank t: previous entries
+        // below it, less the movers among them, plus the movers before it
+        if (t < M) {
+            const uint32_t app = pay[t];
+            const uint32_t src = rS[app], lo = rL[app];
+            const uint32_t below = lds_count_lt(sAsc, M, 1, lo);
+            sSrc[t] = src;
+            sOut[t] = t + lo - below;
+            const uint32_t a = src < lo ? src : lo, b = src < lo ? lo - 1 : src;
+            wk[t] = ((uint64_t)a << 32) | b;
+        } else if (t < P) {
+            wk[t] = ~0ull;
+        }
+        __syncthreads();
+        block_bitonic(wk, nullptr, nullptr, P);  // windows by (first, last)
+        // merged where they overlap or touch: a running maximum of the last
+        // positions decides where a merged window starts and ends
+        const bool valid = t < M;
+        const uint32_t a = valid ? (uint32_t)(wk[t] >> 32) : 0xFFFFFFFFu;
+        const uint32_t b = valid ? (uint32_t)wk[t] : 0u;
+        const uint32_t run = block_incl_max(b, wtmp);
+        if (valid) wA[t] = run;  // (t < M <= FEW)
+        __syncthreads();
+        const bool start = valid && (t == 0 || a > wA[t - 1] + 1);
+        if (valid) wB[t] = start ? 1u : 0u;
+        uint32_t nstart = 0;
+        const uint32_t k = block_excl_sum(start ? 1u : 0u, wtmp, nstart);
+        const bool next_start = t + 1 < M && wB[t + 1] != 0;
+        if (start) sWin[2 * k] = a;
+        if (valid && (t + 1 == M || next_start)) sWin[2 * (k + (start ? 1u : 0u) - 1) + 1] = run;
+        if (t == 0) sNW = nstart;
+        __syncthreads();
     }
     if (mode == 1 && w == 0) {
         // ---- one wave: records in registers (one memory round trip), rank
@@ -1849,18 +1929,18 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         const uint32_t rk = wave_rank(mk, M), rs = wave_rank((uint64_t)msl, M);
         if (rec) {
             sX[rk] = mk;
+            pay[rk] = lane;
             sSrc[rk] = msl;
             sOut[rk] = mpl;  // the insertion point for now
-            sOsz[rk] = mw.x;
-            sOsc[rk] = mw.y;
-            sOkey[rk] = mw.z;
+            rOsz[lane] = mw.x;
+            rOsc[lane] = mw.y;
+            rOkey[lane] = mw.z;
             sAsc[rs] = msl;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // new position and window of the mover of rank `lane`: previous entries
-        // below it, less the movers among them, plus the movers before it
+        // new position and window of the mover of rank `lane`
         uint64_t win = ~0ull;
         uint32_t out = 0;
         if (rec) {
@@ -1872,9 +1952,7 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
             const uint32_t a = src < lo ? src : lo, b = src < lo ? lo - 1 : src;
             win = ((uint64_t)a << 32) | b;
         }
-        // windows by first position, merged where they overlap or touch: a
-        // running maximum of the last positions decides where a merged window
-        // starts and ends
+        // windows by first position, merged where they overlap or touch
         const uint32_t rw = wave_rank(win, M);
         __builtin_amdgcn_wave_barrier();
         if (rec) {
@@ -1901,8 +1979,7 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         const uint32_t nwv = (uint32_t)__popcll(starts);
         if (lane == 0) sNW = nwv;
         if (npv <= 64) {
-            // slot prefixes of the new and previous windows (this wave; the
-            // previous windows were loaded by the whole block above, so wait)
+            // slot prefixes of the new windows (this wave)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1915,67 +1992,10 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
             if (lane == 0) sWpre[0] = 0;
             if (lane == 63) sTot[0] = wl;
         }
-    } else if (mode != 1 && t == 0) {
+    } else if (mode == 0 && t == 0) {
         sNW = 0;
     }
     __syncthreads();
-    if (mode == 2) {
-        // ---- the whole block: ranks by (key, id), storage indices ascending,
-        // movers below the insertion point; new position and window
-        uint32_t wa = 0xFFFFFFFFu, wb = 0;
-        if (t < M) {
-            const uint64_t myX = rX[t];
-            const uint32_t myS = rS[t], myL = rL[t];
-            uint32_t rk = 0, rs = 0, below = 0;
-#pragma unroll 4
-            for (uint32_t r = 0; r < M; ++r) {
-                const uint64_t x = rX[r];
-                const uint32_t q = rS[r];
-                rk += x < myX ? 1u : 0u;
-                rs += q < myS ? 1u : 0u;
-                below += q < myL ? 1u : 0u;
-            }
-            sX[rk] = myX;
-            sSrc[rk] = myS;
-            sOut[rk] = rk + myL - below;
-            sOsz[rk] = mo.x;
-            sOsc[rk] = mo.y;
-            sOkey[rk] = mo.z;
-            sAsc[rs] = myS;
-            wa = myS < myL ? myS : myL;
-            wb = myS < myL ? myL - 1 : myS;
-            wA[t] = wa;
-            wB[t] = wb;
-        }
-        __syncthreads();
-        // windows ranked by (first, last, mover), into rS / rL
-        if (t < M) {
-            uint32_t rw = 0;
-#pragma unroll 4
-            for (uint32_t r = 0; r < M; ++r) {
-                const uint32_t a = wA[r], b = wB[r];
-                rw += (a < wa || (a == wa && (b < wb || (b == wb && r < t)))) ? 1u : 0u;
-            }
-            rS[rw] = wa;
-            rL[rw] = wb;
-        }
-        __syncthreads();
-        // merged where they overlap or touch
-        const bool valid = t < M;
-        const uint32_t a = valid ? rS[t] : 0xFFFFFFFFu, b = valid ? rL[t] : 0u;
-        const uint32_t run = block_incl_max(b, wtmp);
-        if (valid) wA[t] = run;  // (t < M <= FEW)
-        __syncthreads();
-        const bool start = valid && (t == 0 || a > wA[t - 1] + 1);
-        if (valid) wB[t] = start ? 1u : 0u;
-        uint32_t nstart = 0;
-        const uint32_t k = block_excl_sum(start ? 1u : 0u, wtmp, nstart);
-        const bool next_start = t + 1 < M && wB[t + 1] != 0;
-        if (start) sWin[2 * k] = a;
-        if (valid && (t + 1 == M || next_start)) sWin[2 * (k + (start ? 1u : 0u) - 1) + 1] = run;
-        if (t == 0) sNW = nstart;
-        __syncthreads();
-    }
     const uint32_t nw = sNW;
     // ---- slot prefixes of the new windows (one past each too) and of the
     // previous ones: the sweep finds a slot's window by search (the one wave
@@ -2070,11 +2090,11 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         __syncthreads();
         for (uint32_t q = t; q < 27 * nm; q += PATCH_T) {
             const uint32_t e = q / 27;
-            mark_neighbourhood(A.scenes[sOsc[e]], (uint32_t)(sX[e] >> 32), q % 27, A.dirty, true);
+            mark_neighbourhood(A.scenes[rOsc[pay[e]]], (uint32_t)(sX[e] >> 32), q % 27, A.dirty, true);
         }
         if (t < nm) {
             dn[1 + 2 * t] = (uint32_t)(sX[t] >> 32);
-            dn[2 + 2 * t] = sOsc[t];
+            dn[2 + 2 * t] = rOsc[pay[t]];
         }
         if (t == 0) dn[0] = nm;
     }
@@ -2096,13 +2116,13 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         // the movers' marks of record_mover are cleared for the next step; the
         // windows of this sort are recorded
         if (t < M) {
-            const uint32_t src = sSrc[t], ko = sOkey[t];
+            const uint32_t src = sSrc[t], app = pay[t], ko = rOkey[app];
             if (ko < A.ncell_cap) {
                 uint32_t leave = 0;
-                for (uint32_t r = 0; r < M; ++r) leave += sOkey[r] == ko ? 1u : 0u;
+                for (uint32_t r = 0; r < M; ++r) leave += rOkey[r] == ko ? 1u : 0u;
                 const uint32_t enter = lds_count_lt64(sX, M, (uint64_t)(ko + 1) << 32) -
                                        lds_count_lt64(sX, M, (uint64_t)ko << 32);
-                if (sOsz[t] + enter == leave) A.crange[ko] = make_uint2(EMPTY, 0);
+                if (rOsz[app] + enter == leave) A.crange[ko] = make_uint2(EMPTY, 0);
             }
             A.moverbits[src >> 6] = 0;
             A.blockcnt[src / ROWS_T] = 0;
