@@ -66,19 +66,23 @@ def profiled_traffic(kernel_sig):
     return None
 
 
-def run_fp64(scene, steps, warmup, lib):
+def run_fp64(scene, steps, warmup, lib, phase_steps=20):
+    """The plain FP64 path: `steps` steps after `warmup` (timed without
+    events), then `phase_steps` more with HIP-event phase timing."""
     from metakonga_amd.xdem import Simulation
     sim = Simulation(scene, lib)
     t0 = time.perf_counter()
-    sim.step(warmup)
+    if warmup:
+        sim.step(warmup)
     sim.sync()
     log(f"fp64 warmup {warmup} steps {time.perf_counter() - t0:.2f}s")
-    abi.check(lib, lib.xdem_sim_enable_timing(sim.h, 1))
-    sim.sync()
     t0 = time.perf_counter()
     sim.step(steps)
     sim.sync()
     wall = time.perf_counter() - t0
+    abi.check(lib, lib.xdem_sim_enable_timing(sim.h, 1))
+    sim.step(phase_steps)
+    sim.sync()
     ms = np.zeros(8)
     cnt = np.zeros(8, np.uint32)
     abi.check(lib, lib.xdem_sim_phase_times(sim.h, abi.ptr(ms), abi.ptr(cnt)))
@@ -89,25 +93,27 @@ def run_fp64(scene, steps, warmup, lib):
     sim.close()
     n = scene.np
     return dict(wall_s=wall, ms_per_step=wall * 1e3 / steps, value=n * steps / wall,
-                phase_ms=dict(zip(PHASES, (ms / max(steps, 1)).tolist())),
-                phase_launches=dict(zip(PHASES, cnt.tolist())))
+                phase_ms=dict(zip(PHASES, (ms / phase_steps).tolist())),
+                phase_launches=dict(zip(PHASES, cnt.tolist())), phase_steps=phase_steps)
 
 
 def run_batched(nscene, steps, lib, block=100, seed=0xD3E3):
     """Batched box settling (many independent scenes, N_s ~ logU[2k, 200k],
-    one radius per scene), all advanced together on the fused path."""
+    one radius per scene), all advanced together on the fused path: the whole
+    `steps`-step job from the scenes' initial state (resident in HBM first),
+    the launch shape autotuned on a separate batch of the same scenes."""
     from metakonga_amd import batch
     t0 = time.perf_counter()
     scs = scenes.batch_scenes(nscene=nscene, seed=seed, nsteps=steps)
     gen = time.perf_counter() - t0
     ntot = sum(s.np for s in scs)
     cells = sum(int(np.prod(s.grid_size)) for s in scs)
+    tuned = batch.tune_shape(scs, lib)
     b = batch.Batch(ntot, nscene, cells, lib)
+    batch.apply_shape(b, tuned[0])
     for s in scs:
         b.add(s)
-    b.step(block)          # upload + first (full) sort + first block, untimed
-    tuned = b.autotune(steps_per_trial=20)
-    b.step(block)
+    b.prepare()
     b.sync()
     t0 = time.perf_counter()
     done = 0
@@ -136,9 +142,11 @@ def run_settling_8k(lib, steps=2000):
     sim.sync()
     f64 = sc.np * steps / (time.perf_counter() - t0)
     sim.close()
+    tuned = batch.tune_shape([sc], lib)
     b = batch.Batch(sc.np, 1, int(np.prod(sc.grid_size)), lib)
+    batch.apply_shape(b, tuned[0])
     b.add(sc)
-    b.step(100)
+    b.prepare()
     b.sync()
     t0 = time.perf_counter()
     for _ in range(steps // 100):
@@ -159,11 +167,11 @@ def run_c2_drum(lib, steps=200, f64_steps=20):
     t0 = time.perf_counter()
     sc = scenes.drum()
     gen = time.perf_counter() - t0
+    tuned = batch.tune_shape([sc], lib)
     b = batch.Batch(sc.np, 1, int(np.prod(sc.grid_size)), lib)
+    batch.apply_shape(b, tuned[0])
     sid = b.add(sc)
-    b.step(20)
-    tuned = b.autotune(steps_per_trial=20)
-    b.step(100)
+    b.prepare()
     b.sync()
     t0 = time.perf_counter()
     done = 0
@@ -388,8 +396,8 @@ def cpu_model():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=5000, help="timed steps (the whole 5,000-step run)")
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--np", type=int, default=1_000_000, help="particles of the hopper workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
