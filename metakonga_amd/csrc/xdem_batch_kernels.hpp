@@ -1805,35 +1805,42 @@ __device__ inline uint32_t lds_count_lt64(const uint64_t* v, uint32_t m, uint64_
 // whose rank among the stayers is j - (movers placed before j).  On the full
 // merge the blocks sort the movers for k_merge instead.
 constexpr uint32_t PATCH_SLOTS_MAX = (PATCH_GRID - 3u) * PATCH_T;  // one per sweep thread
-// Bitonic sort of P (a power of two, at most PATCH_T) entries of key (with
-// payload val) and, in the same network, of k2 (its own order); one element
-// per thread; every thread of the block calls it.
-__device__ inline void block_bitonic(uint64_t* key, uint32_t* val, uint32_t* k2, uint32_t P) {
+// Bitonic sort of P (a power of two, at most PATCH_T) entries held one per
+// thread in registers: key with payload val and, in the same network, k2 (its
+// own order).  Partners closer than a wave are exchanged by shuffles; the
+// stages with partners 64 or more threads away go through LDS (xk, xv, x2:
+// PATCH_T entries each) between barriers.  Threads at or past P carry
+// padding and take part in the barriers only.  Every thread calls it.
+__device__ inline void block_bitonic(uint64_t& key, uint32_t& val, uint32_t& k2, uint32_t P,
+                                     uint64_t* xk, uint32_t* xv, uint32_t* x2) {
     const uint32_t i = threadIdx.x;
     for (uint32_t size = 2; size <= P; size <<= 1)
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            const uint32_t j = i ^ stride;
-            if (i < P && j > i) {
-                const bool up = (i & size) == 0;
-                const uint64_t a = key[i], b = key[j];
-                if ((a > b) == up) {
-                    key[i] = b;
-                    key[j] = a;
-                    if (val) {
-                        const uint32_t tv = val[i];
-                        val[i] = val[j];
-                        val[j] = tv;
-                    }
-                }
-                if (k2) {
-                    const uint32_t c = k2[i], d = k2[j];
-                    if ((c > d) == up) {
-                        k2[i] = d;
-                        k2[j] = c;
-                    }
-                }
+            uint64_t pk;
+            uint32_t pv, p2;
+            if (stride >= 64) {
+                xk[i] = key;
+                xv[i] = val;
+                x2[i] = k2;
+                __syncthreads();
+                pk = xk[i ^ stride];
+                pv = xv[i ^ stride];
+                p2 = x2[i ^ stride];
+                __syncthreads();
+            } else {
+                const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)key, (int)stride, 64);
+                const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(key >> 32), (int)stride, 64);
+                pk = ((uint64_t)hi << 32) | lo;
+                pv = (uint32_t)__shfl_xor((int)val, (int)stride, 64);
+                p2 = (uint32_t)__shfl_xor((int)k2, (int)stride, 64);
             }
-            __syncthreads();
+            // the lower index of a pair keeps the smaller in an ascending run
+            const bool take_min = ((i & stride) == 0) == ((i & size) == 0);
+            if (take_min ? pk < key : pk > key) {
+                key = pk;
+                val = pv;
+            }
+            k2 = take_min ? min(k2, p2) : max(k2, p2);
         }
 }
 
@@ -1868,25 +1875,29 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         // padded to a power of two for the sorting network
         uint32_t P = 64;
         while (P < M) P <<= 1;
+        const bool rec = t < M;
+        uint64_t key = rec ? A.mkey[t] : ~0ull;
+        uint32_t val = t;
+        uint32_t src0 = rec ? A.ms[t] : 0xFFFFFFFFu;
+        if (rec) {
+            rS[t] = src0;
+            rL[t] = A.mplo[t];
+            const uint4 mo = A.mold[t];
+            rOsz[t] = mo.x;
+            rOsc[t] = mo.y;
+            rOkey[t] = mo.z;
+        }
+        // by (key, id) with the append index as payload; storage indices ascending
+        block_bitonic(key, val, src0, P, wk, wA, wB);
         if (t < P) {
-            const bool rec = t < M;
-            sX[t] = rec ? A.mkey[t] : ~0ull;
-            pay[t] = t;
-            const uint32_t src = rec ? A.ms[t] : 0xFFFFFFFFu;
-            sAsc[t] = src;
-            if (rec) {
-                rS[t] = src;
-                rL[t] = A.mplo[t];
-                const uint4 mo = A.mold[t];
-                rOsz[t] = mo.x;
-                rOsc[t] = mo.y;
-                rOkey[t] = mo.z;
-            }
+            sX[t] = key;
+            pay[t] = val;
+            sAsc[t] = src0;
         }
         __syncthreads();
-        block_bitonic(sX, pay, sAsc, P);  // by (key, id); storage indices ascending
         // new position and window of the mover of rank t: previous entries
         // below it, less the movers among them, plus the movers before it
+        uint64_t wkey = ~0ull;
         if (t < M) {
             const uint32_t app = pay[t];
             const uint32_t src = rS[app], lo = rL[app];
@@ -1894,17 +1905,15 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
             sSrc[t] = src;
             sOut[t] = t + lo - below;
             const uint32_t a = src < lo ? src : lo, b = src < lo ? lo - 1 : src;
-            wk[t] = ((uint64_t)a << 32) | b;
-        } else if (t < P) {
-            wk[t] = ~0ull;
+            wkey = ((uint64_t)a << 32) | b;
         }
-        __syncthreads();
-        block_bitonic(wk, nullptr, nullptr, P);  // windows by (first, last)
+        uint32_t u0 = 0, u1 = 0;
+        block_bitonic(wkey, u0, u1, P, wk, wA, wB);  // windows by (first, last)
         // merged where they overlap or touch: a running maximum of the last
         // positions decides where a merged window starts and ends
         const bool valid = t < M;
-        const uint32_t a = valid ? (uint32_t)(wk[t] >> 32) : 0xFFFFFFFFu;
-        const uint32_t b = valid ? (uint32_t)wk[t] : 0u;
+        const uint32_t a = valid ? (uint32_t)(wkey >> 32) : 0xFFFFFFFFu;
+        const uint32_t b = valid ? (uint32_t)wkey : 0u;
         const uint32_t run = block_incl_max(b, wtmp);
         if (valid) wA[t] = run;  // (t < M <= FEW)
         __syncthreads();

@@ -130,3 +130,33 @@ def test_lists_alone_vs_mix(lib):
         alone = run([k])[0]
         for key in alone:
             assert np.array_equal(alone[key], mixed[k][key]), (k, key)
+
+
+def test_batch_many_movers_window_patch(lib):
+    """The 256 box scenes (1.07e7 spheres) over 2,400 steps: the particles that
+    change cell per step grow past 512 (the block path sorting them with its
+    largest networks), the window patch keeps every list (no scene rebuilds
+    them all), and three sampled scenes end bit-identical to the run that takes
+    the full merge every step."""
+    scs = scenes.batch_scenes()
+    ntot = sum(s.np for s in scs)
+    cells = sum(int(np.prod(s.grid_size)) for s in scs)
+    picks = [0, 128, 255]
+    out = []
+    for patch in (True, False):
+        b = Batch(ntot, len(scs), cells)
+        b.lists(True, 0.05)
+        b.sort_patch(patch)
+        ids = [b.add(s) for s in scs]
+        trace = []
+        for _ in range(24):
+            b.step(100)
+            trace.append((b.last_movers(), b.list_rebuilds()))
+        out.append([b.state(ids[k]) for k in picks])
+        b.close()
+        if patch:
+            print(f"movers and scenes rebuilding, per 100-step block: {trace}")
+            assert any(512 < m <= 1024 and r == 0 for m, r in trace), trace
+    for k in range(len(picks)):
+        for key in out[0][k]:
+            assert np.array_equal(out[0][k][key], out[1][k][key]), (picks[k], key)

@@ -1,0 +1,204 @@
+"""The window-patch re-sort (k_patch) restated on the CPU and checked
+against a full stable re-sort: movers ranked by (key, id), their storage
+indices ascending, windows [src, lo - 1] / [lo, src] merged where they overlap
+or touch, every window position's element found by binary search (the mover
+placed there, or the stayer of rank q at q + #{i : sAsc[i] - i <= q}), cell
+boundaries rewritten at window positions, emptied cells cleared and the
+previous windows reset to the identity.  Random cases: the permutation, the
+keys at every position and the cell ranges must equal the full re-sort's.
+This pins the algorithm the GPU kernel implements; the GPU tests compare the
+kernel with the full merge and with the oracle's sort.
+"""
+import bisect
+import numpy as np
+
+EMPTY = 0xFFFFFFFF
+
+
+def ranges(skeys, ncell):
+    cr = np.full((ncell, 2), [EMPTY, 0], dtype=np.int64)
+    n = len(skeys)
+    for j in range(n):
+        if j == 0 or skeys[j - 1] != skeys[j]:
+            cr[skeys[j], 0] = j
+            if j > 0:
+                cr[skeys[j - 1], 1] = j
+    if n:
+        cr[skeys[n - 1], 1] = n
+    return cr
+
+
+def count_lt(arr, x):  # sorted arr
+    return bisect.bisect_left(arr, x)
+
+
+def count_le(arr, x):
+    return bisect.bisect_right(arr, x)
+
+
+def patch(keys, ids, newkeys, movers, cr_old, prev_windows, perm_in):
+    """keys/ids: old sorted order (storage); movers: storage indices (append
+    order); returns perm, skey, crange, windows — mirroring the kernel."""
+    n = len(keys)
+    M = len(movers)
+    rX = [(int(newkeys[s]) << 32) | int(ids[s]) for s in movers]
+    rS = [int(s) for s in movers]
+    oldX = [(int(keys[q]) << 32) | int(ids[q]) for q in range(n)]
+    # insertion point recorded with the mover: previous entries below X
+    rL = [bisect.bisect_left(oldX, x) for x in rX]
+    # P1: ranks by counting
+    sX = [0] * M; sSrc = [0] * M; sOut = [0] * M; sAsc = [0] * M; sOldKey = [0] * M
+    wA = [0] * M; wB = [0] * M
+    for t in range(M):
+        rk = sum(1 for r in range(M) if rX[r] < rX[t])
+        rs = sum(1 for r in range(M) if rS[r] < rS[t])
+        below = sum(1 for r in range(M) if rS[r] < rL[t])
+        out = rk + rL[t] - below
+        a = rS[t] if rS[t] < rL[t] else rL[t]
+        b = rL[t] - 1 if rS[t] < rL[t] else rS[t]
+        sX[rk] = rX[t]; sSrc[rk] = rS[t]; sOut[rk] = out; sOldKey[rk] = int(keys[rS[t]])
+        sAsc[rs] = rS[t]
+        wA[t] = a; wB[t] = b
+    # P2: windows ranked by (a, b, t)
+    sWa = [0] * M; sWb = [0] * M
+    for t in range(M):
+        rw = sum(1 for r in range(M) if (wA[r], wB[r], r) < (wA[t], wB[t], t))
+        sWa[rw] = wA[t]; sWb[rw] = wB[t]
+    # P3: merge by running max; compaction
+    run = []
+    m = -1
+    for t in range(M):
+        m = max(m, sWb[t])
+        run.append(m)
+    starts = [t == 0 or sWa[t] > run[t - 1] + 1 for t in range(M)]
+    win = []
+    for t in range(M):
+        if starts[t]:
+            win.append([sWa[t], None])
+        if t + 1 == M or starts[t + 1]:
+            win[-1][1] = run[t]
+    nw = len(win)
+    wl = [w[1] - w[0] + 2 for w in win]
+    wpre = [0]
+    for x in wl:
+        wpre.append(wpre[-1] + x)
+    total_new = wpre[-1]
+    pl = [b - a + 1 for a, b in prev_windows]
+    ppre = [0]
+    for x in pl:
+        ppre.append(ppre[-1] + x)
+    total = total_new + ppre[-1]
+    starts_sorted = [w[0] for w in win]
+
+    def elem(j):
+        c1 = count_lt(sOut, j)
+        if c1 < M and sOut[c1] == j:
+            return sSrc[c1], int(sX[c1] >> 32)
+        q0 = j - c1
+        lo, hi = 0, M
+        while lo < hi:
+            mid = (lo + hi) // 2
+            if sAsc[mid] - mid <= q0:
+                lo = mid + 1
+            else:
+                hi = mid
+        s = min(q0 + lo, n - 1)
+        return s, int(keys[s])
+
+    def in_new(j):
+        k = bisect.bisect_right(starts_sorted, j) - 1
+        return k >= 0 and j <= win[k][1]
+
+    perm = perm_in.copy()
+    skey = newkeys.copy()
+    cr = cr_old.copy()
+    # writes collected per position, then applied (the kernel's threads write
+    # disjoint positions; boundaries are written once each)
+    for g in range(total):
+        if g < total_new:
+            k = bisect.bisect_right(wpre, g) - 1
+            a, b = win[k]
+            j = a + (g - wpre[k])
+            if j > n:
+                continue
+            kc = kp = None
+            if j <= b:
+                e, kc = elem(j)
+                perm[j] = e
+                skey[j] = kc
+            elif j < n:
+                kc = int(keys[j])
+            if j > 0:
+                kp = elem(j - 1)[1] if j - 1 >= a else int(keys[j - 1])
+            if j == 0 or j == n or kp != kc:
+                if j < n:
+                    cr[kc, 0] = j
+                if j > 0:
+                    cr[kp, 1] = j
+        else:
+            gg = g - total_new
+            k = bisect.bisect_right(ppre, gg) - 1
+            j = prev_windows[k][0] + (gg - ppre[k])
+            if j < n and not in_new(j):
+                perm[j] = j
+    # emptied cells
+    for t in range(M):
+        ko = sOldKey[t]
+        size = cr_old[ko, 1] - cr_old[ko, 0] if cr_old[ko, 0] != EMPTY else 0
+        leave = sum(1 for r in range(M) if sOldKey[r] == ko)
+        klist = [int(x >> 32) for x in sX]
+        enter = count_le(klist, ko) - count_lt(klist, ko)
+        if size + enter == leave:
+            cr[ko] = [EMPTY, 0]
+    return perm, skey, cr, [tuple(w) for w in win]
+
+
+def trial(rng, n, ncell, M, prev):
+    ids = rng.permutation(n)
+    keys = np.sort(rng.integers(0, ncell, n))
+    o = np.lexsort((ids, keys))
+    keys = keys[o]
+    ids = ids[o]
+    cr_old = ranges(keys, ncell)
+    movers = rng.choice(n, size=M, replace=False)
+    newkeys = keys.copy()
+    for s in movers:
+        k = rng.integers(0, ncell)
+        while k == keys[s]:
+            k = rng.integers(0, ncell)
+        newkeys[s] = k
+    perm_ref = np.lexsort((ids, newkeys))
+    # the previous sort's windows: perm holds garbage there (a permutation of
+    # their own positions), the identity elsewhere
+    perm_in = np.arange(n)
+    for a, b in prev:
+        perm_in[a:b + 1] = rng.permutation(np.arange(a, b + 1))
+    perm, skey, cr, win = patch(keys, ids, newkeys, movers, cr_old, prev, perm_in)
+    assert np.array_equal(perm, perm_ref), (n, M)
+    # keys at every position: inside windows as written, outside unchanged
+    want_keys = newkeys[perm_ref]
+    inU = np.zeros(n, bool)
+    for a, b in win:
+        inU[a:b + 1] = True
+    assert np.array_equal(np.where(inU, skey, keys), want_keys), (n, M)
+    ref = ranges(want_keys, ncell)
+    assert np.array_equal(cr, ref), (n, M)
+    return win
+
+
+
+
+def test_window_patch_matches_full_resort():
+    rng = np.random.default_rng(5)
+    for it in range(1500):
+        n = int(rng.integers(1, 120))
+        ncell = int(rng.integers(2, 50))
+        M = int(rng.integers(0, min(n, 100) + 1))
+        prev = []
+        p = 0
+        while p < n and len(prev) < 5:
+            a = int(rng.integers(p, n))
+            b = int(rng.integers(a, min(n, a + 10)))
+            prev.append((a, b))
+            p = b + 2
+        trial(rng, n, ncell, M, prev)
