@@ -28,8 +28,8 @@ namespace xdem {
 namespace {
 
 constexpr int RADIX = 256;
-constexpr int SORT_THREADS = 256;  // 4 waves
-constexpr int SORT_ITEMS = 16;
+constexpr int SORT_THREADS = 1024;  // 16 waves: a 4096-key tile with 4 keys per lane
+constexpr int SORT_ITEMS = 4;
 constexpr int TILE = SORT_THREADS * SORT_ITEMS;
 constexpr int WAVES = SORT_THREADS / 64;
 
@@ -166,22 +166,24 @@ __global__ __launch_bounds__(SORT_THREADS) void k_scatter(
         }
         gstart[t] = offsets[t * ntiles + blockIdx.x];
     }
-    // digit bases: exclusive scan of the digit totals (every block redoes it)
+    // digit bases: exclusive scan of the digit totals (every block redoes it;
+    // the first RADIX threads, one digit each)
     {
-        uint32_t v = dtot[t];
-        uint32_t inc = wave_incl_scan(v);
-        if (lane == 63) wtot[w] = inc;
+        const uint32_t v = t < RADIX ? dtot[t] : 0u;
+        const uint32_t inc = wave_incl_scan(v);
+        if (lane == 63 && t < RADIX) wtot[w] = inc;
         __syncthreads();
-        uint32_t before = 0;
-        for (uint32_t k = 0; k < w; ++k) before += wtot[k];
-        dbase[t] = before + inc - v;
-        __syncthreads();
-        gstart[t] += dbase[t];
+        if (t < RADIX) {
+            uint32_t before = 0;
+            for (uint32_t k = 0; k < w; ++k) before += wtot[k];
+            dbase[t] = before + inc - v;
+            gstart[t] += dbase[t];
+        }
         __syncthreads();
     }
-    // exclusive scan of totals over the 256 digits (SORT_THREADS == RADIX)
-    uint32_t incl = wave_incl_scan(total);
-    if (lane == 63) wtot[w] = incl;
+    // exclusive scan of the tile's totals over the digits
+    const uint32_t incl = wave_incl_scan(total);
+    if (lane == 63 && t < RADIX) wtot[w] = incl;
     __syncthreads();
     if (t < RADIX) {
         uint32_t before = 0;
