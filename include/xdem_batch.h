@@ -118,16 +118,16 @@ uint64_t xdem_batch_device_bytes(uint32_t max_particles, uint32_t max_scenes, ui
  * instead of walking the 27 cells.  A particle walks its cells again (and
  * rebuilds its list) at the first step of every xdem_batch_step call, when
  * its scene's accumulated displacement bound reaches half the skin, when more
- * than 64 particles of the batch changed cell in the last step, when it moved
+ * than 1024 particles of the batch changed cell in the last step, when it moved
  * in the sorted order, or when a particle changed cell within one cell of its
  * own; contacts and their order, hence the results, are bit-identical to the
  * walk.  Plane-only batches (no history, drums or meshes); the lists take
- * (32 + 1) x 4 bytes per particle of capacity and a bit per cell. */
+ * (32 + 1) x 4 bytes per particle of capacity and 2 bytes per cell. */
 int32_t xdem_batch_set_lists(xdem_batch* b, int32_t on, float skin_fraction);
 /* Scenes whose lists were all rebuilt for the last step (the rest reused
  * theirs, particle by particle as above). */
 int32_t xdem_batch_list_rebuilds(xdem_batch* b, uint32_t* scenes_rebuilding);
-/* The re-sort between steps: with at most 64 particles changed cell (on, the
+/* The re-sort between steps: with at most 1024 particles changed cell (on, the
  * default) only the windows of the sorted order those particles shift are
  * rewritten, in one kernel; off: the full merge every step.  The results are
  * identical. */

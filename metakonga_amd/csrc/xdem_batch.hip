@@ -136,9 +136,8 @@ struct xdem_batch {
     float skin_frac = 0.05f;          // skin = skin_frac x cell size
     uint32_t* vl = nullptr;           // VLK entries per particle, lane-interleaved
     uint32_t* vc = nullptr;           // entries per particle
-    uint8_t* dirty = nullptr;         // byte per cell key
+    uint16_t* dirty = nullptr;        // per cell key: tag of the sort that last marked it
     size_t dirty_bytes = 0;
-    uint32_t* dprev = nullptr;        // marked neighbourhoods of the last two sorts
     float* disp = nullptr;            // per scene (cap_scenes each)
     uint32_t* wdisp = nullptr;        // per block of k_step: its largest displacement
     uint2* d_scene_blk = nullptr;     // per scene: first and end block of the sorted table
@@ -207,7 +206,7 @@ struct xdem_batch {
         free_state(buf[1]);
         if (d_blocks_store == d_blocks_sorted) d_blocks_store = nullptr;
         for (void* p : {(void*)invperm, (void*)win, (void*)mold, (void*)vl, (void*)vc, (void*)dirty,
-                        (void*)dprev, (void*)disp, (void*)wdisp, (void*)d_scene_blk, (void*)force, (void*)rebuild})
+                        (void*)disp, (void*)wdisp, (void*)d_scene_blk, (void*)force, (void*)rebuild})
             if (p) (void)hipFree(p);
         for (void* p : {(void*)skey[0], (void*)skey[1], (void*)perm, (void*)crange,
                         (void*)moverbits, (void*)blockcnt, (void*)counters, (void*)mkey, (void*)ms,
@@ -315,7 +314,6 @@ struct xdem_batch {
         a.win = win;
         a.lists = lists_on && vl ? 1u : 0u;
         a.dirty = dirty;
-        a.dprev = dprev;
         a.scenes = d_scenes;
         a.nscenes = (uint32_t)scenes.size();
         a.disp = disp;
@@ -786,6 +784,7 @@ StepArgs step_args(xdem_batch* b, bool dbg) {
     a.invperm = b->invperm;
     a.rebuild = b->rebuild;
     a.dirty = b->dirty;
+    a.dtag = b->counters + 6 + b->skcur;  // the tag of the last sort (k_patch, parity skcur)
     a.wdisp = b->wdisp;
     a.scenes = b->d_scenes;
     a.planes = b->d_planes;
@@ -926,14 +925,14 @@ uint64_t batch_device_bytes(uint64_t n, uint64_t nscene, uint64_t ncell, bool hi
     b += (nblk + 1) * sizeof(uint32_t) + 8 * sizeof(uint32_t);  // blockcnt, counters
     b += n * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t) + 2 * sizeof(uint32_t)) +
          FEW * sizeof(uint4);                             // movers
-    b += (2 * WIN_WORDS + 2 * DPREV_WORDS) * sizeof(uint32_t);  // windows, marked cells
+    b += 2 * WIN_WORDS * sizeof(uint32_t);              // windows
     b += 5 * nscene * sizeof(uint32_t);                   // disp, force, rebuild, scene blocks
     b += wdisp_words(n, nscene) * sizeof(uint32_t);       // block displacements
     b += 1024 * sizeof(double);                           // d_red
     b += nscene * sizeof(BScene) + 2 * div_up((uint32_t)std::min<uint64_t>(n, 0xFFFFFFFFull), 64) *
                                        sizeof(BlockEntry);
     if (history) b += 2 * n * (HSLOTS * sizeof(float4) + sizeof(uint32_t)) + sizeof(uint32_t);
-    if (lists) b += n * (VLK + 1) * sizeof(uint32_t) + ncell;  // lists, counts, dirty bytes
+    if (lists) b += n * (VLK + 1) * sizeof(uint32_t) + ncell * sizeof(uint16_t);  // lists, counts, dirty tags
     return b;
 }
 }  // namespace
@@ -971,8 +970,8 @@ int32_t xdem_batch_set_lists(xdem_batch* b, int32_t on, float skin_fraction) {
     if (on && !b->vl) {
         XDEM_TRY(dalloc(&b->vl, (size_t)VLK * b->cap_np));
         XDEM_TRY(dalloc(&b->vc, (size_t)b->cap_np));
-        b->dirty_bytes = (size_t)b->cap_cells;
-        XDEM_TRY(dalloc(&b->dirty, b->dirty_bytes));
+        b->dirty_bytes = (size_t)b->cap_cells * sizeof(uint16_t);
+        XDEM_TRY(dalloc(&b->dirty, (size_t)b->cap_cells));
         XDEM_HIP(hipMemsetAsync(b->dirty, 0, b->dirty_bytes, b->st));
         XDEM_HIP(hipStreamSynchronize(b->st));
         b->layout_dirty = true;  // the sort state restarts with the lists
@@ -1021,14 +1020,13 @@ int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t 
         (rc = dalloc(&b->win, 2 * WIN_WORDS)) || (rc = dalloc(&b->mold, (size_t)FEW)) ||
         (rc = dalloc(&b->mkey, n)) || (rc = dalloc(&b->ms, n)) || (rc = dalloc(&b->mkey2, n)) ||
         (rc = dalloc(&b->ms2, n)) || (rc = dalloc(&b->mplo, 2 * n)) ||
-        (rc = dalloc(&b->dprev, 2 * DPREV_WORDS)) || (rc = dalloc(&b->disp, max_scenes)) ||
+        (rc = dalloc(&b->disp, max_scenes)) ||
         (rc = dalloc(&b->wdisp, wdisp_words(max_particles, max_scenes))) ||
         (rc = dalloc(&b->force, max_scenes)) ||
         (rc = dalloc(&b->rebuild, max_scenes)) || (rc = dalloc(&b->d_red, 8 * 1024)))
         return fail(rc);
     if (hipMemset(b->counters, 0, 32) != hipSuccess ||
         hipMemset(b->win, 0, 8 * WIN_WORDS) != hipSuccess ||
-        hipMemset(b->dprev, 0, 8 * DPREV_WORDS) != hipSuccess ||
         hipMemset(b->disp, 0, 4ull * max_scenes) != hipSuccess ||
         hipMemset(b->wdisp, 0, 4 * wdisp_words(max_particles, max_scenes)) != hipSuccess ||
         hipMemset(b->force, 0, 4ull * max_scenes) != hipSuccess ||

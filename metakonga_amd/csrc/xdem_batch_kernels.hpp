@@ -32,7 +32,6 @@ constexpr int LDS_T = 256;                   // lane stride of the per-thread LD
 constexpr int MERGE_LDS = 2048;              // sorted movers staged in LDS by k_merge up to this count
 constexpr int ROWS_T = 256;                  // threads of the row kernels
 constexpr int FEW = 1024;                    // movers re-placed by the window patch (k_patch)
-constexpr int FEW_WAVE = 64;                 // up to this many, one wave orders them
 constexpr int PATCH_T = 1024;                // threads of a k_patch block
 constexpr int PATCH_GRID = 256;              // blocks of k_patch (grid-stride over the windows)
 constexpr int RANK_MAX = 32768;              // movers sorted by counting ranks (k_patch, every block)
@@ -160,7 +159,8 @@ struct MoverOut {
     uint64_t* mkey;            // (key << 32 | id)
     uint32_t* ms;              // storage index
     uint32_t* mplo;            // insertion point: previous entries below the new (key, id)
-    uint4* mold;               // first FEW movers: (particles in the old cell, scene, old key)
+    uint4* mold;               // first FEW movers: (particles in the old cell, scene, old key,
+                               // the old cell's first position)
     const uint2* zi;           // particle ids, storage order
     const uint32_t* perm;      // sorted position of the last sort -> storage index (null: identity)
     uint64_t* moverbits;
@@ -201,7 +201,9 @@ struct StepArgs {
     uint32_t vl_stride;
     const uint32_t* invperm;   // storage index of `in` -> sorted position
     const uint32_t* rebuild;   // per scene: rebuild every list this step
-    const uint8_t* dirty;      // byte per cell key: a particle changed cell within one cell
+    const uint16_t* dirty;     // per cell key: tag of the last sort that saw a particle change
+                               // cell within one cell of it
+    const uint32_t* dtag;      // the last sort's tag
     uint32_t* wdisp;           // per block L of the launch: the largest displacement of this
                                // step (float bits, bounded above)
     // parity probe outputs (per sorted position) — null in production
@@ -304,12 +306,13 @@ __device__ inline void record_mover(const MoverOut& O, uint32_t s, uint32_t nw, 
             while (lo < O.n && O.key_old[lo] == nw && O.zi[O.perm ? O.perm[lo] : lo].y < id) ++lo;
             O.mplo[p] = lo;
             if (p < (uint32_t)FEW) {
-                uint32_t size = 0;
+                uint32_t size = 0, first = 0;
                 if (ko < O.ncell_cap) {
                     const uint2 c = O.crange[ko];
                     size = c.x == EMPTY ? 0u : c.y - c.x;
+                    first = c.x == EMPTY ? 0u : c.x;
                 }
-                O.mold[p] = make_uint4(size, scene, ko, 0u);
+                O.mold[p] = make_uint4(size, scene, ko, first);
             }
         }
         atomicOr((unsigned long long*)&O.moverbits[s >> 6], 1ull << (s & 63));
@@ -828,7 +831,7 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     if constexpr (VLIST) {
         vcnt = A.vc[s];
         const uint32_t kown = A.mv.key_old[j];
-        const bool dirty = A.dirty[kown] != 0;
+        const bool dirty = A.dirty[kown] == *A.dtag;
         rebuild = A.rebuild[be.scene] != 0 || s != j || dirty || vcnt > (uint32_t)VLK;
     }
     f3 Ft = F3(0, 0, 0), Fn = F3(0, 0, 0), M = F3(0, 0, 0);
@@ -1631,7 +1634,6 @@ __global__ __launch_bounds__(ROWS_T) void k_ranges(const uint32_t* skey, uint2* 
 // ones get the identity permutation back.  More movers: block 0 sorts them
 // (sort_movers_block) and k_merge / k_ranges do the rest (patched = 0).
 constexpr int WIN_WORDS = 2 + 2 * FEW;   // per parity: count, then (first, last) pairs
-constexpr int DPREV_WORDS = 1 + 2 * FEW; // per parity: count, then (cell key, scene) pairs
 
 struct PatchArgs {
     uint32_t n, ncell_cap, par;
@@ -1654,8 +1656,7 @@ struct PatchArgs {
     uint32_t* win;             // windows by parity (this sort: par; the previous: 1 - par)
     // candidate lists (lists != 0)
     uint32_t lists;
-    uint8_t* dirty;            // byte per cell key
-    uint32_t* dprev;           // marked neighbourhoods by parity
+    uint16_t* dirty;           // per cell key: tag of the sort that last marked it
     const BScene* scenes;
     uint32_t nscenes;
     float* disp;               // per scene: displacement bound since its lists were built
@@ -1665,59 +1666,70 @@ struct PatchArgs {
     uint32_t* rebuild;         // per scene: out, rebuild every list in the next step
 };
 
-// The 27 cells around cell key k of scene S (the walk's wrap-around) get
-// their dirty bytes set (set = true) or cleared (plain byte stores: no
-// read-modify-write, so marks of different movers never interfere).
+// Cell `which` (0..26) of the 27 around cell key k of scene S (the walk's
+// wrap-around) gets the sort's tag (plain stores of one value: marks of
+// different movers never interfere).
 __device__ inline void mark_neighbourhood(const BScene& S, uint32_t k, uint32_t which,
-                                          uint8_t* dirty, bool set) {
+                                          uint16_t* dirty, uint16_t tag) {
     const uint32_t local = k - S.cell_base;
     const int x = (int)(local % S.gx), y = (int)((local / S.gx) % S.gy),
               z = (int)(local / (S.gx * S.gy));
     const int dz = (int)(which / 9) - 1, dy = (int)((which / 3) % 3) - 1, dx = (int)(which % 3) - 1;
     const uint32_t kk = key_of(S, x + dx, y + dy, z + dz);
-    dirty[kk] = set ? 1u : 0u;
+    dirty[kk] = tag;
 }
 
-// Rank of this lane's key among the keys of lanes 0..M-1 (ties by lane).
-__device__ inline uint32_t wave_rank(uint64_t key, uint32_t M) {
-    const uint32_t lane = __lane_id();
-    const int khi = (int)(uint32_t)(key >> 32), klo = (int)(uint32_t)key;
-    uint32_t rank = 0;
-    for (uint32_t r = 0; r < M; ++r) {
-        const uint64_t o = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(khi, (int)r) << 32) |
-                           (uint32_t)__builtin_amdgcn_readlane(klo, (int)r);
-        rank += (o < key || (o == key && r < lane)) ? 1u : 0u;
-    }
-    return rank;
+// Inclusive scans of a wave's 64 lanes by DPP moves: shifts by 1, 2, 4 and 8
+// lanes inside each row of 16, then lane 15 of a row added into the next row
+// (rows 1 and 3) and lane 31 into the upper half (rows 2 and 3).  Lanes with
+// no source take 0, the identity of both operations.
+__device__ inline uint32_t wave_incl_sum(uint32_t v) {
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return (uint32_t)x;
+}
+__device__ inline uint32_t wave_incl_max(uint32_t v) {
+    int x = (int)v;
+    x = (int)max((uint32_t)x, (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));
+    x = (int)max((uint32_t)x, (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));
+    x = (int)max((uint32_t)x, (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));
+    x = (int)max((uint32_t)x, (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));
+    x = (int)max((uint32_t)x, (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));
+    x = (int)max((uint32_t)x, (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));
+    return (uint32_t)x;
 }
 
 // Block-wide scans over PATCH_T threads (one value per thread): inclusive
-// maximum and exclusive sum; every thread of the block calls them.
+// maximum and exclusive sums; every thread of the block calls them.
 __device__ inline uint32_t block_incl_max(uint32_t v, uint32_t* wtmp) {
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = (uint32_t)__shfl_up((int)v, o, 64);
-        if (lane >= (uint32_t)o) v = max(v, u);
-    }
+    v = wave_incl_max(v);
     if (lane == 63) wtmp[w] = v;
     __syncthreads();
-    for (uint32_t q = 0; q < w; ++q) v = max(v, wtmp[q]);
+#pragma unroll
+    for (uint32_t q = 0; q < PATCH_T / 64; ++q) {
+        const uint32_t u = wtmp[q];
+        if (q < w) v = max(v, u);
+    }
     __syncthreads();
     return v;
 }
 __device__ inline uint32_t block_excl_sum(uint32_t v, uint32_t* wtmp, uint32_t& total) {
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = (uint32_t)__shfl_up((int)x, o, 64);
-        if (lane >= (uint32_t)o) x += u;
-    }
+    const uint32_t x = wave_incl_sum(v);
     if (lane == 63) wtmp[w] = x;
     __syncthreads();
     uint32_t before = 0, all = 0;
+#pragma unroll
     for (uint32_t q = 0; q < PATCH_T / 64; ++q) {
-        before += q < w ? wtmp[q] : 0u;
-        all += wtmp[q];
+        const uint32_t u = wtmp[q];
+        before += q < w ? u : 0u;
+        all += u;
     }
     __syncthreads();
     total = all;
@@ -1726,21 +1738,14 @@ __device__ inline uint32_t block_excl_sum(uint32_t v, uint32_t* wtmp, uint32_t& 
 
 __device__ inline uint2 block_excl_sum2(uint2 v, uint32_t* wtmp, uint2& total) {
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint2 x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t ux = (uint32_t)__shfl_up((int)x.x, o, 64);
-        const uint32_t uy = (uint32_t)__shfl_up((int)x.y, o, 64);
-        if (lane >= (uint32_t)o) {
-            x.x += ux;
-            x.y += uy;
-        }
-    }
+    const uint2 x = make_uint2(wave_incl_sum(v.x), wave_incl_sum(v.y));
     if (lane == 63) {
         wtmp[2 * w] = x.x;
         wtmp[2 * w + 1] = x.y;
     }
     __syncthreads();
     uint2 before = make_uint2(0u, 0u), all = make_uint2(0u, 0u);
+#pragma unroll
     for (uint32_t q = 0; q < PATCH_T / 64; ++q) {
         const uint32_t a = wtmp[2 * q], b = wtmp[2 * q + 1];
         if (q < w) {
@@ -1790,132 +1795,193 @@ __device__ inline uint32_t lds_count_lt64(const uint64_t* v, uint32_t m, uint64_
     return lo;
 }
 
-// Each block works out the movers' order and the merged windows in LDS:
-// up to FEW_WAVE movers one wave does it with lane ranks and shuffles; up to
-// FEW the whole block (thread t for mover t: ranks by counting over the M
-// records, windows ranked by (first, last, mover), merged by a block-wide
-// running maximum of their last positions).  When the new windows hold more
-// than PATCH_SLOTS_MAX positions (movers across far-apart cells) the full
-// merge is taken instead.  Roles then: block 0 the counters, block 1 the
-// emptied cells, the movers' marks and the window record, block 2 the dirty
-// cells, every block a share of the scenes' displacement bounds, blocks 3..
-// the sweep: new window positions (one past each too, for its last boundary)
-// and the previous windows' positions outside the new ones; a position's
+// Ranks among the keys of threads 0..M-1 (M <= PATCH_T; ties by thread
+// index), for one key set or two at once.  The keys form chunks of 64 (by
+// thread); every wave of the block takes a share of the comparisons:
+// first each chunk's 64 x 64 (its keys' ranks inside the chunk; the chunk is
+// then written in that order to cb1 / cb2), then, per pair of chunks, a
+// binary search of each key of one chunk in the other (equal keys of a lower
+// chunk precede).  Partial counts are summed with LDS atomics in R1 / R2.
+// kb*, cb*, R*: PATCH_T entries of LDS each.  Every thread of the block calls
+// it; the buffers may be reused after the next barrier.
+template <bool TWO>
+__device__ inline uint2 block_rank(uint64_t k1, uint32_t k2, uint32_t M, uint64_t* kb1,
+                                   uint64_t* cb1, uint32_t* kb2, uint32_t* cb2, uint32_t* R1,
+                                   uint32_t* R2) {
+    constexpr uint32_t NWV = PATCH_T / 64;
+    const uint32_t t = threadIdx.x, lane = t & 63;
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(t >> 6));
+    const bool has = t < M;
+    if (has) {
+        kb1[t] = k1;
+        if (TWO) kb2[t] = k2;
+    }
+    R1[t] = 0;
+    if (TWO) R2[t] = 0;
+    __syncthreads();
+    const uint32_t nch = (M + 63) >> 6;
+    // ---- inside each chunk: S waves per chunk, each a slice of its 64 keys
+    const uint32_t S = nch ? max(1u, NWV / nch) : 1u;
+    {
+        const uint32_t c = w / S, part = w - c * S;
+        if (c < nch) {
+            const uint32_t b0 = c * 64, cn = min(64u, M - b0);
+            const uint32_t e = b0 + lane;  // the key this lane ranks
+            const bool mine = lane < cn;
+            const uint64_t x1 = mine ? kb1[e] : 0ull;
+            const uint32_t x2 = TWO && mine ? kb2[e] : 0u;
+            const uint32_t r0 = part * 64 / S, r1e = min((part + 1) * 64 / S, cn);
+            uint32_t a1 = 0, a2 = 0;
+            auto cmp = [&](uint32_t r, uint64_t o1, uint32_t o2) {
+                a1 += (o1 < x1 || (o1 == x1 && r < lane)) ? 1u : 0u;
+                if (TWO) a2 += (o2 < x2 || (o2 == x2 && r < lane)) ? 1u : 0u;
+            };
+            // the same entries for every lane (broadcasts), four in flight
+            uint32_t r = r0;
+            for (; r + 4 <= r1e; r += 4) {
+                uint64_t o1[4];
+                uint32_t o2[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    o1[u] = kb1[b0 + r + u];
+                    o2[u] = TWO ? kb2[b0 + r + u] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) cmp(r + u, o1[u], o2[u]);
+            }
+            for (; r < r1e; ++r) cmp(r, kb1[b0 + r], TWO ? kb2[b0 + r] : 0u);
+            if (mine && r0 < r1e) {
+                atomicAdd(&R1[e], a1);
+                if (TWO) atomicAdd(&R2[e], a2);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t c0 = w * 64;
+    if (has) {
+        cb1[c0 + R1[t]] = k1;
+        if (TWO) cb2[c0 + R2[t]] = k2;
+    }
+    __syncthreads();
+    // ---- across chunks: pair q = (chunk of the keys, chunk searched)
+    for (uint32_t q = w; q < nch * nch; q += NWV) {
+        const uint32_t ce = q / nch, cs = q - ce * nch;
+        if (ce == cs) continue;
+        const uint32_t e = ce * 64 + lane;
+        if (e >= M) continue;
+        const uint64_t x1 = kb1[e];
+        const uint32_t x2 = TWO ? kb2[e] : 0u;
+        const uint32_t b0 = cs * 64, m = min(64u, M - b0);
+        const bool lower = cs < ce;
+        uint32_t lo1 = 0, hi1 = m, lo2 = 0, hi2 = TWO ? m : 0u;
+        while (lo1 < hi1 || lo2 < hi2) {  // the two searches in step
+            if (lo1 < hi1) {
+                const uint32_t mid = (lo1 + hi1) >> 1;
+                const uint64_t o = cb1[b0 + mid];
+                if (o < x1 || (lower && o == x1))
+                    lo1 = mid + 1;
+                else
+                    hi1 = mid;
+            }
+            if (TWO && lo2 < hi2) {
+                const uint32_t mid = (lo2 + hi2) >> 1;
+                const uint32_t o = cb2[b0 + mid];
+                if (o < x2 || (lower && o == x2))
+                    lo2 = mid + 1;
+                else
+                    hi2 = mid;
+            }
+        }
+        if (lo1) atomicAdd(&R1[e], lo1);
+        if (TWO && lo2) atomicAdd(&R2[e], lo2);
+    }
+    __syncthreads();
+    return make_uint2(has ? R1[t] : 0u, TWO && has ? R2[t] : 0u);
+}
+
+// Each block works out the movers' order and the merged windows in LDS, thread
+// t holding mover t of the append order: ranks by (key, id) and by storage
+// index (block_rank), each mover's new position (its rank plus the previous
+// entries below its insertion point, less the movers among them) and its
+// window between old and new position, the windows ranked by (first, last)
+// and merged by a block-wide running maximum of their last positions.  More
+// than FEW movers, or new windows holding more than PATCH_SLOTS_MAX positions
+// (movers across far-apart cells), take the full merge instead.  Roles then:
+// block 0 the counters, block 1 the emptied cells, the movers' marks and the
+// window record, blocks 2.. the sweep from the bottom (new window positions,
+// one past each too for its last boundary, then the previous windows'
+// positions outside the new ones) and the dirty marks from the top; then
+// every block a share of the scenes' displacement bounds.  A position's
 // element comes from binary searches: the mover placed there, or the stayer
 // whose rank among the stayers is j - (movers placed before j).  On the full
 // merge the blocks sort the movers for k_merge instead.
-constexpr uint32_t PATCH_SLOTS_MAX = (PATCH_GRID - 3u) * PATCH_T;  // one per sweep thread
-// Bitonic sort of P (a power of two, at most PATCH_T) entries held one per
-// thread in registers: key with payload val and, in the same network, k2 (its
-// own order).  Partners closer than a wave are exchanged by shuffles; the
-// stages with partners 64 or more threads away go through LDS (xk, xv, x2:
-// PATCH_T entries each) between barriers.  Threads at or past P carry
-// padding and take part in the barriers only.  Every thread calls it.
-__device__ inline void block_bitonic(uint64_t& key, uint32_t& val, uint32_t& k2, uint32_t P,
-                                     uint64_t* xk, uint32_t* xv, uint32_t* x2) {
-    const uint32_t i = threadIdx.x;
-    for (uint32_t size = 2; size <= P; size <<= 1)
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            uint64_t pk;
-            uint32_t pv, p2;
-            if (stride >= 64) {
-                xk[i] = key;
-                xv[i] = val;
-                x2[i] = k2;
-                __syncthreads();
-                pk = xk[i ^ stride];
-                pv = xv[i ^ stride];
-                p2 = x2[i ^ stride];
-                __syncthreads();
-            } else {
-                const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)key, (int)stride, 64);
-                const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(key >> 32), (int)stride, 64);
-                pk = ((uint64_t)hi << 32) | lo;
-                pv = (uint32_t)__shfl_xor((int)val, (int)stride, 64);
-                p2 = (uint32_t)__shfl_xor((int)k2, (int)stride, 64);
-            }
-            // the lower index of a pair keeps the smaller in an ascending run
-            const bool take_min = ((i & stride) == 0) == ((i & size) == 0);
-            if (take_min ? pk < key : pk > key) {
-                key = pk;
-                val = pv;
-            }
-            k2 = take_min ? min(k2, p2) : max(k2, p2);
-        }
-}
+constexpr uint32_t PATCH_WSLOTS = 63;  // sweep positions per wave: lane 0 works out the one before
+constexpr uint32_t PATCH_SLOTS_MAX = (PATCH_GRID - 2u) * (PATCH_T / 64) * PATCH_WSLOTS;
 
 __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
     __shared__ uint64_t sX[FEW];                  // by rank: (key, id)
-    __shared__ uint32_t pay[FEW];                 // by rank: append index of the record
     __shared__ uint32_t sSrc[FEW], sOut[FEW];     // by rank: storage index, new position
+    __shared__ uint32_t sScene[FEW];              // by rank: scene
     __shared__ uint32_t sAsc[FEW];                // storage indices ascending
-    __shared__ uint32_t rS[FEW], rL[FEW];         // by append index: storage index, insertion point
-    __shared__ uint32_t rOsz[FEW], rOsc[FEW], rOkey[FEW];  // by append index: old cell size, scene, key
-    __shared__ uint64_t wk[FEW];                  // windows (first << 32 | last), then in order
-    __shared__ uint32_t wA[FEW], wB[FEW];         // scratch
+    __shared__ uint64_t kb1[FEW], cb1[FEW];       // block_rank buffers
+    __shared__ uint32_t kb2[FEW], cb2[FEW];
+    __shared__ uint32_t R1[FEW], R2[FEW];         // block_rank counts
+    __shared__ uint64_t wk[FEW];                  // windows (first << 32 | last) in order
     __shared__ uint32_t sWin[2 * FEW];            // merged windows: (first, last) pairs, ordered
     __shared__ uint32_t sPrev[2 * FEW];           // the previous sort's windows
     __shared__ uint32_t sWpre[FEW + 1], sPpre[FEW + 1];  // slots before each window
     __shared__ uint32_t wtmp[2 * (PATCH_T / 64)];
-    __shared__ uint32_t sNW, sTot[2];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     const uint32_t n = A.n;
     uint32_t* const win_now = A.win + A.par * WIN_WORDS;
     const uint32_t* const win_prev = A.win + (1 - A.par) * WIN_WORDS;
     const uint32_t M = min(A.counters[A.par], n);
     const uint32_t npv = min(win_prev[0], (uint32_t)FEW);
-    // 0: full merge, 1: wave path, 2: block path
-    uint32_t mode = !A.patch_on || M > (uint32_t)FEW ? 0u : M <= (uint32_t)FEW_WAVE ? 1u : 2u;
+    // this sort's dirty marks carry a new tag (the previous sort's plus one,
+    // 1 .. 65535): the next k_step takes the cells holding it as marked, so
+    // older marks need no clearing
+    const uint32_t tag = A.counters[6 + (1 - A.par)] % 65535u + 1u;
+    bool few = A.patch_on && M <= (uint32_t)FEW;
     if (t < npv) {
         sPrev[2 * t] = win_prev[2 + 2 * t];
         sPrev[2 * t + 1] = win_prev[3 + 2 * t];
     }
-    if (mode == 2) {
-        // ---- records (one memory round trip); keys and storage indices
-        // padded to a power of two for the sorting network
-        uint32_t P = 64;
-        while (P < M) P <<= 1;
-        const bool rec = t < M;
-        uint64_t key = rec ? A.mkey[t] : ~0ull;
-        uint32_t val = t;
-        uint32_t src0 = rec ? A.ms[t] : 0xFFFFFFFFu;
+    // this thread's mover (append index t), one memory round trip
+    const bool rec = few && t < M;
+    const uint64_t mk = rec ? A.mkey[t] : 0ull;
+    const uint32_t src = rec ? A.ms[t] : 0u;
+    const uint32_t ins = rec ? A.mplo[t] : 0u;
+    const uint4 mo = rec ? A.mold[t] : make_uint4(0u, 0u, 0u, 0u);
+    uint32_t nw = 0;
+    if (few) {
+        const uint2 rk = block_rank<true>(mk, src, M, kb1, cb1, kb2, cb2, R1, R2);
         if (rec) {
-            rS[t] = src0;
-            rL[t] = A.mplo[t];
-            const uint4 mo = A.mold[t];
-            rOsz[t] = mo.x;
-            rOsc[t] = mo.y;
-            rOkey[t] = mo.z;
-        }
-        // by (key, id) with the append index as payload; storage indices ascending
-        block_bitonic(key, val, src0, P, wk, wA, wB);
-        if (t < P) {
-            sX[t] = key;
-            pay[t] = val;
-            sAsc[t] = src0;
+            sX[rk.x] = mk;
+            sSrc[rk.x] = src;
+            sScene[rk.x] = mo.y;
+            sAsc[rk.y] = src;
         }
         __syncthreads();
-        // new position and window of the mover of rank t: previous entries
-        // below it, less the movers among them, plus the movers before it
-        uint64_t wkey = ~0ull;
-        if (t < M) {
-            const uint32_t app = pay[t];
-            const uint32_t src = rS[app], lo = rL[app];
-            const uint32_t below = lds_count_lt(sAsc, M, 1, lo);
-            sSrc[t] = src;
-            sOut[t] = t + lo - below;
-            const uint32_t a = src < lo ? src : lo, b = src < lo ? lo - 1 : src;
+        uint64_t wkey = 0;
+        if (rec) {
+            const uint32_t below = lds_count_lt(sAsc, M, 1, ins);
+            sOut[rk.x] = rk.x + ins - below;
+            const uint32_t a = src < ins ? src : ins, b = src < ins ? ins - 1 : src;
             wkey = ((uint64_t)a << 32) | b;
         }
-        uint32_t u0 = 0, u1 = 0;
-        block_bitonic(wkey, u0, u1, P, wk, wA, wB);  // windows by (first, last)
+        const uint32_t rw = block_rank<false>(wkey, 0u, M, kb1, cb1, kb2, cb2, R1, R2).x;
+        if (rec) wk[rw] = wkey;
+        __syncthreads();
         // merged where they overlap or touch: a running maximum of the last
         // positions decides where a merged window starts and ends
         const bool valid = t < M;
-        const uint32_t a = valid ? (uint32_t)(wkey >> 32) : 0xFFFFFFFFu;
-        const uint32_t b = valid ? (uint32_t)wkey : 0u;
+        const uint64_t wv = valid ? wk[t] : 0ull;
+        const uint32_t a = valid ? (uint32_t)(wv >> 32) : 0xFFFFFFFFu;
+        const uint32_t b = valid ? (uint32_t)wv : 0u;
         const uint32_t run = block_incl_max(b, wtmp);
-        if (valid) wA[t] = run;  // (t < M <= FEW)
+        uint32_t* const wA = kb2;  // free again
+        uint32_t* const wB = cb2;
+        if (valid) wA[t] = run;
         __syncthreads();
         const bool start = valid && (t == 0 || a > wA[t - 1] + 1);
         if (valid) wB[t] = start ? 1u : 0u;
@@ -1924,107 +1990,13 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         const bool next_start = t + 1 < M && wB[t + 1] != 0;
         if (start) sWin[2 * k] = a;
         if (valid && (t + 1 == M || next_start)) sWin[2 * (k + (start ? 1u : 0u) - 1) + 1] = run;
-        if (t == 0) sNW = nstart;
-        __syncthreads();
-    }
-    if (mode == 1 && w == 0) {
-        // ---- one wave: records in registers (one memory round trip), rank
-        // sorts over the M records by lane reads, scattered to LDS
-        const bool rec = lane < M;
-        const uint64_t mk = rec ? A.mkey[lane] : ~0ull;
-        const uint32_t msl = rec ? A.ms[lane] : 0u;
-        const uint32_t mpl = rec ? A.mplo[lane] : 0u;
-        const uint4 mw = rec ? A.mold[lane] : make_uint4(0u, 0u, 0u, 0u);
-        const uint32_t rk = wave_rank(mk, M), rs = wave_rank((uint64_t)msl, M);
-        if (rec) {
-            sX[rk] = mk;
-            pay[rk] = lane;
-            sSrc[rk] = msl;
-            sOut[rk] = mpl;  // the insertion point for now
-            rOsz[lane] = mw.x;
-            rOsc[lane] = mw.y;
-            rOkey[lane] = mw.z;
-            sAsc[rs] = msl;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // new position and window of the mover of rank `lane`
-        uint64_t win = ~0ull;
-        uint32_t out = 0;
-        if (rec) {
-            const uint32_t lo = sOut[lane], src = sSrc[lane];
-            uint32_t below = 0;
-            for (uint32_t r = 0; r < M; ++r)
-                below += (uint32_t)__builtin_amdgcn_readlane((int)msl, (int)r) < lo ? 1u : 0u;
-            out = lane + lo - below;
-            const uint32_t a = src < lo ? src : lo, b = src < lo ? lo - 1 : src;
-            win = ((uint64_t)a << 32) | b;
-        }
-        // windows by first position, merged where they overlap or touch
-        const uint32_t rw = wave_rank(win, M);
-        __builtin_amdgcn_wave_barrier();
-        if (rec) {
-            sOut[lane] = out;
-            wA[rw] = (uint32_t)(win >> 32);
-            wB[rw] = (uint32_t)win;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint32_t a = rec ? wA[lane] : 0xFFFFFFFFu, b = rec ? wB[lane] : 0xFFFFFFFFu;
-        uint32_t run = b;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t u = (uint32_t)__shfl_up((int)run, o, 64);
-            if (lane >= (uint32_t)o) run = max(run, u);
-        }
-        const uint32_t prev_run = (uint32_t)__shfl_up((int)run, 1, 64);
-        const bool start = rec && (lane == 0 || a > prev_run + 1);
-        const uint64_t starts = __ballot(start);
-        const uint32_t k = (uint32_t)__popcll(starts & (lanemask_lt64() | (1ull << lane))) - 1;
-        const bool next_start = lane + 1 < 64 && ((starts >> (lane + 1)) & 1ull);
-        if (start) sWin[2 * k] = a;
-        if (rec && (lane + 1 == M || next_start)) sWin[2 * k + 1] = run;
-        const uint32_t nwv = (uint32_t)__popcll(starts);
-        if (lane == 0) sNW = nwv;
-        if (npv <= 64) {
-            // slot prefixes of the new windows (this wave)
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            uint32_t wl = lane < nwv ? sWin[2 * lane + 1] - sWin[2 * lane] + 2 : 0u;
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t u = (uint32_t)__shfl_up((int)wl, o, 64);
-                if (lane >= (uint32_t)o) wl += u;
-            }
-            sWpre[lane + 1] = wl;
-            if (lane == 0) sWpre[0] = 0;
-            if (lane == 63) sTot[0] = wl;
-        }
-    } else if (mode == 0 && t == 0) {
-        sNW = 0;
+        nw = nstart;
     }
     __syncthreads();
-    const uint32_t nw = sNW;
     // ---- slot prefixes of the new windows (one past each too) and of the
-    // previous ones: the sweep finds a slot's window by search (the one wave
-    // did the new windows' already when the previous ones were few)
+    // previous ones: the sweep finds a slot's window by search
     uint32_t total_new = 0, total_prev = 0;
-    if (mode == 1 && npv <= 64) {
-        if (w == 1) {
-            uint32_t pl = lane < npv ? sPrev[2 * lane + 1] - sPrev[2 * lane] + 1 : 0u;
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t u = (uint32_t)__shfl_up((int)pl, o, 64);
-                if (lane >= (uint32_t)o) pl += u;
-            }
-            sPpre[lane + 1] = pl;
-            if (lane == 0) sPpre[0] = 0;
-            if (lane == 63) sTot[1] = pl;
-        }
-        __syncthreads();
-        total_new = sTot[0];
-        total_prev = sTot[1];
-    } else {
+    {
         uint2 tot;
         const uint2 e = block_excl_sum2(
             make_uint2(t < nw ? sWin[2 * t + 1] - sWin[2 * t] + 2 : 0u,
@@ -2039,17 +2011,163 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         total_prev = tot.y;
         __syncthreads();
     }
-    if (mode != 0 && total_new > PATCH_SLOTS_MAX) mode = 0;  // the same decision in every block
-    const bool few = mode != 0;
+    if (few && total_new > PATCH_SLOTS_MAX) few = false;  // the same decision in every block
     if (blockIdx.x == 0 && t == 0) {
         A.counters[1 - A.par] = 0;  // the next step's movers go there
         A.counters[2] = M;
         A.counters[3] = few ? 1u : 0u;
         A.counters[4 + (1 - A.par)] = 0;
+        A.counters[6 + A.par] = tag;
         if (!few) {  // the full merge rewrites every position
             win_now[0] = 1;
             win_now[2] = 0;
             win_now[3] = n - 1;
+        }
+    }
+    if (!few) {
+        // the full merge follows: the movers are sorted for it — by counting
+        // ranks spread over the blocks (64 movers per block at a time), or
+        // beyond RANK_MAX by one block (large admissions)
+        if (M <= (uint32_t)RANK_MAX) {
+            // the last block scans the per-block mover counts for k_merge
+            if (blockIdx.x == gridDim.x - 1) scan_blockcnt<PATCH_T>(SA);
+            rank_movers(SA, M);
+        } else if (blockIdx.x == 0) {
+            sort_movers_block<PATCH_T>(SA, M);
+        }
+    } else if (blockIdx.x == 1) {
+        // ---- a cell every particle left (all its positions carry a mover
+        // mark) and none entered becomes empty; then the movers' marks of
+        // record_mover are cleared for the next step and the windows of this
+        // sort recorded
+        if (rec) {
+            const uint32_t ko = mo.z, size = mo.x, first = mo.w;
+            if (ko < A.ncell_cap && size > 0) {
+                bool all = true;
+                for (uint32_t p = first; all && p < first + size;) {
+                    const uint32_t b0 = p & 63, nb = min(64u - b0, first + size - p);
+                    const uint64_t mask = (nb == 64 ? ~0ull : ((1ull << nb) - 1ull)) << b0;
+                    all = (A.moverbits[p >> 6] & mask) == mask;
+                    p += nb;
+                }
+                if (all && lds_count_lt64(sX, M, (uint64_t)(ko + 1) << 32) ==
+                               lds_count_lt64(sX, M, (uint64_t)ko << 32))
+                    A.crange[ko] = make_uint2(EMPTY, 0);
+            }
+        }
+        __syncthreads();
+        if (rec) {
+            A.moverbits[src >> 6] = 0;
+            A.blockcnt[src / ROWS_T] = 0;
+        }
+        if (t < nw) {
+            win_now[2 + 2 * t] = sWin[2 * t];
+            win_now[3 + 2 * t] = sWin[2 * t + 1];
+        }
+        if (t == 0) win_now[0] = nw;
+    } else if (blockIdx.x >= 2) {
+        // ---- dirty cells (blocks from the top): the 27 cells around each
+        // mover's new cell get this sort's tag (the movers' records are at
+        // hand on the patch path only; otherwise every list is rebuilt)
+        if (A.lists)
+            for (uint32_t q = (gridDim.x - 1 - blockIdx.x) * PATCH_T + t; q < 27 * M;
+                 q += (gridDim.x - 2) * PATCH_T) {
+                const uint32_t e = q / 27;
+                mark_neighbourhood(A.scenes[sScene[e]], (uint32_t)(sX[e] >> 32), q % 27, A.dirty,
+                                   (uint16_t)tag);
+            }
+        // ---- the sweep (blocks from the bottom), PATCH_WSLOTS positions per
+        // wave: lane 0 works out the element before the wave's first position
+        // and every lane passes its key to the next, which needs it for the
+        // cell boundary.  The keys around a position are loaded before its
+        // element is worked out (a stayer in a window comes from at most one
+        // position away per overlapping mover, usually one).
+        auto elem = [&](uint32_t j, const uint4 q, uint32_t& key) -> uint32_t {
+            const uint32_t c1 = lds_count_lt(sOut, M, 1, j);  // movers placed before j
+            if (c1 < M && sOut[c1] == j) {
+                key = (uint32_t)(sX[c1] >> 32);
+                return sSrc[c1];
+            }
+            // the stayer of rank q0 = j - c1: q0 plus the movers before it, i.e.
+            // those whose storage index exceeds the stayers before it by at most
+            // q0 (sAsc[i] - i, the stayers before mover i, does not decrease)
+            const uint32_t q0 = j - c1;
+            uint32_t lo = 0, hi = M;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sAsc[mid] - mid <= q0)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            const uint32_t s = min(q0 + lo, n - 1);  // defensive: the windows hold their own stayers
+            if (s + 2 >= j && s <= j + 1) {
+                const uint32_t d = s + 2 - j;
+                key = d == 0 ? q.x : d == 1 ? q.y : d == 2 ? q.z : q.w;
+            } else {
+                key = A.key_old[s];
+            }
+            return s;
+        };
+        // the previous order's keys at j + d (outside every window a position
+        // keeps its key)
+        auto kat = [&](uint32_t j, int d) -> uint32_t {
+            const int64_t y = (int64_t)j + d;
+            return y >= 0 && y < (int64_t)n ? A.key_old[(uint32_t)y] : EMPTY;
+        };
+        const uint32_t total = total_new + total_prev;
+        constexpr uint32_t BS = (PATCH_T / 64) * PATCH_WSLOTS;
+        for (uint32_t g0 = (blockIdx.x - 2) * BS; g0 < total; g0 += (gridDim.x - 2) * BS) {
+            const uint32_t gw = g0 + w * PATCH_WSLOTS;  // the wave's first slot
+            const uint32_t g = gw + lane - 1;           // lane 0: the slot before (not written)
+            const bool real = lane != 0 && g < total;
+            const bool seen = (lane != 0 || gw != 0) && g < total_new;
+            uint32_t j = 0, a = 0, kc = EMPTY;
+            uint4 q = make_uint4(EMPTY, EMPTY, EMPTY, EMPTY);
+            bool inr = false;
+            if (seen) {
+                const uint32_t c = lds_count_le(sWpre, nw, 1, g) - 1;  // the window holding slot g
+                a = sWin[2 * c];
+                const uint32_t b = sWin[2 * c + 1];
+                j = a + (g - sWpre[c]);
+                inr = j <= n;
+                if (inr) {
+                    q = make_uint4(kat(j, -2), kat(j, -1), kat(j, 0), kat(j, 1));
+                    if (j <= b) {
+                        const uint32_t e = elem(j, q, kc);
+                        if (real) {
+                            A.perm[j] = e;
+                            A.invperm[e] = j;
+                            A.skey[j] = kc;
+                        }
+                    } else if (j < n) {
+                        kc = q.z;  // one past the window: unchanged
+                    }
+                }
+            }
+            // the key before j: the lane before worked out slot g - 1, which is
+            // position j - 1 of the same window when j - 1 >= a
+            const uint32_t kn = (uint32_t)__shfl_up((int)kc, 1, 64);
+            if (!real) continue;
+            if (g < total_new) {
+                if (!inr) continue;
+                const uint32_t kp = j == 0 ? EMPTY : (j - 1 >= a ? kn : q.y);
+                if (j == 0 || j == n || kp != kc) {
+                    if (j < n && kc < A.ncell_cap) A.crange[kc].x = j;
+                    if (j > 0 && kp < A.ncell_cap) A.crange[kp].y = j;
+                }
+            } else {
+                const uint32_t gp = g - total_new;
+                const uint32_t c = lds_count_le(sPpre, npv, 1, gp) - 1;
+                const uint32_t jj = sPrev[2 * c] + (gp - sPpre[c]);
+                // inside a new window? (merged windows are disjoint and ordered)
+                const uint32_t cn = lds_count_le(sWin, nw, 2, jj);
+                const bool in_new = cn > 0 && jj <= sWin[2 * (cn - 1) + 1];
+                if (jj < n && !in_new) {
+                    A.perm[jj] = jj;
+                    A.invperm[jj] = jj;
+                }
+            }
         }
     }
     // ---- per scene (blocks in turn): the displacement bound since its lists
@@ -2084,152 +2202,6 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
             }
             __syncthreads();
         }
-    if (blockIdx.x == 2 && A.lists) {
-        // ---- dirty cells (one block): clear the previous sort's marks, then
-        // mark the 27 cells around each mover's new cell (the movers' records
-        // are at hand on the patch path only; otherwise every list is rebuilt)
-        uint32_t* const dp = A.dprev + (1 - A.par) * DPREV_WORDS;
-        uint32_t* const dn = A.dprev + A.par * DPREV_WORDS;
-        const uint32_t npd = min(dp[0], (uint32_t)FEW);
-        for (uint32_t q = t; q < 27 * npd; q += PATCH_T) {
-            const uint32_t e = q / 27;
-            mark_neighbourhood(A.scenes[dp[2 + 2 * e]], dp[1 + 2 * e], q % 27, A.dirty, false);
-        }
-        const uint32_t nm = few ? M : 0u;
-        __syncthreads();
-        for (uint32_t q = t; q < 27 * nm; q += PATCH_T) {
-            const uint32_t e = q / 27;
-            mark_neighbourhood(A.scenes[rOsc[pay[e]]], (uint32_t)(sX[e] >> 32), q % 27, A.dirty, true);
-        }
-        if (t < nm) {
-            dn[1 + 2 * t] = (uint32_t)(sX[t] >> 32);
-            dn[2 + 2 * t] = rOsc[pay[t]];
-        }
-        if (t == 0) dn[0] = nm;
-    }
-    if (!few) {
-        // the full merge follows: the movers are sorted for it — by counting
-        // ranks spread over the blocks (64 movers per block at a time), or
-        // beyond RANK_MAX by one block (large admissions)
-        if (M <= (uint32_t)RANK_MAX) {
-            // the last block scans the per-block mover counts for k_merge
-            if (blockIdx.x == gridDim.x - 1) scan_blockcnt<PATCH_T>(SA);
-            rank_movers(SA, M);
-        } else if (blockIdx.x == 0) {
-            sort_movers_block<PATCH_T>(SA, M);
-        }
-        return;
-    }
-    if (blockIdx.x == 1) {
-        // ---- a cell every particle left (and none entered) becomes empty;
-        // the movers' marks of record_mover are cleared for the next step; the
-        // windows of this sort are recorded
-        if (t < M) {
-            const uint32_t src = sSrc[t], app = pay[t], ko = rOkey[app];
-            if (ko < A.ncell_cap) {
-                uint32_t leave = 0;
-                for (uint32_t r = 0; r < M; ++r) leave += rOkey[r] == ko ? 1u : 0u;
-                const uint32_t enter = lds_count_lt64(sX, M, (uint64_t)(ko + 1) << 32) -
-                                       lds_count_lt64(sX, M, (uint64_t)ko << 32);
-                if (rOsz[app] + enter == leave) A.crange[ko] = make_uint2(EMPTY, 0);
-            }
-            A.moverbits[src >> 6] = 0;
-            A.blockcnt[src / ROWS_T] = 0;
-        }
-        if (t < nw) {
-            win_now[2 + 2 * t] = sWin[2 * t];
-            win_now[3 + 2 * t] = sWin[2 * t + 1];
-        }
-        if (t == 0) win_now[0] = nw;
-        return;
-    }
-    if (blockIdx.x < 3) return;
-    // ---- the sweep (blocks 3..).  Up to FEW_WAVE movers their storage
-    // indices are held in registers (lane r: mover r) and read by uniform
-    // lane index; more, by binary search.  The keys around a position are
-    // loaded before its element is worked out (a stayer in a window comes
-    // from at most one position away per overlapping mover, usually one).
-    const uint32_t vAsc = lane < M ? sAsc[lane] : 0xFFFFFFFFu;
-    auto elem = [&](uint32_t j, uint32_t j0, const uint4 q, uint32_t& key) -> uint32_t {
-        const uint32_t c1 = lds_count_lt(sOut, M, 1, j);  // movers placed before j
-        if (c1 < M && sOut[c1] == j) {
-            key = (uint32_t)(sX[c1] >> 32);
-            return sSrc[c1];
-        }
-        // the stayer of rank q = j - c1: q plus the movers before it, i.e.
-        // those whose storage index exceeds the stayers before it by at most q
-        // (sAsc[i] - i, the stayers before mover i, does not decrease)
-        const uint32_t q0 = j - c1;
-        uint32_t s;
-        if (mode == 1) {
-            s = q0;
-            for (uint32_t r = 0; r < M; ++r)
-                s += (uint32_t)__builtin_amdgcn_readlane((int)vAsc, (int)r) <= s ? 1u : 0u;
-        } else {
-            uint32_t lo = 0, hi = M;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (sAsc[mid] - mid <= q0)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            s = q0 + lo;
-        }
-        s = min(s, n - 1);  // defensive: the windows hold their own stayers
-        if (s + 2 >= j0 && s <= j0 + 1) {
-            const uint32_t d = s + 2 - j0;
-            key = d == 0 ? q.x : d == 1 ? q.y : d == 2 ? q.z : q.w;
-        } else {
-            key = A.key_old[s];
-        }
-        return s;
-    };
-    const uint32_t total = total_new + total_prev;
-    for (uint32_t g = (blockIdx.x - 3) * PATCH_T + t; g < total; g += (gridDim.x - 3) * PATCH_T) {
-        if (g < total_new) {
-            const uint32_t c = lds_count_le(sWpre, nw, 1, g) - 1;  // the window holding slot g
-            const uint32_t a = sWin[2 * c], b = sWin[2 * c + 1], j = a + (g - sWpre[c]);
-            if (j > n) continue;
-            // the previous order's keys at j - 2 .. j + 1 (outside every window
-            // a position keeps its key)
-            auto kat = [&](int d) -> uint32_t {
-                const int64_t y = (int64_t)j + d;
-                return y >= 0 && y < (int64_t)n ? A.key_old[(uint32_t)y] : EMPTY;
-            };
-            const uint4 q = make_uint4(kat(-2), kat(-1), kat(0), kat(1));
-            uint32_t kc = EMPTY, kp = EMPTY;
-            if (j <= b) {
-                const uint32_t e = elem(j, j, q, kc);
-                A.perm[j] = e;
-                A.invperm[e] = j;
-                A.skey[j] = kc;
-            } else if (j < n) {
-                kc = q.z;  // outside every window: unchanged
-            }
-            if (j > 0) {
-                if (j - 1 >= a)
-                    (void)elem(j - 1, j, q, kp);
-                else
-                    kp = q.y;
-            }
-            if (j == 0 || j == n || kp != kc) {
-                if (j < n && kc < A.ncell_cap) A.crange[kc].x = j;
-                if (j > 0 && kp < A.ncell_cap) A.crange[kp].y = j;
-            }
-        } else {
-            const uint32_t gp = g - total_new;
-            const uint32_t c = lds_count_le(sPpre, npv, 1, gp) - 1;
-            const uint32_t j = sPrev[2 * c] + (gp - sPpre[c]);
-            // inside a new window? (merged windows are disjoint and ordered)
-            const uint32_t cn = lds_count_le(sWin, nw, 2, j);
-            const bool in_new = cn > 0 && j <= sWin[2 * (cn - 1) + 1];
-            if (j < n && !in_new) {
-                A.perm[j] = j;
-                A.invperm[j] = j;
-            }
-        }
-    }
 }
 
 // Reset of the sort and list state after a layout change or a regather:
@@ -2245,7 +2217,6 @@ __global__ __launch_bounds__(256) void k_sort_reset(PatchArgs A) {
         A.win[t * WIN_WORDS] = A.n ? 1u : 0u;
         A.win[t * WIN_WORDS + 2] = 0;
         A.win[t * WIN_WORDS + 3] = A.n ? A.n - 1 : 0u;
-        if (A.lists) A.dprev[t * DPREV_WORDS] = 0;
     }
     if (A.lists)
         for (uint32_t sc = t; sc < A.nscenes; sc += 256) {

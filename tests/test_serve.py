@@ -106,9 +106,9 @@ def test_footprint_is_the_library_allocation_arithmetic():
     per_p = (b2 - b1) / 100_000
     per_c = (b3 - b1) / (1 << 20)
     assert 200 < per_p < 400 and per_c == 8
-    # candidate lists: 32 entries and a count per particle, a byte per cell
+    # candidate lists: 32 entries and a count per particle, a 16-bit tag per cell
     bl = serve.device_bytes(100_000, 1, 1 << 20)
-    assert bl - b1 == 100_000 * 33 * 4 + (1 << 20)
+    assert bl - b1 == 100_000 * 33 * 4 + 2 * (1 << 20)
     assert serve.device_bytes(100_000, 1, 1 << 20, history=True) - b1 >= 100_000 * 2 * 16 * 16
     clock = serve.FakeClock()
     be = FakeBackend(clock)

@@ -4,8 +4,12 @@ indices ascending, windows [src, lo - 1] / [lo, src] merged where they overlap
 or touch, every window position's element found by binary search (the mover
 placed there, or the stayer of rank q at q + #{i : sAsc[i] - i <= q}), cell
 boundaries rewritten at window positions, emptied cells cleared and the
-previous windows reset to the identity.  Random cases: the permutation, the
-keys at every position and the cell ranges must equal the full re-sort's.
+previous windows reset to the identity.  Ranks are worked out as the kernel's
+block_rank does (per chunk by counting, across chunks by binary search), a
+position's cell boundary takes the key before it from the slot before, and a
+cell is emptied when every position of it holds a mover and none enters.
+Random cases: the permutation, the keys at every position and the cell ranges
+must equal the full re-sort's.
 This pins the algorithm the GPU kernel implements; the GPU tests compare the
 kernel with the full merge and with the oracle's sort.
 """
@@ -36,7 +40,25 @@ def count_le(arr, x):
     return bisect.bisect_right(arr, x)
 
 
-def patch(keys, ids, newkeys, movers, cr_old, prev_windows, perm_in):
+def block_rank(keys, ch=64):
+    """Rank of each key among all (ties by index), as the kernel's block_rank:
+    counting inside the key's chunk of ch, binary search over the other
+    chunks in key order (equal keys of a lower chunk precede)."""
+    m = len(keys)
+    chunks = [sorted(keys[c:c + ch]) for c in range(0, m, ch)]
+    out = []
+    for t in range(m):
+        w, k = t // ch, keys[t]
+        r = sum(1 for i in range(w * ch, min(m, w * ch + ch))
+                if keys[i] < k or (keys[i] == k and i < t))
+        for c, srt in enumerate(chunks):
+            if c != w:
+                r += bisect.bisect_right(srt, k) if c < w else bisect.bisect_left(srt, k)
+        out.append(r)
+    return out
+
+
+def patch(keys, ids, newkeys, movers, cr_old, prev_windows, perm_in, ch=64):
     """keys/ids: old sorted order (storage); movers: storage indices (append
     order); returns perm, skey, crange, windows — mirroring the kernel."""
     n = len(keys)
@@ -46,24 +68,24 @@ def patch(keys, ids, newkeys, movers, cr_old, prev_windows, perm_in):
     oldX = [(int(keys[q]) << 32) | int(ids[q]) for q in range(n)]
     # insertion point recorded with the mover: previous entries below X
     rL = [bisect.bisect_left(oldX, x) for x in rX]
-    # P1: ranks by counting
-    sX = [0] * M; sSrc = [0] * M; sOut = [0] * M; sAsc = [0] * M; sOldKey = [0] * M
+    # P1: ranks by (key, id) and by storage index (block_rank)
+    sX = [0] * M; sSrc = [0] * M; sOut = [0] * M; sAsc = [0] * M
     wA = [0] * M; wB = [0] * M
+    rks, rss = block_rank(rX, ch), block_rank(rS, ch)
     for t in range(M):
-        rk = sum(1 for r in range(M) if rX[r] < rX[t])
-        rs = sum(1 for r in range(M) if rS[r] < rS[t])
-        below = sum(1 for r in range(M) if rS[r] < rL[t])
-        out = rk + rL[t] - below
-        a = rS[t] if rS[t] < rL[t] else rL[t]
-        b = rL[t] - 1 if rS[t] < rL[t] else rS[t]
-        sX[rk] = rX[t]; sSrc[rk] = rS[t]; sOut[rk] = out; sOldKey[rk] = int(keys[rS[t]])
-        sAsc[rs] = rS[t]
-        wA[t] = a; wB[t] = b
-    # P2: windows ranked by (a, b, t)
+        sX[rks[t]] = rX[t]; sSrc[rks[t]] = rS[t]
+        sAsc[rss[t]] = rS[t]
+    for t in range(M):
+        rk = rks[t]
+        below = count_lt(sAsc, rL[t])
+        sOut[rk] = rk + rL[t] - below
+        wA[t] = rS[t] if rS[t] < rL[t] else rL[t]
+        wB[t] = rL[t] - 1 if rS[t] < rL[t] else rS[t]
+    # P2: windows ranked by (a, b), ties by append index (block_rank)
     sWa = [0] * M; sWb = [0] * M
+    rws = block_rank([(wA[t] << 32) | wB[t] for t in range(M)], ch)
     for t in range(M):
-        rw = sum(1 for r in range(M) if (wA[r], wB[r], r) < (wA[t], wB[t], t))
-        sWa[rw] = wA[t]; sWb[rw] = wB[t]
+        sWa[rws[t]] = wA[t]; sWb[rws[t]] = wB[t]
     # P3: merge by running max; compaction
     run = []
     m = -1
@@ -112,6 +134,17 @@ def patch(keys, ids, newkeys, movers, cr_old, prev_windows, perm_in):
     perm = perm_in.copy()
     skey = newkeys.copy()
     cr = cr_old.copy()
+    # the key each new-window slot works out (the kernel's lanes pass it on
+    # to the lane of the next slot)
+    kcs = [None] * total_new
+    for g in range(total_new):
+        k = bisect.bisect_right(wpre, g) - 1
+        a, b = win[k]
+        j = a + (g - wpre[k])
+        if j <= b:
+            kcs[g] = elem(j)[1]
+        elif j < n:
+            kcs[g] = int(keys[j])
     # writes collected per position, then applied (the kernel's threads write
     # disjoint positions; boundaries are written once each)
     for g in range(total):
@@ -121,15 +154,13 @@ def patch(keys, ids, newkeys, movers, cr_old, prev_windows, perm_in):
             j = a + (g - wpre[k])
             if j > n:
                 continue
-            kc = kp = None
+            kp = None
+            kc = kcs[g]
             if j <= b:
-                e, kc = elem(j)
-                perm[j] = e
+                perm[j] = elem(j)[0]
                 skey[j] = kc
-            elif j < n:
-                kc = int(keys[j])
             if j > 0:
-                kp = elem(j - 1)[1] if j - 1 >= a else int(keys[j - 1])
+                kp = kcs[g - 1] if j - 1 >= a else int(keys[j - 1])
             if j == 0 or j == n or kp != kc:
                 if j < n:
                     cr[kc, 0] = j
@@ -141,19 +172,20 @@ def patch(keys, ids, newkeys, movers, cr_old, prev_windows, perm_in):
             j = prev_windows[k][0] + (gg - ppre[k])
             if j < n and not in_new(j):
                 perm[j] = j
-    # emptied cells
+    # emptied cells: every position of the old cell holds a mover, none enters
+    mark = set(rS)
+    klist = [int(x >> 32) for x in sX]
     for t in range(M):
-        ko = sOldKey[t]
-        size = cr_old[ko, 1] - cr_old[ko, 0] if cr_old[ko, 0] != EMPTY else 0
-        leave = sum(1 for r in range(M) if sOldKey[r] == ko)
-        klist = [int(x >> 32) for x in sX]
-        enter = count_le(klist, ko) - count_lt(klist, ko)
-        if size + enter == leave:
+        ko = int(keys[rS[t]])
+        first = int(cr_old[ko, 0])
+        size = int(cr_old[ko, 1]) - first
+        if all(q in mark for q in range(first, first + size)) and \
+                count_le(klist, ko) == count_lt(klist, ko):
             cr[ko] = [EMPTY, 0]
     return perm, skey, cr, [tuple(w) for w in win]
 
 
-def trial(rng, n, ncell, M, prev):
+def trial(rng, n, ncell, M, prev, ch=64):
     ids = rng.permutation(n)
     keys = np.sort(rng.integers(0, ncell, n))
     o = np.lexsort((ids, keys))
@@ -173,7 +205,7 @@ def trial(rng, n, ncell, M, prev):
     perm_in = np.arange(n)
     for a, b in prev:
         perm_in[a:b + 1] = rng.permutation(np.arange(a, b + 1))
-    perm, skey, cr, win = patch(keys, ids, newkeys, movers, cr_old, prev, perm_in)
+    perm, skey, cr, win = patch(keys, ids, newkeys, movers, cr_old, prev, perm_in, ch)
     assert np.array_equal(perm, perm_ref), (n, M)
     # keys at every position: inside windows as written, outside unchanged
     want_keys = newkeys[perm_ref]
@@ -186,6 +218,18 @@ def trial(rng, n, ncell, M, prev):
     return win
 
 
+
+
+def test_block_rank_is_the_rank_with_ties_by_index():
+    rng = np.random.default_rng(3)
+    for it in range(300):
+        m = int(rng.integers(0, 200))
+        keys = [int(x) for x in rng.integers(0, int(rng.integers(1, 40)), m)]
+        want = sorted(range(m), key=lambda t: (keys[t], t))
+        for ch in (1, 5, 64):
+            got = block_rank(keys, ch)
+            assert sorted(got) == list(range(m))
+            assert [t for _, t in sorted(zip(got, range(m)))] == want
 
 
 def test_window_patch_matches_full_resort():
@@ -201,4 +245,5 @@ def test_window_patch_matches_full_resort():
             b = int(rng.integers(a, min(n, a + 10)))
             prev.append((a, b))
             p = b + 2
-        trial(rng, n, ncell, M, prev)
+        # chunks of 64 as on the GPU, and of 7 (many chunks, ties across them)
+        trial(rng, n, ncell, M, prev, 64 if it % 2 else 7)
