@@ -1953,7 +1953,7 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
     const uint32_t ins = rec ? A.mplo[t] : 0u;
     const uint4 mo = rec ? A.mold[t] : make_uint4(0u, 0u, 0u, 0u);
     uint32_t nw = 0;
-    if (few) {
+    if (few && M > 0) {  // (no movers: no new windows)
         const uint2 rk = block_rank<true>(mk, src, M, kb1, cb1, kb2, cb2, R1, R2);
         if (rec) {
             sX[rk.x] = mk;
@@ -1996,7 +1996,7 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
     // ---- slot prefixes of the new windows (one past each too) and of the
     // previous ones: the sweep finds a slot's window by search
     uint32_t total_new = 0, total_prev = 0;
-    {
+    if (nw | npv) {
         uint2 tot;
         const uint2 e = block_excl_sum2(
             make_uint2(t < nw ? sWin[2 * t + 1] - sWin[2 * t] + 2 : 0u,
