@@ -354,6 +354,7 @@ BScene make_bscene(const HostScene& h, uint32_t plane0, float skin_frac = 0.05f)
     s.woy = d.world_origin.y;
     s.woz = d.world_origin.z;
     s.cs = d.cell_size;
+    s.inv_cs = 1.0 / d.cell_size;
     s.dt = d.dt;
     s.half2dt = 0.5 * d.dt * d.dt;
     s.gx = d.grid_size.x;

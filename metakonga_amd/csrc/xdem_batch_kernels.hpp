@@ -60,6 +60,7 @@ struct BScene {
     float skin;               // candidate-list skin (distance beyond contact kept in a list)
     uint32_t drum0, ndrum;    // into the drum table (applied after the planes)
     uint32_t obj0, nobj;      // into the polygon-object and pose tables
+    double inv_cs;            // 1 / cs (rounded): the cell of a coordinate by product first
 };
 
 struct BPlane {
@@ -221,11 +222,23 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nblk) {
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// floor(fl(d / cs)), the reference's cell coordinate (calcGridPos), from the
+// product with the rounded reciprocal when that is safely away from an
+// integer: the product and the rounded quotient differ by at most 4 ulp of
+// q (|q| < 2^26 here, so by less than 2^-24), so both have the same floor
+// when the product's fraction lies in [2^-20, 1 - 2^-20]; otherwise (and for
+// non-finite or large values) the division itself.
+__device__ inline int cell_coord(double d, double cs, double inv_cs) {
+    const double q = d * inv_cs;
+    const double f = q - floor(q);
+    if (f >= 0x1p-20 && f <= 1.0 - 0x1p-20 && fabs(q) < 0x1p26) return (int)floor(q);
+    return (int)floor(d / cs);
+}
 __device__ inline void cell_of(const BScene& S, double x, double y, double z, int& cx, int& cy,
                                int& cz) {
-    cx = (int)floor((x - S.wox) / S.cs);
-    cy = (int)floor((y - S.woy) / S.cs);
-    cz = (int)floor((z - S.woz) / S.cs);
+    cx = cell_coord(x - S.wox, S.cs, S.inv_cs);
+    cy = cell_coord(y - S.woy, S.cs, S.inv_cs);
+    cz = cell_coord(z - S.woz, S.cs, S.inv_cs);
 }
 
 __device__ inline uint32_t key_of(const BScene& S, int cx, int cy, int cz) {
