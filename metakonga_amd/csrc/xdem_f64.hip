@@ -167,18 +167,49 @@ __global__ __launch_bounds__(512) void k_hash(KParams k, uint32_t* hash, uint32_
     index[sid + id] = sid + id;
 }
 
+// FP32 screening record of a particle: its position relative to the world
+// origin and its radius, x NaN when a coordinate lies farther than pf_range
+// from the origin (such a particle is never screened out: NaN distances fail
+// every rejection).  Each coordinate is off by at most 2^-24 pf_range.
+__device__ inline float4 pf_screen(const KParams& k, double x, double y, double z, double r,
+                                   double pf_range) {
+    const double rx = x - k.wox, ry = y - k.woy, rz = z - k.woz;
+    const bool in = fabs(rx) <= pf_range && fabs(ry) <= pf_range && fabs(rz) <= pf_range;
+    return make_float4(in ? (float)rx : __builtin_nanf(""), (float)ry, (float)rz, (float)r);
+}
+
+// The sorted screening copy for k_p2p: per sorted position the record above;
+// polygon spheres (ids >= np, skipped by the particle pass) far away.
+__global__ __launch_bounds__(256) void k_sorted_screen(KParams k, const double* pos,
+                                                       const uint32_t* sorted, uint32_t np,
+                                                       uint32_t nsorted, double pf_range,
+                                                       float4* pfs) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nsorted) return;
+    const uint32_t id = sorted[j];
+    float4 f = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+    if (id < np) {
+        const d4 p = ld4(pos, id);
+        f = pf_screen(k, p.x, p.y, p.z, p.w, pf_range);
+    }
+    pfs[j] = f;
+}
+
 // K8 calculate_p2p_kernel<TCM> — mphysics_cuda_impl.cuh:403-485.  SORTED: thread t
 // evaluates the particle at sorted position t (of nsorted entries, triangle
 // spheres included and skipped) instead of particle t, so a wave's particles
-// are neighbours in space and share their cells; each particle's arithmetic
-// and neighbour order are the same either way.
+// are neighbours in space and share their cells, and with pfs (the sorted
+// screening copy, error bound pf_eps = 2^-20 pf_range on distances) screens
+// each candidate before loading it; each particle's arithmetic and neighbour
+// order are the same either way.
 template <int TCM, bool SORTED>
 __global__ __launch_bounds__(256) void k_p2p(KParams k, const double* pos, const double* vel,
                                              const double* omega, double* force, double* moment,
                                              const double* mass, const uint32_t* sorted_index,
                                              const uint32_t* cstart, const uint32_t* cend,
                                              const xdem_device_contact_property* cpp, uint32_t np,
-                                             uint32_t nsorted) {
+                                             uint32_t nsorted, const float4* pfs, double pf_range,
+                                             float pf_eps) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t id = t;
     if (SORTED) {
@@ -195,6 +226,11 @@ __global__ __launch_bounds__(256) void k_p2p(KParams k, const double* pos, const
     double ir = ip4.w, im = mass[id];
     d3 Ft = mk3(0, 0, 0), Fn = mk3(0, 0, 0), M = mk3(0, 0, 0);
     d3 sumF = mk3(0, 0, 0), sumM = mk3(0, 0, 0);
+    // with the sorted screening copy: a candidate whose FP32 distance exceeds
+    // ir + jr by more than twice its error bound cannot touch (the FP64 test
+    // below would reject it), so neither its index nor its record is loaded
+    const float4 ownf = pf_screen(k, ipos.x, ipos.y, ipos.z, ir, pf_range);
+    const float sr0 = ownf.w + pf_eps;
     for (int z = -1; z <= 1; z++)
         for (int y = -1; y <= 1; y++)
             for (int x = -1; x <= 1; x++) {
@@ -203,6 +239,13 @@ __global__ __launch_bounds__(256) void k_p2p(KParams k, const double* pos, const
                 if (s == 0xFFFFFFFFu) continue;
                 uint32_t e = cend[h];
                 for (uint32_t j = s; j < e; j++) {
+                    if (SORTED && pfs) {
+                        const float4 f = pfs[j];
+                        const float dx = f.x - ownf.x, dy = f.y - ownf.y, dz = f.z - ownf.z;
+                        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        const float sr = sr0 + f.w;
+                        if (d2 >= sr * sr * (1.0f + 0x1p-19f)) continue;
+                    }
                     uint32_t kk = sorted_index[j];
                     if (id == kk || kk >= np) continue;
                     d4 jp = ld4(pos, kk);
@@ -766,22 +809,32 @@ int32_t launch_p2p(int tcm, const KParams& k, const double* pos, const double* v
                    const double* omega, double* force, double* moment, const double* mass,
                    const uint32_t* sorted, const uint32_t* cstart, const uint32_t* cend,
                    const xdem_device_contact_property* cp, uint32_t np, uint32_t nsorted,
-                   hipStream_t s) {
+                   hipStream_t s, float4* pfs) {
     if (!np) return XDEM_OK;
     const dim3 g(div_up(nsorted ? nsorted : np, 256));
+    // screening range: twice the grid's extent from the origin
+    const double pf_range = 2.0 * k.cs * std::max(k.gx, std::max(k.gy, k.gz));
+    const float pf_eps = (float)std::ldexp(pf_range, -20);
     if (nsorted) {
+        if (pfs) {
+            hipLaunchKernelGGL(k_sorted_screen, dim3(div_up(nsorted, 256)), dim3(256), 0, s, k, pos,
+                               sorted, np, nsorted, pf_range, pfs);
+            XDEM_LAUNCHED();
+        }
         if (tcm == 0)
             hipLaunchKernelGGL((k_p2p<0, true>), g, dim3(256), 0, s, k, pos, vel, omega, force,
-                               moment, mass, sorted, cstart, cend, cp, np, nsorted);
+                               moment, mass, sorted, cstart, cend, cp, np, nsorted, pfs, pf_range,
+                               pf_eps);
         else
             hipLaunchKernelGGL((k_p2p<1, true>), g, dim3(256), 0, s, k, pos, vel, omega, force,
-                               moment, mass, sorted, cstart, cend, cp, np, nsorted);
+                               moment, mass, sorted, cstart, cend, cp, np, nsorted, pfs, pf_range,
+                               pf_eps);
     } else if (tcm == 0) {
         hipLaunchKernelGGL((k_p2p<0, false>), g, dim3(256), 0, s, k, pos, vel, omega, force,
-                           moment, mass, sorted, cstart, cend, cp, np, 0u);
+                           moment, mass, sorted, cstart, cend, cp, np, 0u, nullptr, pf_range, pf_eps);
     } else {
         hipLaunchKernelGGL((k_p2p<1, false>), g, dim3(256), 0, s, k, pos, vel, omega, force,
-                           moment, mass, sorted, cstart, cend, cp, np, 0u);
+                           moment, mass, sorted, cstart, cend, cp, np, 0u, nullptr, pf_range, pf_eps);
     }
     XDEM_LAUNCHED();
     return XDEM_OK;

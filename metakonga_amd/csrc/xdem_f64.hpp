@@ -16,7 +16,7 @@ int32_t launch_p2p(int tcm, const KParams& k, const double* pos, const double* v
                    const double* omega, double* force, double* moment, const double* mass,
                    const uint32_t* sorted, const uint32_t* cstart, const uint32_t* cend,
                    const xdem_device_contact_property* cp, uint32_t np, uint32_t nsorted,
-                   hipStream_t s);
+                   hipStream_t s, float4* pfs = nullptr);
 int32_t launch_planes(int tcm, const KParams& k, const xdem_device_plane_info* planes,
                       const xdem_device_contact_property* cps, uint32_t nplane, const double* pos,
                       const double* vel, const double* omega, double* force, double* moment,

@@ -282,13 +282,15 @@ struct xdem_sim {
     }
     bool dirty = true;
     PhaseTimer timer;
+    float4* d_pfs = nullptr;  // sorted FP32 screening copy for the particle pass
+    uint32_t pfs_cap = 0;
 
     ~xdem_sim() {
         for (void* p : {(void*)pos, (void*)vel, (void*)omega, (void*)acc, (void*)alpha,
                         (void*)force, (void*)moment, (void*)mass, (void*)iner, (void*)d_cp_pp,
                         (void*)d_planes, (void*)d_plane_cp, (void*)d_cyl, (void*)d_cyl_cp,
                         (void*)d_tri, (void*)d_sph, (void*)d_obj, (void*)d_obj_cp,
-                        (void*)d_drum, (void*)d_drum_cp, (void*)d_vlist})
+                        (void*)d_drum, (void*)d_drum_cp, (void*)d_vlist, (void*)d_pfs})
             if (p) (void)hipFree(p);
     }
     KParams kparams() const { return params_from(prm); }
@@ -382,9 +384,18 @@ static int32_t sim_step_once(xdem_sim* s, hipStream_t st) {
     const uint32_t* sid = s->grid.d_sorted_id;
     const uint32_t* cs0 = s->grid.d_cell_start;
     const uint32_t* ce0 = s->grid.d_cell_end;
+    // the sorted screening copy (grown with the triangle spheres)
+    const uint32_t nsorted = np + s->ntri;
+    if (nsorted > s->pfs_cap) {
+        if (s->d_pfs) (void)hipFree(s->d_pfs);
+        s->d_pfs = nullptr;
+        s->pfs_cap = 0;
+        XDEM_HIP(hipMalloc(&s->d_pfs, sizeof(float4) * nsorted));
+        s->pfs_cap = nsorted;
+    }
     XDEM_TRY(T.begin(4, st));
     XDEM_TRY(launch_p2p(tcm, k, s->pos, s->vel, s->omega, s->force, s->moment, s->mass, sid, cs0,
-                        ce0, s->d_cp_pp, np, np + s->ntri, st));
+                        ce0, s->d_cp_pp, np, nsorted, st, s->d_pfs));
     XDEM_TRY(T.end(st));
     XDEM_TRY(T.begin(5, st));
     for (const WallOp& op : s->ops) {
