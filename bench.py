@@ -114,6 +114,7 @@ def run_batched(nscene, steps, lib, block=100, seed=0xD3E3):
     for s in scs:
         b.add(s)
     b.prepare()
+    b.capture(block)
     b.sync()
     t0 = time.perf_counter()
     done = 0
@@ -147,6 +148,7 @@ def run_settling_8k(lib, steps=2000):
     batch.apply_shape(b, tuned[0])
     b.add(sc)
     b.prepare()
+    b.capture(100)
     b.sync()
     t0 = time.perf_counter()
     for _ in range(steps // 100):
@@ -172,6 +174,7 @@ def run_c2_drum(lib, steps=200, f64_steps=20):
     batch.apply_shape(b, tuned[0])
     sid = b.add(sc)
     b.prepare()
+    b.capture(100)
     b.sync()
     t0 = time.perf_counter()
     done = 0

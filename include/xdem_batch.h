@@ -132,6 +132,12 @@ int32_t xdem_batch_list_rebuilds(xdem_batch* b, uint32_t* scenes_rebuilding);
  * rewritten, in one kernel; off: the full merge every step.  The results are
  * identical. */
 int32_t xdem_batch_set_sort_patch(xdem_batch* b, int32_t on);
+/* After xdem_batch_prepare: capture and instantiate the graph that replays an
+ * nsteps block (both buffer parities when nsteps is odd) without running it,
+ * so the first xdem_batch_step(b, nsteps) of a timed run only launches.  A
+ * change of launch shape, walk, lists or scenes drops the graphs.  No-op with
+ * graphs off. */
+int32_t xdem_batch_capture(xdem_batch* b, uint32_t nsteps);
 int32_t xdem_batch_destroy(xdem_batch* b);
 /* Add a scene (host arrays, reference layout: pos4 = x,y,z,r; vel3/omega3 may be
  * NULL for zero).  Takes effect at the next xdem_batch_step.  Fails with

@@ -1180,6 +1180,51 @@ int32_t xdem_batch_set_walk(xdem_batch* b, int32_t per_lane) {
     return XDEM_OK;
 }
 
+// The graph of an nsteps block from the current buffer parities: captured
+// and instantiated on first use (nothing runs), then kept.
+static int32_t block_graph(xdem_batch* b, uint32_t nsteps, hipGraphExec_t* out) {
+    auto key = std::make_pair(nsteps, b->cur * 2 + b->skcur);
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+        int cur0 = b->cur, sk0 = b->skcur;
+        XDEM_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        int32_t rc = enqueue_block(b, nsteps, false);
+        hipGraph_t gg = nullptr;
+        hipError_t e2 = hipStreamEndCapture(b->st, &gg);
+        b->cur = cur0;
+        b->skcur = sk0;
+        if (rc) {
+            if (gg) (void)hipGraphDestroy(gg);
+            return rc;
+        }
+        XDEM_HIP(e2);
+        hipGraphExec_t ex;
+        hipError_t e3 = hipGraphInstantiate(&ex, gg, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gg);
+        XDEM_HIP(e3);
+        it = b->graphs.emplace(key, ex).first;
+    }
+    *out = it->second;
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_capture(xdem_batch* b, uint32_t nsteps) {
+    XDEM_REQUIRE(b, "null batch");
+    XDEM_REQUIRE(b->prepared, "call xdem_batch_prepare first");
+    if (nsteps == 0 || !b->n_total || !b->use_graphs || b->timer.on) return XDEM_OK;
+    hipGraphExec_t ex = nullptr;
+    XDEM_TRY(block_graph(b, nsteps, &ex));
+    if (nsteps & 1) {  // the block after this one starts from the other parities
+        b->cur = 1 - b->cur;
+        b->skcur = 1 - b->skcur;
+        const int32_t rc = block_graph(b, nsteps, &ex);
+        b->cur = 1 - b->cur;
+        b->skcur = 1 - b->skcur;
+        XDEM_TRY(rc);
+    }
+    return XDEM_OK;
+}
+
 int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps) {
     XDEM_REQUIRE(b, "null batch");
     if (nsteps == 0) return XDEM_OK;
@@ -1202,29 +1247,9 @@ int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps) {
         if (b->nobj_total) b->pose_steps += nsteps;
         return XDEM_OK;
     }
-    auto key = std::make_pair(nsteps, b->cur * 2 + b->skcur);
-    auto it = b->graphs.find(key);
-    if (it == b->graphs.end()) {
-        hipGraph_t g;
-        int cur0 = b->cur, sk0 = b->skcur;
-        XDEM_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-        int32_t rc = enqueue_block(b, nsteps, false);
-        hipGraph_t gg = nullptr;
-        hipError_t e2 = hipStreamEndCapture(b->st, &gg);
-        b->cur = cur0;
-        b->skcur = sk0;
-        if (rc) {
-            if (gg) (void)hipGraphDestroy(gg);
-            return rc;
-        }
-        XDEM_HIP(e2);
-        g = gg;
-        hipGraphExec_t ex;
-        XDEM_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-        XDEM_HIP(hipGraphDestroy(g));
-        it = b->graphs.emplace(key, ex).first;
-    }
-    XDEM_HIP(hipGraphLaunch(it->second, b->st));
+    hipGraphExec_t ex = nullptr;
+    XDEM_TRY(block_graph(b, nsteps, &ex));
+    XDEM_HIP(hipGraphLaunch(ex, b->st));
     // the captured block flips buffers nsteps times (state) and nsteps times (keys)
     if (nsteps & 1) {
         b->cur = 1 - b->cur;

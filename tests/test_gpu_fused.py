@@ -291,6 +291,27 @@ def test_block_size_does_not_change_results(lib):
     b.close()
 
 
+def test_capture_ahead_is_the_same_run(lib):
+    """Graphs instantiated ahead (xdem_batch_capture, an odd block size too, so
+    both buffer parities) replay exactly what the blocks captured at first use do."""
+    sc, vel, om = dense_scene(n_side=9, seed=23)
+    outs = []
+    for ahead in (False, True):
+        b = Batch(sc.np, 1, int(np.prod(sc.grid_size)))
+        b.lists(True, 0.05)
+        sid = b.add(sc, vel, om)
+        b.prepare()
+        if ahead:
+            b.capture(7)
+            b.capture(10)
+        for k in (7, 7, 10, 7, 10):
+            b.step(k)
+        outs.append(b.state(sid))
+        b.close()
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
 def test_many_contacts_rescan(lib, orc):
     """A sphere touched by 40 smaller ones (more contacts than the 16-entry
     list): the further chunks are found by rescanning the cells; the pair list
