@@ -119,8 +119,9 @@ struct xdem_batch {
     uint32_t* invperm = nullptr;      // storage index -> sorted position
     uint2* crange = nullptr;
     // sort scratch
-    uint64_t* moverbits = nullptr;
-    uint32_t* blockcnt = nullptr;
+    uint64_t* moverbits = nullptr;    // two buffers (by sort parity) of mb_words
+    uint32_t* blockcnt = nullptr;     // two buffers of bc_words
+    size_t mb_words = 0, bc_words = 0;
     uint32_t* counters = nullptr;     // [0],[1] movers by key parity, [2] movers of the last
                                       // sort, [3] patched (the window patch placed them)
     uint32_t* win = nullptr;          // windows of the last two sorts (k_patch)
@@ -244,6 +245,16 @@ struct xdem_batch {
 
     // where k_step / k_pos write the keys of the positions they compute
     uint32_t* knext() const { return skey[1 - skcur]; }
+    // mover marks and per-block counts of the sort of parity p (the movers of
+    // the keys in skey[p]); the other parity's are cleared during a sort
+    uint64_t* mbits(int p) const { return moverbits + (size_t)p * mb_words; }
+    uint32_t* bcnt(int p) const { return blockcnt + (size_t)p * bc_words; }
+    // both parities cleared (layout changes)
+    int32_t clear_mover_marks(hipStream_t st) {
+        XDEM_HIP(hipMemsetAsync(moverbits, 0, 2 * mb_words * sizeof(uint64_t), st));
+        XDEM_HIP(hipMemsetAsync(blockcnt, 0, 2 * bc_words * sizeof(uint32_t), st));
+        return XDEM_OK;
+    }
 
     // Where k_step / k_pos record the movers of the keys they compute, against
     // the keys of the last sort (on = 0 when a full sort follows instead); the
@@ -259,8 +270,8 @@ struct xdem_batch {
         m.mkey = mkey;
         m.ms = ms;
         m.mplo = mplo;
-        m.moverbits = moverbits;
-        m.blockcnt = blockcnt;
+        m.moverbits = mbits(1 - skcur);
+        m.blockcnt = bcnt(1 - skcur);
         m.n = n_total;
         m.ncell_cap = (uint32_t)cap_cells;
         m.on = on ? 1u : 0u;
@@ -275,8 +286,8 @@ struct xdem_batch {
         a.zi = zi;
         a.patched = counters + 3;
         a.crange = crange;
-        a.moverbits = moverbits;
-        a.blockcnt = blockcnt;
+        a.moverbits = mbits(1 - skcur);
+        a.blockcnt = bcnt(1 - skcur);
         a.msorted = counters + 2;
         a.mkey = mkey;
         a.ms = ms;
@@ -321,6 +332,12 @@ struct xdem_batch {
         a.scene_blk = d_scene_blk;
         a.force = force;
         a.rebuild = rebuild;
+        a.moverbits = mbits(1 - skcur);
+        a.blockcnt = bcnt(1 - skcur);
+        a.mb_clear = mbits(skcur);
+        a.bc_clear = bcnt(skcur);
+        a.mb_words = div_up(std::max<uint32_t>(n_total, 1), 64);
+        a.bc_words = nblk_rows() + 1;
         return a;
     }
 };
@@ -496,8 +513,7 @@ int32_t apply_layout(xdem_batch* b) {
         }
         if (n_old) {
             hipLaunchKernelGGL(k_ranges, dim3(div_up(n_old, ROWS_T)), dim3(ROWS_T), 0, st,
-                               b->skey[b->skcur], b->crange, n_old, (uint32_t)b->cap_cells,
-                               (uint64_t*)nullptr, (uint32_t*)nullptr, (const uint32_t*)nullptr);
+                               b->skey[b->skcur], b->crange, n_old, (uint32_t)b->cap_cells);
             XDEM_LAUNCHED();
         }
     }
@@ -697,6 +713,7 @@ int32_t apply_layout(xdem_batch* b) {
     }
     // the storage moved: the sort and list state starts over
     if (b->dirty) XDEM_HIP(hipMemsetAsync(b->dirty, 0, b->dirty_bytes, st));
+    XDEM_TRY(b->clear_mover_marks(st));
     hipLaunchKernelGGL(k_sort_reset, dim3(1), dim3(256), 0, st, b->patch_args(b->buf[b->cur].zi));
     XDEM_LAUNCHED();
     XDEM_HIP(hipStreamSynchronize(st));
@@ -728,7 +745,6 @@ int32_t run_sort(xdem_batch* b) {
     if (!n) return XDEM_OK;
     hipStream_t st = b->st;
     PhaseTimer& T = b->timer;
-    uint32_t* knew_sorted = b->knext();
     const uint2* zi = b->buf[b->cur].zi;
     SortArgs a = b->sort_args(zi);
     // at most FEW movers: k_patch re-places them and the two kernels after it
@@ -740,12 +756,6 @@ int32_t run_sort(xdem_batch* b) {
     XDEM_TRY(T.end(st));
     XDEM_TRY(T.begin(4, st));
     hipLaunchKernelGGL(k_merge, dim3(SWEEP_GRID + 64), dim3(ROWS_T), 0, st, a);
-    XDEM_LAUNCHED();
-    XDEM_TRY(T.end(st));
-    XDEM_TRY(T.begin(5, st));
-    hipLaunchKernelGGL(k_ranges, dim3(std::min<uint32_t>(SWEEP_GRID, a.nblk)), dim3(ROWS_T), 0, st,
-                       knew_sorted, b->crange, n, (uint32_t)b->cap_cells, b->moverbits,
-                       b->blockcnt, (const uint32_t*)(b->counters + 3));
     XDEM_LAUNCHED();
     XDEM_TRY(T.end(st));
     b->skcur = 1 - b->skcur;
@@ -763,7 +773,7 @@ int32_t run_full_sort(xdem_batch* b) {
     XDEM_HIP(hipStreamSynchronize(st));  // sort_pairs may grow the library workspace
     XDEM_TRY(sort_pairs(out, b->perm, n, b->key_bits(), st));
     hipLaunchKernelGGL(k_ranges, dim3(div_up(n, ROWS_T)), dim3(ROWS_T), 0, st, out, b->crange, n,
-                       (uint32_t)b->cap_cells, b->moverbits, b->blockcnt, (const uint32_t*)nullptr);
+                       (uint32_t)b->cap_cells);
     XDEM_LAUNCHED();
     hipLaunchKernelGGL(k_invert, dim3(div_up(n, 256)), dim3(256), 0, st, b->perm, b->invperm, n);
     XDEM_LAUNCHED();
@@ -922,8 +932,8 @@ uint64_t batch_device_bytes(uint64_t n, uint64_t nscene, uint64_t ncell, bool hi
     b += 2 * n * (sizeof(double4) + 3 * sizeof(float4) + sizeof(uint2) + sizeof(float4));  // buf[2]
     b += 4 * n * sizeof(uint32_t);                        // skey[2], perm, invperm
     b += ncell * sizeof(uint2);                           // crange
-    b += (nblk * (ROWS_T / 64) + 1) * sizeof(uint64_t);   // moverbits
-    b += (nblk + 1) * sizeof(uint32_t) + 8 * sizeof(uint32_t);  // blockcnt, counters
+    b += 2 * (nblk * (ROWS_T / 64) + 1) * sizeof(uint64_t);    // moverbits (two parities)
+    b += 2 * (nblk + 1) * sizeof(uint32_t) + 8 * sizeof(uint32_t);  // blockcnt (two), counters
     b += n * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t) + 2 * sizeof(uint32_t)) +
          FEW * sizeof(uint4);                             // movers
     b += 2 * WIN_WORDS * sizeof(uint32_t);              // windows
@@ -1012,12 +1022,14 @@ int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t 
     }
     size_t n = max_particles;
     uint32_t nblk = div_up(max_particles, ROWS_T);
+    b->mb_words = (size_t)nblk * (ROWS_T / 64) + 1;
+    b->bc_words = (size_t)nblk + 1;
     if ((rc = alloc_state(b->buf[0], n)) || (rc = alloc_state(b->buf[1], n)) ||
         (rc = dalloc(&b->skey[0], n)) || (rc = dalloc(&b->skey[1], n)) ||
         (rc = dalloc(&b->perm, n)) || (rc = dalloc(&b->invperm, n)) ||
         (rc = dalloc(&b->crange, (size_t)max_cells)) ||
-        (rc = dalloc(&b->moverbits, (size_t)nblk * (ROWS_T / 64) + 1)) ||
-        (rc = dalloc(&b->blockcnt, (size_t)nblk + 1)) || (rc = dalloc(&b->counters, 8)) ||
+        (rc = dalloc(&b->moverbits, 2 * ((size_t)nblk * (ROWS_T / 64) + 1))) ||
+        (rc = dalloc(&b->blockcnt, 2 * ((size_t)nblk + 1))) || (rc = dalloc(&b->counters, 8)) ||
         (rc = dalloc(&b->win, 2 * WIN_WORDS)) || (rc = dalloc(&b->mold, (size_t)FEW)) ||
         (rc = dalloc(&b->mkey, n)) || (rc = dalloc(&b->ms, n)) || (rc = dalloc(&b->mkey2, n)) ||
         (rc = dalloc(&b->ms2, n)) || (rc = dalloc(&b->mplo, 2 * n)) ||
@@ -1032,8 +1044,8 @@ int32_t xdem_batch_create(uint32_t max_particles, uint32_t max_scenes, uint64_t 
         hipMemset(b->wdisp, 0, 4 * wdisp_words(max_particles, max_scenes)) != hipSuccess ||
         hipMemset(b->force, 0, 4ull * max_scenes) != hipSuccess ||
         hipMemset(b->rebuild, 0, 4ull * max_scenes) != hipSuccess ||
-        hipMemset(b->moverbits, 0, 8 * ((size_t)nblk * (ROWS_T / 64) + 1)) != hipSuccess ||
-        hipMemset(b->blockcnt, 0, 4 * ((size_t)nblk + 1)) != hipSuccess)
+        hipMemset(b->moverbits, 0, 16 * ((size_t)nblk * (ROWS_T / 64) + 1)) != hipSuccess ||
+        hipMemset(b->blockcnt, 0, 8 * ((size_t)nblk + 1)) != hipSuccess)
         return fail(XDEM_ERR_HIP);
     *out = b;
     return XDEM_OK;

@@ -6,14 +6,15 @@
 //            the scene's planes) + Verlet velocity update + next Verlet position
 //            update + next cell key, written in the new sorted order (the
 //            physical reorder rides on the writes);
-//   k_patch / k_merge / k_ranges
+//   k_patch / k_merge
 //            re-sort by (cell key, particle id): only the particles whose cell
 //            changed ("movers", recorded by k_step / k_pos together with their
-//            insertion point) are sorted; the rest keep their order.  Up to 64
-//            movers, k_patch rewrites only the windows of positions they shift
-//            (k_merge and k_ranges exit); otherwise one block of k_patch sorts
-//            the movers and k_merge / k_ranges merge them with the rest and
-//            rebuild the cell ranges.  The result is the unique (key, id)
+//            insertion point) are sorted; the rest keep their order.  Up to
+//            1024 movers, k_patch rewrites only the windows of positions they
+//            shift (k_merge exits); otherwise k_patch ranks the movers and
+//            clears the old cell ranges, and k_merge merges them with the rest
+//            and writes the cell ranges (atomic minimum and maximum of the
+//            positions per cell).  The result is the unique (key, id)
 //            order, i.e. what the reference's stable sort of the id-ordered
 //            keys produces (source/mphysics_cuda_def.cu:92-94).
 #pragma once
@@ -152,7 +153,9 @@ struct BlockEntry {
 // computes the new keys (k_step / k_pos), for the incremental sort: appended
 // in arrival order with their insertion point in the previous sorted order,
 // marked in moverbits and counted per ROWS_T block (the many-movers path of
-// k_merge).  k_ranges clears moverbits, blockcnt and the count for the next sort.
+// k_merge).  The marks and counts alternate between two buffers with the sort
+// parity; each sort's k_patch clears the other parity's (the previous sort's)
+// for the next one.
 struct MoverOut {
     const uint32_t* key_old;   // sorted keys of the previous sort, storage order
     const uint2* crange;       // cell ranges of the previous sort
@@ -1543,8 +1546,9 @@ __device__ inline uint32_t mover_prefix(const SortArgs& A, uint32_t P) {
 // Stayers count the sorted movers below their (key, id) by a binary search in
 // LDS (global memory when more than MERGE_LDS movers).  A mover starts from
 // its insertion point (recorded with it) and steps over the entries of its new cell
-// with a smaller id; it also clears its old cell's range (k_ranges rewrites
-// every non-empty cell).
+// with a smaller id.  Every placed element takes its position into its cell's
+// range (atomic minimum of the first, maximum of one past the last; k_patch
+// cleared the ranges of the previous order's cells).
 __device__ inline uint32_t movers_below(const uint64_t* mk, uint32_t M, uint64_t X) {
     uint32_t lo = 0, hi = M;
     while (lo < hi) {
@@ -1584,6 +1588,10 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
                 A.perm[out] = s;
                 A.invperm[s] = out;
                 A.skey[out] = kn;
+                if (kn < A.ncell_cap) {
+                    atomicMin(&A.crange[kn].x, out);
+                    atomicMax(&A.crange[kn].y, out + 1);
+                }
             }
         }
         return;
@@ -1602,24 +1610,19 @@ __global__ __launch_bounds__(ROWS_T) void k_merge(SortArgs A) {
             A.perm[out] = s;
             A.invperm[s] = out;
             A.skey[out] = kn;
-            const uint32_t old = A.key_old[s];
-            if (old < A.ncell_cap) A.crange[old] = make_uint2(EMPTY, 0);
+            if (kn < A.ncell_cap) {
+                atomicMin(&A.crange[kn].x, out);
+                atomicMax(&A.crange[kn].y, out + 1);
+            }
         }
     }
 }
 
-// Cell ranges of the sorted keys (reorderDataAndFindCellStart_kernel, :166-212).
-// Also clears the mover marks of the sort just finished (when given).
-// Grid-stride over the positions; exits when k_patch already updated the
-// ranges (patched non-null and set).
+// Cell ranges of the sorted keys (reorderDataAndFindCellStart_kernel, :166-212)
+// after a full sort or a layout change.  Grid-stride over the positions.
 __global__ __launch_bounds__(ROWS_T) void k_ranges(const uint32_t* skey, uint2* crange, uint32_t n,
-                                                  uint32_t ncell_cap, uint64_t* moverbits,
-                                                  uint32_t* blockcnt, const uint32_t* patched) {
-    if (patched && *patched) return;
+                                                  uint32_t ncell_cap) {
     for (uint32_t j = blockIdx.x * ROWS_T + threadIdx.x; j < n; j += gridDim.x * ROWS_T) {
-        if (moverbits && (j & 63) == 0) moverbits[j >> 6] = 0;
-        if (blockcnt && j % ROWS_T == 0) blockcnt[j / ROWS_T] = 0;
-        if (j == 0 && blockcnt) blockcnt[(n + ROWS_T - 1) / ROWS_T] = 0;
         const uint32_t h = skey[j];
         if (h >= ncell_cap) continue;  // defensive: keys are < the cell capacity
         if (j == 0 || skey[j - 1] != h) {
@@ -1644,8 +1647,8 @@ __global__ __launch_bounds__(ROWS_T) void k_ranges(const uint32_t* skey, uint2* 
 // mover placed at j, or the stayer of rank j - (movers placed before j) in the
 // old order), and the cell boundaries at window positions are rewritten from
 // the new keys.  Positions of the previous sort's windows outside the new
-// ones get the identity permutation back.  More movers: block 0 sorts them
-// (sort_movers_block) and k_merge / k_ranges do the rest (patched = 0).
+// ones get the identity permutation back.  More movers: they are ranked for
+// k_merge, which does the rest (patched = 0).
 constexpr int WIN_WORDS = 2 + 2 * FEW;   // per parity: count, then (first, last) pairs
 
 struct PatchArgs {
@@ -1677,6 +1680,9 @@ struct PatchArgs {
     const uint2* scene_blk;    // per scene: first and end block of the launch table
     uint32_t* force;           // per scene: a block start asks for a rebuild
     uint32_t* rebuild;         // per scene: out, rebuild every list in the next step
+    uint64_t* mb_clear;        // the other parity's mover marks and block counts, cleared
+    uint32_t* bc_clear;        // here for the next sort
+    uint32_t mb_words, bc_words;
 };
 
 // Cell `which` (0..26) of the 27 around cell key k of scene S (the walk's
@@ -1955,6 +1961,11 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
     // older marks need no clearing
     const uint32_t tag = A.counters[6 + (1 - A.par)] % 65535u + 1u;
     bool few = A.patch_on && M <= (uint32_t)FEW;
+    // the previous sort's mover marks and block counts, cleared for the next
+    for (uint32_t q = blockIdx.x * PATCH_T + t; q < A.mb_words; q += gridDim.x * PATCH_T)
+        A.mb_clear[q] = 0;
+    for (uint32_t q = blockIdx.x * PATCH_T + t; q < A.bc_words; q += gridDim.x * PATCH_T)
+        A.bc_clear[q] = 0;
     if (t < npv) {
         sPrev[2 * t] = win_prev[2 + 2 * t];
         sPrev[2 * t + 1] = win_prev[3 + 2 * t];
@@ -2048,11 +2059,17 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
         } else if (blockIdx.x == 0) {
             sort_movers_block<PATCH_T>(SA, M);
         }
+        // and the previous order's cell ranges are cleared (k_merge writes
+        // every cell's range of the new order)
+        for (uint32_t j = blockIdx.x * PATCH_T + t; j < n; j += gridDim.x * PATCH_T) {
+            const uint32_t k = A.key_old[j];
+            if (k < A.ncell_cap && (j == 0 || A.key_old[j - 1] != k))
+                A.crange[k] = make_uint2(EMPTY, 0);
+        }
     } else if (blockIdx.x == 1) {
         // ---- a cell every particle left (all its positions carry a mover
-        // mark) and none entered becomes empty; then the movers' marks of
-        // record_mover are cleared for the next step and the windows of this
-        // sort recorded
+        // mark) and none entered becomes empty; the windows of this sort are
+        // recorded
         if (rec) {
             const uint32_t ko = mo.z, size = mo.x, first = mo.w;
             if (ko < A.ncell_cap && size > 0) {
@@ -2067,11 +2084,6 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
                                lds_count_lt64(sX, M, (uint64_t)ko << 32))
                     A.crange[ko] = make_uint2(EMPTY, 0);
             }
-        }
-        __syncthreads();
-        if (rec) {
-            A.moverbits[src >> 6] = 0;
-            A.blockcnt[src / ROWS_T] = 0;
         }
         if (t < nw) {
             win_now[2 + 2 * t] = sWin[2 * t];
