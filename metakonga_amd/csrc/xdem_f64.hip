@@ -231,45 +231,63 @@ __global__ __launch_bounds__(256) void k_p2p(KParams k, const double* pos, const
     // below would reject it), so neither its index nor its record is loaded
     const float4 ownf = pf_screen(k, ipos.x, ipos.y, ipos.z, ir, pf_range);
     const float sr0 = ownf.w + pf_eps;
-    for (int z = -1; z <= 1; z++)
-        for (int y = -1; y <= 1; y++)
-            for (int x = -1; x <= 1; x++) {
-                uint32_t h = grid_hash(k, gx + x, gy + y, gz + z);
-                uint32_t s = cstart[h];
-                if (s == 0xFFFFFFFFu) continue;
-                uint32_t e = cend[h];
-                for (uint32_t j = s; j < e; j++) {
-                    if (SORTED && pfs) {
-                        const float4 f = pfs[j];
-                        const float dx = f.x - ownf.x, dy = f.y - ownf.y, dz = f.z - ownf.z;
-                        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                        const float sr = sr0 + f.w;
-                        if (d2 >= sr * sr * (1.0f + 0x1p-19f)) continue;
-                    }
-                    uint32_t kk = sorted_index[j];
-                    if (id == kk || kk >= np) continue;
-                    d4 jp = ld4(pos, kk);
-                    d3 rp = mk3(jp.x - ipos.x, jp.y - ipos.y, jp.z - ipos.z);
-                    double dist = length(rp);
-                    double jr = jp.w;
-                    double cdist = (ir + jr) - dist;
-                    if (cdist > 0) {
-                        d3 jvel = ld3(vel, kk), jomega = ld3(omega, kk);
-                        double jm = mass[kk];
-                        double rcon = ir - 0.5 * cdist;
-                        d3 unit = rp / dist;
-                        d3 rv = k.rolling ? jvel + cross(jomega, -jr * unit) -
-                                                (ivel + cross(iomega, ir * unit))
-                                          : (jvel - ivel);
-                        ForceConst c = get_constant<TCM>(ir, jr, im, jm, cp);
-                        double fsn = -c.kn * pow(cdist, 1.5);
-                        double fca = cohesion_force(ir, jr, cp.Ei, cp.Ej, cp.pri, cp.prj, cp.coh, fsn);
-                        force_law<TCM>(c, k.dt, ir, fca, rcon, cdist, iomega, rv, unit, Ft, Fn, M);
-                        sumF += Fn + Ft;
-                        sumM += M;
-                    }
-                }
+    // candidate at sorted position j
+    auto visit = [&](uint32_t j) {
+        if (SORTED && pfs) {
+            const float4 f = pfs[j];
+            const float dx = f.x - ownf.x, dy = f.y - ownf.y, dz = f.z - ownf.z;
+            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            const float sr = sr0 + f.w;
+            if (d2 >= sr * sr * (1.0f + 0x1p-19f)) return;
+        }
+        uint32_t kk = sorted_index[j];
+        if (id == kk || kk >= np) return;
+        d4 jp = ld4(pos, kk);
+        d3 rp = mk3(jp.x - ipos.x, jp.y - ipos.y, jp.z - ipos.z);
+        double dist = length(rp);
+        double jr = jp.w;
+        double cdist = (ir + jr) - dist;
+        if (cdist > 0) {
+            d3 jvel = ld3(vel, kk), jomega = ld3(omega, kk);
+            double jm = mass[kk];
+            double rcon = ir - 0.5 * cdist;
+            d3 unit = rp / dist;
+            d3 rv = k.rolling ? jvel + cross(jomega, -jr * unit) - (ivel + cross(iomega, ir * unit))
+                              : (jvel - ivel);
+            ForceConst c = get_constant<TCM>(ir, jr, im, jm, cp);
+            double fsn = -c.kn * pow(cdist, 1.5);
+            double fca = cohesion_force(ir, jr, cp.Ei, cp.Ej, cp.pri, cp.prj, cp.coh, fsn);
+            force_law<TCM>(c, k.dt, ir, fca, rcon, cdist, iomega, rv, unit, Ft, Fn, M);
+            sumF += Fn + Ft;
+            sumM += M;
+        }
+    };
+    if (SORTED && gx >= 1 && gx + 1 < (int)k.gx) {
+        // Away from the x wrap the three cells of an x-row have consecutive
+        // hashes, so their particles are one run of sorted positions: the
+        // row's cells in x order, each in ascending sorted position — the
+        // cell loop's order.  The six range words of a row are loaded together.
+        for (int z = -1; z <= 1; z++)
+            for (int y = -1; y <= 1; y++) {
+                const uint32_t h = grid_hash(k, gx, gy + y, gz + z);
+                const uint32_t s0 = cstart[h - 1], s1 = cstart[h], s2 = cstart[h + 1];
+                const uint32_t e0 = cend[h - 1], e1 = cend[h], e2 = cend[h + 1];
+                const uint32_t rb = s0 != 0xFFFFFFFFu ? s0 : (s1 != 0xFFFFFFFFu ? s1 : s2);
+                if (rb == 0xFFFFFFFFu) continue;
+                const uint32_t re = s2 != 0xFFFFFFFFu ? e2 : (s1 != 0xFFFFFFFFu ? e1 : e0);
+                for (uint32_t j = rb; j < re; j++) visit(j);
             }
+    } else {
+        for (int z = -1; z <= 1; z++)
+            for (int y = -1; y <= 1; y++)
+                for (int x = -1; x <= 1; x++) {
+                    uint32_t h = grid_hash(k, gx + x, gy + y, gz + z);
+                    uint32_t s = cstart[h];
+                    if (s == 0xFFFFFFFFu) continue;
+                    uint32_t e = cend[h];
+                    for (uint32_t j = s; j < e; j++) visit(j);
+                }
+    }
     add3(force, id, sumF);
     add3(moment, id, sumM);
 }
