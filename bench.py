@@ -417,6 +417,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB, 20 times (profiler "
                          "byte-counter calibration), then exit")
+    ap.add_argument("--block-log", default="",
+                    help="only run the hopper with per-block JSON metrics (contacts per particle, "
+                         "largest overlap, kinetic energy, momentum) written to this file, then exit")
+    ap.add_argument("--block-log-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -435,6 +439,18 @@ def main():
             abi.check(lib, lib.xdem_copy_kernel(dst, src, nb))
         abi.check(lib, lib.xdem_synchronize())
         print(json.dumps({"copy_calibration_bytes": nb, "launches": 20}))
+        return
+    if args.block_log:
+        from metakonga_amd import batch, blocklog
+        sc = scenes.hopper(n=args.np)
+        b = batch.Batch(sc.np, 1, int(np.prod(sc.grid_size)), lib)
+        b.lists(True, 0.05)
+        sid = b.add(sc)
+        b.prepare()
+        with open(args.block_log, "w") as f:
+            recs = blocklog.run_logged(b, [sid], args.block_log_steps, 100, f)
+        b.close()
+        print(json.dumps({"block_log": args.block_log, "blocks": len(recs), "last": recs[-1]}))
         return
     t0 = time.perf_counter()
     scene = scenes.hopper(n=args.np)

@@ -214,6 +214,13 @@ uint32_t xdem_batch_num_scenes(const xdem_batch* b);
 /* Kinetic energy (translational + rotational) and linear momentum of one
  * scene, reduced on the device in FP64 (deterministic order). */
 int32_t xdem_batch_energy(xdem_batch* b, uint32_t scene_id, double* kinetic, double momentum[3]);
+/* Contact metrics of a scene at its current positions (after
+ * xdem_batch_prepare): particle-particle contacts, each pair once (the
+ * reference predicate (ri + rj) - |pj - pi| > 0 in FP64 over the 27 cells of
+ * the walk), and the largest overlap (0 without contacts).  No counterpart in
+ * the reference, which has no contact counters; for per-block logs. */
+int32_t xdem_batch_contact_metrics(xdem_batch* b, uint32_t scene_id, uint64_t* contact_pairs,
+                                   double* max_overlap);
 /* Reactions of the triangle-mesh objects, the batched counterpart of the
  * per-object force/moment the reference accumulates in dpmi[] during the
  * triangle contact kernel (particle_polygonObject_collision_kernel,

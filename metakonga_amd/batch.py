@@ -69,6 +69,7 @@ def declare(lib):
         "xdem_batch_last_movers": ([vp, C.POINTER(u32)], i32),
         "xdem_batch_prepare": ([vp], i32),
         "xdem_batch_capture": ([vp, u32], i32),
+        "xdem_batch_contact_metrics": ([vp, u32, C.POINTER(C.c_uint64), C.POINTER(C.c_double)], i32),
         "xdem_batch_probe": ([vp, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp], i32),
     }
     for n, (a, r) in sig.items():
@@ -249,6 +250,15 @@ class Batch:
         mom = np.zeros(3)
         abi.check(self.lib, self.lib.xdem_batch_energy(self.h, sid, C.byref(ke), abi.ptr(mom)))
         return ke.value, mom
+
+    def contact_metrics(self, sid):
+        """(particle-particle contact pairs, largest overlap) of a scene at its
+        current positions; call prepare() first."""
+        pairs = C.c_uint64()
+        ov = C.c_double()
+        abi.check(self.lib, self.lib.xdem_batch_contact_metrics(self.h, sid, C.byref(pairs),
+                                                                C.byref(ov)))
+        return pairs.value, ov.value
 
     def enable_reactions(self, on: bool = True):
         """Evaluate the mesh objects' reactions at the last step of every step call."""

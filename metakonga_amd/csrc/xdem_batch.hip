@@ -169,6 +169,8 @@ struct xdem_batch {
     BlockEntry* d_blocks_store = nullptr;
     uint32_t nblk_sorted = 0, nblk_store = 0;
     double* d_red = nullptr;          // energy partials
+    unsigned long long* d_metrics = nullptr;  // contact metrics: counts, then overlap bits
+    size_t metrics_cap = 0;
     // tangential-spring history (allocated with the first scene that uses it)
     float4* hxi[2] = {nullptr, nullptr};  // HSLOTS slots per particle, parity = cur
     uint32_t* hcnt[2] = {nullptr, nullptr};
@@ -212,7 +214,7 @@ struct xdem_batch {
         for (void* p : {(void*)skey[0], (void*)skey[1], (void*)perm, (void*)crange,
                         (void*)moverbits, (void*)blockcnt, (void*)counters, (void*)mkey, (void*)ms,
                         (void*)mkey2, (void*)ms2, (void*)mplo, (void*)d_scenes, (void*)d_planes,
-                        (void*)d_blocks_sorted, (void*)d_blocks_store, (void*)d_red, (void*)dbg_f,
+                        (void*)d_blocks_sorted, (void*)d_blocks_store, (void*)d_red, (void*)d_metrics, (void*)dbg_f,
                         (void*)dbg_cnt, (void*)dbg_part, (void*)hxi[0], (void*)hxi[1],
                         (void*)hcnt[0], (void*)hcnt[1], (void*)hover, (void*)d_drums,
                         (void*)d_objs, (void*)d_tris, (void*)d_tcells, (void*)d_poses,
@@ -1563,6 +1565,55 @@ __global__ __launch_bounds__(1024) void k_energy(BState st, uint32_t p0, uint32_
     if (threadIdx.x < 4) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
+// Contact metrics at the current positions (after xdem_batch_prepare: storage
+// in sorted order, cell ranges current): per scene the particle-particle
+// contacts found from each side (the reference predicate, (ri + rj) - |pj -
+// pi| > 0 in FP64, over the 27 cells of the walk) and the largest such
+// overlap, as the bits of a non-negative double (an integer maximum orders
+// them).  A wave sums and maximises first, then one atomic per wave.
+__global__ __launch_bounds__(256) void k_metrics(BState st, const uint2* crange, const BScene* scenes,
+                                                 const BlockEntry* blocks, uint32_t n,
+                                                 unsigned long long* cnt, unsigned long long* maxov) {
+    const BlockEntry be = blocks[blockIdx.x];
+    const uint32_t j = be.j0 + threadIdx.x;
+    const bool live = j < be.j1 && j < n;
+    uint32_t c = 0;
+    double mo = 0.0;
+    if (live) {
+        const BScene S = scenes[be.scene];
+        const double4 p = st.pr[j];
+        int cx, cy, cz;
+        cell_of(S, p.x, p.y, p.z, cx, cy, cz);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const uint2 cr = crange[key_of(S, cx + dx, cy + dy, cz + dz)];
+                    if (cr.x == EMPTY) continue;
+                    const uint32_t e1 = min(cr.y, n);
+                    for (uint32_t q = cr.x; q < e1; ++q) {
+                        if (q == j) continue;
+                        const double4 o = st.pr[q];
+                        const double ex = o.x - p.x, ey = o.y - p.y, ez = o.z - p.z;
+                        const double cd = (p.w + o.w) - sqrt(ex * ex + ey * ey + ez * ez);
+                        if (cd > 0) {
+                            ++c;
+                            mo = fmax(mo, cd);
+                        }
+                    }
+                }
+    }
+    unsigned long long m = (unsigned long long)__double_as_longlong(mo);
+    for (int o = 32; o > 0; o >>= 1) {
+        c += (uint32_t)__shfl_xor((int)c, o, 64);
+        const unsigned long long u = __shfl_xor(m, o, 64);
+        m = u > m ? u : m;
+    }
+    if ((threadIdx.x & 63) == 0 && j < n && be.j0 < be.j1) {
+        if (c) atomicAdd(&cnt[be.scene], (unsigned long long)c);
+        if (m) atomicMax(&maxov[be.scene], m);
+    }
+}
+
 // Gather the state into sorted order (perm) — used after a parity prepare so
 // that storage is sorted again.
 __global__ void k_gather(BState in, BState out, const uint32_t* perm, uint32_t n) {
@@ -1652,6 +1703,46 @@ int32_t xdem_batch_energy(xdem_batch* b, uint32_t scene_id, double* kinetic, dou
         momentum[0] = h[1];
         momentum[1] = h[2];
         momentum[2] = h[3];
+    }
+    return XDEM_OK;
+}
+
+int32_t xdem_batch_contact_metrics(xdem_batch* b, uint32_t scene_id, uint64_t* contact_pairs,
+                                   double* max_overlap) {
+    XDEM_REQUIRE(b, "null batch");
+    XDEM_REQUIRE(b->prepared, "call xdem_batch_prepare first");
+    HostScene* s = find_scene(b, scene_id);
+    XDEM_REQUIRE(s, "no scene %u", scene_id);
+    XDEM_REQUIRE(!s->pending, "scene %u not on the device yet (step first)", scene_id);
+    const size_t ns = std::max<size_t>(b->scenes.size(), 1);
+    if (ns > b->metrics_cap) {
+        if (b->d_metrics) (void)hipFree(b->d_metrics);
+        b->d_metrics = nullptr;
+        b->metrics_cap = 0;
+        XDEM_TRY(dalloc(&b->d_metrics, 2 * ns));
+        b->metrics_cap = ns;
+    }
+    XDEM_HIP(hipMemsetAsync(b->d_metrics, 0, 2 * ns * sizeof(unsigned long long), b->st));
+    if (b->nblk_sorted) {
+        hipLaunchKernelGGL(k_metrics, dim3(b->nblk_sorted), dim3(b->block_threads), 0, b->st,
+                           b->buf[b->cur], b->crange, b->d_scenes, b->d_blocks_sorted, b->n_total,
+                           b->d_metrics, b->d_metrics + ns);
+        XDEM_LAUNCHED();
+    }
+    uint32_t k = 0;  // the scene's row in the device scene table
+    for (; k < b->scenes.size(); ++k)
+        if (&b->scenes[k] == s) break;
+    unsigned long long h[2] = {0, 0};
+    if (k < b->scenes.size()) {
+        XDEM_HIP(hipMemcpyAsync(&h[0], b->d_metrics + k, 8, hipMemcpyDeviceToHost, b->st));
+        XDEM_HIP(hipMemcpyAsync(&h[1], b->d_metrics + ns + k, 8, hipMemcpyDeviceToHost, b->st));
+    }
+    XDEM_HIP(hipStreamSynchronize(b->st));
+    if (contact_pairs) *contact_pairs = h[0] / 2;
+    if (max_overlap) {
+        double v;
+        std::memcpy(&v, &h[1], 8);
+        *max_overlap = v;
     }
     return XDEM_OK;
 }
