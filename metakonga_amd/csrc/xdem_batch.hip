@@ -123,9 +123,10 @@ struct xdem_batch {
     uint32_t* blockcnt = nullptr;     // two buffers of bc_words
     size_t mb_words = 0, bc_words = 0;
     uint32_t* counters = nullptr;     // [0],[1] movers by key parity, [2] movers of the last
-                                      // sort, [3] patched (the window patch placed them)
+                                      // sort, [3] patched (the window patch placed them),
+                                      // [4],[5] scenes rebuilding every list, [6],[7] dirty tags
     uint32_t* win = nullptr;          // windows of the last two sorts (k_patch)
-    uint4* mold = nullptr;            // first FEW movers: old cell size, scene, old key
+    uint4* mold = nullptr;            // first FEW movers: old cell size, scene, old key, first
     uint64_t* mkey = nullptr;
     uint32_t* ms = nullptr;
     uint64_t* mkey2 = nullptr;

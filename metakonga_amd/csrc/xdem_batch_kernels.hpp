@@ -1662,7 +1662,7 @@ struct PatchArgs {
     uint32_t* invperm;
     uint32_t* counters;        // [0], [1] movers by key-buffer parity, [2] movers of the
                                // sort, [3] patched, [4], [5] scenes rebuilding every list
-                               // by parity
+                               // by parity, [6], [7] the dirty tag of each parity's sort
     const uint64_t* mkey;      // mover records, append order (record_mover)
     const uint32_t* ms;
     const uint32_t* mplo;
@@ -2231,8 +2231,8 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
 
 // Reset of the sort and list state after a layout change or a regather:
 // both mover counters zero, both window sets the whole range (the next patch
-// restores the identity everywhere), no marked neighbourhoods recorded (the
-// dirty bytes themselves are cleared by the host), every scene's lists rebuilt.
+// restores the identity everywhere), every scene's lists rebuilt (the dirty
+// tags and the mover marks are cleared by the host).
 // One block.
 __global__ __launch_bounds__(256) void k_sort_reset(PatchArgs A) {
     const uint32_t t = threadIdx.x;
