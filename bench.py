@@ -444,7 +444,7 @@ def main():
         from metakonga_amd import batch, blocklog
         sc = scenes.hopper(n=args.np)
         b = batch.Batch(sc.np, 1, int(np.prod(sc.grid_size)), lib)
-        b.lists(True, 0.05)
+        b.lists(True)
         sid = b.add(sc)
         b.prepare()
         with open(args.block_log, "w") as f:

@@ -113,7 +113,7 @@ uint64_t xdem_batch_device_bytes(uint32_t max_particles, uint32_t max_scenes, ui
                                  int32_t options);
 /* Candidate lists (no counterpart in the reference): the contact kernel keeps,
  * per particle, the neighbours within ri + rj + skin (skin = skin_fraction x
- * the scene's cell size; 0 keeps the current fraction, 0.05 at creation) in
+ * the scene's cell size; 0 keeps the current fraction, 0.02 at creation) in
  * the reference traversal order and evaluates the contact predicate on them
  * instead of walking the 27 cells.  A particle walks its cells again (and
  * rebuilds its list) at the first step of every xdem_batch_step call, when

@@ -197,7 +197,7 @@ class Batch:
 
     def lists(self, on=True, skin_fraction=0.0):
         """Candidate lists in the contact kernel (plane-only batches); skin as a
-        fraction of the cell size (0: keep the current one, 0.05 initially)."""
+        fraction of the cell size (0: keep the current one, 0.02 initially)."""
         abi.check(self.lib, self.lib.xdem_batch_set_lists(self.h, int(bool(on)),
                                                           float(skin_fraction)))
 

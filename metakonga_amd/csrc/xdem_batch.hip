@@ -135,7 +135,7 @@ struct xdem_batch {
     // candidate lists (plane-only scenes)
     bool lists_on = false;
     bool patch_on = true;             // the window patch for few movers (else the full merge)
-    float skin_frac = 0.05f;          // skin = skin_frac x cell size
+    float skin_frac = 0.02f;          // skin = skin_frac x cell size
     uint32_t* vl = nullptr;           // VLK entries per particle, lane-interleaved
     uint32_t* vc = nullptr;           // entries per particle
     uint16_t* dirty = nullptr;        // per cell key: tag of the sort that last marked it
@@ -367,7 +367,7 @@ int32_t alloc_cells(xdem_batch* b, uint32_t ncell, uint32_t* base) {
     return XDEM_OK;
 }
 
-BScene make_bscene(const HostScene& h, uint32_t plane0, float skin_frac = 0.05f) {
+BScene make_bscene(const HostScene& h, uint32_t plane0, float skin_frac = 0.02f) {
     const xdem_batch_scene_desc& d = h.desc;
     BScene s{};
     s.wox = d.world_origin.x;

@@ -83,6 +83,7 @@ def _run(kind, steps=90, block=30, patch=True, lists=False, skin=0.0, threads=25
 VARIANTS = [
     dict(patch=True),
     dict(patch=True, lists=True, skin=0.05),
+    dict(patch=True, lists=True, skin=0.01),
     dict(patch=True, lists=True, skin=0.3),
     dict(patch=True, lists=True, skin=0.05, threads=64, unroll=4),
     dict(patch=True, lists=True, skin=0.05, unroll=1, ahead=True),
