@@ -57,8 +57,11 @@ typedef struct xdem_batch_scene_desc {
 /* Tangential-spring history (tangential_history = 1; no counterpart in the
  * reference).  Per particle up to XDEM_BATCH_HISTORY_SLOTS contacts, in
  * evaluation order (particle contacts in the reference traversal order, then
- * planes in order), each a key (partner particle index within the scene, or
- * XDEM_BATCH_WALL_KEY | plane index) and a spring xi (FP32).  Per step: xi is
+ * planes in order, then drums in order: shell, then the cap on the particle's
+ * side), each a key (partner particle index within the scene,
+ * XDEM_BATCH_WALL_KEY | plane index, or XDEM_BATCH_DRUM_KEY | drum index << 2
+ * | side with side 0 the shell, 1 the cap at +half_length, 2 the cap at
+ * -half_length) and a spring xi (FP32).  Per step: xi is
  * projected onto the current tangent plane and advanced by dt e (e the
  * tangential relative velocity), Ft = ks xi + vs e; beyond the Coulomb cap
  * mu |Fn| the force is capped and xi reset to (Ft - vs e) / ks.  A contact
@@ -66,11 +69,13 @@ typedef struct xdem_batch_scene_desc {
  * zero spring and counted (xdem_batch_history_overflow). */
 #define XDEM_BATCH_HISTORY_SLOTS 16
 #define XDEM_BATCH_WALL_KEY 0x80000000u
+#define XDEM_BATCH_DRUM_KEY 0xC0000000u
 
 /* Walls beyond the planes, and triangle meshes, for a scene that has not
  * stepped yet (no counterpart in the reference's batched form; the contact
- * rules are those of the FP64 entry points in xdem.h).  Not available with
- * tangential_history.
+ * rules are those of the FP64 entry points in xdem.h).  Polygon objects are
+ * not available with tangential_history; drums are (their contacts keep
+ * springs as above).
  *
  * xdem_batch_add_drum: a rotating drum (xdem_drum_info, as
  * xdem_drum_contact_force_f64), applied after the scene's planes, in the order
@@ -238,7 +243,8 @@ int32_t xdem_batch_object_reactions(xdem_batch* b, uint32_t scene_id, xdem_doubl
 /* The same for the scene's drum walls (order of xdem_batch_add_drum): minus the
  * summed drum contact forces and their moment about the drum origin (the drive
  * torque is its component along the axis), as xdem_drum_contact_force_f64's
- * out_mf/out_mm; evaluated with the object reactions when those are on. */
+ * out_mf/out_mm; evaluated with the object reactions when those are on.
+ * Refused for a scene with tangential_history (that evaluation has no springs). */
 int32_t xdem_batch_drum_reactions(xdem_batch* b, uint32_t scene_id, xdem_double3* force,
                                   xdem_double3* moment, uint32_t cap, uint32_t* ndrum);
 /* Per-phase device times (HIP events), phases: 0 position update + hash,

@@ -26,6 +26,7 @@ class BatchSceneDesc(C.Structure):
 
 HISTORY_SLOTS = 16
 WALL_KEY = 0x80000000
+DRUM_KEY = 0xC0000000
 
 
 def declare(lib):

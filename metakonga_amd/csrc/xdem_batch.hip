@@ -29,6 +29,7 @@ using namespace xdem::fused;
 
 static_assert(HSLOTS == XDEM_BATCH_HISTORY_SLOTS, "history slots");
 static_assert(WALL_KEY == XDEM_BATCH_WALL_KEY, "wall key");
+static_assert(DRUM_KEY == XDEM_BATCH_DRUM_KEY, "drum key");
 
 namespace {
 
@@ -857,7 +858,10 @@ int32_t launch_step(xdem_batch* b, bool adv) {
         else                                                                                  \
             hipLaunchKernelGGL((k_step<false, false, U, P, H, G, false>), g, t, 0, b->st, a); \
     } while (0)
-    if (b->geom_active()) {
+    if (b->geom_active() && b->hist_active) {
+        // drums or meshes and some scene with history (its drum springs)
+        XDEM_STEP_LAUNCH(2, false, true, true);
+    } else if (b->geom_active()) {
         // drums or meshes: the GEOM variants (contact records not loaded ahead)
         if (b->unroll == 1)
             XDEM_STEP_LAUNCH(1, false, false, true);
@@ -1095,9 +1099,14 @@ int32_t xdem_batch_add_scene_history(xdem_batch* b, const xdem_batch_scene_desc*
             XDEM_REQUIRE(hist_count[i] <= XDEM_BATCH_HISTORY_SLOTS, "history count %u > %d",
                          hist_count[i], XDEM_BATCH_HISTORY_SLOTS);
             for (uint32_t k = 0; k < hist_count[i]; ++k) {
+                // drum keys: the scene's drums are added after it (a key of a
+                // drum never added matches no contact and is dropped)
                 uint32_t key = hist_key[(size_t)i * XDEM_BATCH_HISTORY_SLOTS + k];
-                XDEM_REQUIRE(key & WALL_KEY ? (key & ~WALL_KEY) < d->nplane : key < d->np,
-                             "history key %#x of particle %u out of range", key, i);
+                const bool ok = (key & DRUM_KEY) == DRUM_KEY
+                                    ? (key & 3u) <= 2u && ((key & ~DRUM_KEY) >> 2) < (1u << 20)
+                                : key & WALL_KEY ? (key & ~WALL_KEY) < d->nplane
+                                                 : key < d->np;
+                XDEM_REQUIRE(ok, "history key %#x of particle %u out of range", key, i);
             }
         }
     uint32_t active_np = 0;
@@ -1663,6 +1672,9 @@ static int32_t body_reactions(xdem_batch* b, uint32_t scene_id, bool drums, xdem
                      k < b->h_scenes.size(),
                  "no reactions of scene %u yet (step first)", scene_id);
     const BScene& bs = b->h_scenes[k];
+    // the reaction pass evaluates contacts without springs
+    XDEM_REQUIRE(!drums || !bs.history || !bs.ndrum,
+                 "scene %u: drum reactions are not available with tangential history", scene_id);
     const uint32_t n = drums ? bs.ndrum : bs.nobj;
     const size_t first = drums ? (size_t)b->nobj_total + bs.drum0 : (size_t)bs.obj0;
     if (count) *count = n;
@@ -1878,13 +1890,14 @@ d3 circumcenter_host(d3 P, d3 Q, d3 R) {
     return M1 + t * D1;
 }
 
-int32_t pending_scene(xdem_batch* b, uint32_t scene_id, HostScene** out) {
+// mesh: a polygon object is being added (no tangential history with meshes;
+// drums carry their springs in the particles' history)
+int32_t pending_scene(xdem_batch* b, uint32_t scene_id, bool mesh, HostScene** out) {
     HostScene* s = find_scene(b, scene_id);
     XDEM_REQUIRE(s, "no scene %u", scene_id);
     XDEM_REQUIRE(s->pending, "scene %u has stepped: add its walls before its first step", scene_id);
-    XDEM_REQUIRE(!s->desc.tangential_history,
-                 "scene %u: tangential history is not available with drums or polygon objects",
-                 scene_id);
+    XDEM_REQUIRE(!mesh || !s->desc.tangential_history,
+                 "scene %u: tangential history is not available with polygon objects", scene_id);
     *out = s;
     return XDEM_OK;
 }
@@ -1895,7 +1908,8 @@ int32_t xdem_batch_add_drum(xdem_batch* b, uint32_t scene_id, const xdem_drum_in
                             const xdem_contact_pair_desc* wall) {
     XDEM_REQUIRE(b && drum && wall, "null argument");
     HostScene* s = nullptr;
-    XDEM_TRY(pending_scene(b, scene_id, &s));
+    XDEM_TRY(pending_scene(b, scene_id, false, &s));
+    XDEM_REQUIRE(s->drums.size() < (1u << 20), "scene %u: too many drums", scene_id);
     const xdem_double3& a = drum->axis;
     double an = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
     XDEM_REQUIRE(std::fabs(an - 1.0) < 1e-12, "drum axis must be a unit vector (|axis| = %.17g)", an);
@@ -1915,7 +1929,7 @@ int32_t xdem_batch_add_polygon_object(xdem_batch* b, uint32_t scene_id,
                                       const xdem_contact_pair_desc* wall) {
     XDEM_REQUIRE(b && tris && ntri && origin && ep && wall, "null argument");
     HostScene* s = nullptr;
-    XDEM_TRY(pending_scene(b, scene_id, &s));
+    XDEM_TRY(pending_scene(b, scene_id, true, &s));
     HostObject ho{};
     BObject& ob = ho.ob;
     ob.origin0 = *origin;

@@ -29,6 +29,7 @@ constexpr int CMAX = 16;                     // contact-list slots per particle 
 constexpr int HSLOTS = 16;                   // tangential-history slots per particle
 constexpr int RTAB = 7;                      // neighbour rows kept in LDS during the walk
 constexpr uint32_t WALL_KEY = 0x80000000u;   // history key of plane p: WALL_KEY | p
+constexpr uint32_t DRUM_KEY = 0xC0000000u;   // drum q: DRUM_KEY | q << 2 | side
 constexpr int LDS_T = 256;                   // lane stride of the per-thread LDS lists
 constexpr int MERGE_LDS = 2048;              // sorted movers staged in LDS by k_merge up to this count
 constexpr int ROWS_T = 256;                  // threads of the row kernels
@@ -680,10 +681,11 @@ __device__ inline void mesh_object(const StepArgs& A, const BScene& S, const Own
 // Drum wall q of the scene with the particle `o` (FP64 geometry, FP32 DHS with
 // ir = 0).  REACT: the reaction on the drum is accumulated too: -(Fn + Ft) and
 // its moment about the drum origin, (contact point - origin) x -(Fn + Ft), as
-// xdem_drum_contact_force_f64 reports it.
-template <bool REACT>
+// xdem_drum_contact_force_f64 reports it.  HIST: each surface carries a
+// tangential spring in the particle's history (key DRUM_KEY | q << 2 | side).
+template <bool REACT, bool HIST = false>
 __device__ inline void drum_wall(const StepArgs& A, const BScene& S, const Own& o, double ri,
-                                 uint32_t q, f3& F, f3* rf, f3* rm) {
+                                 uint32_t q, f3& F, f3* rf, f3* rm, HistCtx* hc = nullptr) {
     const d3 p = mk3(o.x, o.y, o.z);
     const BDrum& bd = A.drums[S.drum0 + q];
     const d3 ax = of(bd.d.axis), c0 = of(bd.d.origin);
@@ -701,14 +703,23 @@ __device__ inline void drum_wall(const StepArgs& A, const BScene& S, const Own& 
     const double rho = sqrt(q2);
     const float kn = bd.ckn * fsqrt(o.r);
     const float vn = fsqrt(bd.cvn * o.m * kn);
-    auto contact = [&](d3 u, d3 cpt, double cdist) {
+    // side: 0 shell, 1 the cap at +half_length, 2 the cap at -half_length
+    auto contact = [&](d3 u, d3 cpt, double cdist, uint32_t side) {
         const f3 uf = F3((float)u.x, (float)u.y, (float)u.z);
         const d3 lever = cpt - c0;
         const d3 wv = bd.d.omega * cross(ax, lever);
         const f3 dv = F3((float)wv.x, (float)wv.y, (float)wv.z) - (o.v + crossf(o.w, o.r * uf));
         f3 Ft = F3(0, 0, 0), Fn = F3(0, 0, 0), M = F3(0, 0, 0);
-        dhs_f32(kn, vn, kn * bd.sr, vn * bd.sr, bd.mu, S.dtf, 0.0f, (float)cdist, dv, uf, Ft, Fn,
-                M);
+        if constexpr (HIST) {
+            const uint32_t key = DRUM_KEY | q << 2 | side;
+            f3 xi = hc->lookup(key);
+            hist_f32(kn, vn, kn * bd.sr, vn * bd.sr, bd.mu, S.dtf, 0.0f, (float)cdist, dv, uf, xi,
+                     Ft, Fn, M);
+            hc->store(key, xi);
+        } else {
+            dhs_f32(kn, vn, kn * bd.sr, vn * bd.sr, bd.mu, S.dtf, 0.0f, (float)cdist, dv, uf, Ft,
+                    Fn, M);
+        }
         F += Fn + Ft;
         if (REACT) {
             *rf += -(Fn + Ft);
@@ -718,12 +729,12 @@ __device__ inline void drum_wall(const StepArgs& A, const BScene& S, const Own& 
     const double cs = ri + rho - bd.d.radius;
     if (cs > 0 && rho > 0) {
         const d3 u = qv / rho;
-        contact(u, c0 + a * ax + bd.d.radius * u, cs);
+        contact(u, c0 + a * ax + bd.d.radius * u, cs, 0u);
     }
     const double cc = ri - (bd.d.half_length - fabs(a));
     if (bd.d.caps && cc > 0 && rho < bd.d.radius) {
         const double sg = a >= 0 ? 1.0 : -1.0;
-        contact(sg * ax, c0 + (sg * bd.d.half_length) * ax + qv, cc);
+        contact(sg * ax, c0 + (sg * bd.d.half_length) * ax + qv, cc, a >= 0 ? 1u : 2u);
     }
 }
 
@@ -737,7 +748,7 @@ __device__ inline void walls_and_meshes(const StepArgs& A, const BScene& S, cons
 // Waves per SIMD the register allocation targets for each variant (the
 // counts its register footprint nearly reaches without spilling).
 constexpr int step_waves(int unr, bool pipe, bool hist, bool geom) {
-    return hist || pipe || geom ? 4 : 5;
+    return hist && geom ? 3 : hist || pipe || geom ? 4 : 5;
 }
 
 // GEOM: the variant that also evaluates drum walls and triangle meshes (used
@@ -1177,7 +1188,16 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
             F += pFn + pFt;
         }
     }
-    if (GEOM && !HIST && (S.ndrum | S.nobj)) walls_and_meshes(A, S, o, ri, F);
+    if (GEOM && (S.ndrum | S.nobj)) {
+        if (HIST && hc.on) {
+            // a scene with history has drums and no meshes: drum springs
+            // after the plane springs
+            for (uint32_t q = 0; q < S.ndrum; ++q)
+                drum_wall<false, HIST>(A, S, o, ri, q, F, nullptr, nullptr, &hc);
+        } else {
+            walls_and_meshes(A, S, o, ri, F);
+        }
+    }
     if (DBG) {
         float* d = A.dbg_f + 6 * (size_t)j;
         d[0] = F.x - o.m * S.gvx;

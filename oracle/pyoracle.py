@@ -51,6 +51,7 @@ class OrcHist(C.Structure):
 
 
 ORC_WALL_KEY = 0x80000000
+ORC_DRUM_KEY = 0xC0000000
 HIST_SLOTS = 16
 
 

@@ -1328,7 +1328,7 @@ struct HistCursor {
 
 // Contacts of particle id with history.  F and Mo enter with the values the
 // contacts are added to (the kernels' order: F += particle sum, then F += each
-// plane's force).
+// plane's force, then each drum contact's).
 void hist_one(const orc_world* w, const orc_hist* in, orc_hist* out, uint32_t id, V3& F, V3& Mo,
               uint64_t* overflow) {
     const xdem_device_parameters* p = &w->prm;
@@ -1399,6 +1399,42 @@ void hist_one(const orc_world* w, const orc_hist* in, orc_hist* out, uint32_t id
         hc.store(key, xi, overflow);
         add_mag(id, Fn + Ft, term_scale(c, cdist, 0.0, length(ivel) + length(iomega) * ir, Ft));
         F += Fn + Ft;
+    }
+    // drums in order: orc_drum's shell and cap contacts (the cylinder kernel's
+    // evaluation, ir = 0) with a spring per surface
+    for (uint32_t q = 0; q < w->nwall; ++q) {
+        const orc_wall& wl = w->walls[q];
+        if (wl.kind != ORC_WALL_DRUM) continue;
+        const xdem_drum_info* dr = &w->drums[wl.index];
+        V3 ax = of(dr->axis), o = of(dr->origin);
+        V3 d = ipos - o;
+        double a = dot(d, ax);
+        if (!(std::fabs(a) <= dr->half_length)) continue;
+        V3 qv = d - a * ax;
+        double rho = std::sqrt(dot(qv, qv));
+        auto contact = [&](V3 un, V3 cpt, double cdist, uint32_t side) {
+            V3 wv = dr->omega * cross(ax, cpt - o);
+            V3 dv = wv - (ivel + cross(iomega, ir * un));
+            xdem_device_force_constant c = force_constant(1, ir, 0.0, im, 0.0, wl.cp);
+            uint32_t key = ORC_DRUM_KEY | (wl.index << 2) | side;
+            V3 xi = hc.lookup(key), Ft, Fn, M;
+            hist_law(c, p->dt, 0.0, cdist, dv, un, xi, Ft, Fn, M);
+            hc.store(key, xi, overflow);
+            add_mag(id, Fn + Ft,
+                    term_scale(c, cdist, 0.0,
+                               length(ivel) + length(iomega) * ir + length(wv), Ft));
+            F += Fn + Ft;
+        };
+        double cs = ir + rho - dr->radius;
+        if (cs > 0 && rho > 0) {
+            V3 u = qv / rho;
+            contact(u, o + a * ax + dr->radius * u, cs, 0u);
+        }
+        double cc = ir - (dr->half_length - std::fabs(a));
+        if (dr->caps && cc > 0 && rho < dr->radius) {
+            double sg = a >= 0 ? 1.0 : -1.0;
+            contact(sg * ax, o + (sg * dr->half_length) * ax + qv, cc, a >= 0 ? 1u : 2u);
+        }
     }
     out->count[id] = hc.nout;
 }

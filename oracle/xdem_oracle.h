@@ -196,14 +196,18 @@ void orc_step_omp(orc_world* w, uint32_t nsteps, int32_t nthreads);
  * counterpart in the reference, whose DHS tangential force is memoryless
  * (mphysics_cuda_impl.cuh:379-401, contact.cpp:20-46).  Per particle, up to
  * `slots` contacts in evaluation order (particle contacts in the traversal
- * order of calculate_p2p_kernel, then planes in application order); key =
- * partner particle id, or ORC_WALL_KEY | wall index; xi = spring (3 doubles).
+ * order of calculate_p2p_kernel, then planes in application order, then
+ * drums in order: shell, then the cap on the particle's side); key = partner
+ * particle id, ORC_WALL_KEY | wall index for a plane, or ORC_DRUM_KEY |
+ * (drum index << 2 | side) for a drum (side 0 shell, 1 cap at +half_length,
+ * 2 cap at -half_length); xi = spring (3 doubles).
  * Law (own definition): Fn as DHS; e = dv - (dv.n)n; xi <- xi - (xi.n)n + dt e;
  * Ft = ks xi + vs e; if |Ft| > mu |Fn| then Ft <- mu |Fn| Ft / |Ft| and
  * xi <- (Ft - vs e) / ks (the spring that reproduces the capped force);
  * M = (ri n) x Ft (walls: ri = 0).  With xi = 0 on entry this
  * is the reference DHS tangential force whenever |e| != 0. */
 #define ORC_WALL_KEY 0x80000000u
+#define ORC_DRUM_KEY 0xC0000000u
 typedef struct orc_hist {
     uint32_t slots;
     uint32_t pad_;

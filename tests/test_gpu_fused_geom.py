@@ -239,12 +239,23 @@ def test_refusals(lib):
     with pytest.raises(RuntimeError):       # 4.1 mm < r_max + circumradius = 4.12 mm
         b.add(small)
     b.close()
+    # tangential history: with drums yes (drum springs), with meshes no
     hist = scenes.Scene(name="h", pos=sc.pos, dt=sc.dt, nsteps=1, world_origin=sc.world_origin,
                         cell_size=0.0, grid_size=sc.grid_size, drums=sc.drums,
                         tangential_history=True)
     b = _batch_for([hist])
+    hid = b.add(hist)
+    b.enable_reactions(True)
+    b.step(1)
+    with pytest.raises(RuntimeError):       # drum reactions are evaluated without springs
+        b.drum_reactions(hid)
+    b.close()
+    mesh = scenes.Scene(name="hm", pos=sc.pos, dt=sc.dt, nsteps=1, world_origin=sc.world_origin,
+                        cell_size=0.0, grid_size=sc.grid_size, tri=sc.tri, objects=sc.objects,
+                        tangential_history=True)
+    b = _batch_for([mesh])
     with pytest.raises(RuntimeError):
-        b.add(hist)
+        b.add(mesh)
     b.close()
 
 

@@ -11,14 +11,17 @@ own state and history and runs one step; the two must agree
     relative to the particle's summed contact-force magnitudes plus weight).
 Also: a history scene alone and in a mix with memoryless scenes, graph replay
 vs eager, checkpoint/resume with history — all bit-identical; slot overflow
-counted as the oracle counts it.
+counted as the oracle counts it.  Drums carry springs too (shell and end caps,
+after the planes); the same teacher-forced comparison covers a history scene
+inside a spinning drum, and such a scene alone and in a mix with a drum and
+mesh scene without history is bit-identical.
 """
 import numpy as np
 import pytest
 
 from metakonga_amd import scenes
 from metakonga_amd.batch import Batch
-from xdem_testutil import FP32C_TOL, acc_error, dense_scene
+from xdem_testutil import FP32C_TOL, acc_error, dense_scene, drum_history_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -164,8 +167,9 @@ def test_history_alone_vs_mix_and_graphs(lib):
         assert mixed[k][1][0].sum() == 0
 
 
-def test_history_checkpoint_resume(lib):
-    sc, vel, om = history_scene(n_side=8, seed=37)
+@pytest.mark.parametrize("drum", [False, True])
+def test_history_checkpoint_resume(lib, drum):
+    sc, vel, om = (drum_history_scene if drum else history_scene)(n_side=8, seed=37)
     ref = _run([sc], 40, 20, vel=[vel], om=[om])[0]
     b = Batch(sc.np, 1, int(np.prod(sc.grid_size)))
     sid = b.add(sc, vel, om)
@@ -180,6 +184,16 @@ def test_history_checkpoint_resume(lib):
     out = (b2.state(sid2), b2.history(sid2))
     b2.close()
     _same(ref, out)
+    if drum:
+        assert (hist[1] >= 0xC0000000).any()
+        bad = hist[1].copy()
+        i = int(np.argmax(hist[0] > 0))
+        bad[i, 0] = 0xC0000003            # no drum surface has side 3
+        b3 = Batch(sc.np, 1, int(np.prod(sc.grid_size)))
+        with pytest.raises(RuntimeError):
+            b3.add(sc, pos=st["pos"], vel=st["vel"], omega=st["omega"], acc=st["acc"],
+                   alpha=st["alpha"], history=(hist[0], bad, hist[2]))
+        b3.close()
 
 
 def test_history_overflow_matches_oracle(lib, orc):
@@ -239,3 +253,43 @@ def test_history_walks_and_blocks_bitexact(lib):
                 ref = out
                 continue
             _same(ref, out)
+
+
+def test_history_drum_steps_vs_oracle(lib, orc):
+    """Teacher-forced steps of a history scene inside a spinning drum (caps on
+    the lattice's top and bottom layers, shell on its vertical edges) and an
+    open box of planes: keys bit-exact, springs and forces within FP32
+    rounding of the oracle."""
+    sc, vel, om = drum_history_scene(n_side=9, seed=53)
+    b = Batch(sc.np, 1, int(np.prod(sc.grid_size)))
+    b.graphs(False)
+    sid = b.add(sc, vel, om)
+    prev = (b.state(sid), b.history(sid))
+    kept, sides = 0, set()
+    for k in range(12):
+        b.step(1)
+        st, hist = b.state(sid), b.history(sid)
+        w, hout = oracle_step(orc, sc, prev[0], prev[1])
+        assert np.array_equal(st["pos"], w.pos)
+        compare(orc, sc, st, hist, w, hout)
+        for i in range(sc.np):
+            c0, c1 = int(prev[1][0][i]), int(hist[0][i])
+            now = hist[1][i, :c1]
+            kept += len({x for x in prev[1][1][i, :c0] if x >= 0xC0000000} & set(now))
+            sides |= {int(x) & 3 for x in now if x >= 0xC0000000}
+        prev = (st, hist)
+    assert sides == {0, 1, 2}      # shell and both caps
+    assert kept > 12 * 20          # drum springs carried from step to step
+    assert b.history_overflow() == 0
+    b.close()
+
+
+def test_history_drum_alone_vs_mix_and_graphs(lib):
+    from test_gpu_drum import _drum_scene
+    h, vh, oh = drum_history_scene(n_side=8, seed=55)
+    d, vd = _drum_scene()
+    mixed = _run([d, h], 60, 20, vel=[vd, vh], om=[None, oh])
+    _same(_run([h], 60, 20, vel=[vh], om=[oh])[0], mixed[1])
+    _same(_run([h], 60, 20, graphs=False, vel=[vh], om=[oh])[0], mixed[1])
+    _same(_run([d], 60, 20, vel=[vd])[0], mixed[0])
+    assert mixed[0][1][0].sum() == 0 and mixed[1][1][0].sum() > 0

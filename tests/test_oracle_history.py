@@ -11,11 +11,16 @@ mphysics_cuda_impl.cuh:379-401), so the history model is pinned by:
     mu |Fn|, where Ft saturates along e;
   * separation zeroes the spring (the key disappears and a new contact
     starts from zero).
+Drum surfaces (shell, end caps) carry springs too: with an empty history the
+force equals the memoryless drum contact (orc_drum); the keys follow the
+planes, shell before cap, side by side of the drum's middle; and a particle
+held against the spinning shell or a cap loads its spring linearly, K dt e
+with e the wall's surface velocity, up to the Coulomb cap.
 """
 import numpy as np
 
 from metakonga_amd import scenes
-from xdem_testutil import dense_scene
+from xdem_testutil import dense_scene, drum_history_scene
 
 
 def _world(orc, sc, vel, om):
@@ -148,3 +153,115 @@ def test_history_slots_overflow_is_counted(orc):
     for i in range(sc.np):
         c = int(small_out.count[i])
         assert np.array_equal(small_out.key[i, :c], big_out.key[i, :c])
+
+
+def _drum_keys(orc, sc, pos):
+    """Drum surfaces each particle touches (FP64 predicate), in order: the
+    shell (side 0), then the cap on its side (1 above the drum's middle, 2
+    below)."""
+    out = [[] for _ in range(sc.np)]
+    for q, dr in enumerate(sc.drums):
+        o = np.array([dr.origin.x, dr.origin.y, dr.origin.z])
+        ax = np.array([dr.axis.x, dr.axis.y, dr.axis.z])
+        for i in range(sc.np):
+            d = pos[i, :3] - o
+            a = d @ ax
+            if not abs(a) <= dr.half_length:
+                continue
+            qv = d - a * ax
+            rho, r = np.sqrt(qv @ qv), pos[i, 3]
+            if r + rho - dr.radius > 0 and rho > 0:
+                out[i].append(orc.ORC_DRUM_KEY | q << 2)
+            if dr.caps and r - (dr.half_length - abs(a)) > 0 and rho < dr.radius:
+                out[i].append(orc.ORC_DRUM_KEY | q << 2 | (1 if a >= 0 else 2))
+    return out
+
+
+def test_drum_empty_history_equals_reference_law(orc):
+    sc, vel, om = drum_history_scene(n_side=7, seed=52)
+    w = _world(orc, sc, vel, om)
+    hin, hout = orc.History(sc.np), orc.History(sc.np)
+    f, m = w.hist_contacts(hin, hout)
+    p = sc.params()
+    fr = np.zeros((sc.np, 3))
+    mr = np.zeros((sc.np, 3))
+    mass, _ = sc.mass_inertia()
+    pairs = orc.p2p(p, 1, w.pos, w.vel, w.omega, fr, mr, mass, w.sorted, w.cstart, w.cend,
+                    sc.contact_pp(), want_pairs=True)
+    for pl in sc.planes:
+        orc.plane(p, 1, pl, w.pos, w.vel, w.omega, fr, mr, mass, sc.contact_pp())
+    fd = np.zeros((sc.np, 3))
+    for dr in sc.drums:
+        orc.drum(p, 1, dr, w.pos, w.vel, w.omega, fd, mr, mass, sc.contact_pp())
+    assert (np.abs(fd).max(axis=1) > 0).sum() > sc.np // 8
+    fr += fd
+    scale = np.abs(fr).max() + 1e-30
+    assert np.abs(f - fr).max() / scale < 1e-12
+    assert np.abs(m - mr).max() / (np.abs(mr).max() + 1e-30) < 1e-12
+    # keys: partners in traversal order, planes in order, then drum surfaces
+    pk, dk = _plane_keys(orc, sc, w.pos), _drum_keys(orc, sc, w.pos)
+    sides = {k & 3 for ks in dk for k in ks}
+    assert sides == {0, 1, 2}
+    for i in range(sc.np):
+        want = list(pairs[pairs[:, 0] == i, 1]) + pk[i] + dk[i]
+        keys, _ = hout.entries(i)
+        assert list(keys) == want, i
+        # touching the drum <=> a nonzero memoryless drum force
+        assert bool(dk[i]) == bool(np.abs(fd[i]).max() > 0), i
+
+
+def test_drum_spring_closed_form(orc):
+    """One particle on the shell at the bottom, one on the top cap, both at
+    rest, no gravity, the drum spinning at omega about z: each contact's
+    relative tangential velocity is the wall's surface velocity e (shell:
+    omega R x_hat; cap at radius rc: omega rc y_hat), so xi_K = K dt e and
+    Ft = (ks K dt + vs) e until mu |Fn|, where Ft saturates along e."""
+    r = float(scenes.as_f32_exact(2e-3))
+    R, HL, ov = 0.04, 0.03, 2e-6
+    rc = 0.01
+    pos = np.array([[0.0, -(R - r + ov), 0.0, r], [rc, 0.0, HL - r + ov, r]])
+    mu = 0.3
+    base = scenes.Scene(name="d", pos=pos, dt=1e-5, nsteps=1, world_origin=(-0.1, -0.1, -0.1),
+                        cell_size=2 * r, grid_size=(64, 64, 64), gravity=(0.0, 0.0, 0.0),
+                        contact=scenes.ContactSpec(friction=mu), tangential_history=True)
+    mass, _ = base.mass_inertia()
+    c = orc.force_constant(1, r, 0.0, mass[0], 0.0, base.contact_pp())
+    # overlaps as rounded in FP64 (shell: r + rho - R; cap: r - (HL - |a|));
+    # no normal velocity: |Fn| = kn delta^1.5
+    delta = [(r + (R - r + ov)) - R, r - (HL - (HL - r + ov))]
+    fns = [c.kn * d ** 1.5 for d in delta]
+    fn = fns[0]
+    # spin so that the shell contact saturates near K = 40
+    omega = mu * fn / ((c.ks * 40 * base.dt + c.vs) * R)
+    base.drums = [scenes.drum_info((0, 0, 0), (0, 0, 1), R, HL, omega, True)]
+    w = _world(orc, base, np.zeros((2, 3)), np.zeros((2, 3)))
+    e = [np.array([omega * R, 0.0, 0.0]), np.array([0.0, omega * rc, 0.0])]
+    unit = [np.array([0.0, -1.0, 0.0]), np.array([0.0, 0.0, 1.0])]
+    want_key = [orc.ORC_DRUM_KEY | 0, orc.ORC_DRUM_KEY | 1]
+    h = [orc.History(2), orc.History(2)]
+    sat = [None, None]
+    for K in range(1, 400):
+        f, m = w.hist_contacts(h[0], h[1])
+        assert np.abs(m).max() == 0  # wall contacts: ir = 0
+        for i in range(2):
+            fn = fns[i]
+            keys, xi = h[1].entries(i)
+            assert list(keys) == [want_key[i]]
+            ft = f[i] - (f[i] @ unit[i]) * unit[i]
+            fni = abs(f[i] @ unit[i])
+            assert abs(fni - fn) < 1e-12 * fn
+            en = np.linalg.norm(e[i])
+            lin = (c.ks * K * base.dt + c.vs) * en
+            if lin < mu * fn * (1 - 1e-9):
+                assert np.allclose(xi[0], K * base.dt * e[i], rtol=1e-12, atol=0)
+                assert np.allclose(ft, lin * e[i] / en, rtol=1e-12, atol=1e-18)
+            elif lin > mu * fn * (1 + 1e-9):
+                sat[i] = sat[i] or K
+                assert abs(np.linalg.norm(ft) - mu * fn) < 1e-12 * mu * fn
+                assert ft @ e[i] > 0 and np.linalg.norm(np.cross(ft, e[i])) < 1e-9 * mu * fn * en
+                assert abs(c.ks * (xi[0] @ e[i]) / en + c.vs * en - mu * fn) < 1e-12 * mu * fn
+        h = h[::-1]
+        if sat[0] and sat[1] and K > max(sat) + 5:
+            break
+    assert sat[0] is not None and 35 <= sat[0] <= 45
+    assert sat[1] is not None and sat[1] > sat[0]  # the cap point moves slower

@@ -95,6 +95,25 @@ def dense_scene(n_side=12, seed=11, pitch=1.97, jitter=0.04, vel_scale=0.05, ome
     return sc, vel, om
 
 
+def drum_history_scene(n_side=9, seed=51, omega=3.0, margin=0.05, planes=True, **kw):
+    """The dense lattice of dense_scene inside a drum spinning about z, history
+    on: the drum's end caps press on the lattice's top and bottom layers and
+    its shell on the particles of the four vertical edges (overlaps up to
+    margin radii); with planes, the open box of planes as well (plane springs
+    come before drum springs)."""
+    sc, vel, om = dense_scene(n_side=n_side, seed=seed, **kw)
+    pos, r = sc.pos, sc.pos[0, 3]
+    c = 0.5 * (pos[:, :3].min(0) + pos[:, :3].max(0))
+    d = pos[:, :3] - c
+    half = np.abs(d[:, 2]).max() + (1 - margin) * r
+    rad = np.hypot(d[:, 0], d[:, 1]).max() + (1 - margin) * r
+    sc.drums = [scenes.drum_info(tuple(c), (0, 0, 1), rad, half, omega, True)]
+    if not planes:
+        sc.planes = []
+    sc.tangential_history = True
+    return sc, vel, om
+
+
 # ------------------------------------------------ the older entry points
 def setb_params(sc, np_, nsp=0, kn=0.0, vn=0.0, ks=0.0, vs=0.0, mu=0.3, cohesive=0.0):
     """Parameter block of the older entry points (include/xdem_setb.h) for a
