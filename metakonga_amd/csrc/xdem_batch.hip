@@ -731,12 +731,18 @@ int32_t launch_pos(xdem_batch* b, bool adv, const BlockEntry* blocks, uint32_t n
     if (!nblk) return XDEM_OK;
     BState& A = b->buf[b->cur];
     const MoverOut mv = b->mover_out(movers, A.zi, nullptr);
+    // over the contact kernel's block table the advance records its
+    // displacement for the list bound (the lists carry over); otherwise every
+    // list is rebuilt at the next sort
+    const bool same = adv && blocks == b->d_blocks_sorted && nblk == b->nblk_sorted;
+    uint32_t* wd = same ? b->wdisp : nullptr;
+    uint32_t* fc = same ? nullptr : b->force;
     if (adv)
         hipLaunchKernelGGL(k_pos<true>, dim3(nblk), dim3(b->block_threads), 0, b->st, A, b->knext(),
-                           b->d_scenes, blocks, mv, b->force);
+                           b->d_scenes, blocks, mv, fc, wd);
     else
         hipLaunchKernelGGL(k_pos<false>, dim3(nblk), dim3(b->block_threads), 0, b->st, A,
-                           b->knext(), b->d_scenes, blocks, mv, b->force);
+                           b->knext(), b->d_scenes, blocks, mv, fc, wd);
     XDEM_LAUNCHED();
     return XDEM_OK;
 }
@@ -801,6 +807,7 @@ StepArgs step_args(xdem_batch* b, bool dbg) {
     a.dirty = b->dirty;
     a.dtag = b->counters + 6 + b->skcur;  // the tag of the last sort (k_patch, parity skcur)
     a.wdisp = b->wdisp;
+    a.force = b->force;
     a.scenes = b->d_scenes;
     a.planes = b->d_planes;
     a.drums = b->d_drums;
@@ -877,10 +884,18 @@ int32_t launch_step(xdem_batch* b, bool adv) {
             XDEM_STEP_LAUNCH(4, false, true, false);
         else
             XDEM_STEP_LAUNCH(2, false, true, false);
-    } else if (adv && b->flat && b->lists_on && b->vl) {
-        // candidate lists (per-lane walk where a list is rebuilt)
-#define XDEM_VLIST_LAUNCH(U, P) \
-    hipLaunchKernelGGL((k_step<true, false, U, P, false, false, true, true>), g, t, 0, b->st, a)
+    } else if (b->flat && b->lists_on && b->vl) {
+        // candidate lists (per-lane walk where a list is rebuilt); the lists
+        // are kept through a block's last step (no advance) too
+#define XDEM_VLIST_LAUNCH(U, P)                                                                  \
+    do {                                                                                       \
+        if (adv)                                                                               \
+            hipLaunchKernelGGL((k_step<true, false, U, P, false, false, true, true>), g, t, 0,  \
+                               b->st, a);                                                      \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_step<false, false, U, P, false, false, true, true>), g, t, 0, \
+                               b->st, a);                                                      \
+    } while (0)
         const int v = b->unroll * 2 + (b->pipe ? 1 : 0);
         switch (v) {
             case 2: XDEM_VLIST_LAUNCH(1, false); break;
@@ -1292,8 +1307,8 @@ int32_t xdem_batch_step(xdem_batch* b, uint32_t nsteps) {
 // unroll + 10000 * pipelined + 100000 * per-lane + 1000000 * lists) and the
 // per-candidate kernel time per launch (54 entries: walk-major, then block,
 // unroll, pipelined; -1 where a shape does not apply).  The step time of the
-// candidate-list shapes includes their list rebuilds (the first timed step of
-// each trial rebuilds every list).
+// candidate-list shapes includes their list rebuilds (the first step after a
+// shape without lists rebuilds every list).
 int32_t xdem_batch_autotune(xdem_batch* b, uint32_t steps_per_trial, uint32_t* chosen,
                             double* ms_per_candidate) {
     XDEM_REQUIRE(b, "null batch");

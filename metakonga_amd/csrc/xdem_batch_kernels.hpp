@@ -211,6 +211,8 @@ struct StepArgs {
     const uint32_t* dtag;      // the last sort's tag
     uint32_t* wdisp;           // per block L of the launch: the largest displacement of this
                                // step (float bits, bounded above)
+    uint32_t* force;           // per scene: set by the variants that do not keep the lists
+                               // (the next sort then rebuilds every list of the scene)
     // parity probe outputs (per sorted position) — null in production
     float* dbg_f;              // 6 floats: force xyz, torque xyz
     uint32_t* dbg_cnt;
@@ -756,8 +758,8 @@ constexpr int step_waves(int unr, bool pipe, bool hist, bool geom) {
 template <bool ADV, bool DBG, int UNR, bool PIPE, bool HIST, bool GEOM = false, bool FLAT = true,
           bool VLIST = false>
 __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step(StepArgs A) {
-    static_assert(!VLIST || (ADV && !DBG && !HIST && !GEOM && FLAT),
-                  "candidate lists: advancing plane-only per-lane-walk variants");
+    static_assert(!VLIST || (!DBG && !HIST && !GEOM && FLAT),
+                  "candidate lists: plane-only per-lane-walk variants");
     // per-thread LDS lists, lane-contiguous (entry c of thread t at c * LDS_T + t)
     __shared__ uint32_t clist[LDS_T * CMAX];
     __shared__ uint32_t hkeys[HIST ? LDS_T * HSLOTS : 1];
@@ -795,6 +797,9 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     const BlockEntry be = A.blocks[L];
     const BScene S = A.scenes[be.scene];
     const uint32_t j = be.j0 + threadIdx.x;
+    // a variant without lists leaves them stale: the next sort rebuilds them
+    if constexpr (!VLIST)
+        if (A.force && threadIdx.x == 0) A.force[be.scene] = 1u;
     // Every lane arrives once with its wave's largest displacement (0 for a
     // lane without a particle); the lane that completes the block's count
     // writes the block's maximum (LDS atomics, one per wave and path).
@@ -1237,6 +1242,8 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
             const float d = fsqrt(ddx * ddx + ddy * ddy + ddz * ddz) * 1.0001f;
             disp_arrive(wave_max_hi16(__float_as_uint(d)));
         }
+    } else if constexpr (VLIST) {
+        disp_arrive(0u);  // no advance (a block's last step)
     }
     A.out.pr[j] = make_double4(nx, ny, nz, ip.w);
     A.out.pf[j] = pf_of(S, nx, ny, nz, o.r);
@@ -1322,33 +1329,51 @@ __global__ __launch_bounds__(256) void k_react_sum(const double* part, uint32_t 
 }
 
 // Verlet position update (optional) + cell key of every particle, in place,
-// in storage order (block start / parity prepare).
+// in storage order (block start / parity prepare).  force: every candidate
+// list of the block's scene is rebuilt at the next sort.  wdisp (with ADV,
+// when the block table is the contact kernel's): the block's largest
+// displacement instead, as the contact kernel records it, so that the lists
+// carry over the block start.
 template <bool ADV>
 __global__ __launch_bounds__(256) void k_pos(BState st, uint32_t* keynew, const BScene* scenes,
                                              const BlockEntry* blocks, MoverOut mv,
-                                             uint32_t* force) {
+                                             uint32_t* force, uint32_t* wdisp) {
+    __shared__ uint32_t smax;
     const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
     const BlockEntry be = blocks[L];
     const BScene S = scenes[be.scene];
-    // a step block starts: every candidate list of the scene is rebuilt
     if (force && threadIdx.x == 0) force[be.scene] = 1;
-    const uint32_t j = be.j0 + threadIdx.x;
-    if (j >= be.j1) return;
-    double4 p = st.pr[j];
-    double x = p.x, y = p.y, z = p.z;
-    if (ADV) {
-        float4 vm = st.vm[j], aa = st.aa[j];
-        x += S.dt * (double)vm.x + S.half2dt * (double)aa.x;
-        y += S.dt * (double)vm.y + S.half2dt * (double)aa.y;
-        z += S.dt * (double)vm.z + S.half2dt * (double)aa.z;
-        st.pr[j] = make_double4(x, y, z, p.w);
-        st.pf[j] = pf_of(S, x, y, z, (float)p.w);
+    if (wdisp) {  // uniform
+        if (threadIdx.x == 0) smax = 0;
+        __syncthreads();
     }
-    int cx, cy, cz;
-    cell_of(S, x, y, z, cx, cy, cz);
-    const uint32_t kn = key_of(S, cx, cy, cz);
-    keynew[j] = kn;
-    if (mv.on) record_mover(mv, j, kn, st.zi[j].y, be.scene);
+    const uint32_t j = be.j0 + threadIdx.x;
+    float d = 0.0f;
+    if (j < be.j1) {
+        double4 p = st.pr[j];
+        double x = p.x, y = p.y, z = p.z;
+        if (ADV) {
+            float4 vm = st.vm[j], aa = st.aa[j];
+            x += S.dt * (double)vm.x + S.half2dt * (double)aa.x;
+            y += S.dt * (double)vm.y + S.half2dt * (double)aa.y;
+            z += S.dt * (double)vm.z + S.half2dt * (double)aa.z;
+            st.pr[j] = make_double4(x, y, z, p.w);
+            st.pf[j] = pf_of(S, x, y, z, (float)p.w);
+            const float ddx = (float)(x - p.x), ddy = (float)(y - p.y), ddz = (float)(z - p.z);
+            d = fsqrt(ddx * ddx + ddy * ddy + ddz * ddz) * 1.0001f;
+        }
+        int cx, cy, cz;
+        cell_of(S, x, y, z, cx, cy, cz);
+        const uint32_t kn = key_of(S, cx, cy, cz);
+        keynew[j] = kn;
+        if (mv.on) record_mover(mv, j, kn, st.zi[j].y, be.scene);
+    }
+    if (wdisp) {
+        const uint32_t m = wave_max_hi16(__float_as_uint(d));  // every lane of the wave
+        if (__lane_id() == 0) atomicMax(&smax, m);
+        __syncthreads();
+        if (threadIdx.x == 0) wdisp[L] = smax;
+    }
 }
 
 // ------------------------------------------------------------------ sort
@@ -2218,7 +2243,8 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
     // ---- per scene (blocks in turn): the displacement bound since its lists
     // were built, the step's largest block value added; every list is rebuilt
     // when it reaches half the skin (both particles of a pair may have moved),
-    // at a block start, or on the full merge
+    // after a contact kernel without lists or a position update that did not
+    // record its displacement (force), or on the full merge
     if (A.lists)
         for (uint32_t sc = blockIdx.x; sc < A.nscenes; sc += gridDim.x) {
             const uint2 br = A.scene_blk[sc];

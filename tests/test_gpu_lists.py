@@ -5,8 +5,9 @@ every step): bit-identical states after every run.
 Lists keep, per particle, its neighbours within ri + rj + skin in the
 reference traversal order; contacts found from a list are the walk's
 contacts in the walk's order, so every result must match bit for bit —
-across skins, launch shapes, graph replay, scene mixes, few movers (the
-window patch) and many (the full merge and a rebuild of every list).
+across skins, launch shapes, graph replay, scene mixes, step-block lengths
+(the lists carry over block boundaries), few movers (the window patch) and
+many (the full merge and a rebuild of every list).
 """
 import numpy as np
 import pytest
@@ -88,6 +89,10 @@ VARIANTS = [
     dict(patch=True, lists=True, skin=0.05, threads=64, unroll=4),
     dict(patch=True, lists=True, skin=0.05, unroll=1, ahead=True),
     dict(patch=True, lists=True, skin=0.05, graphs=False),
+    # lists kept across many block boundaries (a block's last step does not
+    # advance; the next block's first position update records its displacement)
+    dict(patch=True, lists=True, skin=0.05, block=7),
+    dict(patch=True, lists=True, skin=0.02, block=1, graphs=False),
 ]
 
 
