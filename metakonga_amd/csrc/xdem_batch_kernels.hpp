@@ -211,8 +211,8 @@ struct StepArgs {
     const uint32_t* dtag;      // the last sort's tag
     uint32_t* wdisp;           // per block L of the launch: the largest displacement of this
                                // step (float bits, bounded above)
-    uint32_t* force;           // per scene: set by the variants that do not keep the lists
-                               // (the next sort then rebuilds every list of the scene)
+    uint32_t* force;           // per scene: set by the non-advancing variants that do not
+                               // keep the lists (the next sort rebuilds every list)
     // parity probe outputs (per sorted position) — null in production
     float* dbg_f;              // 6 floats: force xyz, torque xyz
     uint32_t* dbg_cnt;
@@ -797,8 +797,11 @@ __global__ __launch_bounds__(256, step_waves(UNR, PIPE, HIST, GEOM)) void k_step
     const BlockEntry be = A.blocks[L];
     const BScene S = A.scenes[be.scene];
     const uint32_t j = be.j0 + threadIdx.x;
-    // a variant without lists leaves them stale: the next sort rebuilds them
-    if constexpr (!VLIST)
+    // a variant without lists leaves them stale: the next sort rebuilds them.
+    // Every step block ends with a step that does not advance, so marking in
+    // those variants covers a whole block (and keeps the advancing variants'
+    // registers as they are)
+    if constexpr (!VLIST && !ADV)
         if (A.force && threadIdx.x == 0) A.force[be.scene] = 1u;
     // Every lane arrives once with its wave's largest displacement (0 for a
     // lane without a particle); the lane that completes the block's count
