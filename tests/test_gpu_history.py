@@ -229,15 +229,18 @@ def test_history_overflow_matches_oracle(lib, orc):
     b.close()
 
 
-def test_history_walks_and_blocks_bitexact(lib):
+@pytest.mark.parametrize("drum", [False, True])
+def test_history_walks_and_blocks_bitexact(lib, drum):
     """Tangential history under both neighbour walks and block sizes 64, 128
     and 256: identical states, history counts, keys and springs.  The per-lane
     walk keeps its rows past the second in the thread's own history-key slots
     (rows in slots 2i, 2i+1); in a dense lattice (pitch just under the
     diameter, cell size = the diameter) most particles have more than two
     non-empty neighbour rows, so that table is used on nearly every lane
-    while other waves of the block write their keys."""
-    sc, vel, om = history_scene(n_side=11, seed=41, jitter=0.08)
+    while other waves of the block write their keys.  With drum: the same
+    lattice inside a spinning drum (the history and drum variant)."""
+    sc, vel, om = (drum_history_scene if drum else history_scene)(n_side=11, seed=41,
+                                                                 jitter=0.08)
     ref = None
     for per_lane in (True, False):
         for block in (64, 128, 256):

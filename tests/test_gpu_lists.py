@@ -166,3 +166,33 @@ def test_batch_many_movers_window_patch(lib):
     for k in range(len(picks)):
         for key in out[0][k]:
             assert np.array_equal(out[0][k][key], out[1][k][key]), (picks[k], key)
+
+
+def test_lists_carry_over_step_calls(lib):
+    """A quiet scene keeps its lists from one step call to the next (the
+    block's last step keeps them, the next block's position update records
+    its displacement): no scene rebuilds every list at a block start, and the
+    result equals one long call and the walk."""
+    sc, vel, om = dense_scene(n_side=10, seed=21, vel_scale=0.05)
+
+    def run(lists, blocks):
+        b = Batch(sc.np, 1, int(np.prod(sc.grid_size)))
+        if lists:
+            b.lists(True, 0.05)
+        sid = b.add(sc, vel, om)
+        rebuilds = []
+        for k in blocks:
+            b.step(k)
+            rebuilds.append(b.list_rebuilds())
+        out = b.state(sid)
+        b.close()
+        return out, rebuilds
+
+    ref, _ = run(False, [40])
+    one, _ = run(True, [40])
+    many, rb = run(True, [10, 1, 1, 1, 7, 20])
+    for out in (one, many):
+        for key in ref:
+            assert np.array_equal(ref[key], out[key]), key
+    # the first call builds the lists; every later block start reuses them
+    assert rb[1:] == [0] * (len(rb) - 1), rb
