@@ -44,7 +44,7 @@ typedef struct xdem_batch_scene_desc {
     xdem_contact_pair_desc pp;     /* particle-particle pair (pp.wall = particle material);
                                       pp.cohesion >= 0 adds cohesionForce
                                       (mphysics_cuda_impl.cuh:259-279) to the normal
-                                      force; refused with tangential_history */
+                                      force (with tangential_history too) */
     xdem_contact_pair_desc wall;   /* particle-plane pair (wall.wall = wall material);
                                       wall contacts carry no cohesion term, as the
                                       FP64 entry points (xdem.h) */

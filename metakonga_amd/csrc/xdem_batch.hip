@@ -1107,8 +1107,6 @@ int32_t xdem_batch_add_scene_history(xdem_batch* b, const xdem_batch_scene_desc*
                  "history arrays need tangential_history on and all three arrays");
     XDEM_REQUIRE(d->pp.cohesion >= 0 && std::isfinite(d->pp.cohesion),
                  "cohesion must be finite and >= 0");
-    XDEM_REQUIRE(!(d->tangential_history && d->pp.cohesion != 0),
-                 "cohesion is not available together with tangential_history");
     if (hist_count)
         for (uint32_t i = 0; i < d->np; ++i) {
             XDEM_REQUIRE(hist_count[i] <= XDEM_BATCH_HISTORY_SLOTS, "history count %u > %d",

@@ -394,10 +394,11 @@ __device__ inline void dhs_f32(float kn, float vn, float ks, float vs, float mu,
 // force is capped and the spring reset to (Ft - vs e) / ks.  With xi = 0 on
 // entry this is dhs_f32's force whenever |e| != 0.
 __device__ inline void hist_f32(float kn, float vn, float ks, float vs, float mu, float dt, float ir,
-                                float delta, f3 dv, f3 u, f3& xi, f3& Ft, f3& Fn, f3& M) {
+                                float delta, f3 dv, f3 u, f3& xi, f3& Ft, f3& Fn, f3& M,
+                                float fca = 0.0f) {
     float fsn = -kn * (delta * fsqrt(delta));
     float fdn = vn * dotf(dv, u);
-    Fn = (fsn + fdn) * u;
+    Fn = ((fsn + fca) + fdn) * u;
     f3 e = dv - dotf(dv, u) * u;
     xi = xi - dotf(xi, u) * u;
     xi = xi + dt * e;
@@ -480,7 +481,7 @@ __device__ inline void pp_contact(const StepArgs& A, const BScene& S, const Own&
     if (HIST && hc.on) {
         f3 xi = hc.lookup(jid), cFt, cFn, cM;
         hist_f32(kn, vn, kn * S.sr, vn * S.sr, S.mu, S.dtf, o.r, cdist, dv, u, xi, cFt, cFn,
-                 cM);
+                 cM, fca);
         hc.store(jid, xi);
         sumF += cFn + cFt;
         sumM += cM;

@@ -1282,10 +1282,10 @@ namespace {
 // Tangential-spring DHS (no reference counterpart; the law is stated in
 // xdem_oracle.h).  xi is in/out.
 void hist_law(const xdem_device_force_constant& c, double dt, double ir, double cdist, V3 dv,
-              V3 unit, V3& xi, V3& Ft, V3& Fn, V3& M) {
+              V3 unit, V3& xi, V3& Ft, V3& Fn, V3& M, double fca = 0.0) {
     double fsn = -c.kn * std::pow(cdist, 1.5);
     double fdn = c.vn * dot(dv, unit);
-    Fn = (fsn + fdn) * unit;
+    Fn = (fsn + fca + fdn) * unit;
     V3 e = dv - dot(dv, unit) * unit;
     xi = xi - dot(xi, unit) * unit;
     xi = xi + dt * e;
@@ -1365,11 +1365,15 @@ void hist_one(const orc_world* w, const orc_hist* in, orc_hist* out, uint32_t id
                                 ? jvel + cross(jomega, -jr * unit) - (ivel + cross(iomega, ir * unit))
                                 : (jvel - ivel);
                     xdem_device_force_constant c = force_constant(1, ir, jr, im, w->mass[k], w->cp_pp);
+                    // cohesionForce on particle pairs, as the DHS kernel adds it
+                    const xdem_device_contact_property& cp = w->cp_pp;
+                    double fca = cohesion_force(ir, jr, cp.Ei, cp.Ej, cp.pri, cp.prj, cp.coh,
+                                                -c.kn * std::pow(cdist, 1.5));
                     V3 xi = hc.lookup(k), Ft, Fn, M;
-                    hist_law(c, p->dt, ir, cdist, rv, unit, xi, Ft, Fn, M);
+                    hist_law(c, p->dt, ir, cdist, rv, unit, xi, Ft, Fn, M, fca);
                     hc.store(k, xi, overflow);
                     add_mag(id, Fn + Ft,
-                            term_scale(c, cdist, 0.0,
+                            term_scale(c, cdist, fca,
                                        length(ivel) + length(iomega) * ir + length(jvel) +
                                            length(jomega) * jr,
                                        Ft));

@@ -201,7 +201,8 @@ void orc_step_omp(orc_world* w, uint32_t nsteps, int32_t nthreads);
  * particle id, ORC_WALL_KEY | wall index for a plane, or ORC_DRUM_KEY |
  * (drum index << 2 | side) for a drum (side 0 shell, 1 cap at +half_length,
  * 2 cap at -half_length); xi = spring (3 doubles).
- * Law (own definition): Fn as DHS; e = dv - (dv.n)n; xi <- xi - (xi.n)n + dt e;
+ * Law (own definition): Fn as DHS (particle pairs with the cohesionForce term);
+ * e = dv - (dv.n)n; xi <- xi - (xi.n)n + dt e;
  * Ft = ks xi + vs e; if |Ft| > mu |Fn| then Ft <- mu |Fn| Ft / |Ft| and
  * xi <- (Ft - vs e) / ks (the spring that reproduces the capped force);
  * M = (ri n) x Ft (walls: ri = 0).  With xi = 0 on entry this

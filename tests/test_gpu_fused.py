@@ -105,16 +105,6 @@ def test_probe_cohesion(lib, orc):
     assert rel_error(out[1e5]["force"], out[0.0]["force"], scale) > 1e-2
 
 
-def test_cohesion_refused_with_history(lib):
-    sc, vel, om = dense_scene(n_side=4, seed=7)
-    sc.contact = scenes.ContactSpec(cohesion=1e3)
-    sc.tangential_history = True
-    b = Batch(sc.np, 1, int(np.prod(sc.grid_size)), lib)
-    with pytest.raises(abi.XdemError, match="cohesion"):
-        b.add(sc, vel, om)
-    b.close()
-
-
 def test_first_step_vs_oracle(lib, orc):
     """One full step from an identical state: velocities within the FP32c bound."""
     sc, vel, om = dense_scene(n_side=10, seed=5)

@@ -76,8 +76,11 @@ def history_scene(**kw):
     return sc, vel, om
 
 
-def test_history_steps_vs_oracle(lib, orc):
+@pytest.mark.parametrize("coh", [0.0, 1e5])
+def test_history_steps_vs_oracle(lib, orc, coh):
+    """coh > 0: the particle pairs' cohesionForce term with the springs."""
     sc, vel, om = history_scene(n_side=9, seed=31)
+    sc.contact = scenes.ContactSpec(cohesion=coh)
     b = Batch(sc.np, 1, int(np.prod(sc.grid_size)))
     b.graphs(False)
     sid = b.add(sc, vel, om)

@@ -18,6 +18,7 @@ held against the spinning shell or a cap loads its spring linearly, K dt e
 with e the wall's surface velocity, up to the Coulomb cap.
 """
 import numpy as np
+import pytest
 
 from metakonga_amd import scenes
 from xdem_testutil import dense_scene, drum_history_scene
@@ -46,8 +47,12 @@ def _plane_keys(orc, sc, pos):
     return out
 
 
-def test_empty_history_equals_reference_law(orc):
+@pytest.mark.parametrize("coh", [0.0, 1e5])
+def test_empty_history_equals_reference_law(orc, coh):
+    """With coh > 0 the particle pairs carry the reference's cohesionForce term
+    (walls none, as the GPU kernels)."""
     sc, vel, om = dense_scene(n_side=7, seed=21)  # the outer layer touches the walls
+    sc.contact = scenes.ContactSpec(cohesion=coh)
     w = _world(orc, sc, vel, om)
     hin, hout = orc.History(sc.np), orc.History(sc.np)
     f, m = w.hist_contacts(hin, hout)
