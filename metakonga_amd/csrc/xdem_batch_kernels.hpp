@@ -1864,7 +1864,8 @@ __device__ inline uint32_t lds_count_lt64(const uint64_t* v, uint32_t m, uint64_
 }
 
 // Ranks among the keys of threads 0..M-1 (M <= PATCH_T; ties by thread
-// index), for one key set or two at once.  The keys form chunks of 64 (by
+// index), for one key set or two at once (TWO: both sets hold distinct keys,
+// as the movers' (key, id) and storage indices do, and no tie is tested).  The keys form chunks of 64 (by
 // thread); every wave of the block takes a share of the comparisons:
 // first each chunk's 64 x 64 (its keys' ranks inside the chunk; the chunk is
 // then written in that order to cb1 / cb2), then, per pair of chunks, a
@@ -1900,9 +1901,14 @@ __device__ inline uint2 block_rank(uint64_t k1, uint32_t k2, uint32_t M, uint64_
             const uint32_t x2 = TWO && mine ? kb2[e] : 0u;
             const uint32_t r0 = part * 64 / S, r1e = min((part + 1) * 64 / S, cn);
             uint32_t a1 = 0, a2 = 0;
+            // TWO (the movers): both keys are unique, so no tie needs breaking
             auto cmp = [&](uint32_t r, uint64_t o1, uint32_t o2) {
-                a1 += (o1 < x1 || (o1 == x1 && r < lane)) ? 1u : 0u;
-                if (TWO) a2 += (o2 < x2 || (o2 == x2 && r < lane)) ? 1u : 0u;
+                if constexpr (TWO) {
+                    a1 += o1 < x1 ? 1u : 0u;
+                    a2 += o2 < x2 ? 1u : 0u;
+                } else {
+                    a1 += (o1 < x1 || (o1 == x1 && r < lane)) ? 1u : 0u;
+                }
             };
             // the same entries for every lane (broadcasts), four in flight
             uint32_t r = r0;
@@ -1946,7 +1952,7 @@ __device__ inline uint2 block_rank(uint64_t k1, uint32_t k2, uint32_t M, uint64_
             if (lo1 < hi1) {
                 const uint32_t mid = (lo1 + hi1) >> 1;
                 const uint64_t o = cb1[b0 + mid];
-                if (o < x1 || (lower && o == x1))
+                if (o < x1 || (!TWO && lower && o == x1))
                     lo1 = mid + 1;
                 else
                     hi1 = mid;
@@ -1954,7 +1960,7 @@ __device__ inline uint2 block_rank(uint64_t k1, uint32_t k2, uint32_t M, uint64_
             if (TWO && lo2 < hi2) {
                 const uint32_t mid = (lo2 + hi2) >> 1;
                 const uint32_t o = cb2[b0 + mid];
-                if (o < x2 || (lower && o == x2))
+                if (o < x2)
                     lo2 = mid + 1;
                 else
                     hi2 = mid;
