@@ -1863,14 +1863,14 @@ __device__ inline uint32_t lds_count_lt64(const uint64_t* v, uint32_t m, uint64_
     return lo;
 }
 
-// Ranks among the keys of threads 0..M-1 (M <= PATCH_T; ties by thread
-// index), for one key set or two at once (TWO: both sets hold distinct keys,
-// as the movers' (key, id) and storage indices do, and no tie is tested).  The keys form chunks of 64 (by
+// Ranks among the distinct keys of threads 0..M-1 (M <= PATCH_T), for one
+// key set or two at once (the movers' (key, id) and storage indices, the
+// windows' (start, append index): no key repeats, so no tie is tested).  The keys form chunks of 64 (by
 // thread); every wave of the block takes a share of the comparisons:
 // first each chunk's 64 x 64 (its keys' ranks inside the chunk; the chunk is
 // then written in that order to cb1 / cb2), then, per pair of chunks, a
-// binary search of each key of one chunk in the other (equal keys of a lower
-// chunk precede).  Partial counts are summed with LDS atomics in R1 / R2.
+// binary search of each key of one chunk in the other.  Partial counts are
+// summed with LDS atomics in R1 / R2.
 // kb*, cb*, R*: PATCH_T entries of LDS each.  Every thread of the block calls
 // it; the buffers may be reused after the next barrier.
 template <bool TWO>
@@ -1901,14 +1901,9 @@ __device__ inline uint2 block_rank(uint64_t k1, uint32_t k2, uint32_t M, uint64_
             const uint32_t x2 = TWO && mine ? kb2[e] : 0u;
             const uint32_t r0 = part * 64 / S, r1e = min((part + 1) * 64 / S, cn);
             uint32_t a1 = 0, a2 = 0;
-            // TWO (the movers): both keys are unique, so no tie needs breaking
-            auto cmp = [&](uint32_t r, uint64_t o1, uint32_t o2) {
-                if constexpr (TWO) {
-                    a1 += o1 < x1 ? 1u : 0u;
-                    a2 += o2 < x2 ? 1u : 0u;
-                } else {
-                    a1 += (o1 < x1 || (o1 == x1 && r < lane)) ? 1u : 0u;
-                }
+            auto cmp = [&](uint64_t o1, uint32_t o2) {
+                a1 += o1 < x1 ? 1u : 0u;
+                if (TWO) a2 += o2 < x2 ? 1u : 0u;
             };
             // the same entries for every lane (broadcasts), four in flight
             uint32_t r = r0;
@@ -1921,9 +1916,9 @@ __device__ inline uint2 block_rank(uint64_t k1, uint32_t k2, uint32_t M, uint64_
                     o2[u] = TWO ? kb2[b0 + r + u] : 0u;
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) cmp(r + u, o1[u], o2[u]);
+                for (int u = 0; u < 4; ++u) cmp(o1[u], o2[u]);
             }
-            for (; r < r1e; ++r) cmp(r, kb1[b0 + r], TWO ? kb2[b0 + r] : 0u);
+            for (; r < r1e; ++r) cmp(kb1[b0 + r], TWO ? kb2[b0 + r] : 0u);
             if (mine && r0 < r1e) {
                 atomicAdd(&R1[e], a1);
                 if (TWO) atomicAdd(&R2[e], a2);
@@ -1946,13 +1941,12 @@ __device__ inline uint2 block_rank(uint64_t k1, uint32_t k2, uint32_t M, uint64_
         const uint64_t x1 = kb1[e];
         const uint32_t x2 = TWO ? kb2[e] : 0u;
         const uint32_t b0 = cs * 64, m = min(64u, M - b0);
-        const bool lower = cs < ce;
         uint32_t lo1 = 0, hi1 = m, lo2 = 0, hi2 = TWO ? m : 0u;
         while (lo1 < hi1 || lo2 < hi2) {  // the two searches in step
             if (lo1 < hi1) {
                 const uint32_t mid = (lo1 + hi1) >> 1;
                 const uint64_t o = cb1[b0 + mid];
-                if (o < x1 || (!TWO && lower && o == x1))
+                if (o < x1)
                     lo1 = mid + 1;
                 else
                     hi1 = mid;
@@ -2048,7 +2042,10 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
             const uint32_t a = src < ins ? src : ins, b = src < ins ? ins - 1 : src;
             wkey = ((uint64_t)a << 32) | b;
         }
-        const uint32_t rw = block_rank<false>(wkey, 0u, M, kb1, cb1, kb2, cb2, R1, R2).x;
+        // windows in start order (equal starts by append index: the merge
+        // below does not depend on their order)
+        const uint64_t wrank = ((uint64_t)(uint32_t)(wkey >> 32) << 32) | t;
+        const uint32_t rw = block_rank<false>(wrank, 0u, M, kb1, cb1, kb2, cb2, R1, R2).x;
         if (rec) wk[rw] = wkey;
         __syncthreads();
         // merged where they overlap or touch: a running maximum of the last

@@ -41,19 +41,19 @@ def count_le(arr, x):
 
 
 def block_rank(keys, ch=64):
-    """Rank of each key among all (ties by index), as the kernel's block_rank:
+    """Rank of each of distinct keys among all, as the kernel's block_rank:
     counting inside the key's chunk of ch, binary search over the other
-    chunks in key order (equal keys of a lower chunk precede)."""
+    chunks in key order (no tie is tested: the keys must be distinct)."""
     m = len(keys)
+    assert len(set(keys)) == m
     chunks = [sorted(keys[c:c + ch]) for c in range(0, m, ch)]
     out = []
     for t in range(m):
         w, k = t // ch, keys[t]
-        r = sum(1 for i in range(w * ch, min(m, w * ch + ch))
-                if keys[i] < k or (keys[i] == k and i < t))
+        r = sum(1 for i in range(w * ch, min(m, w * ch + ch)) if keys[i] < k)
         for c, srt in enumerate(chunks):
             if c != w:
-                r += bisect.bisect_right(srt, k) if c < w else bisect.bisect_left(srt, k)
+                r += bisect.bisect_left(srt, k)
         out.append(r)
     return out
 
@@ -81,9 +81,9 @@ def patch(keys, ids, newkeys, movers, cr_old, prev_windows, perm_in, ch=64):
         sOut[rk] = rk + rL[t] - below
         wA[t] = rS[t] if rS[t] < rL[t] else rL[t]
         wB[t] = rL[t] - 1 if rS[t] < rL[t] else rS[t]
-    # P2: windows ranked by (a, b), ties by append index (block_rank)
+    # P2: windows ranked by (a, append index) (block_rank; distinct keys)
     sWa = [0] * M; sWb = [0] * M
-    rws = block_rank([(wA[t] << 32) | wB[t] for t in range(M)], ch)
+    rws = block_rank([(wA[t] << 32) | t for t in range(M)], ch)
     for t in range(M):
         sWa[rws[t]] = wA[t]; sWb[rws[t]] = wB[t]
     # P3: merge by running max; compaction
@@ -220,12 +220,13 @@ def trial(rng, n, ncell, M, prev, ch=64):
 
 
 
-def test_block_rank_is_the_rank_with_ties_by_index():
+def test_block_rank_is_the_rank_of_distinct_keys():
     rng = np.random.default_rng(3)
     for it in range(300):
         m = int(rng.integers(0, 200))
-        keys = [int(x) for x in rng.integers(0, int(rng.integers(1, 40)), m)]
-        want = sorted(range(m), key=lambda t: (keys[t], t))
+        # distinct keys, clustered (a small first part and the index below it)
+        keys = [(int(x) << 32) | t for t, x in enumerate(rng.integers(0, int(rng.integers(1, 40)), m))]
+        want = sorted(range(m), key=lambda t: keys[t])
         for ch in (1, 5, 64):
             got = block_rank(keys, ch)
             assert sorted(got) == list(range(m))
@@ -245,5 +246,5 @@ def test_window_patch_matches_full_resort():
             b = int(rng.integers(a, min(n, a + 10)))
             prev.append((a, b))
             p = b + 2
-        # chunks of 64 as on the GPU, and of 7 (many chunks, ties across them)
+        # chunks of 64 as on the GPU, and of 7 (many chunks)
         trial(rng, n, ncell, M, prev, 64 if it % 2 else 7)
