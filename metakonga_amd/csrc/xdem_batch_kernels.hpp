@@ -1863,30 +1863,32 @@ __device__ inline uint32_t lds_count_lt64(const uint64_t* v, uint32_t m, uint64_
     return lo;
 }
 
-// Ranks among the distinct keys of threads 0..M-1 (M <= PATCH_T), for one
-// key set or two at once (the movers' (key, id) and storage indices, the
-// windows' (start, append index): no key repeats, so no tie is tested).  The keys form chunks of 64 (by
-// thread); every wave of the block takes a share of the comparisons:
-// first each chunk's 64 x 64 (its keys' ranks inside the chunk; the chunk is
-// then written in that order to cb1 / cb2), then, per pair of chunks, a
-// binary search of each key of one chunk in the other.  Partial counts are
-// summed with LDS atomics in R1 / R2.
+// Ranks among the keys of threads 0..M-1 (M <= PATCH_T) in three key sets at
+// once: the movers' (key, id), their storage indices and the windows' (start,
+// append index).  Each set holds distinct keys, so no tie is tested.  The keys
+// form chunks of 64 (by thread); every wave of the block takes a share of the
+// comparisons: first each chunk's 64 x 64 (its keys' ranks inside the chunk;
+// the chunk is then written in that order to cb*), then, per pair of chunks,
+// a binary search of each key of one chunk in the other (the three searches
+// in step).  Partial counts are summed with LDS atomics in R*.
 // kb*, cb*, R*: PATCH_T entries of LDS each.  Every thread of the block calls
 // it; the buffers may be reused after the next barrier.
-template <bool TWO>
-__device__ inline uint2 block_rank(uint64_t k1, uint32_t k2, uint32_t M, uint64_t* kb1,
-                                   uint64_t* cb1, uint32_t* kb2, uint32_t* cb2, uint32_t* R1,
-                                   uint32_t* R2) {
+__device__ inline uint3 block_rank(uint64_t k1, uint32_t k2, uint64_t k3, uint32_t M,
+                                   uint64_t* kb1, uint64_t* cb1, uint32_t* kb2, uint32_t* cb2,
+                                   uint64_t* kb3, uint64_t* cb3, uint32_t* R1, uint32_t* R2,
+                                   uint32_t* R3) {
     constexpr uint32_t NWV = PATCH_T / 64;
     const uint32_t t = threadIdx.x, lane = t & 63;
     const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(t >> 6));
     const bool has = t < M;
     if (has) {
         kb1[t] = k1;
-        if (TWO) kb2[t] = k2;
+        kb2[t] = k2;
+        kb3[t] = k3;
     }
     R1[t] = 0;
-    if (TWO) R2[t] = 0;
+    R2[t] = 0;
+    R3[t] = 0;
     __syncthreads();
     const uint32_t nch = (M + 63) >> 6;
     // ---- inside each chunk: S waves per chunk, each a slice of its 64 keys
@@ -1898,30 +1900,37 @@ __device__ inline uint2 block_rank(uint64_t k1, uint32_t k2, uint32_t M, uint64_
             const uint32_t e = b0 + lane;  // the key this lane ranks
             const bool mine = lane < cn;
             const uint64_t x1 = mine ? kb1[e] : 0ull;
-            const uint32_t x2 = TWO && mine ? kb2[e] : 0u;
+            const uint32_t x2 = mine ? kb2[e] : 0u;
+            const uint64_t x3 = mine ? kb3[e] : 0ull;
             const uint32_t r0 = part * 64 / S, r1e = min((part + 1) * 64 / S, cn);
-            uint32_t a1 = 0, a2 = 0;
-            auto cmp = [&](uint64_t o1, uint32_t o2) {
-                a1 += o1 < x1 ? 1u : 0u;
-                if (TWO) a2 += o2 < x2 ? 1u : 0u;
-            };
+            uint32_t a1 = 0, a2 = 0, a3 = 0;
             // the same entries for every lane (broadcasts), four in flight
             uint32_t r = r0;
             for (; r + 4 <= r1e; r += 4) {
-                uint64_t o1[4];
+                uint64_t o1[4], o3[4];
                 uint32_t o2[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     o1[u] = kb1[b0 + r + u];
-                    o2[u] = TWO ? kb2[b0 + r + u] : 0u;
+                    o2[u] = kb2[b0 + r + u];
+                    o3[u] = kb3[b0 + r + u];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) cmp(o1[u], o2[u]);
+                for (int u = 0; u < 4; ++u) {
+                    a1 += o1[u] < x1 ? 1u : 0u;
+                    a2 += o2[u] < x2 ? 1u : 0u;
+                    a3 += o3[u] < x3 ? 1u : 0u;
+                }
             }
-            for (; r < r1e; ++r) cmp(kb1[b0 + r], TWO ? kb2[b0 + r] : 0u);
+            for (; r < r1e; ++r) {
+                a1 += kb1[b0 + r] < x1 ? 1u : 0u;
+                a2 += kb2[b0 + r] < x2 ? 1u : 0u;
+                a3 += kb3[b0 + r] < x3 ? 1u : 0u;
+            }
             if (mine && r0 < r1e) {
                 atomicAdd(&R1[e], a1);
-                if (TWO) atomicAdd(&R2[e], a2);
+                atomicAdd(&R2[e], a2);
+                atomicAdd(&R3[e], a3);
             }
         }
     }
@@ -1929,7 +1938,8 @@ __device__ inline uint2 block_rank(uint64_t k1, uint32_t k2, uint32_t M, uint64_
     const uint32_t c0 = w * 64;
     if (has) {
         cb1[c0 + R1[t]] = k1;
-        if (TWO) cb2[c0 + R2[t]] = k2;
+        cb2[c0 + R2[t]] = k2;
+        cb3[c0 + R3[t]] = k3;
     }
     __syncthreads();
     // ---- across chunks: pair q = (chunk of the keys, chunk searched)
@@ -1938,33 +1948,39 @@ __device__ inline uint2 block_rank(uint64_t k1, uint32_t k2, uint32_t M, uint64_
         if (ce == cs) continue;
         const uint32_t e = ce * 64 + lane;
         if (e >= M) continue;
-        const uint64_t x1 = kb1[e];
-        const uint32_t x2 = TWO ? kb2[e] : 0u;
+        const uint64_t x1 = kb1[e], x3 = kb3[e];
+        const uint32_t x2 = kb2[e];
         const uint32_t b0 = cs * 64, m = min(64u, M - b0);
-        uint32_t lo1 = 0, hi1 = m, lo2 = 0, hi2 = TWO ? m : 0u;
-        while (lo1 < hi1 || lo2 < hi2) {  // the two searches in step
+        uint32_t lo1 = 0, hi1 = m, lo2 = 0, hi2 = m, lo3 = 0, hi3 = m;
+        while (lo1 < hi1 || lo2 < hi2 || lo3 < hi3) {  // the three searches in step
             if (lo1 < hi1) {
                 const uint32_t mid = (lo1 + hi1) >> 1;
-                const uint64_t o = cb1[b0 + mid];
-                if (o < x1)
+                if (cb1[b0 + mid] < x1)
                     lo1 = mid + 1;
                 else
                     hi1 = mid;
             }
-            if (TWO && lo2 < hi2) {
+            if (lo2 < hi2) {
                 const uint32_t mid = (lo2 + hi2) >> 1;
-                const uint32_t o = cb2[b0 + mid];
-                if (o < x2)
+                if (cb2[b0 + mid] < x2)
                     lo2 = mid + 1;
                 else
                     hi2 = mid;
             }
+            if (lo3 < hi3) {
+                const uint32_t mid = (lo3 + hi3) >> 1;
+                if (cb3[b0 + mid] < x3)
+                    lo3 = mid + 1;
+                else
+                    hi3 = mid;
+            }
         }
         if (lo1) atomicAdd(&R1[e], lo1);
-        if (TWO && lo2) atomicAdd(&R2[e], lo2);
+        if (lo2) atomicAdd(&R2[e], lo2);
+        if (lo3) atomicAdd(&R3[e], lo3);
     }
     __syncthreads();
-    return make_uint2(has ? R1[t] : 0u, TWO && has ? R2[t] : 0u);
+    return has ? make_uint3(R1[t], R2[t], R3[t]) : make_uint3(0u, 0u, 0u);
 }
 
 // Each block works out the movers' order and the merged windows in LDS, thread
@@ -1993,7 +2009,8 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
     __shared__ uint32_t sAsc[FEW];                // storage indices ascending
     __shared__ uint64_t kb1[FEW], cb1[FEW];       // block_rank buffers
     __shared__ uint32_t kb2[FEW], cb2[FEW];
-    __shared__ uint32_t R1[FEW], R2[FEW];         // block_rank counts
+    __shared__ uint64_t kb3[FEW], cb3[FEW];
+    __shared__ uint32_t R1[FEW], R2[FEW], R3[FEW];  // block_rank counts
     __shared__ uint64_t wk[FEW];                  // windows (first << 32 | last) in order
     __shared__ uint32_t sWin[2 * FEW];            // merged windows: (first, last) pairs, ordered
     __shared__ uint32_t sPrev[2 * FEW];           // the previous sort's windows
@@ -2027,27 +2044,26 @@ __global__ __launch_bounds__(PATCH_T) void k_patch(PatchArgs A, SortArgs SA) {
     const uint4 mo = rec ? A.mold[t] : make_uint4(0u, 0u, 0u, 0u);
     uint32_t nw = 0;
     if (few && M > 0) {  // (no movers: no new windows)
-        const uint2 rk = block_rank<true>(mk, src, M, kb1, cb1, kb2, cb2, R1, R2);
+        // the mover's window: the positions between its old storage index and
+        // its insertion point in the previous order; windows go in start order
+        // (equal starts by append index: the merge below does not depend on
+        // their order)
+        const uint32_t wa = src < ins ? src : ins, wb = src < ins ? ins - 1 : src;
+        const uint64_t wkey = ((uint64_t)wa << 32) | wb;
+        const uint64_t wrank = ((uint64_t)wa << 32) | t;
+        const uint3 rk = block_rank(mk, src, wrank, M, kb1, cb1, kb2, cb2, kb3, cb3, R1, R2, R3);
         if (rec) {
             sX[rk.x] = mk;
             sSrc[rk.x] = src;
             sScene[rk.x] = mo.y;
             sAsc[rk.y] = src;
+            wk[rk.z] = wkey;
         }
         __syncthreads();
-        uint64_t wkey = 0;
         if (rec) {
             const uint32_t below = lds_count_lt(sAsc, M, 1, ins);
             sOut[rk.x] = rk.x + ins - below;
-            const uint32_t a = src < ins ? src : ins, b = src < ins ? ins - 1 : src;
-            wkey = ((uint64_t)a << 32) | b;
         }
-        // windows in start order (equal starts by append index: the merge
-        // below does not depend on their order)
-        const uint64_t wrank = ((uint64_t)(uint32_t)(wkey >> 32) << 32) | t;
-        const uint32_t rw = block_rank<false>(wrank, 0u, M, kb1, cb1, kb2, cb2, R1, R2).x;
-        if (rec) wk[rw] = wkey;
-        __syncthreads();
         // merged where they overlap or touch: a running maximum of the last
         // positions decides where a merged window starts and ends
         const bool valid = t < M;
